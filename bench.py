@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the MDWT hot path on one MI355X.
+
+A step is one 1-spp frame of the workload `sync_4ch256`: 4 channels of 256^3
+uint8 (seeded fBm + 16 Gaussian blobs each), 8-point transfer functions with
+distinct hues, 16^3 macrocells, 1024x1024 pixels, plain syncDDA (-m 2), every
+frame accumulated. The volume, majorants and accumulation buffer are resident
+in HBM before the timed region; the timed region is `--steps` back-to-back
+frames of the frame kernel.
+
+Prints one JSON line. The roofline object prices the frame kernel against
+HBM: algorithmic bytes per frame B = 8*bpv*S + 4*k*V + 32*P (S tentative
+collisions, each one channel's 8-voxel trilinear footprint; V macrocell
+visits, k majorant floats per visit: C for -m 2, 1 for -m 0 passes and -m 3;
+P pixels, float4 accumulator read + write), S and V from the kernel's
+counting pass. The CPU baseline is the oracle (a scalar C++ port of the
+reference's device code) timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "ms/frame @1spp 1024²; time-to-RMSE 1e-3, denoised vs plain, 1 GPU"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def frame_counts(r, frames: int) -> dict:
+    """Average per-frame work counters over `frames` consecutive frames."""
+    tot = {"samples": 0, "visits": 0, "rejections": 0, "rays": 0}
+    for _ in range(frames):
+        c = r.count_frame()
+        for k in tot:
+            tot[k] += c[k]
+        r.render(1)
+    return {k: v / frames for k, v in tot.items()}
+
+
+def algorithmic_bytes(counts: dict, scene, mode: int) -> float:
+    bpv = scene.bytes_per_voxel()
+    k = scene.channels if mode == 2 else 1
+    return 8.0 * bpv * counts["samples"] + 4.0 * k * counts["visits"] + 32.0 * scene.width * scene.height
+
+
+def read_traffic(workload: str, mode: int):
+    p = ROOT / "profiles" / f"traffic_{workload}_m{mode}.json"
+    if not p.exists():
+        return None, None
+    d = json.loads(p.read_text())
+    return d.get("bytes_per_launch"), str(p.relative_to(ROOT))
+
+
+def cpu_baseline(scene, mode: int, frames: int) -> dict:
+    from oracle import pyoracle
+    from mdwtracker_amd import scenes
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = pyoracle.OracleRenderer(threads=threads)
+    scenes.load_scene(o, scene, mode)
+    t0 = time.perf_counter()
+    o.render(frames, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(dt * 1000.0 / frames, 3), "unit": "ms/frame", "cores": threads,
+            "kind": "port",
+            "sample": f"{scene.name} -m {mode}, {scene.width}x{scene.height}, {frames} full frames "
+                      f"(of {scene.frames}), OpenMP over pixel rows, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=256)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--mode", type=int, default=2, choices=(0, 2, 3))
+    ap.add_argument("--workload", default="sync_4ch256")
+    ap.add_argument("--cpu-frames", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra-modes", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from mdwtracker_amd import capi, scenes
+
+    t = time.perf_counter()
+    scene = scenes.make_scene(args.workload)
+    log(f"[bench] scene {scene.name}: {scene.channels} x {scene.volumes[0].shape} "
+        f"{scene.volumes[0].dtype}, {scene.width}x{scene.height}, mc {scene.mc} "
+        f"({time.perf_counter() - t:.1f} s)")
+    r = capi.MdwtRenderer(0)
+    scenes.load_scene(r, scene, args.mode)
+    bw = r.stream_copy(2 << 30, 5)
+    log(f"[bench] measured copy bandwidth {bw:.0f} GB/s (2 GiB read + 2 GiB write)")
+
+    r.render(args.warmup)
+    counts = frame_counts(r, 4)
+    r.reset_accumulation()
+    t0 = time.perf_counter()
+    gpu_ms = r.render(args.steps)
+    t1 = time.perf_counter()
+    kernel_ms = r.last_kernel_ms()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    log(f"[bench] -m {args.mode}: {args.steps} frames, wall {ms_per_step:.4f} ms/frame, "
+        f"kernel {kernel_ms:.4f} ms/frame, counts/frame {counts}")
+
+    B = algorithmic_bytes(counts, scene, args.mode)
+    achieved = B / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_src = read_traffic(scene.name, args.mode)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
+                "traffic": traffic, "algorithmic_bytes": round(B),
+                "measured_copy_gbps": round(bw, 1),
+                "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
+                "kernel": "render_kernel<2,u8>" if args.mode == 2 else f"render_kernel<{args.mode},u8>"}
+    if traffic_src:
+        roofline["traffic_source"] = traffic_src
+
+    extra_modes = {}
+    if not args.no_extra_modes:
+        for m in (0, 2, 3):
+            if m == args.mode:
+                continue
+            r.set_mode(m)
+            r.render(4)
+            r.reset_accumulation()
+            r.render(32)
+            extra_modes[f"m{m}_ms_per_frame"] = round(r.last_kernel_ms(), 4)
+        r.set_mode(args.mode)
+
+    line = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (u8 voxels)",
+        "data": "synthetic (seeded fBm + Gaussian blobs, seeded transfer functions)",
+        "config": {"workload": scene.name, "channels": scene.channels,
+                   "volume": list(scene.volumes[0].shape), "voxel": str(scene.volumes[0].dtype),
+                   "image": [scene.width, scene.height], "spp": scene.spp, "mode": args.mode,
+                   "macrocells": list(scene.mc), "camera": "orbit r=1.8",
+                   "parallelism": "single GPU"},
+        "kernel_ms_per_frame": round(kernel_ms, 4),
+        "samples_per_s": round(counts["samples"] / (kernel_ms * 1e-3)),
+        "cell_visits_per_s": round(counts["visits"] / (kernel_ms * 1e-3)),
+        "work_per_frame": {k: round(v) for k, v in counts.items()},
+        "time_to_rmse": None,
+        "roofline": roofline,
+    }
+    line.update(extra_modes)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(scene, args.mode, args.cpu_frames)
+        log(f"[bench] cpu baseline {line['cpu_baseline']}")
+    r.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

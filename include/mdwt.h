@@ -1,0 +1,152 @@
+/*
+ * mdwt.h — C ABI of the MI355X multi-density Woodcock tracking (MDWT) library
+ * (libmdwt.so, built from mdwtracker_amd/csrc).
+ *
+ * Drop-in boundary for the RAW-volume render path of alpers-git/MDWTracker.
+ * Each entry point replaces one member of dtracker::Renderer (renderer.h:52-222)
+ * or one step of its OWL launch; the reference interface it replaces is cited
+ * beside it. Conventions:
+ *   - every function returns an int status (MDWT_OK = 0); nothing throws or
+ *     exits across the ABI; mdwt_last_error() describes the last failure;
+ *   - plain pointers and sizes only; all device memory is owned by the handle;
+ *     caller buffers are host memory and are copied, never retained;
+ *   - one handle = one HIP device and one HIP stream; every call is
+ *     synchronous with respect to the host; a handle is not thread-safe
+ *     (as the reference Renderer is not).
+ */
+#ifndef MDWT_H
+#define MDWT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDWT_MAX_CHANNELS 12 /* deviceCode.h:13 MAX_CHANNELS */
+
+enum mdwt_status {
+  MDWT_OK = 0,
+  MDWT_ERR_ARG = 1,   /* invalid argument (bad channel id, size, dtype, NaN ...) */
+  MDWT_ERR_STATE = 2, /* call not valid in the current state (e.g. render before init) */
+  MDWT_ERR_HIP = 3,   /* HIP runtime error (no device, launch failure ...) */
+  MDWT_ERR_LIMIT = 4, /* a documented size limit was exceeded */
+  MDWT_ERR_OOM = 5    /* device allocation failed */
+};
+
+/* voxel types; value conversion follows rawFile.cpp:100-123 (u8 -> v*255,
+ * u16 -> v*65535, f16/f32 as is) */
+enum mdwt_dtype { MDWT_U8 = 0, MDWT_U16 = 1, MDWT_F16 = 2, MDWT_F32 = 3 };
+
+/* -m values (renderer.h:24-36 Mode): 0 N-DDA, 2 syncDDA, 3 cumulative majorant */
+enum mdwt_mode { MDWT_MODE_NDDA = 0, MDWT_MODE_SYNCDDA = 2, MDWT_MODE_CUMULATIVE = 3 };
+
+typedef struct mdwt_renderer mdwt_renderer;
+
+/* Renderer::Renderer (renderer.cpp:316) — binds HIP device `hip_device`.
+ * Touches no GPU until mdwt_init. */
+int mdwt_create(mdwt_renderer** out, int hip_device);
+/* Renderer::Terminate (renderer.cpp:1010-1014) + destructor */
+int mdwt_destroy(mdwt_renderer* h);
+/* text of the last error on this handle ("" if none); h may be NULL for
+ * failures of mdwt_create */
+const char* mdwt_last_error(const mdwt_renderer* h);
+
+/* Renderer::PushMesh(shared_ptr<RawR>) (renderer.cpp:1031-1044) with the RawR
+ * load (rawFile.cpp:31-128): copies nx*ny*nz voxels (x fastest). At most
+ * MDWT_MAX_CHANNELS channels (the reference's off-by-one 13th is rejected);
+ * all channels must share dims and dtype. */
+int mdwt_push_channel(mdwt_renderer* h, const void* voxels, int dtype, int nx, int ny, int nz);
+/* RawR::reshapeBounds(remap) (rawFile.cpp:228-237, the -cb flag,
+ * viewer.cpp:305-317): world box [0, remap] instead of [0,1]^3. Before init. */
+int mdwt_set_bounds(mdwt_renderer* h, const float remap[3]);
+/* Renderer::Init(mode, autoSetCamera) (renderer.cpp:324-942, RAW 638-903):
+ * uploads the volume, builds the value-range grid (renderer.cu:1116-1179).
+ * mc = macrocells per side (renderer.h:146 macrocellDims, -mc); 0 entries
+ * take CalculateMCGridDims(4) (renderer.cpp:1317-1361). */
+int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera);
+/* Switch -m without re-uploading (no reference counterpart: the reference
+ * re-runs Init); rebuilds the majorant grid. */
+int mdwt_set_mode(mdwt_renderer* h, int mode);
+
+/* SetXFColormap + SetXFOpacityScale + SetXFRange in one call
+ * (renderer.cpp:1094-1193): rgba = n_texels float4 (alpha is cubed on upload,
+ * renderer.cpp:1098-1101); rebuilds majorants (renderer.cu:206-274) once all
+ * channels have a transfer function. */
+int mdwt_set_tf(mdwt_renderer* h, int ch, const float* rgba, int n_texels, float opacity_scale,
+                float xf_lo, float xf_hi);
+/* the three reference setters individually */
+int mdwt_set_xf_colormap(mdwt_renderer* h, int ch, const float* rgba, int n_texels);
+int mdwt_set_xf_opacity_scale(mdwt_renderer* h, int ch, float opacity_scale);
+int mdwt_set_xf_range(mdwt_renderer* h, int ch, float xf_lo, float xf_hi);
+
+/* camera.setOrientation(from, at, up, fovy) + UpdateCamera()
+ * (camera.h:83-110, renderer.cpp:1058-1092) */
+int mdwt_set_camera(mdwt_renderer* h, const float from[3], const float at[3], const float up[3],
+                    float fovy_deg);
+/* Setdt (renderer.cpp:1460-1466) / Resetdt (renderer.cpp:1444-1453) */
+int mdwt_set_dt(mdwt_renderer* h, float dt);
+int mdwt_reset_dt(mdwt_renderer* h);
+/* fbSize/Resize (renderer.cpp:1046-1056), spp, enableAccumulation, bgColor */
+int mdwt_set_frame(mdwt_renderer* h, int width, int height, int spp, int accumulate,
+                   const float bg[3]);
+/* SetLightIntensity / SetAmbient (renderer.cpp:1476-1486) */
+int mdwt_set_lighting(mdwt_renderer* h, float light_intensity, float ambient);
+/* seed of the counter-based RNG (replaces the LCG seeding, deviceCode.cu:296-297) */
+int mdwt_set_seed(mdwt_renderer* h, uint32_t seed);
+
+/* Render() (renderer.cpp:944-973) n_frames times, synchronously. ms_out
+ * (may be NULL) receives the GPU time of the n_frames launches (HIP events). */
+int mdwt_render(mdwt_renderer* h, int n_frames, float* ms_out);
+/* ResetAccumulation (renderer.cpp:1488-1495) */
+int mdwt_reset_accumulation(mdwt_renderer* h);
+/* accumBuffer (float4 per pixel, device-resident in the reference) and the
+ * per-pixel running mean of squares (an addition, for confidence intervals);
+ * either pointer may be NULL. */
+int mdwt_read_accum(mdwt_renderer* h, float* rgba, float* m2);
+/* frameBuffer (uint32 RGBA8, renderer.cpp:908; make_rgba of the accumulated
+ * colour, deviceCode.cu:463-469) */
+int mdwt_read_fb(mdwt_renderer* h, uint32_t* rgba8);
+
+/* ---- inspection hooks (tests, bench) ---- */
+/* value-range grid: float2 (min, max) per (cell, channel), index cell*C + ch */
+int mdwt_read_ranges(mdwt_renderer* h, float* out);
+/* majorants: float per (cell, channel) for modes 0/2, float per cell for 3 */
+int mdwt_read_majorants(mdwt_renderer* h, float* out);
+/* out = {mc[3], dims[3], channels, mode, frame_id, accum_id} */
+int mdwt_get_info(mdwt_renderer* h, int out[10]);
+int mdwt_get_dt(mdwt_renderer* h, float* dt);
+/* origin, lower_left_corner, horizontal, vertical (LaunchParams camera) */
+int mdwt_get_camera(mdwt_renderer* h, float out[12]);
+/* per-channel value domain (renderer.cpp:743-745), 2 floats per channel */
+int mdwt_get_domains(mdwt_renderer* h, float* out);
+/* DDA over the majorant grid for n world rays (grid transform of the
+ * closest-hit programs + dda.h:82-203): up to max_cells (cell id, t0, t1) per
+ * ray, counts[r] = number of cells the DDA emits. */
+int mdwt_trace_cells(mdwt_renderer* h, const float* org, const float* dir, int n, int max_cells,
+                     int32_t* cells, float* t01, int32_t* counts);
+/* camera rays (origin, direction) of every pixel for (frame, sample)
+ * (deviceCode.cu:293-300 jitter + generateRay 61-76) */
+int mdwt_gen_rays(mdwt_renderer* h, uint32_t frame, uint32_t sample, float* org, float* dir);
+/* root-box clip (volumeIntersection, deviceCode.cu:1676-1726) of n rays */
+int mdwt_clip_rays(mdwt_renderer* h, const float* org, const float* dir, int n, float* t01,
+                   int32_t* hit);
+/* Counting pass: renders the next frame's work without writing the image or
+ * advancing the frame; out = {samples, cell visits, rejections, rays}
+ * (prd.samples / prd.rejections, deviceCode.cu:590,611; cell visits added). */
+int mdwt_count_frame(mdwt_renderer* h, uint64_t out[4]);
+/* Average duration in ms of the last mdwt_render's frame kernel launches. */
+int mdwt_last_kernel_ms(mdwt_renderer* h, float* ms);
+/* Tile order of the frame kernel: 0 = row-major, 1 = Hilbert (default);
+ * tile = 8, 16 or 32 pixels per side. Images do not depend on either. */
+int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile);
+/* Read-dominated device copy of `bytes` bytes, `iters` times; GB/s of
+ * (read + write) bytes. Measures the achievable HBM bandwidth. */
+int mdwt_stream_copy(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps);
+/* 2-D Hilbert curve of side 2^bits: index d -> (x, y) (host function, no GPU) */
+int mdwt_hilbert_d2xy(int bits, uint32_t d, uint32_t* x, uint32_t* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDWT_H */

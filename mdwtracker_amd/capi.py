@@ -1,0 +1,303 @@
+"""ctypes binding of libmdwt.so (include/mdwt.h) and a thin object wrapper.
+
+The library is the product: there is no Python or CPU fallback. Importing
+this module without a built libmdwt.so raises ImportError (build it with
+``python -m mdwtracker_amd.build`` or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libmdwt.so"
+
+U8, U16, F16, F32 = 0, 1, 2, 3
+MODE_NDDA, MODE_SYNCDDA, MODE_CUMULATIVE = 0, 2, 3
+MAX_CHANNELS = 12
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_STATE", 3: "ERR_HIP", 4: "ERR_LIMIT", 5: "ERR_OOM"}
+
+_DTYPES = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.float16): F16,
+           np.dtype(np.float32): F32}
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_h = C.c_void_p
+
+# name -> argtypes; every function returns int status unless listed in _RESTYPES
+SIGNATURES = {
+    "mdwt_create": [C.POINTER(C.c_void_p), C.c_int],
+    "mdwt_destroy": [_h],
+    "mdwt_last_error": [_h],
+    "mdwt_push_channel": [_h, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int],
+    "mdwt_set_bounds": [_h, _fp],
+    "mdwt_init": [_h, C.c_int, _ip, C.c_int],
+    "mdwt_set_mode": [_h, C.c_int],
+    "mdwt_set_tf": [_h, C.c_int, _fp, C.c_int, C.c_float, C.c_float, C.c_float],
+    "mdwt_set_xf_colormap": [_h, C.c_int, _fp, C.c_int],
+    "mdwt_set_xf_opacity_scale": [_h, C.c_int, C.c_float],
+    "mdwt_set_xf_range": [_h, C.c_int, C.c_float, C.c_float],
+    "mdwt_set_camera": [_h, _fp, _fp, _fp, C.c_float],
+    "mdwt_set_dt": [_h, C.c_float],
+    "mdwt_reset_dt": [_h],
+    "mdwt_set_frame": [_h, C.c_int, C.c_int, C.c_int, C.c_int, _fp],
+    "mdwt_set_lighting": [_h, C.c_float, C.c_float],
+    "mdwt_set_seed": [_h, C.c_uint32],
+    "mdwt_render": [_h, C.c_int, _fp],
+    "mdwt_reset_accumulation": [_h],
+    "mdwt_read_accum": [_h, _fp, _fp],
+    "mdwt_read_fb": [_h, _up],
+    "mdwt_read_ranges": [_h, _fp],
+    "mdwt_read_majorants": [_h, _fp],
+    "mdwt_get_info": [_h, _ip],
+    "mdwt_get_dt": [_h, _fp],
+    "mdwt_get_camera": [_h, _fp],
+    "mdwt_get_domains": [_h, _fp],
+    "mdwt_trace_cells": [_h, _fp, _fp, C.c_int, C.c_int, _ip, _fp, _ip],
+    "mdwt_gen_rays": [_h, C.c_uint32, C.c_uint32, _fp, _fp],
+    "mdwt_clip_rays": [_h, _fp, _fp, C.c_int, _fp, _ip],
+    "mdwt_count_frame": [_h, _u64p],
+    "mdwt_last_kernel_ms": [_h, _fp],
+    "mdwt_set_tile_order": [_h, C.c_int, C.c_int],
+    "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
+    "mdwt_hilbert_d2xy": [C.c_int, C.c_uint32, _up, _up],
+}
+_RESTYPES = {"mdwt_last_error": C.c_char_p}
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m mdwtracker_amd.build` "
+            "(hipcc, gfx950). There is no CPU fallback.")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    return lib
+
+
+lib = _load()
+
+
+class MdwtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _f(a) -> C.POINTER(C.c_float):
+    return a.ctypes.data_as(_fp)
+
+
+def _i(a) -> C.POINTER(C.c_int32):
+    return a.ctypes.data_as(_ip)
+
+
+def _vec3(v) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(3))
+
+
+def hilbert_d2xy(bits: int, d: int) -> tuple[int, int]:
+    x, y = C.c_uint32(), C.c_uint32()
+    rc = lib.mdwt_hilbert_d2xy(bits, d, C.byref(x), C.byref(y))
+    if rc:
+        raise MdwtError(rc, "hilbert_d2xy")
+    return x.value, y.value
+
+
+class MdwtRenderer:
+    """One libmdwt handle (one HIP device, one stream). Methods map 1:1 onto
+    the C ABI and raise MdwtError on a non-zero status."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib.mdwt_create(C.byref(h), device)
+        if rc:
+            raise MdwtError(rc, lib.mdwt_last_error(None).decode())
+        self._h = h
+        self.channels = 0
+        self.width, self.height = 1024, 1024
+
+    def close(self) -> None:
+        if self._h:
+            lib.mdwt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc: int) -> None:
+        if rc:
+            raise MdwtError(rc, lib.mdwt_last_error(self._h).decode())
+
+    # -- Renderer set-up --------------------------------------------------
+    def push_channel(self, vox: np.ndarray) -> None:
+        """vox: array of shape (nz, ny, nx) (x fastest), dtype u8/u16/f16/f32."""
+        vox = np.ascontiguousarray(vox)
+        if vox.ndim != 3 or vox.dtype not in _DTYPES:
+            raise ValueError("voxels must be a 3-D uint8/uint16/float16/float32 array")
+        nz, ny, nx = vox.shape
+        self._chk(lib.mdwt_push_channel(self._h, vox.ctypes.data, _DTYPES[vox.dtype], nx, ny, nz))
+        self.channels += 1
+
+    def set_bounds(self, remap) -> None:
+        self._chk(lib.mdwt_set_bounds(self._h, _f(_vec3(remap))))
+
+    def init(self, mode: int = 0, mc=(0, 0, 0), auto_camera: bool = True) -> None:
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mc, dtype=np.int32), (3,)))
+        self._chk(lib.mdwt_init(self._h, mode, _i(m), int(auto_camera)))
+
+    def set_mode(self, mode: int) -> None:
+        self._chk(lib.mdwt_set_mode(self._h, mode))
+
+    def set_tf(self, ch: int, rgba, opacity_scale: float = 1.0, xf_range=(0.0, 1.0)) -> None:
+        t = np.ascontiguousarray(np.asarray(rgba, dtype=np.float32).reshape(-1, 4))
+        self._chk(lib.mdwt_set_tf(self._h, ch, _f(t), t.shape[0], opacity_scale,
+                                  float(xf_range[0]), float(xf_range[1])))
+
+    def set_xf_colormap(self, ch: int, rgba) -> None:
+        t = np.ascontiguousarray(np.asarray(rgba, dtype=np.float32).reshape(-1, 4))
+        self._chk(lib.mdwt_set_xf_colormap(self._h, ch, _f(t), t.shape[0]))
+
+    def set_xf_opacity_scale(self, ch: int, s: float) -> None:
+        self._chk(lib.mdwt_set_xf_opacity_scale(self._h, ch, s))
+
+    def set_xf_range(self, ch: int, lo: float, hi: float) -> None:
+        self._chk(lib.mdwt_set_xf_range(self._h, ch, lo, hi))
+
+    def set_camera(self, frm, at, up, fovy: float) -> None:
+        self._chk(lib.mdwt_set_camera(self._h, _f(_vec3(frm)), _f(_vec3(at)), _f(_vec3(up)), fovy))
+
+    def set_dt(self, dt: float) -> None:
+        self._chk(lib.mdwt_set_dt(self._h, dt))
+
+    def reset_dt(self) -> None:
+        self._chk(lib.mdwt_reset_dt(self._h))
+
+    def set_frame(self, width: int, height: int, spp: int = 1, accumulate: bool = True,
+                  bg=(0.0, 0.0, 0.0)) -> None:
+        self._chk(lib.mdwt_set_frame(self._h, width, height, spp, int(accumulate), _f(_vec3(bg))))
+        self.width, self.height = width, height
+
+    def set_lighting(self, light_intensity: float, ambient: float) -> None:
+        self._chk(lib.mdwt_set_lighting(self._h, light_intensity, ambient))
+
+    def set_seed(self, seed: int) -> None:
+        self._chk(lib.mdwt_set_seed(self._h, seed & 0xFFFFFFFF))
+
+    def set_tile_order(self, order: int, tile: int = 16) -> None:
+        self._chk(lib.mdwt_set_tile_order(self._h, order, tile))
+
+    # -- frames -------------------------------------------------------------
+    def render(self, n_frames: int = 1) -> float:
+        ms = C.c_float()
+        self._chk(lib.mdwt_render(self._h, n_frames, C.byref(ms)))
+        return ms.value
+
+    def reset_accumulation(self) -> None:
+        self._chk(lib.mdwt_reset_accumulation(self._h))
+
+    def read_accum(self, with_m2: bool = False):
+        a = np.empty((self.height, self.width, 4), np.float32)
+        m = np.empty_like(a) if with_m2 else None
+        self._chk(lib.mdwt_read_accum(self._h, _f(a), _f(m) if with_m2 else None))
+        return (a, m) if with_m2 else a
+
+    def read_fb(self) -> np.ndarray:
+        a = np.empty((self.height, self.width), np.uint32)
+        self._chk(lib.mdwt_read_fb(self._h, a.ctypes.data_as(_up)))
+        return a
+
+    # -- inspection ---------------------------------------------------------
+    def info(self) -> dict:
+        o = np.zeros(10, np.int32)
+        self._chk(lib.mdwt_get_info(self._h, _i(o)))
+        return {"mc": tuple(int(v) for v in o[0:3]), "dims": tuple(int(v) for v in o[3:6]),
+                "channels": int(o[6]), "mode": int(o[7]), "frame_id": int(o[8]),
+                "accum_id": int(o[9])}
+
+    def read_ranges(self) -> np.ndarray:
+        i = self.info()
+        n = int(np.prod(i["mc"])) * i["channels"]
+        a = np.empty((n, 2), np.float32)
+        self._chk(lib.mdwt_read_ranges(self._h, _f(a)))
+        return a
+
+    def read_majorants(self) -> np.ndarray:
+        i = self.info()
+        n = int(np.prod(i["mc"])) * (i["channels"] if i["mode"] <= 2 else 1)
+        a = np.empty(n, np.float32)
+        self._chk(lib.mdwt_read_majorants(self._h, _f(a)))
+        return a
+
+    def get_dt(self) -> float:
+        v = C.c_float()
+        self._chk(lib.mdwt_get_dt(self._h, C.byref(v)))
+        return v.value
+
+    def get_camera(self) -> np.ndarray:
+        a = np.empty(12, np.float32)
+        self._chk(lib.mdwt_get_camera(self._h, _f(a)))
+        return a.reshape(4, 3)
+
+    def get_domains(self) -> np.ndarray:
+        a = np.empty(2 * max(self.channels, 1), np.float32)
+        self._chk(lib.mdwt_get_domains(self._h, _f(a)))
+        return a.reshape(-1, 2)[: self.channels]
+
+    def trace_cells(self, org, dirs, max_cells: int):
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = org.shape[0]
+        cells = np.full((n, max_cells), -1, np.int32)
+        t01 = np.zeros((n, max_cells, 2), np.float32)
+        counts = np.zeros(n, np.int32)
+        self._chk(lib.mdwt_trace_cells(self._h, _f(org), _f(dirs), n, max_cells, _i(cells),
+                                       _f(t01), _i(counts)))
+        return cells, t01, counts
+
+    def gen_rays(self, frame: int, sample: int):
+        o = np.empty((self.height * self.width, 3), np.float32)
+        d = np.empty_like(o)
+        self._chk(lib.mdwt_gen_rays(self._h, frame, sample, _f(o), _f(d)))
+        return o, d
+
+    def clip_rays(self, org, dirs):
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = org.shape[0]
+        t01 = np.zeros((n, 2), np.float32)
+        hit = np.zeros(n, np.int32)
+        self._chk(lib.mdwt_clip_rays(self._h, _f(org), _f(dirs), n, _f(t01), _i(hit)))
+        return t01, hit
+
+    def count_frame(self) -> dict:
+        o = np.zeros(4, np.uint64)
+        self._chk(lib.mdwt_count_frame(self._h, o.ctypes.data_as(_u64p)))
+        return {"samples": int(o[0]), "visits": int(o[1]), "rejections": int(o[2]),
+                "rays": int(o[3])}
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_float()
+        self._chk(lib.mdwt_last_kernel_ms(self._h, C.byref(v)))
+        return v.value
+
+    def stream_copy(self, nbytes: int, iters: int = 10) -> float:
+        v = C.c_float()
+        self._chk(lib.mdwt_stream_copy(self._h, nbytes, iters, C.byref(v)))
+        return v.value

@@ -1,0 +1,1064 @@
+// C ABI of libmdwt (include/mdwt.h): the host side of the dtracker::Renderer
+// RAW path (renderer.h:52-222, renderer.cpp) re-designed for one HIP device and
+// one stream per handle. Host-side math that feeds the kernels (camera basis,
+// dt, macrocell dims, projection windows) follows the reference's fp32
+// expressions in order; the kernels do the rest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mdwt.h"
+#include "mdwt_internal.hpp"
+#include "mdwt_types.hpp"
+
+namespace {
+
+struct F3 {
+  float x, y, z;
+};
+inline F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 operator*(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline F3 operator-(F3 a) { return {-a.x, -a.y, -a.z}; }
+inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 normalize(F3 a) { return a * (1.f / std::sqrt(dot(a, a))); }
+inline F3 cross(F3 a, F3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+struct Float4 {
+  float x, y, z, w;
+};
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct mdwt_renderer {
+  int device = 0;
+  std::string err;
+  bool hip_ready = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // channels staged on the host until init (RawR keeps a host copy too)
+  std::vector<std::vector<uint8_t>> staging;
+  int dtype = -1;
+  int dims[3] = {0, 0, 0};
+  int C = 0;
+  float vmin[MDWT_MAX_CHANNELS] = {}, vmax[MDWT_MAX_CHANNELS] = {};
+  F3 remap{1.f, 1.f, 1.f};
+
+  bool inited = false;
+  int mode = 0;
+  int mc[3] = {0, 0, 0};
+
+  // transfer functions (TFData, renderer.h:39-50)
+  std::vector<std::vector<Float4>> cm;  // alpha cubed
+  float os[MDWT_MAX_CHANNELS], xlo[MDWT_MAX_CHANNELS], xhi[MDWT_MAX_CHANNELS];
+  bool cm_set[MDWT_MAX_CHANNELS] = {};
+  int tf_stride = 0;
+  bool maj_valid = false;
+
+  // camera (camera.h:46-142 state) and LaunchParams::camera
+  F3 cam_pos{0, -1, 0}, fvx{1, 0, 0}, fvy{0, 1, 0}, fvz{0, 0, 1}, cam_up{0, 1, 0};
+  float fovy = 60.f;
+  mdwt::CameraParams cam{};
+
+  // frame state
+  int W = 1024, H = 1024, spp = 1;
+  bool accumulate = true;
+  float bg[3] = {0.f, 0.f, 0.f};
+  float light = 1.0f, ambient = 0.1f;   // renderer.h:124-126
+  float spec[3] = {1.f, 1.f, 1.f};      // renderer.h:159 material specularColor
+  uint32_t seed = 0x5eed1234u;
+  uint32_t frame_id = 0, accum_id = 0;
+  float dt = 0.05f;
+  int order_mode = 1;
+  float last_kernel_ms = 0.f;
+
+  // device memory
+  uint8_t* d_vol = nullptr;
+  uint64_t chan_stride = 0;
+  float* d_ranges = nullptr;
+  float* d_maj = nullptr;
+  Float4* d_tf = nullptr;
+  size_t d_tf_texels = 0;
+  Float4* d_accum = nullptr;
+  Float4* d_m2 = nullptr;
+  size_t fb_pixels = 0;
+  unsigned long long* d_counters = nullptr;
+  uint32_t* d_order = nullptr;
+  int n_blocks = 0;
+  int* d_win = nullptr;
+};
+
+namespace {
+
+int fail(mdwt_renderer* h, int code, const std::string& msg) {
+  h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((h), MDWT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t bytes_per_voxel(int dt) { return dt == MDWT_U8 ? 1 : (dt == MDWT_F32 ? 4 : 2); }
+
+float half_to_float(uint16_t h) {
+  const uint32_t s = uint32_t(h >> 15) << 31;
+  uint32_t e = (h >> 10) & 0x1f, m = h & 0x3ff, bits;
+  if (e == 0) {
+    if (m == 0) {
+      bits = s;
+    } else {
+      int ee = -1;
+      do {
+        m <<= 1;
+        ++ee;
+      } while ((m & 0x400) == 0);
+      m &= 0x3ff;
+      bits = s | (uint32_t(127 - 15 - ee) << 23) | (m << 13);
+    }
+  } else if (e == 31) {
+    bits = s | 0x7f800000u | (m << 13);
+  } else {
+    bits = s | ((e + 127 - 15) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+float value_scale(int dt) { return dt == MDWT_U8 ? 255.f : (dt == MDWT_U16 ? 65535.f : 1.f); }
+
+// camera.h:83-110 setOrientation with forceUpFrame (camera.h:72-80)
+void set_orientation(mdwt_renderer* h, F3 origin, F3 interest, F3 up, float fovy) {
+  h->fovy = fovy;
+  h->cam_pos = origin;
+  h->cam_up = up;
+  const bool same = origin.x == interest.x && origin.y == interest.y && origin.z == interest.z;
+  h->fvz = same ? F3{0, 0, 1} : -normalize(interest - origin);
+  h->fvx = cross(up, h->fvz);
+  if (dot(h->fvx, h->fvx) < 1e-8f)
+    h->fvx = F3{0, 1, 0};
+  else
+    h->fvx = normalize(h->fvx);
+  h->fvy = normalize(cross(h->fvz, h->fvx));
+  if (std::fabs(dot(h->fvz, h->cam_up)) < 1e-6f) return;
+  h->fvx = normalize(cross(h->cam_up, h->fvz));
+  h->fvy = normalize(cross(h->fvz, h->fvx));
+}
+
+// renderer.cpp:1058-1092 UpdateCamera (+ ResetAccumulation)
+void update_camera(mdwt_renderer* h) {
+  const F3 lookFrom = h->cam_pos;
+  const F3 lookAt = h->cam_pos - h->fvz;
+  const F3 lookUp = h->fvy;
+  const float cosFovy = std::cos(h->fovy * float(M_PI / 180.0));
+  const float vfov = std::acos(cosFovy) * float(180.0 / M_PI);
+  const F3 vup = lookUp;
+  const float aspect = float(h->W) / float(h->H);
+  const float theta = vfov * float(M_PI) / 180.0f;
+  const float half_height = std::tan(theta / 2.0f);
+  const float half_width = aspect * half_height;
+  const float focusDist = 10.f;
+  const F3 origin = lookFrom;
+  const F3 w = normalize(lookFrom - lookAt);
+  const F3 u = normalize(cross(vup, w));
+  const F3 v = cross(w, u);
+  const F3 llc = origin - half_width * focusDist * u - half_height * focusDist * v - focusDist * w;
+  const F3 hor = 2.0f * half_width * focusDist * u;
+  const F3 ver = 2.0f * half_height * focusDist * v;
+  const F3 all[4] = {origin, llc, hor, ver};
+  float* dst[4] = {h->cam.org, h->cam.llc, h->cam.hor, h->cam.ver};
+  for (int i = 0; i < 4; ++i) {
+    dst[i][0] = all[i].x;
+    dst[i][1] = all[i].y;
+    dst[i][2] = all[i].z;
+  }
+  h->accum_id = 0;
+}
+
+// renderer.cu:387-391 project
+int project(float f, float lo, float hi, int dim) {
+  const float q = float(dim) * (f - lo) / (hi - lo);
+  const int i = int(q);
+  return std::max(0, std::min(dim - 1, i));
+}
+
+// Per-axis voxel windows of the range grid. Voxel i's widened box along this
+// axis (rasterElements RAW, renderer.cu:853-890: own box extended by the
+// lower corners of its in-range neighbours) projects to macrocells
+// [project(lower), min(project(upper), M-1)] (rasterBox float2,
+// renderer.cu:487-519). The voxels covering macrocell m form an interval
+// [a, b]; the cell's range is the min/max over [a-1, b+1] (each voxel brings
+// its 3x3x3 neighbourhood). Returns false if a window is not an interval.
+bool axis_windows(int n, int m, float wlo, float whi, std::vector<int>& out) {
+  const float boxLen = (whi - wlo) / float(n);
+  std::vector<int> clo(n), chi(n);
+  std::vector<char> valid(n);
+  for (int i = 0; i < n; ++i) {
+    const float vxl = wlo + float(i) * boxLen;
+    float lower = vxl, upper = vxl + boxLen;
+    for (int d = -1; d <= 1; ++d) {
+      const int q = i + d;
+      if (q < 0 || q >= n) continue;
+      const float nl = wlo + float(q) * boxLen;
+      lower = std::fmin(lower, nl);
+      upper = std::fmax(upper, nl);
+    }
+    valid[i] = lower < upper;
+    clo[i] = project(lower, wlo, whi, m);
+    chi[i] = std::min(project(upper, wlo, whi, m), m - 1);
+  }
+  out.assign(2 * size_t(m), 0);
+  for (int c = 0; c < m; ++c) {
+    int a = -1, b = -2;
+    bool ended = false;
+    for (int i = 0; i < n; ++i) {
+      const bool in = valid[i] && clo[i] <= c && c <= chi[i];
+      if (in) {
+        if (ended) return false;
+        if (a < 0) a = i;
+        b = i;
+      } else if (a >= 0) {
+        ended = true;
+      }
+    }
+    if (a < 0) {
+      out[2 * c] = 1;  // empty
+      out[2 * c + 1] = 0;
+    } else {
+      out[2 * c] = std::max(a - 1, 0);
+      out[2 * c + 1] = std::min(b + 1, n - 1);
+    }
+  }
+  return true;
+}
+
+// 2-D Hilbert curve, iterative quadrant rotation (Hamilton; the textbook
+// d -> (x, y) walk), reported as (y, x) so that it traces the same curve as
+// the reference's hilbert_i2c with nDims = 2 (hilbert.h:128-170).
+void hilbert_d2xy(int bits, uint32_t d, uint32_t& ox, uint32_t& oy) {
+  uint32_t x = 0, y = 0, t = d;
+  const uint32_t n = 1u << bits;
+  for (uint32_t s = 1; s < n; s <<= 1) {
+    const uint32_t rx = 1u & (t >> 1);
+    const uint32_t ry = 1u & (t ^ rx);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = s - 1 - x;
+        y = s - 1 - y;
+      }
+      std::swap(x, y);
+    }
+    x += s * rx;
+    y += s * ry;
+    t >>= 2;
+  }
+  ox = y;
+  oy = x;
+}
+
+void free_frame(mdwt_renderer* h) {
+  if (h->d_accum) (void)hipFree(h->d_accum);
+  if (h->d_m2) (void)hipFree(h->d_m2);
+  if (h->d_order) (void)hipFree(h->d_order);
+  h->d_accum = h->d_m2 = nullptr;
+  h->d_order = nullptr;
+  h->fb_pixels = 0;
+}
+
+// frame buffers and the workgroup -> pixel-block order of the frame kernel
+int alloc_frame(mdwt_renderer* h) {
+  free_frame(h);
+  const size_t n = size_t(h->W) * h->H;
+  HIPCHK(h, hipMalloc(&h->d_accum, n * sizeof(Float4)));
+  HIPCHK(h, hipMalloc(&h->d_m2, n * sizeof(Float4)));
+  HIPCHK(h, hipMemsetAsync(h->d_accum, 0, n * sizeof(Float4), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_m2, 0, n * sizeof(Float4), h->stream));
+  h->fb_pixels = n;
+  const int bx = (h->W + mdwt::kBlockPixels - 1) / mdwt::kBlockPixels;
+  const int by = (h->H + mdwt::kBlockPixels - 1) / mdwt::kBlockPixels;
+  std::vector<uint32_t> order;
+  order.reserve(size_t(bx) * by);
+  if (h->order_mode == 1) {
+    int bits = 0;
+    while ((1 << bits) < std::max(bx, by)) ++bits;
+    const uint64_t total = uint64_t(1) << (2 * bits);
+    for (uint64_t d = 0; d < total; ++d) {
+      uint32_t x, y;
+      hilbert_d2xy(bits, uint32_t(d), x, y);
+      if (int(x) < bx && int(y) < by) order.push_back(x | (y << 16));
+    }
+  } else {
+    for (int y = 0; y < by; ++y)
+      for (int x = 0; x < bx; ++x) order.push_back(uint32_t(x) | (uint32_t(y) << 16));
+  }
+  h->n_blocks = int(order.size());
+  HIPCHK(h, hipMalloc(&h->d_order, order.size() * sizeof(uint32_t)));
+  HIPCHK(h, hipMemcpyAsync(h->d_order, order.data(), order.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MDWT_OK;
+}
+
+bool all_tf_set(const mdwt_renderer* h) {
+  for (int c = 0; c < h->C; ++c)
+    if (!h->cm_set[c]) return false;
+  return true;
+}
+
+mdwt::TfParams tf_params(const mdwt_renderer* h) {
+  mdwt::TfParams t{};
+  t.tab = h->d_tf;
+  t.stride = h->tf_stride;
+  for (int c = 0; c < h->C; ++c) {
+    t.n[c] = int(h->cm[c].size());
+    t.opacity[c] = h->os[c];
+    t.xf_lo[c] = h->xlo[c];
+    t.xf_hi[c] = h->xhi[c];
+    t.dom_lo[c] = h->vmin[c];
+    t.dom_hi[c] = h->vmax[c];
+  }
+  return t;
+}
+
+// RecalculateDensityRanges (renderer.cu:206-274): upload the TF texels and
+// rebuild the majorant grid for the current mode.
+int rebuild_majorants(mdwt_renderer* h) {
+  h->maj_valid = false;
+  if (!h->inited || !all_tf_set(h)) return MDWT_OK;
+  int stride = 0;
+  for (int c = 0; c < h->C; ++c) stride = std::max(stride, int(h->cm[c].size()));
+  stride = (stride + 3) & ~3;
+  if (size_t(h->C) * stride > size_t(mdwt::kMaxTexelsLds))
+    return fail(h, MDWT_ERR_LIMIT, "transfer functions exceed 160 KiB of LDS (channels * texels)");
+  const size_t texels = size_t(h->C) * stride;
+  if (texels > h->d_tf_texels) {
+    if (h->d_tf) (void)hipFree(h->d_tf);
+    h->d_tf = nullptr;
+    HIPCHK(h, hipMalloc(&h->d_tf, texels * sizeof(Float4)));
+    h->d_tf_texels = texels;
+  }
+  h->tf_stride = stride;
+  std::vector<Float4> tab(texels, Float4{0, 0, 0, 0});
+  for (int c = 0; c < h->C; ++c)
+    std::copy(h->cm[c].begin(), h->cm[c].end(), tab.begin() + size_t(c) * stride);
+  HIPCHK(h, hipMemcpyAsync(h->d_tf, tab.data(), texels * sizeof(Float4), hipMemcpyHostToDevice,
+                           h->stream));
+  mdwt::MajorantParams p{};
+  p.ranges = h->d_ranges;
+  p.tf = tf_params(h);
+  p.channels = h->C;
+  p.n_cells = h->mc[0] * h->mc[1] * h->mc[2];
+  p.mode = h->mode;
+  p.maj = h->d_maj;
+  HIPCHK(h, mdwt::launch_majorants(p, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->maj_valid = true;
+  return MDWT_OK;
+}
+
+mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
+  mdwt::VolumeParams v{};
+  v.vol = h->d_vol;
+  v.chan_stride = h->chan_stride;
+  v.nx = h->dims[0];
+  v.ny = h->dims[1];
+  v.nz = h->dims[2];
+  v.channels = h->C;
+  v.scale = value_scale(h->dtype);
+  v.offset = 0.f;
+  v.blo[0] = v.blo[1] = v.blo[2] = 0.f;
+  v.bhi[0] = h->remap.x;
+  v.bhi[1] = h->remap.y;
+  v.bhi[2] = h->remap.z;
+  return v;
+}
+
+mdwt::RenderParams render_params(const mdwt_renderer* h) {
+  mdwt::RenderParams p{};
+  p.vol = volume_params(h);
+  p.tf = tf_params(h);
+  p.cam = h->cam;
+  for (int i = 0; i < 3; ++i) p.mc[i] = h->mc[i];
+  p.maj = h->d_maj;
+  p.dt = h->dt;
+  p.fr.width = h->W;
+  p.fr.height = h->H;
+  p.fr.spp = h->spp;
+  p.fr.accumulate = h->accumulate ? 1 : 0;
+  for (int i = 0; i < 3; ++i) {
+    p.fr.bg[i] = h->bg[i];
+    p.fr.spec[i] = h->spec[i];
+  }
+  p.fr.light_intensity = h->light;
+  p.fr.ambient = h->ambient;
+  p.fr.seed = h->seed;
+  p.fr.frame_id = h->frame_id;
+  p.fr.accum_id = h->accum_id;
+  p.fr.accum = h->d_accum;
+  p.fr.m2 = h->d_m2;
+  p.fr.counters = h->d_counters;
+  p.fr.block_order = h->d_order;
+  p.fr.n_blocks = h->n_blocks;
+  return p;
+}
+
+int ready_to_render(mdwt_renderer* h) {
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (!all_tf_set(h) || !h->maj_valid)
+    return fail(h, MDWT_ERR_STATE, "transfer functions not set for every channel");
+  return MDWT_OK;
+}
+
+bool finite3(const float* v) {
+  return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdwt_create(mdwt_renderer** out, int hip_device) {
+  if (!out) {
+    g_create_error = "null output pointer";
+    return MDWT_ERR_ARG;
+  }
+  *out = nullptr;
+  if (hip_device < 0) {
+    g_create_error = "negative device id";
+    return MDWT_ERR_ARG;
+  }
+  mdwt_renderer* h = new (std::nothrow) mdwt_renderer();
+  if (!h) {
+    g_create_error = "out of host memory";
+    return MDWT_ERR_OOM;
+  }
+  h->device = hip_device;
+  for (int c = 0; c < MDWT_MAX_CHANNELS; ++c) {
+    h->os[c] = 1.0f;  // TFData defaults (renderer.h:45-49)
+    h->xlo[c] = 0.f;
+    h->xhi[c] = 1.f;
+  }
+  *out = h;
+  return MDWT_OK;
+}
+
+int mdwt_destroy(mdwt_renderer* h) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->hip_ready) {
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_frame(h);
+    void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+  }
+  delete h;
+  return MDWT_OK;
+}
+
+const char* mdwt_last_error(const mdwt_renderer* h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int mdwt_push_channel(mdwt_renderer* h, const void* voxels, int dtype, int nx, int ny, int nz) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->inited) return fail(h, MDWT_ERR_STATE, "channels must be pushed before init");
+  if (h->C >= MDWT_MAX_CHANNELS)
+    return fail(h, MDWT_ERR_LIMIT, "at most 12 channels (MAX_CHANNELS)");
+  if (!voxels) return fail(h, MDWT_ERR_ARG, "null voxel pointer");
+  if (dtype < MDWT_U8 || dtype > MDWT_F32) return fail(h, MDWT_ERR_ARG, "unknown dtype");
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(h, MDWT_ERR_ARG, "dims must be positive");
+  const uint64_t nvox = uint64_t(nx) * uint64_t(ny) * uint64_t(nz);
+  if (nvox > 0xFFFFFFFFull / 2)
+    return fail(h, MDWT_ERR_LIMIT, "a channel must hold fewer than 2^31 voxels");
+  if (h->C > 0 && (nx != h->dims[0] || ny != h->dims[1] || nz != h->dims[2]))
+    return fail(h, MDWT_ERR_ARG, "all channels must share dims");
+  if (h->C > 0 && dtype != h->dtype) return fail(h, MDWT_ERR_ARG, "all channels must share dtype");
+  const size_t bytes = size_t(nvox) * bytes_per_voxel(dtype);
+  std::vector<uint8_t> buf;
+  try {
+    buf.assign(static_cast<const uint8_t*>(voxels), static_cast<const uint8_t*>(voxels) + bytes);
+  } catch (...) {
+    return fail(h, MDWT_ERR_OOM, "out of host memory staging the channel");
+  }
+  // value domain: bounds_.w starts at {0, 1} and widens (rawFile.h:162, rawFile.cpp:119-123)
+  const float sc = value_scale(dtype);
+  float lo = 0.f, hi = 1.f;
+  for (size_t i = 0; i < size_t(nvox); ++i) {
+    float r;
+    if (dtype == MDWT_U8) {
+      r = float(buf[i]);
+    } else if (dtype == MDWT_U16) {
+      uint16_t v;
+      std::memcpy(&v, &buf[2 * i], 2);
+      r = float(v);
+    } else if (dtype == MDWT_F16) {
+      uint16_t v;
+      std::memcpy(&v, &buf[2 * i], 2);
+      r = half_to_float(v);
+    } else {
+      std::memcpy(&r, &buf[4 * i], 4);
+      if (!std::isfinite(r)) return fail(h, MDWT_ERR_ARG, "non-finite voxel value");
+    }
+    const float v = r * sc + 0.f;
+    if (v > hi) hi = v;
+    if (v < lo) lo = v;
+  }
+  if (dtype == MDWT_F16)
+    for (size_t i = 0; i < size_t(nvox); ++i) {
+      uint16_t v;
+      std::memcpy(&v, &buf[2 * i], 2);
+      if ((v & 0x7c00) == 0x7c00) return fail(h, MDWT_ERR_ARG, "non-finite voxel value");
+    }
+  h->vmin[h->C] = lo;
+  h->vmax[h->C] = hi;
+  h->staging.push_back(std::move(buf));
+  h->dtype = dtype;
+  h->dims[0] = nx;
+  h->dims[1] = ny;
+  h->dims[2] = nz;
+  h->C += 1;
+  return MDWT_OK;
+}
+
+int mdwt_set_bounds(mdwt_renderer* h, const float remap[3]) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->inited) return fail(h, MDWT_ERR_STATE, "bounds must be set before init");
+  if (!remap || !finite3(remap) || !(remap[0] > 0.f) || !(remap[1] > 0.f) || !(remap[2] > 0.f))
+    return fail(h, MDWT_ERR_ARG, "bounds must be finite and positive");
+  h->remap = {remap[0], remap[1], remap[2]};
+  return MDWT_OK;
+}
+
+int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->inited) return fail(h, MDWT_ERR_STATE, "already initialized");
+  if (h->C == 0) return fail(h, MDWT_ERR_STATE, "no data source (push a channel first)");
+  if (mode != 0 && mode != 2 && mode != 3)
+    return fail(h, MDWT_ERR_ARG, "mode must be 0 (N-DDA), 2 (syncDDA) or 3 (cumulative)");
+  int m[3];
+  for (int i = 0; i < 3; ++i) {
+    const int v = mc ? mc[i] : 0;
+    if (v < 0) return fail(h, MDWT_ERR_ARG, "macrocell count must be positive");
+    if (v > 0) {
+      m[i] = v;
+    } else {  // CalculateMCGridDims(4), renderer.cpp:1342-1355
+      unsigned acc = 0;
+      for (int c = 0; c < h->C; ++c) acc += unsigned(float(h->dims[i]) / 4.f);
+      const unsigned avg = unsigned(float(acc) / float(h->C));
+      m[i] = avg < 1 ? 1 : int(avg);
+    }
+  }
+  const uint64_t n_cells = uint64_t(m[0]) * m[1] * m[2];
+  if (n_cells * uint64_t(h->C) > (1ull << 28))
+    return fail(h, MDWT_ERR_LIMIT, "macrocell grid too large");
+  std::vector<int> wx, wy, wz;
+  if (!axis_windows(h->dims[0], m[0], 0.f, h->remap.x, wx) ||
+      !axis_windows(h->dims[1], m[1], 0.f, h->remap.y, wy) ||
+      !axis_windows(h->dims[2], m[2], 0.f, h->remap.z, wz))
+    return fail(h, MDWT_ERR_ARG, "macrocell projection windows are not intervals");
+
+  HIPCHK(h, hipSetDevice(h->device));
+  if (!h->hip_ready) {
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreate(&h->ev0));
+    HIPCHK(h, hipEventCreate(&h->ev1));
+    h->hip_ready = true;
+  }
+  for (int i = 0; i < 3; ++i) h->mc[i] = m[i];
+  h->mode = mode;
+  const size_t chan_bytes = size_t(h->dims[0]) * h->dims[1] * h->dims[2] * bytes_per_voxel(h->dtype);
+  h->chan_stride = (chan_bytes + 255) & ~size_t(255);
+  if (hipMalloc(&h->d_vol, h->chan_stride * h->C) != hipSuccess)
+    return fail(h, MDWT_ERR_OOM, "device allocation of the volume failed");
+  for (int c = 0; c < h->C; ++c)
+    HIPCHK(h, hipMemcpyAsync(h->d_vol + size_t(c) * h->chan_stride, h->staging[c].data(), chan_bytes,
+                             hipMemcpyHostToDevice, h->stream));
+  std::vector<int> win;
+  win.insert(win.end(), wx.begin(), wx.end());
+  win.insert(win.end(), wy.begin(), wy.end());
+  win.insert(win.end(), wz.begin(), wz.end());
+  HIPCHK(h, hipMalloc(&h->d_win, win.size() * sizeof(int)));
+  HIPCHK(h, hipMemcpyAsync(h->d_win, win.data(), win.size() * sizeof(int), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMalloc(&h->d_ranges, n_cells * h->C * 2 * sizeof(float)));
+  HIPCHK(h, hipMalloc(&h->d_maj, n_cells * h->C * sizeof(float)));
+  HIPCHK(h, hipMemsetAsync(h->d_maj, 0, n_cells * h->C * sizeof(float), h->stream));
+  HIPCHK(h, hipMalloc(&h->d_counters, 4 * sizeof(unsigned long long)));
+  mdwt::RangeParams rp{};
+  rp.vol = h->d_vol;
+  rp.chan_stride = h->chan_stride;
+  rp.nx = h->dims[0];
+  rp.ny = h->dims[1];
+  rp.nz = h->dims[2];
+  rp.channels = h->C;
+  rp.scale = value_scale(h->dtype);
+  rp.offset = 0.f;
+  for (int i = 0; i < 3; ++i) rp.mc[i] = m[i];
+  rp.win = h->d_win;
+  rp.ranges = h->d_ranges;
+  HIPCHK(h, mdwt::launch_ranges(rp, h->dtype, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  // the reference drops its scalar buffers after the build (renderer.cpp:738-739)
+  std::vector<std::vector<uint8_t>>().swap(h->staging);
+  h->cm.assign(h->C, std::vector<Float4>());
+  if (auto_camera) {  // renderer.cpp:856-863
+    set_orientation(h, F3{0.5f, 0.5f, 0.5f + 2.5f}, F3{0.5f, 0.5f, 0.5f}, F3{0, 1, 0}, 45.0f);
+  }
+  h->inited = true;
+  update_camera(h);
+  int rc = alloc_frame(h);
+  if (rc != MDWT_OK) return rc;
+  h->frame_id = 0;
+  h->accum_id = 0;
+  return mdwt_reset_dt(h);
+}
+
+int mdwt_set_mode(mdwt_renderer* h, int mode) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (mode != 0 && mode != 2 && mode != 3)
+    return fail(h, MDWT_ERR_ARG, "mode must be 0 (N-DDA), 2 (syncDDA) or 3 (cumulative)");
+  HIPCHK(h, hipSetDevice(h->device));
+  h->mode = mode;
+  h->accum_id = 0;
+  return rebuild_majorants(h);
+}
+
+int mdwt_set_xf_colormap(mdwt_renderer* h, int ch, const float* rgba, int n_texels) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (ch < 0 || ch >= h->C) return fail(h, MDWT_ERR_ARG, "channel id out of range");
+  if (!rgba || n_texels < 1 || n_texels > 4096)
+    return fail(h, MDWT_ERR_ARG, "colormap needs 1..4096 texels");
+  std::vector<Float4> t(n_texels);
+  for (int i = 0; i < n_texels; ++i) {
+    const float* p = rgba + 4 * size_t(i);
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || !std::isfinite(p[3]))
+      return fail(h, MDWT_ERR_ARG, "non-finite colormap entry");
+    // opacities come from a [0,1] widget in the reference; a negative one
+    // would make free-flight steps run backwards
+    if (p[3] < 0.f || p[3] > 1.f) return fail(h, MDWT_ERR_ARG, "colormap alpha must lie in [0, 1]");
+    t[i] = Float4{p[0], p[1], p[2], std::pow(p[3], 3.f)};  // renderer.cpp:1098-1101
+  }
+  h->cm[ch] = std::move(t);
+  h->cm_set[ch] = true;
+  h->accum_id = 0;
+  HIPCHK(h, hipSetDevice(h->device));
+  return rebuild_majorants(h);
+}
+
+int mdwt_set_xf_opacity_scale(mdwt_renderer* h, int ch, float opacity_scale) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (ch < 0 || ch >= h->C) return fail(h, MDWT_ERR_ARG, "channel id out of range");
+  if (!(opacity_scale >= 0.f && opacity_scale <= 1000.f))
+    return fail(h, MDWT_ERR_ARG, "opacity scale must lie in [0, 1000]");
+  h->os[ch] = opacity_scale;
+  h->accum_id = 0;
+  HIPCHK(h, hipSetDevice(h->device));
+  return rebuild_majorants(h);
+}
+
+int mdwt_set_xf_range(mdwt_renderer* h, int ch, float xf_lo, float xf_hi) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (ch < 0 || ch >= h->C) return fail(h, MDWT_ERR_ARG, "channel id out of range");
+  if (!std::isfinite(xf_lo) || !std::isfinite(xf_hi) || xf_lo == xf_hi)
+    return fail(h, MDWT_ERR_ARG, "transfer-function range must be finite and non-empty");
+  h->xlo[ch] = xf_lo;
+  h->xhi[ch] = xf_hi;
+  h->accum_id = 0;
+  HIPCHK(h, hipSetDevice(h->device));
+  return rebuild_majorants(h);
+}
+
+int mdwt_set_tf(mdwt_renderer* h, int ch, const float* rgba, int n_texels, float opacity_scale,
+                float xf_lo, float xf_hi) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (ch < 0 || ch >= h->C) return fail(h, MDWT_ERR_ARG, "channel id out of range");
+  if (!(opacity_scale >= 0.f && opacity_scale <= 1000.f))
+    return fail(h, MDWT_ERR_ARG, "opacity scale must lie in [0, 1000]");
+  if (!std::isfinite(xf_lo) || !std::isfinite(xf_hi) || xf_lo == xf_hi)
+    return fail(h, MDWT_ERR_ARG, "transfer-function range must be finite and non-empty");
+  h->os[ch] = opacity_scale;
+  h->xlo[ch] = xf_lo;
+  h->xhi[ch] = xf_hi;
+  return mdwt_set_xf_colormap(h, ch, rgba, n_texels);
+}
+
+int mdwt_set_camera(mdwt_renderer* h, const float from[3], const float at[3], const float up[3],
+                    float fovy_deg) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!from || !at || !up || !finite3(from) || !finite3(at) || !finite3(up) ||
+      !std::isfinite(fovy_deg))
+    return fail(h, MDWT_ERR_ARG, "camera vectors must be finite");
+  set_orientation(h, F3{from[0], from[1], from[2]}, F3{at[0], at[1], at[2]}, F3{up[0], up[1], up[2]},
+                  fovy_deg);
+  if (h->inited) update_camera(h);
+  return MDWT_OK;
+}
+
+int mdwt_set_dt(mdwt_renderer* h, float dt) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!std::isfinite(dt)) return fail(h, MDWT_ERR_ARG, "non-finite dt");
+  h->dt = std::fmax(dt, 1e-4f);  // renderer.cpp:1460-1466
+  h->accum_id = 0;
+  return MDWT_OK;
+}
+
+int mdwt_reset_dt(mdwt_renderer* h) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->C == 0) return fail(h, MDWT_ERR_STATE, "no data source");
+  // renderer.cpp:1444-1453: the loop assigns, so the last channel wins
+  float minVoxelSideLength = 3.402823466e38f;
+  for (int c = 0; c < h->C; ++c)
+    minVoxelSideLength =
+        std::fmin(h->remap.x / float(h->dims[0]),
+                  std::fmin(h->remap.y / float(h->dims[1]), h->remap.z / float(h->dims[2])));
+  return mdwt_set_dt(h, minVoxelSideLength * 0.5f);
+}
+
+int mdwt_set_frame(mdwt_renderer* h, int width, int height, int spp, int accumulate,
+                   const float bg[3]) {
+  if (!h) return MDWT_ERR_ARG;
+  if (width <= 0 || height <= 0 || spp <= 0) return fail(h, MDWT_ERR_ARG, "frame sizes must be positive");
+  if (width > 16384 * 4 || height > 16384 * 4) return fail(h, MDWT_ERR_LIMIT, "frame too large");
+  if (bg && !finite3(bg)) return fail(h, MDWT_ERR_ARG, "non-finite background");
+  const bool resize = width != h->W || height != h->H;
+  h->W = width;
+  h->H = height;
+  h->spp = spp;
+  h->accumulate = accumulate != 0;
+  if (bg)
+    for (int i = 0; i < 3; ++i) h->bg[i] = bg[i];
+  h->accum_id = 0;
+  if (h->inited) {
+    update_camera(h);
+    if (resize) {
+      HIPCHK(h, hipSetDevice(h->device));
+      return alloc_frame(h);
+    }
+  }
+  return MDWT_OK;
+}
+
+int mdwt_set_lighting(mdwt_renderer* h, float light_intensity, float ambient) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!std::isfinite(light_intensity) || !std::isfinite(ambient))
+    return fail(h, MDWT_ERR_ARG, "non-finite lighting");
+  h->light = light_intensity;
+  h->ambient = ambient;
+  h->accum_id = 0;
+  return MDWT_OK;
+}
+
+int mdwt_set_seed(mdwt_renderer* h, uint32_t seed) {
+  if (!h) return MDWT_ERR_ARG;
+  h->seed = seed;
+  h->accum_id = 0;
+  return MDWT_OK;
+}
+
+int mdwt_render(mdwt_renderer* h, int n_frames, float* ms_out) {
+  if (!h) return MDWT_ERR_ARG;
+  int rc = ready_to_render(h);
+  if (rc != MDWT_OK) return rc;
+  if (n_frames < 0) return fail(h, MDWT_ERR_ARG, "negative frame count");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int f = 0; f < n_frames; ++f) {
+    const mdwt::RenderParams p = render_params(h);
+    HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, false, h->stream));
+    h->accum_id++;
+    h->frame_id++;
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
+  if (ms_out) *ms_out = ms;
+  return MDWT_OK;
+}
+
+int mdwt_reset_accumulation(mdwt_renderer* h) {
+  if (!h) return MDWT_ERR_ARG;
+  h->accum_id = 0;
+  return MDWT_OK;
+}
+
+int mdwt_read_accum(mdwt_renderer* h, float* rgba, float* m2) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t bytes = h->fb_pixels * sizeof(Float4);
+  if (rgba) HIPCHK(h, hipMemcpyAsync(rgba, h->d_accum, bytes, hipMemcpyDeviceToHost, h->stream));
+  if (m2) HIPCHK(h, hipMemcpyAsync(m2, h->d_m2, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MDWT_OK;
+}
+
+int mdwt_read_fb(mdwt_renderer* h, uint32_t* rgba8) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (!rgba8) return fail(h, MDWT_ERR_ARG, "null output");
+  HIPCHK(h, hipSetDevice(h->device));
+  uint32_t* d = nullptr;
+  HIPCHK(h, hipMalloc(&d, h->fb_pixels * sizeof(uint32_t)));
+  hipError_t e = mdwt::launch_rgba8(h->d_accum, d, int(h->fb_pixels), h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rgba8, d, h->fb_pixels * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("read_fb: ") + hipGetErrorString(e));
+  return MDWT_OK;
+}
+
+int mdwt_read_ranges(mdwt_renderer* h, float* out) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t n = size_t(h->mc[0]) * h->mc[1] * h->mc[2] * h->C * 2;
+  HIPCHK(h, hipMemcpy(out, h->d_ranges, n * sizeof(float), hipMemcpyDeviceToHost));
+  return MDWT_OK;
+}
+
+int mdwt_read_majorants(mdwt_renderer* h, float* out) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (!h->maj_valid) return fail(h, MDWT_ERR_STATE, "majorants not built (set every transfer function)");
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t n = size_t(h->mc[0]) * h->mc[1] * h->mc[2] * (h->mode <= 2 ? h->C : 1);
+  HIPCHK(h, hipMemcpy(out, h->d_maj, n * sizeof(float), hipMemcpyDeviceToHost));
+  return MDWT_OK;
+}
+
+int mdwt_get_info(mdwt_renderer* h, int out[10]) {
+  if (!h || !out) return MDWT_ERR_ARG;
+  out[0] = h->mc[0];
+  out[1] = h->mc[1];
+  out[2] = h->mc[2];
+  out[3] = h->dims[0];
+  out[4] = h->dims[1];
+  out[5] = h->dims[2];
+  out[6] = h->C;
+  out[7] = h->mode;
+  out[8] = int(h->frame_id);
+  out[9] = int(h->accum_id);
+  return MDWT_OK;
+}
+
+int mdwt_get_dt(mdwt_renderer* h, float* dt) {
+  if (!h || !dt) return MDWT_ERR_ARG;
+  *dt = h->dt;
+  return MDWT_OK;
+}
+
+int mdwt_get_camera(mdwt_renderer* h, float out[12]) {
+  if (!h || !out) return MDWT_ERR_ARG;
+  const float* src[4] = {h->cam.org, h->cam.llc, h->cam.hor, h->cam.ver};
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = src[i][k];
+  return MDWT_OK;
+}
+
+int mdwt_get_domains(mdwt_renderer* h, float* out) {
+  if (!h || !out) return MDWT_ERR_ARG;
+  for (int c = 0; c < h->C; ++c) {
+    out[2 * c] = h->vmin[c];
+    out[2 * c + 1] = h->vmax[c];
+  }
+  return MDWT_OK;
+}
+
+int mdwt_trace_cells(mdwt_renderer* h, const float* org, const float* dir, int n, int max_cells,
+                     int32_t* cells, float* t01, int32_t* counts) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (n < 0 || max_cells < 0 || (n > 0 && (!org || !dir || !counts)) ||
+      (n > 0 && max_cells > 0 && (!cells || !t01)))
+    return fail(h, MDWT_ERR_ARG, "bad trace_cells arguments");
+  if (n == 0) return MDWT_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t nc = size_t(n) * std::max(max_cells, 1);
+  float *d_org = nullptr, *d_dir = nullptr, *d_t = nullptr;
+  int *d_cells = nullptr, *d_cnt = nullptr;
+  hipError_t e = hipMalloc(&d_org, size_t(n) * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_dir, size_t(n) * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_t, nc * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_cells, nc * 4);
+  if (e == hipSuccess) e = hipMalloc(&d_cnt, size_t(n) * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_org, org, size_t(n) * 12, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_dir, dir, size_t(n) * 12, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const mdwt::VolumeParams v = volume_params(h);
+    e = mdwt::launch_trace_cells(v, h->mc, d_org, d_dir, n, max_cells, d_cells, d_t, d_cnt, h->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && max_cells > 0) {
+    e = hipMemcpy(cells, d_cells, size_t(n) * max_cells * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(t01, d_t, size_t(n) * max_cells * 8, hipMemcpyDeviceToHost);
+  }
+  if (e == hipSuccess) e = hipMemcpy(counts, d_cnt, size_t(n) * 4, hipMemcpyDeviceToHost);
+  void* ptrs[] = {d_org, d_dir, d_t, d_cells, d_cnt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("trace_cells: ") + hipGetErrorString(e));
+  return MDWT_OK;
+}
+
+int mdwt_gen_rays(mdwt_renderer* h, uint32_t frame, uint32_t sample, float* org, float* dir) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (!org || !dir) return fail(h, MDWT_ERR_ARG, "null output");
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t n = size_t(h->W) * h->H;
+  float *d_org = nullptr, *d_dir = nullptr;
+  hipError_t e = hipMalloc(&d_org, n * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_dir, n * 12);
+  if (e == hipSuccess)
+    e = mdwt::launch_gen_rays(h->cam, h->W, h->H, h->seed, frame, sample, d_org, d_dir, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(org, d_org, n * 12, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(dir, d_dir, n * 12, hipMemcpyDeviceToHost);
+  if (d_org) (void)hipFree(d_org);
+  if (d_dir) (void)hipFree(d_dir);
+  if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("gen_rays: ") + hipGetErrorString(e));
+  return MDWT_OK;
+}
+
+int mdwt_clip_rays(mdwt_renderer* h, const float* org, const float* dir, int n, float* t01,
+                   int32_t* hit) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (n < 0 || (n > 0 && (!org || !dir || !t01 || !hit)))
+    return fail(h, MDWT_ERR_ARG, "bad clip_rays arguments");
+  if (n == 0) return MDWT_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  float *d_org = nullptr, *d_dir = nullptr, *d_t = nullptr;
+  int* d_hit = nullptr;
+  hipError_t e = hipMalloc(&d_org, size_t(n) * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_dir, size_t(n) * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_t, size_t(n) * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_hit, size_t(n) * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_org, org, size_t(n) * 12, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_dir, dir, size_t(n) * 12, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = mdwt::launch_clip(volume_params(h), d_org, d_dir, n, d_t, d_hit, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(t01, d_t, size_t(n) * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(hit, d_hit, size_t(n) * 4, hipMemcpyDeviceToHost);
+  void* ptrs[] = {d_org, d_dir, d_t, d_hit};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("clip_rays: ") + hipGetErrorString(e));
+  return MDWT_OK;
+}
+
+int mdwt_count_frame(mdwt_renderer* h, uint64_t out[4]) {
+  if (!h || !out) return MDWT_ERR_ARG;
+  int rc = ready_to_render(h);
+  if (rc != MDWT_OK) return rc;
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
+  const mdwt::RenderParams p = render_params(h);
+  HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, true, h->stream));
+  unsigned long long c[4];
+  HIPCHK(h, hipMemcpyAsync(c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+  return MDWT_OK;
+}
+
+int mdwt_last_kernel_ms(mdwt_renderer* h, float* ms) {
+  if (!h || !ms) return MDWT_ERR_ARG;
+  *ms = h->last_kernel_ms;
+  return MDWT_OK;
+}
+
+int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile) {
+  if (!h) return MDWT_ERR_ARG;
+  if (order != 0 && order != 1) return fail(h, MDWT_ERR_ARG, "order must be 0 (rows) or 1 (Hilbert)");
+  if (tile != mdwt::kBlockPixels) return fail(h, MDWT_ERR_ARG, "tile must be 16");
+  h->order_mode = order;
+  if (h->inited) {
+    HIPCHK(h, hipSetDevice(h->device));
+    const uint32_t keep = h->accum_id;
+    // keep the accumulated image: only the block order changes
+    Float4* acc = h->d_accum;
+    Float4* m2 = h->d_m2;
+    h->d_accum = nullptr;
+    h->d_m2 = nullptr;
+    if (h->d_order) (void)hipFree(h->d_order);
+    h->d_order = nullptr;
+    int rc = alloc_frame(h);
+    if (rc != MDWT_OK) return rc;
+    HIPCHK(h, hipMemcpy(h->d_accum, acc, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
+    HIPCHK(h, hipMemcpy(h->d_m2, m2, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
+    (void)hipFree(acc);
+    (void)hipFree(m2);
+    h->accum_id = keep;
+  }
+  return MDWT_OK;
+}
+
+int mdwt_stream_copy(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps) {
+  if (!h || !gbps) return MDWT_ERR_ARG;
+  if (bytes < 16 || iters < 1) return fail(h, MDWT_ERR_ARG, "bad copy size");
+  if (!h->hip_ready) {
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreate(&h->ev0));
+    HIPCHK(h, hipEventCreate(&h->ev1));
+    h->hip_ready = true;
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t n16 = size_t(bytes / 16);
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n16 * 16) != hipSuccess) return fail(h, MDWT_ERR_OOM, "copy buffer");
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(h, MDWT_ERR_OOM, "copy buffer");
+  }
+  hipError_t e = hipMemsetAsync(a, 1, n16 * 16, h->stream);
+  if (e == hipSuccess) e = mdwt::launch_copy(a, b, n16, h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = mdwt::launch_copy(a, b, n16, h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("stream_copy: ") + hipGetErrorString(e));
+  *gbps = float(2.0 * double(n16 * 16) * iters / (double(ms) * 1e-3) / 1e9);
+  return MDWT_OK;
+}
+
+int mdwt_hilbert_d2xy(int bits, uint32_t d, uint32_t* x, uint32_t* y) {
+  if (bits < 0 || bits > 15 || !x || !y) return MDWT_ERR_ARG;
+  if (bits < 16 && uint64_t(d) >= (uint64_t(1) << (2 * bits))) return MDWT_ERR_ARG;
+  hilbert_d2xy(bits, d, *x, *y);
+  return MDWT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+// Host-side launchers of the MDWT kernels (defined in the .hip files, called
+// by the C ABI in mdwt_api.cpp). Each returns the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mdwt_types.hpp"
+
+namespace mdwt {
+
+hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
+hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);
+hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
+hipError_t launch_gen_rays(const CameraParams& cam, int W, int H, uint32_t seed, uint32_t frame,
+                           uint32_t sample, float* org, float* dir, hipStream_t s);
+hipError_t launch_clip(const VolumeParams& vol, const float* org, const float* dir, int n,
+                       float* t01, int* hit, hipStream_t s);
+hipError_t launch_trace_cells(const VolumeParams& vol, const int mc[3], const float* org,
+                              const float* dir, int n, int max_cells, int* cells, float* t01,
+                              int* counts, hipStream_t s);
+hipError_t launch_rgba8(const void* accum, uint32_t* out, int n, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
+
+}  // namespace mdwt

@@ -1,0 +1,86 @@
+// Launch-parameter structs shared by the host library and the kernels. They
+// take the place of the reference's LaunchParams (deviceCode.h:61-136): one
+// by-value kernel argument per launch instead of OWL named variables.
+#pragma once
+#include <stdint.h>
+
+namespace mdwt {
+
+enum { MDWT_DT_U8 = 0, MDWT_DT_U16 = 1, MDWT_DT_F16 = 2, MDWT_DT_F32 = 3 };
+constexpr int kMaxChannels = 12;   // deviceCode.h:13
+constexpr int kBlockPixels = 16;   // a frame workgroup covers 16x16 pixels
+constexpr int kThreads = 256;      // 4 wave64, each an 8x8 pixel quad
+constexpr int kMaxTexelsLds = 10240;  // C * stride float4 texels (160 KiB)
+
+struct CameraParams {
+  float org[3], llc[3], hor[3], ver[3];  // LaunchParams::camera
+};
+
+// Volume in HBM: channel c's voxels (x fastest, native dtype) start at
+// vol + c * chan_stride bytes. value = raw * scale + offset (rawFile.cpp:100-123).
+struct VolumeParams {
+  const uint8_t* vol;
+  uint64_t chan_stride;
+  int nx, ny, nz;
+  int channels;
+  float scale, offset;
+  float blo[3], bhi[3];  // world box (root box = union of channel bounds)
+};
+
+// Per-channel transfer function (LaunchParams::transferFunction): texels of
+// channel c at tab + c * stride (alpha already cubed).
+struct TfParams {
+  const void* tab;  // float4
+  int stride;
+  int n[kMaxChannels];
+  float opacity[kMaxChannels];
+  float xf_lo[kMaxChannels], xf_hi[kMaxChannels];
+  float dom_lo[kMaxChannels], dom_hi[kMaxChannels];
+};
+
+struct FrameParams {
+  int width, height, spp, accumulate;
+  float bg[3];
+  float light_intensity, ambient;
+  float spec[3];
+  uint32_t seed, frame_id, accum_id;
+  void* accum;  // float4 per pixel
+  void* m2;     // float4 per pixel
+  unsigned long long* counters;  // samples, visits, rejections, rays
+  const uint32_t* block_order;   // (bx | by << 16) per workgroup, in launch order
+  int n_blocks;
+};
+
+struct RenderParams {
+  VolumeParams vol;
+  TfParams tf;
+  CameraParams cam;
+  int mc[3];
+  const float* maj;
+  float dt;
+  FrameParams fr;
+};
+
+// Range-grid build: per axis and macrocell index m, the inclusive voxel
+// window [lo, hi] whose values bound the cell (lo > hi: no voxel covers it).
+struct RangeParams {
+  const uint8_t* vol;
+  uint64_t chan_stride;
+  int nx, ny, nz;
+  int channels;
+  float scale, offset;
+  int mc[3];
+  const int* win;  // 2*(mx + my + mz) ints: x windows, then y, then z
+  void* ranges;    // float2 per (cell, channel)
+};
+
+struct MajorantParams {
+  const void* ranges;  // float2 per (cell, channel)
+  TfParams tf;
+  int channels;
+  int n_cells;
+  int mode;
+  float* maj;
+};
+
+}  // namespace mdwt

@@ -1,0 +1,1161 @@
+// CPU oracle for multi-density Woodcock tracking. TEST INFRASTRUCTURE ONLY:
+// loaded by tests/, __graft_entry__.smoke() and bench.py's CPU baseline leg as
+// the checker; the HIP product never links or calls it.
+//
+// Scalar restatement of the reference's RAW render path. Each function cites
+// the reference file:line it follows. Build: see oracle/Makefile
+// (g++ -O2 -ffp-contract=off, no fast-math, so every fp32 operation is rounded
+// exactly once in source order, like the HIP kernels built with
+// -ffp-contract=off).
+#include "mdwt_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int kMaxChannels = 12;  // deviceCode.h:13 MAX_CHANNELS
+
+// ---------------------------------------------------------------- vectors ---
+// OWL vec3f/vec4f stand-ins. Binary vector ops are component-wise and
+// evaluated left to right, as C++ evaluates the reference's expressions.
+struct vec3f { float x, y, z; };
+struct vec4f { float x, y, z, w; };
+struct vec3i { int x, y, z; };
+
+inline vec3f operator+(vec3f a, vec3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline vec3f operator-(vec3f a, vec3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline vec3f operator*(vec3f a, vec3f b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline vec3f operator*(float s, vec3f a) { return {s * a.x, s * a.y, s * a.z}; }
+inline vec3f operator*(vec3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline vec3f operator/(float s, vec3f a) { return {s / a.x, s / a.y, s / a.z}; }
+inline vec3f operator-(vec3f a) { return {-a.x, -a.y, -a.z}; }
+inline float dot(vec3f a, vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float length(vec3f a) { return sqrtf(dot(a, a)); }
+inline vec3f normalize(vec3f a) { return a * (1.f / sqrtf(dot(a, a))); }
+inline vec3f cross(vec3f a, vec3f b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline vec3f vmin(vec3f a, vec3f b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+inline vec3f vmax(vec3f a, vec3f b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+inline vec3f vabs(vec3f a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+inline vec3f vfloor(vec3f a) { return {floorf(a.x), floorf(a.y), floorf(a.z)}; }
+inline float reduce_min(vec3f a) { return fminf(fminf(a.x, a.y), a.z); }
+inline float reduce_max(vec3f a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
+
+inline vec4f operator+(vec4f a, vec4f b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+inline vec4f operator*(float s, vec4f a) { return {s * a.x, s * a.y, s * a.z, s * a.w}; }
+inline vec4f operator/(vec4f a, float s) { return {a.x / s, a.y / s, a.z / s, a.w / s}; }
+
+// ------------------------------------------------------------------- RNG ----
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11). Replaces OWL's LCG<4> (deviceCode.h:145,
+// deviceCode.cu:296-297), whose streams are keyed by threadIdx; here a stream
+// is keyed by (seed, pixel) and counted by (frame, sample, block, stream).
+inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+  const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+  const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+  const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+  const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+  const uint32_t n0 = hi1 ^ c[1] ^ k[0];
+  const uint32_t n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+}
+inline void philox10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  uint32_t k[2] = {key[0], key[1]};
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+    philox_round(c, k);
+  }
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+// One stream of uniforms in [0,1): draw n is word n%4 of block n/4.
+struct Rng {
+  uint32_t key[2];
+  uint32_t ctr[4];
+  uint32_t buf[4];
+  int pos;
+  Rng(uint32_t seed, uint32_t pixel, uint32_t frame, uint32_t sample, uint32_t stream) {
+    key[0] = seed; key[1] = pixel;
+    ctr[0] = frame; ctr[1] = sample; ctr[2] = 0; ctr[3] = stream;
+    pos = 4;
+  }
+  float operator()() {
+    if (pos == 4) { philox10(ctr, key, buf); ctr[2] += 1; pos = 0; }
+    const uint32_t w = buf[pos++];
+    return float(w >> 8) * 0x1p-24f;  // exact: 24-bit integer times 2^-24
+  }
+};
+
+// --------------------------------------------------------------- logf -------
+// Portable fp32 natural log for x > 0 normal: x = 2^e m, m in [sqrt(1/2),
+// sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1), atanh series to s^9
+// (Abramowitz & Stegun 4.1.27); e ln2 split in a Cody-Waite hi/lo pair.
+// Used in place of log() at deviceCode.cu:563,797,913 so that CPU and GPU
+// free-flight distances round identically.
+inline float plogf(float x) {
+  uint32_t bits;
+  std::memcpy(&bits, &x, 4);
+  int e = int((bits >> 23) & 0xffu) - 127;
+  uint32_t mb = (bits & 0x007fffffu) | 0x3f800000u;
+  float m;
+  std::memcpy(&m, &mb, 4);
+  if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+  const float f = m - 1.0f;
+  const float s = f / (2.0f + f);
+  const float z = s * s;
+  const float p = ((((0.11111111f * z + 0.14285715f) * z + 0.2f) * z + 0.33333334f) * z);
+  const float r = (s + s) + (s + s) * p;
+  const float fe = float(e);
+  return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
+}
+
+// ------------------------------------------------------------- textures ----
+// Clamp-to-edge linear lookup of a 1D float4 table at normalized u: texel
+// coordinate x = u*N - 0.5, lerp a + f*(b-a) (renderer.cpp:1140-1152 texture
+// descriptor: linear filter, clamp, normalized coordinates).
+inline int clamp_index(float fl, int n) {
+  // clamp in float first so that int conversion is always defined
+  float c = fminf(fmaxf(fl, -1.0f), float(n));
+  int i = int(c);
+  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+inline int clamp_index_plus1(float fl, int n) {
+  float c = fminf(fmaxf(fl, -1.0f), float(n));
+  int i = int(c) + 1;
+  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+
+struct TF {
+  std::vector<vec4f> texels;  // alpha already cubed (renderer.cpp:1098-1101)
+  float opacity_scale = 1.f;
+  float xf_lo = 0.f, xf_hi = 1.f;
+  bool set = false;
+};
+
+inline vec4f tf_lookup_u(const TF& tf, float u) {
+  const int n = int(tf.texels.size());
+  const float x = u * float(n) - 0.5f;
+  const float fl = floorf(x);
+  const float f = x - fl;
+  const vec4f a = tf.texels[clamp_index(fl, n)];
+  const vec4f b = tf.texels[clamp_index_plus1(fl, n)];
+  return {a.x + f * (b.x - a.x), a.y + f * (b.y - a.y), a.z + f * (b.z - a.z),
+          a.w + f * (b.w - a.w)};
+}
+
+inline float half_to_float(uint16_t h) {
+  const uint32_t s = uint32_t(h >> 15) << 31;
+  uint32_t e = (h >> 10) & 0x1f;
+  uint32_t m = h & 0x3ff;
+  uint32_t bits;
+  if (e == 0) {
+    if (m == 0) {
+      bits = s;
+    } else {  // subnormal half -> normal float
+      int ee = -1;
+      do { m <<= 1; ++ee; } while ((m & 0x400) == 0);
+      m &= 0x3ff;
+      bits = s | (uint32_t(127 - 15 - ee) << 23) | (m << 13);
+    }
+  } else if (e == 31) {
+    bits = s | 0x7f800000u | (m << 13);
+  } else {
+    bits = s | ((e + 127 - 15) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+struct Channel {
+  int dtype = 0;
+  int dims[3] = {0, 0, 0};
+  std::vector<uint8_t> raw;
+  float scale = 1.f, offset = 0.f;  // value = raw * scale + offset (rawFile.cpp:100-123)
+  float vmin = 0.f, vmax = 0.f;     // converted min/max
+  float raw_at(size_t i) const {
+    switch (dtype) {
+      case ORACLE_U8: return float(raw[i]);
+      case ORACLE_U16: { uint16_t v; std::memcpy(&v, &raw[2 * i], 2); return float(v); }
+      case ORACLE_F16: { uint16_t v; std::memcpy(&v, &raw[2 * i], 2); return half_to_float(v); }
+      default: { float v; std::memcpy(&v, &raw[4 * i], 4); return v; }
+    }
+  }
+  // converted voxel value (rawFile.cpp:100-123): u8 -> v*255, u16 -> v*65535
+  float value_at(size_t i) const { return raw_at(i) * scale + offset; }
+};
+
+struct Hit {
+  bool missed = true;
+  vec4f rgba{0, 0, 0, 0};
+  float tHit = 0.f;
+  int channel = 0;
+  uint64_t samples = 0, rejections = 0, visits = 0;
+};
+
+}  // namespace
+
+struct oracle_renderer {
+  std::string err;
+  std::vector<Channel> ch;
+  vec3f remap{1.f, 1.f, 1.f};
+  bool remap_set = false;
+  bool inited = false;
+  int mode = 0;
+  int mc[3] = {0, 0, 0};
+  vec3f blo{0, 0, 0}, bhi{1, 1, 1};       // world box (rawFile.h:162, reshapeBounds)
+  std::vector<float> ranges;              // float2 per (cell, ch)
+  std::vector<float> maj;
+  std::vector<TF> tf;
+  std::vector<float> dom_lo, dom_hi;      // value domains (renderer.cpp:743-745)
+  // camera (camera.h state) and derived launch values (renderer.cpp:1058-1092)
+  vec3f cam_pos{0, -1, 0};
+  vec3f fvx{1, 0, 0}, fvy{0, 1, 0}, fvz{0, 0, 1};
+  vec3f cam_up{0, 1, 0};
+  float fovy = 60.f;
+  vec3f c_org{0, 0, 0}, c_llc{0, 0, 0}, c_hor{0, 0, 0}, c_ver{0, 0, 0};
+  float dt = 0.05f;
+  int W = 1024, H = 1024, spp = 1;
+  bool accumulate = true;
+  vec3f bg{0, 0, 0};
+  float light_intensity = 1.0f, ambient = 0.1f;   // renderer.h:124-126
+  vec3f spec_color{1.f, 1.f, 1.f};                 // renderer.h:159
+  uint32_t seed = 0x5eed1234u;
+  uint32_t frame_id = 0, accum_id = 0;
+  std::vector<vec4f> accum, m2;
+  uint64_t counters[4] = {0, 0, 0, 0};
+  int build_threads = 1;
+};
+
+namespace {
+
+int fail(oracle_renderer* o, const std::string& m) {
+  o->err = m;
+  return 1;
+}
+
+// ------------------------------------------------------------ camera -------
+// camera.h:83-110 setOrientation (+ forceUpFrame camera.h:72-80)
+void cam_set_orientation(oracle_renderer* o, vec3f origin, vec3f interest, vec3f up,
+                         float fovy) {
+  o->fovy = fovy;
+  o->cam_pos = origin;
+  o->cam_up = up;
+  const bool same = origin.x == interest.x && origin.y == interest.y && origin.z == interest.z;
+  o->fvz = same ? vec3f{0, 0, 1} : -normalize(interest - origin);
+  o->fvx = cross(up, o->fvz);
+  if (dot(o->fvx, o->fvx) < 1e-8f)
+    o->fvx = vec3f{0, 1, 0};
+  else
+    o->fvx = normalize(o->fvx);
+  o->fvy = normalize(cross(o->fvz, o->fvx));
+  // forceUpFrame
+  if (fabsf(dot(o->fvz, o->cam_up)) < 1e-6f) return;
+  o->fvx = normalize(cross(o->cam_up, o->fvz));
+  o->fvy = normalize(cross(o->fvz, o->fvx));
+}
+
+inline float to_radian(float d) { return d * float(M_PI / 180.0); }
+inline float to_degrees(float r) { return r * float(180.0 / M_PI); }
+
+// renderer.cpp:1058-1092 UpdateCamera
+void update_camera(oracle_renderer* o) {
+  const vec3f lookFrom = o->cam_pos;
+  const vec3f lookAt = o->cam_pos - o->fvz;  // camera.h:59 getAt
+  const vec3f lookUp = o->fvy;               // camera.h:61 getUp
+  const float cosFovy = cosf(to_radian(o->fovy));
+  const float vfov = to_degrees(acosf(cosFovy));
+  const vec3f vup = lookUp;
+  const float aspect = float(o->W) / float(o->H);
+  const float theta = vfov * float(M_PI) / 180.0f;
+  const float half_height = tanf(theta / 2.0f);
+  const float half_width = aspect * half_height;
+  const float focusDist = 10.f;
+  const vec3f origin = lookFrom;
+  const vec3f w = normalize(lookFrom - lookAt);
+  const vec3f u = normalize(cross(vup, w));
+  const vec3f v = cross(w, u);
+  o->c_org = origin;
+  o->c_llc = origin - half_width * focusDist * u - half_height * focusDist * v - focusDist * w;
+  o->c_hor = 2.0f * half_width * focusDist * u;
+  o->c_ver = 2.0f * half_height * focusDist * v;
+  o->accum_id = 0;  // ResetAccumulation (renderer.cpp:1091)
+}
+
+// ------------------------------------------------------ range grid ---------
+// renderer.cu:387-391 project
+inline int project(float f, float lo, float hi, int dim) {
+  const float q = float(dim) * (f - lo) / (hi - lo);
+  const int i = int(q);
+  return std::max(0, std::min(dim - 1, i));
+}
+
+// renderer.cu:1116-1179 buildSpatialMacrocells (RAW), sizeMCs 330-347,
+// rasterElements (RAW) 835-893, rasterBox (float2) 487-519.
+void build_ranges(oracle_renderer* o) {
+  const int C = int(o->ch.size());
+  const int Mx = o->mc[0], My = o->mc[1], Mz = o->mc[2];
+  const size_t ncell = size_t(Mx) * My * Mz;
+  o->ranges.assign(ncell * C * 2, 0.f);
+  for (size_t i = 0; i < ncell * C; ++i) {  // sizeMCs: (1e20, -1e20)
+    o->ranges[2 * i] = 1e20f;
+    o->ranges[2 * i + 1] = -1e20f;
+  }
+  const vec3f wlo = o->blo, whi = o->bhi;
+  for (int c = 0; c < C; ++c) {
+    const Channel& chn = o->ch[c];
+    const int nx = chn.dims[0], ny = chn.dims[1], nz = chn.dims[2];
+    const size_t nvox = size_t(nx) * ny * nz;
+    std::vector<float> val(nvox);  // the reference's float scalar buffer (renderer.cpp:690-696)
+    for (size_t i = 0; i < nvox; ++i) val[i] = chn.value_at(i);
+    const vec3f size = whi - wlo;
+    const vec3f boxLen = {size.x / float(nx), size.y / float(ny), size.z / float(nz)};
+    // One thread per voxel in the reference, merged by order-independent CAS
+    // min/max; here z-slabs per OpenMP thread, each into a private grid that
+    // is min/max-merged afterwards (same result for any thread count).
+    const int nthr = o->build_threads > 0 ? o->build_threads : 1;
+    std::vector<std::vector<float>> part(nthr);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthr)
+#endif
+    {
+#ifdef _OPENMP
+      const int tid = omp_get_thread_num();
+#else
+      const int tid = 0;
+#endif
+      std::vector<float>& g = part[tid];
+      g.assign(ncell * 2, 0.f);
+      for (size_t i = 0; i < ncell; ++i) {
+        g[2 * i] = 1e20f;
+        g[2 * i + 1] = -1e20f;
+      }
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+      for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+          for (int ix = 0; ix < nx; ++ix) {
+            const size_t idx = size_t(ix) + size_t(iy) * nx + size_t(iz) * nx * ny;
+            const vec3f vxlLower = wlo + vec3f{float(ix), float(iy), float(iz)} * boxLen;
+            const float s = val[idx];
+            vec3f lower = vxlLower, upper = vxlLower + boxLen;
+            float lw = s, uw = s;
+            for (int dz = -1; dz <= 1; ++dz)
+              for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                  const int qx = ix + dx, qy = iy + dy, qz = iz + dz;
+                  if (qx < 0 || qx >= nx) continue;
+                  if (qy < 0 || qy >= ny) continue;
+                  if (qz < 0 || qz >= nz) continue;
+                  const size_t q = size_t(qx) + size_t(qy) * nx + size_t(qz) * nx * ny;
+                  const vec3f nl = wlo + vec3f{float(qx), float(qy), float(qz)} * boxLen;
+                  const float sn = val[q];
+                  lower = vmin(lower, nl);
+                  upper = vmax(upper, nl);
+                  lw = fminf(lw, sn);
+                  uw = fmaxf(uw, sn);
+                }
+            // rasterBox (float2)
+            if (lower.x >= upper.x || lower.y >= upper.y || lower.z >= upper.z) continue;
+            const int lx = project(lower.x, wlo.x, whi.x, Mx);
+            const int ly = project(lower.y, wlo.y, whi.y, My);
+            const int lz = project(lower.z, wlo.z, whi.z, Mz);
+            const int hx = std::min(project(upper.x, wlo.x, whi.x, Mx), Mx - 1);
+            const int hy = std::min(project(upper.y, wlo.y, whi.y, My), My - 1);
+            const int hz = std::min(project(upper.z, wlo.z, whi.z, Mz), Mz - 1);
+            for (int cz = lz; cz <= hz; ++cz)
+              for (int cy = ly; cy <= hy; ++cy)
+                for (int cx = lx; cx <= hx; ++cx) {
+                  const size_t cell = size_t(cx) + size_t(cy) * Mx + size_t(cz) * Mx * My;
+                  float* r = &g[2 * cell];
+                  if (lw < r[0]) r[0] = lw;  // CAS atomicMin (renderer.cu:362-371)
+                  if (uw > r[1]) r[1] = uw;  // CAS atomicMax (renderer.cu:374-384)
+                }
+          }
+    }
+    for (int t = 0; t < nthr; ++t)
+      for (size_t cell = 0; cell < ncell; ++cell) {
+        float* r = &o->ranges[2 * (cell * C + c)];
+        const float* p = &part[t][2 * cell];
+        if (p[0] < r[0]) r[0] = p[0];
+        if (p[1] > r[1]) r[1] = p[1];
+      }
+  }
+}
+
+// renderer.cu:159-203 (_recalculateDensityRangesMM) per-channel majorant of
+// one (cell, channel); renderer.cu:111-157 sums the same quantity for mode 3.
+float channel_majorant(const oracle_renderer* o, int c, float mn, float mx) {
+  const TF& tf = o->tf[c];
+  const float vlo = o->dom_lo[c], vhi = o->dom_hi[c];
+  const int n = int(tf.texels.size());
+  const float remappedMin1 = (mn - vlo) / (vhi - vlo);
+  const float remappedMin = (remappedMin1 - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  const float remappedMax1 = (mx - vlo) / (vhi - vlo);
+  const float remappedMax = (remappedMax1 - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  const float addr1 = remappedMin * float(n);
+  const float addr2 = remappedMax * float(n);
+  const int addrMin = clamp_index(fminf(floorf(addr1), floorf(addr2)), n);
+  const int addrMax = clamp_index(fmaxf(ceilf(addr1), ceilf(addr2)), n);
+  float m = 0.f;
+  for (int i = addrMin; i <= addrMax; ++i) {
+    const float d = tf_lookup_u(tf, (float(i) + 0.5f) / float(n)).w * tf.opacity_scale;
+    m = (i == addrMin) ? d : fmaxf(m, d);
+  }
+  float d = tf_lookup_u(tf, (addr1 + 0.5f) / float(n)).w * tf.opacity_scale;
+  m = fmaxf(m, d);
+  d = tf_lookup_u(tf, (addr2 + 0.5f) / float(n)).w * tf.opacity_scale;
+  m = fmaxf(m, d);
+  return m;
+}
+
+// renderer.cu:206-274 RecalculateDensityRanges (grid part)
+void build_majorants(oracle_renderer* o) {
+  const int C = int(o->ch.size());
+  const size_t ncell = size_t(o->mc[0]) * o->mc[1] * o->mc[2];
+  const bool per_channel = o->mode <= 2;
+  o->maj.assign(per_channel ? ncell * C : ncell, 0.f);
+  for (size_t cell = 0; cell < ncell; ++cell) {
+    float sum = 0.f;
+    for (int c = 0; c < C; ++c) {
+      const float mn = o->ranges[2 * (cell * C + c)];
+      const float mx = o->ranges[2 * (cell * C + c) + 1];
+      if (mx < mn) continue;  // empty box
+      const float m = channel_majorant(o, c, mn, mx);
+      if (per_channel)
+        o->maj[cell * C + c] = m;
+      else
+        sum += m;
+    }
+    if (!per_channel) o->maj[cell] = sum;
+  }
+}
+
+bool tfs_ready(const oracle_renderer* o) {
+  for (const TF& t : o->tf)
+    if (!t.set) return false;
+  return true;
+}
+
+// ------------------------------------------------------ device programs ----
+// deviceCode.cu:27-42 transferFunction
+inline vec4f transfer_function(const oracle_renderer* o, float f, int c) {
+  const float vlo = o->dom_lo[c], vhi = o->dom_hi[c];
+  if (f < vlo || f > vhi) return {1.f, 0.f, 1.f, 0.f};
+  const TF& tf = o->tf[c];
+  const float r1 = (f - vlo) / (vhi - vlo);
+  const float r2 = (r1 - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  vec4f xf = tf_lookup_u(tf, r2);
+  xf.w = xf.w * tf.opacity_scale;
+  return xf;
+}
+
+// deviceCode.cu:122-132 sampleVolumeTexture -> sample 91-120 (one texture):
+// NaN outside [0,1]^3, else clamp-to-edge trilinear at texel coordinate p*N-0.5.
+inline float sample_volume(const oracle_renderer* o, vec3f p, int c) {
+  if (p.x < 0.0f || p.x > 1.0f || p.y < 0.0f || p.y > 1.0f || p.z < 0.0f || p.z > 1.0f)
+    return NAN;
+  const Channel& chn = o->ch[c];
+  const int nx = chn.dims[0], ny = chn.dims[1], nz = chn.dims[2];
+  const float x = p.x * float(nx) - 0.5f;
+  const float y = p.y * float(ny) - 0.5f;
+  const float z = p.z * float(nz) - 0.5f;
+  const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+  const float fx = x - flx, fy = y - fly, fz = z - flz;
+  const int x0 = clamp_index(flx, nx), x1 = clamp_index_plus1(flx, nx);
+  const int y0 = clamp_index(fly, ny), y1 = clamp_index_plus1(fly, ny);
+  const int z0 = clamp_index(flz, nz), z1 = clamp_index_plus1(flz, nz);
+  auto at = [&](int i, int j, int k) {
+    return chn.raw_at(size_t(i) + size_t(j) * nx + size_t(k) * size_t(nx) * ny);
+  };
+  const float v000 = at(x0, y0, z0), v100 = at(x1, y0, z0);
+  const float v010 = at(x0, y1, z0), v110 = at(x1, y1, z0);
+  const float v001 = at(x0, y0, z1), v101 = at(x1, y0, z1);
+  const float v011 = at(x0, y1, z1), v111 = at(x1, y1, z1);
+  const float c00 = v000 + fx * (v100 - v000);
+  const float c10 = v010 + fx * (v110 - v010);
+  const float c01 = v001 + fx * (v101 - v001);
+  const float c11 = v011 + fx * (v111 - v011);
+  const float c0 = c00 + fy * (c10 - c00);
+  const float c1 = c01 + fy * (c11 - c01);
+  const float r = c0 + fz * (c1 - c0);
+  return r * chn.scale + chn.offset;
+}
+
+// Grid-space ray set-up shared by the closest-hit programs
+// (deviceCode.cu:866-888; identical at 521-543 and 755-777).
+struct GridRay {
+  vec3f worldOrg, worldDir, worldToUnit, org, dir;
+  float worldToGridT, gridToWorldT;
+};
+inline GridRay grid_ray(const oracle_renderer* o, vec3f worldOrg, vec3f worldDir) {
+  GridRay g;
+  g.worldOrg = worldOrg;
+  g.worldDir = worldDir;
+  vec3f org = worldOrg - o->blo;
+  g.worldToUnit = 1.f / (o->bhi - o->blo);
+  const vec3f unitToGrid = {float(o->mc[0]), float(o->mc[1]), float(o->mc[2])};
+  g.org = unitToGrid * g.worldToUnit * org;
+  vec3f dir = g.worldToUnit * unitToGrid * worldDir;
+  g.worldToGridT = length(dir);
+  g.gridToWorldT = 1.f / g.worldToGridT;
+  g.dir = normalize(dir);
+  return g;
+}
+
+// dda.h:82-203 dda3. The lambda returns true to continue. Reproduces the
+// reference's zero-direction handling (dda.h:96-116: t_nr written twice,
+// t_fr untouched), which drops rays with an exactly zero grid direction.
+template <typename F>
+void dda3(vec3f org, vec3f dir, float tMax, const int grid[3], F&& lambda) {
+  const vec3f f_size = {float(grid[0]), float(grid[1]), float(grid[2])};
+  const vec3f rcp_dir = 1.f / dir;
+  const vec3f t_lo = (vec3f{0.f, 0.f, 0.f} - org) * rcp_dir;
+  const vec3f t_hi = (f_size - org) * rcp_dir;
+  vec3f t_nr = vmin(t_lo, t_hi);
+  const vec3f t_fr = vmax(t_lo, t_hi);
+  if (dir.x == 0.f) {
+    if (org.x < 0.f || org.x > f_size.x) return;
+    t_nr.x = INFINITY;
+  }
+  if (dir.y == 0.f) {
+    if (org.y < 0.f || org.y > f_size.y) return;
+    t_nr.y = INFINITY;
+  }
+  if (dir.z == 0.f) {
+    if (org.z < 0.f || org.z > f_size.z) return;
+    t_nr.z = INFINITY;
+  }
+  const float ray_t0 = fmaxf(0.f, reduce_max(t_nr));
+  const float ray_t1 = fminf(tMax, reduce_min(t_fr));
+  if (ray_t0 > ray_t1) return;
+  const vec3f org_in_volume = org + ray_t0 * dir;
+  const vec3f f_cell = vmax(vec3f{0.f, 0.f, 0.f},
+                            vmin(f_size - vec3f{1.f, 1.f, 1.f}, vfloor(org_in_volume)));
+  const vec3f f_cell_end = {dir.x > 0.f ? f_cell.x + 1.f : f_cell.x,
+                            dir.y > 0.f ? f_cell.y + 1.f : f_cell.y,
+                            dir.z > 0.f ? f_cell.z + 1.f : f_cell.z};
+  const vec3f t_step = vabs(rcp_dir);
+  vec3f t_next = {
+      (dir.x == 0.f) ? INFINITY : fabsf(f_cell_end.x - org_in_volume.x) * t_step.x,
+      (dir.y == 0.f) ? INFINITY : fabsf(f_cell_end.y - org_in_volume.y) * t_step.y,
+      (dir.z == 0.f) ? INFINITY : fabsf(f_cell_end.z - org_in_volume.z) * t_step.z};
+  const vec3i stop = {dir.x > 0.f ? grid[0] : -1, dir.y > 0.f ? grid[1] : -1,
+                      dir.z > 0.f ? grid[2] : -1};
+  const vec3i delta = {dir.x > 0.f ? 1 : -1, dir.y > 0.f ? 1 : -1, dir.z > 0.f ? 1 : -1};
+  vec3i cell = {int(f_cell.x), int(f_cell.y), int(f_cell.z)};
+  float next_cell_begin = 0.f;
+  while (true) {
+    const float t_closest = reduce_min(t_next);
+    const float cell_t0 = ray_t0 + next_cell_begin;
+    const float cell_t1 = ray_t0 + fminf(t_closest, tMax);
+    if (!lambda(cell, cell_t0, cell_t1)) return;
+    next_cell_begin = t_closest;
+    // a NaN ray steps no axis; end it instead of spinning (finite rays
+    // always step at least one axis)
+    bool stepped = false;
+    if (t_next.x == t_closest) {
+      t_next.x += t_step.x;
+      cell.x += delta.x;
+      stepped = true;
+      if (cell.x == stop.x) return;
+    }
+    if (t_next.y == t_closest) {
+      t_next.y += t_step.y;
+      cell.y += delta.y;
+      stepped = true;
+      if (cell.y == stop.y) return;
+    }
+    if (t_next.z == t_closest) {
+      t_next.z += t_step.z;
+      cell.z += delta.z;
+      stepped = true;
+      if (cell.z == stop.z) return;
+    }
+    if (!stepped) return;
+  }
+}
+
+inline int cell_id(const oracle_renderer* o, vec3i c) {
+  return c.x + c.y * o->mc[0] + c.z * o->mc[0] * o->mc[1];
+}
+
+inline void record_hit(Hit& prd, float tWorld, vec4f s, int c) {
+  prd.tHit = tWorld;
+  prd.rgba = s;
+  prd.rgba.w = 1.0f;
+  prd.missed = false;
+  prd.channel = c;
+}
+
+// deviceCode.cu:858-970 baseLineDTCH (-m 0, N-DDA)
+void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
+  const int C = int(o->ch.size());
+  const float unit = o->dt;
+  int curMesh = 0;
+  float tMax = 1e20f;
+  auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
+    prd.visits++;
+    const int cid = cell_id(o, cellIdx);
+    const float majorant = o->maj[size_t(cid) * C + curMesh];
+    if (majorant == 0.00f) return true;
+    float t = t0;
+    while (true) {
+      t = t - (plogf(1.0f - rng()) / majorant) * unit * g.worldToGridT;
+      if (t >= t1) break;
+      if (tMax < t) return false;
+      const float tWorld = t * g.gridToWorldT;
+      const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+      if (tWorld >= prd_t1) {
+        prd.rejections++;
+        return false;
+      }
+      const float nullColThreshold = rng() * majorant;
+      const float value = sample_volume(o, xTexture, curMesh);
+      prd.samples++;
+      if (std::isnan(value)) continue;
+      const vec4f s = transfer_function(o, value, curMesh);
+      if (s.w > 0.0f && nullColThreshold < s.w) {
+        record_hit(prd, tWorld, s, curMesh);
+        tMax = fminf(tMax, t);
+        return false;
+      }
+      prd.rejections++;
+    }
+    return true;
+  };
+  for (int i = 0; i < C; ++i) {
+    curMesh = i;
+    dda3(g.org, g.dir, 1e20f, o->mc, lambda);
+  }
+}
+
+// deviceCode.cu:747-856 altMultiMajDTCH (-m 2, syncDDA)
+void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
+  const int C = int(o->ch.size());
+  const float unit = o->dt;
+  auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
+    prd.visits++;
+    const int cid = cell_id(o, cellIdx);
+    float minT = t1;
+    for (int c = 0; c < C; ++c) {
+      const float majorant = o->maj[size_t(cid) * C + c];
+      float t = t0;
+      if (majorant == 0.0f) continue;
+      while (true) {
+        t = t - (plogf(1.0f - rng()) / majorant) * unit * g.worldToGridT;
+        if (t >= minT) break;
+        const float tWorld = t * g.gridToWorldT;
+        const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+        if (tWorld >= prd_t1) {
+          prd.rejections++;
+          if (c == C - 1) return false;
+          break;
+        }
+        const float nullColThreshold = rng() * majorant;
+        const float value = sample_volume(o, xTexture, c);
+        prd.samples++;
+        if (!std::isnan(value)) {
+          const vec4f s = transfer_function(o, value, c);
+          if (s.w > 0.0f && nullColThreshold < s.w) {
+            record_hit(prd, tWorld, s, c);
+            minT = fminf(t, minT);
+          }
+        }
+        prd.rejections++;
+      }
+    }
+    return prd.missed;
+  };
+  dda3(g.org, g.dir, 1e20f, o->mc, lambda);
+}
+
+// deviceCode.cu:513-617 cummilativeDTCH (-m 3, cumulative majorant)
+void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng,
+                      Hit& prd) {
+  const int C = int(o->ch.size());
+  const float unit = o->dt;
+  auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
+    prd.visits++;
+    const int cid = cell_id(o, cellIdx);
+    const float majorant = o->maj[size_t(cid)];
+    if (majorant == 0.00f) return true;
+    float t = t0;
+    while (true) {
+      t = t - (plogf(1.0f - rng()) / majorant) * unit * g.worldToGridT;
+      if (t >= t1) break;
+      const float tWorld = t * g.gridToWorldT;
+      const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+      if (tWorld >= prd_t1) {
+        prd.rejections++;
+        return false;
+      }
+      float nullColThreshold = rng() * majorant;
+      for (int c = 0; c < C; ++c) {
+        const float value = sample_volume(o, xTexture, c);
+        prd.samples++;
+        if (std::isnan(value)) continue;
+        const vec4f s = transfer_function(o, value, c);
+        if (s.w > 0.0f && nullColThreshold < s.w) {
+          record_hit(prd, tWorld, s, c);
+          return false;
+        }
+        nullColThreshold -= s.w;
+      }
+      prd.rejections++;
+    }
+    return true;
+  };
+  dda3(g.org, g.dir, 1e20f, o->mc, lambda);
+}
+
+// deviceCode.cu:61-76 generateRay
+inline void generate_ray(const oracle_renderer* o, float su, float sv, vec3f& org, vec3f& dir) {
+  org = o->c_org;
+  vec3f d = o->c_llc + su * o->c_hor + sv * o->c_ver;
+  if (fabsf(d.x) < 1e-5f) d.x = 1e-5f;
+  if (fabsf(d.y) < 1e-5f) d.y = 1e-5f;
+  if (fabsf(d.z) < 1e-5f) d.z = 1e-5f;
+  dir = normalize(d - org);
+}
+
+// deviceCode.cu:1676-1726 volumeIntersection against the root box
+// (renderer.cpp:700-704: union of channel bounds); ray tmin = 0.
+inline bool clip_root(const oracle_renderer* o, vec3f org, vec3f dir, float& t0, float& t1) {
+  const vec3f lb = o->blo, rt = o->bhi;
+  const float dfx = 1.0f / dir.x, dfy = 1.0f / dir.y, dfz = 1.0f / dir.z;
+  const float a1 = (lb.x - org.x) * dfx, a2 = (rt.x - org.x) * dfx;
+  const float a3 = (lb.y - org.y) * dfy, a4 = (rt.y - org.y) * dfy;
+  const float a5 = (lb.z - org.z) * dfz, a6 = (rt.z - org.z) * dfz;
+  float thit0 = fmaxf(fmaxf(fminf(a1, a2), fminf(a3, a4)), fminf(a5, a6));
+  const float thit1 = fminf(fminf(fmaxf(a1, a2), fmaxf(a3, a4)), fmaxf(a5, a6));
+  if (thit1 < 0) return false;
+  if (thit0 >= thit1) return false;
+  thit0 = fmaxf(thit0, 0.f);
+  t0 = fmaxf(0.f, thit0);
+  t1 = fminf(1e20f, thit1);
+  return true;
+}
+
+inline uint32_t make_8bit(float f) {
+  const float s = f * 256.f;
+  // clamp in float first so that the conversion is always defined
+  const float c = fminf(fmaxf(s, -1.f), 256.f);
+  const int i = int(c);
+  return uint32_t(std::min(255, std::max(0, i)));
+}
+inline uint32_t make_rgba(vec4f c) {
+  return make_8bit(c.x) | (make_8bit(c.y) << 8) | (make_8bit(c.z) << 16) | (make_8bit(c.w) << 24);
+}
+
+// deviceCode.cu:283-479 mainRG for one pixel and one frame (shadows and
+// gradient shading off, renderer.h:121-122; no surfaces, tMax = 1e20).
+void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
+  const size_t fbOfs = size_t(px) + size_t(o->W) * py;
+  vec4f finalColor = {0.f, 0.f, 0.f, 1.f};
+  const uint32_t pixel = uint32_t(fbOfs);
+  for (int sampleID = 0; sampleID < o->spp; ++sampleID) {
+    Rng rng(o->seed, pixel, o->frame_id, uint32_t(sampleID), 0u);
+    const float jit = rng();
+    const float su = (float(px) + jit) / float(o->W);
+    const float sv = (float(py) + jit) / float(o->H);
+    vec3f org, dir;
+    generate_ray(o, su, sv, org, dir);
+    vec4f color = {0.f, 0.f, 0.f, 0.f};
+    float t0, t1;
+    Hit prd;
+    if (clip_root(o, org, dir, t0, t1)) {
+      const GridRay g = grid_ray(o, org, dir);
+      if (o->mode == 0)
+        trace_ndda(o, g, t1, rng, prd);
+      else if (o->mode == 2)
+        trace_sync(o, g, t1, rng, prd);
+      else
+        trace_cumulative(o, g, t1, rng, prd);
+      cnt[0] += prd.samples;
+      cnt[1] += prd.visits;
+      cnt[2] += prd.rejections;
+    }
+    cnt[3] += 1;
+    if (!prd.missed) {
+      // deviceCode.cu:326-433 with diffuse = 1, specular = 0, shadow = 1
+      const vec3f albedo = {prd.rgba.x, prd.rgba.y, prd.rgba.z};
+      const float transparency = prd.rgba.w;
+      const float diffuse = 1.0f, specular = 0.0f, shadow = 1.0f;
+      const vec3f sc = o->spec_color;
+      const vec3f mixedSpecularColor = albedo * 0.75f + sc * 0.25f;
+      const vec3f ambientColor = albedo * 0.75f + sc * 0.25f;
+      const vec3f diffuseContrib = diffuse * shadow * o->light_intensity * albedo;
+      const vec3f specularContrib = specular * shadow * o->light_intensity * mixedSpecularColor;
+      const vec3f ambientContrib = o->ambient * ambientColor;
+      const vec3f shaded = diffuseContrib + specularContrib + ambientContrib;
+      color = {shaded.x, shaded.y, shaded.z, transparency};
+    }
+    // over(color, vec4f(bg, 1)) (deviceCode.cu:44-59, 433)
+    const float Ain = color.w, Ax = 1.f;
+    const vec3f Cin = {color.x, color.y, color.z};
+    const vec3f c = Cin + o->bg * Ax * (1.f - Ain);
+    const float a = Ain + (1.f - Ain) * Ax;
+    finalColor = finalColor + vec4f{c.x, c.y, c.z, a};
+  }
+  finalColor = finalColor / float(o->spp);
+  // deviceCode.cu:463-469 running mean; the second moment is an addition
+  const float k = float(o->accum_id);
+  vec4f& acc = o->accum[fbOfs];
+  vec4f& sq = o->m2[fbOfs];
+  const vec4f c2 = {finalColor.x * finalColor.x, finalColor.y * finalColor.y,
+                    finalColor.z * finalColor.z, finalColor.w * finalColor.w};
+  if (o->accumulate) {
+    acc = (finalColor + k * acc) / float(o->accum_id + 1);
+    sq = (c2 + k * sq) / float(o->accum_id + 1);
+  } else {
+    acc = finalColor;
+    sq = c2;
+  }
+}
+
+}  // namespace
+
+// ================================================================= C API ===
+extern "C" {
+
+oracle_renderer* oracle_create(void) { return new oracle_renderer(); }
+void oracle_destroy(oracle_renderer* o) { delete o; }
+const char* oracle_last_error(oracle_renderer* o) { return o->err.c_str(); }
+
+// renderer.cpp:1031-1044 PushMesh; values converted as rawFile.cpp:100-123
+int oracle_push_channel(oracle_renderer* o, const void* vox, int dtype, int nx, int ny, int nz) {
+  if (o->inited) return fail(o, "push after init");
+  if (int(o->ch.size()) >= kMaxChannels) return fail(o, "too many channels");
+  if (dtype < 0 || dtype > 3) return fail(o, "bad dtype");
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(o, "bad dims");
+  if (!o->ch.empty() && (o->ch[0].dims[0] != nx || o->ch[0].dims[1] != ny || o->ch[0].dims[2] != nz))
+    return fail(o, "channel dims differ");
+  Channel c;
+  c.dtype = dtype;
+  c.dims[0] = nx; c.dims[1] = ny; c.dims[2] = nz;
+  const size_t n = size_t(nx) * ny * nz;
+  const size_t bpv = dtype == ORACLE_U8 ? 1 : (dtype == ORACLE_F32 ? 4 : 2);
+  c.raw.assign(static_cast<const uint8_t*>(vox), static_cast<const uint8_t*>(vox) + n * bpv);
+  c.scale = dtype == ORACLE_U8 ? 255.f : (dtype == ORACLE_U16 ? 65535.f : 1.f);
+  c.offset = 0.f;
+  // bounds_.w starts at {0, 1} (rawFile.h:162) and is widened (rawFile.cpp:119-123)
+  float lo = 0.f, hi = 1.f;
+  for (size_t i = 0; i < n; ++i) {
+    const float v = c.value_at(i);
+    if (v > hi) hi = v;
+    if (v < lo) lo = v;
+  }
+  c.vmin = lo;
+  c.vmax = hi;
+  o->ch.push_back(std::move(c));
+  return 0;
+}
+
+// rawFile.cpp:228-237 reshapeBounds(remap): box [0,1]^3 -> [0, remap]
+int oracle_set_bounds(oracle_renderer* o, const float remap[3]) {
+  if (o->inited) return fail(o, "set_bounds after init");
+  for (int i = 0; i < 3; ++i)
+    if (!(remap[i] > 0.f) || !std::isfinite(remap[i])) return fail(o, "bad bounds");
+  o->remap = {remap[0], remap[1], remap[2]};
+  o->remap_set = true;
+  return 0;
+}
+
+// renderer.cpp:324-942 Init (RAW path 638-903, camera 856-863, dt 935)
+int oracle_init(oracle_renderer* o, int mode, const int mc[3], int auto_camera) {
+  if (o->ch.empty()) return fail(o, "no channels");
+  if (mode != 0 && mode != 2 && mode != 3) return fail(o, "unsupported mode");
+  o->mode = mode;
+  // CalculateMCGridDims(4) (renderer.cpp:1342-1355) for any zero entry
+  for (int i = 0; i < 3; ++i) {
+    if (mc[i] < 0) return fail(o, "bad mc");
+    if (mc[i] > 0) {
+      o->mc[i] = mc[i];
+    } else {
+      unsigned acc = 0;
+      for (const Channel& c : o->ch) acc += unsigned(float(c.dims[i]) / 4.f);
+      unsigned avg = unsigned(float(acc) / float(o->ch.size()));
+      o->mc[i] = avg < 1 ? 1 : int(avg);
+    }
+  }
+  o->blo = {0.f, 0.f, 0.f};
+  o->bhi = o->remap_set ? o->remap : vec3f{1.f, 1.f, 1.f};
+  const int C = int(o->ch.size());
+  o->dom_lo.resize(C);
+  o->dom_hi.resize(C);
+  o->tf.assign(C, TF());
+  for (int c = 0; c < C; ++c) {
+    o->dom_lo[c] = o->ch[c].vmin;
+    o->dom_hi[c] = o->ch[c].vmax;
+  }
+  build_ranges(o);
+  o->maj.assign(size_t(o->mc[0]) * o->mc[1] * o->mc[2] * (mode <= 2 ? C : 1), 0.f);
+  if (auto_camera) {
+    const vec3f center = {0.5f, 0.5f, 0.5f};
+    const vec3f eye = {center.x, center.y, center.z + 2.5f};
+    cam_set_orientation(o, eye, center, vec3f{0, 1, 0}, 45.0f);
+  }
+  o->inited = true;
+  o->accum.assign(size_t(o->W) * o->H, vec4f{0, 0, 0, 0});
+  o->m2.assign(size_t(o->W) * o->H, vec4f{0, 0, 0, 0});
+  update_camera(o);
+  o->frame_id = 0;
+  o->accum_id = 0;
+  oracle_reset_dt(o);
+  return 0;
+}
+
+// renderer.cpp:1094-1193 SetXFColormap (alpha cubed), SetXFOpacityScale, SetXFRange
+int oracle_set_tf(oracle_renderer* o, int ch, const float* rgba, int n_texels,
+                  float opacity_scale, float xf_lo, float xf_hi) {
+  if (!o->inited) return fail(o, "set_tf before init");
+  if (ch < 0 || ch >= int(o->ch.size())) return fail(o, "bad channel");
+  if (n_texels < 1 || n_texels > 4096) return fail(o, "bad texel count");
+  if (!(xf_hi != xf_lo) || !std::isfinite(xf_lo) || !std::isfinite(xf_hi))
+    return fail(o, "bad xf range");
+  if (!(opacity_scale >= 0.f && opacity_scale <= 1000.f)) return fail(o, "bad opacity scale");
+  for (int i = 0; i < n_texels; ++i)
+    if (!(rgba[4 * i + 3] >= 0.f && rgba[4 * i + 3] <= 1.f)) return fail(o, "alpha outside [0,1]");
+  TF& t = o->tf[ch];
+  t.texels.resize(n_texels);
+  for (int i = 0; i < n_texels; ++i)
+    t.texels[i] = {rgba[4 * i], rgba[4 * i + 1], rgba[4 * i + 2], powf(rgba[4 * i + 3], 3.f)};
+  t.opacity_scale = opacity_scale;
+  t.xf_lo = xf_lo;
+  t.xf_hi = xf_hi;
+  t.set = true;
+  o->accum_id = 0;
+  if (tfs_ready(o)) build_majorants(o);
+  return 0;
+}
+
+int oracle_set_camera(oracle_renderer* o, const float from[3], const float at[3],
+                      const float up[3], float fovy_deg) {
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(from[i]) || !std::isfinite(at[i]) || !std::isfinite(up[i]))
+      return fail(o, "non-finite camera");
+  cam_set_orientation(o, {from[0], from[1], from[2]}, {at[0], at[1], at[2]},
+                      {up[0], up[1], up[2]}, fovy_deg);
+  if (o->inited) update_camera(o);
+  return 0;
+}
+
+// renderer.cpp:1460-1466 Setdt
+int oracle_set_dt(oracle_renderer* o, float dt) {
+  o->dt = fmaxf(dt, 1e-4f);
+  o->accum_id = 0;
+  return 0;
+}
+
+// renderer.cpp:1444-1453 Resetdt (RAW): the loop assigns, so the last channel wins
+int oracle_reset_dt(oracle_renderer* o) {
+  if (o->ch.empty()) return fail(o, "no channels");
+  float minVoxelSideLength = 3.402823466e38f;
+  const vec3f span = o->bhi - o->blo;
+  for (const Channel& c : o->ch)
+    minVoxelSideLength = fminf(span.x / float(c.dims[0]),
+                               fminf(span.y / float(c.dims[1]), span.z / float(c.dims[2])));
+  return oracle_set_dt(o, minVoxelSideLength * 0.5f);
+}
+
+int oracle_set_frame(oracle_renderer* o, int w, int h, int spp, int accumulate, const float bg[3]) {
+  if (w <= 0 || h <= 0 || spp <= 0) return fail(o, "bad frame");
+  o->W = w;
+  o->H = h;
+  o->spp = spp;
+  o->accumulate = accumulate != 0;
+  o->bg = {bg[0], bg[1], bg[2]};
+  o->accum.assign(size_t(w) * h, vec4f{0, 0, 0, 0});
+  o->m2.assign(size_t(w) * h, vec4f{0, 0, 0, 0});
+  if (o->inited) update_camera(o);
+  o->accum_id = 0;
+  return 0;
+}
+
+int oracle_set_seed(oracle_renderer* o, uint32_t seed) {
+  o->seed = seed;
+  o->accum_id = 0;
+  return 0;
+}
+
+int oracle_set_threads(oracle_renderer* o, int n) {
+  o->build_threads = n < 1 ? 1 : n;
+  return 0;
+}
+
+int oracle_set_lighting(oracle_renderer* o, float light_intensity, float ambient) {
+  o->light_intensity = light_intensity;
+  o->ambient = ambient;
+  o->accum_id = 0;
+  return 0;
+}
+
+// renderer.cpp:944-973 Render: frame k of an accumulation uses accumID = k
+int oracle_render(oracle_renderer* o, int n_frames, int y0, int y1, int nthreads) {
+  if (!o->inited) return fail(o, "render before init");
+  if (!tfs_ready(o)) return fail(o, "transfer functions not set");
+  if (y1 <= 0 || y1 > o->H) y1 = o->H;
+  if (y0 < 0) y0 = 0;
+  for (int i = 0; i < 4; ++i) o->counters[i] = 0;
+  for (int f = 0; f < n_frames; ++f) {
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) \
+    reduction(+ : s0, s1, s2, s3)
+#endif
+    for (int py = y0; py < y1; ++py) {
+      uint64_t cnt[4] = {0, 0, 0, 0};
+      for (int px = 0; px < o->W; ++px) render_pixel(o, px, py, cnt);
+      s0 += cnt[0]; s1 += cnt[1]; s2 += cnt[2]; s3 += cnt[3];
+    }
+    o->counters[0] += s0; o->counters[1] += s1; o->counters[2] += s2; o->counters[3] += s3;
+    o->accum_id++;
+    o->frame_id++;
+  }
+  return 0;
+}
+
+int oracle_reset_accumulation(oracle_renderer* o) {
+  o->accum_id = 0;
+  return 0;
+}
+
+int oracle_read_accum(oracle_renderer* o, float* rgba, float* m2) {
+  const size_t n = size_t(o->W) * o->H;
+  if (rgba) std::memcpy(rgba, o->accum.data(), n * 16);
+  if (m2) std::memcpy(m2, o->m2.data(), n * 16);
+  return 0;
+}
+
+int oracle_read_fb(oracle_renderer* o, uint32_t* out) {
+  const size_t n = size_t(o->W) * o->H;
+  for (size_t i = 0; i < n; ++i) out[i] = make_rgba(o->accum[i]);
+  return 0;
+}
+
+int oracle_read_ranges(oracle_renderer* o, float* out) {
+  if (!o->inited) return fail(o, "not initialized");
+  std::memcpy(out, o->ranges.data(), o->ranges.size() * 4);
+  return 0;
+}
+
+int oracle_read_majorants(oracle_renderer* o, float* out) {
+  if (!o->inited) return fail(o, "not initialized");
+  std::memcpy(out, o->maj.data(), o->maj.size() * 4);
+  return 0;
+}
+
+int oracle_read_counters(oracle_renderer* o, uint64_t out[4]) {
+  for (int i = 0; i < 4; ++i) out[i] = o->counters[i];
+  return 0;
+}
+
+int oracle_get_info(oracle_renderer* o, int out[10]) {
+  out[0] = o->mc[0]; out[1] = o->mc[1]; out[2] = o->mc[2];
+  out[3] = o->ch.empty() ? 0 : o->ch[0].dims[0];
+  out[4] = o->ch.empty() ? 0 : o->ch[0].dims[1];
+  out[5] = o->ch.empty() ? 0 : o->ch[0].dims[2];
+  out[6] = int(o->ch.size());
+  out[7] = o->mode;
+  out[8] = int(o->frame_id);
+  out[9] = int(o->accum_id);
+  return 0;
+}
+
+float oracle_get_dt(oracle_renderer* o) { return o->dt; }
+
+int oracle_get_camera(oracle_renderer* o, float out[12]) {
+  const vec3f v[4] = {o->c_org, o->c_llc, o->c_hor, o->c_ver};
+  for (int i = 0; i < 4; ++i) {
+    out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z;
+  }
+  return 0;
+}
+
+int oracle_get_domains(oracle_renderer* o, float* out) {
+  for (size_t c = 0; c < o->ch.size(); ++c) {
+    out[2 * c] = o->dom_lo[c];
+    out[2 * c + 1] = o->dom_hi[c];
+  }
+  return 0;
+}
+
+int oracle_trace_cells(oracle_renderer* o, const float* org, const float* dir, int n,
+                       int max_cells, int32_t* cells, float* t01, int32_t* counts) {
+  if (!o->inited) return fail(o, "not initialized");
+  for (int r = 0; r < n; ++r) {
+    const GridRay g = grid_ray(o, {org[3 * r], org[3 * r + 1], org[3 * r + 2]},
+                               {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]});
+    int k = 0;
+    dda3(g.org, g.dir, 1e20f, o->mc, [&](vec3i c, float t0, float t1) -> bool {
+      if (k < max_cells) {
+        cells[size_t(r) * max_cells + k] = cell_id(o, c);
+        t01[2 * (size_t(r) * max_cells + k)] = t0;
+        t01[2 * (size_t(r) * max_cells + k) + 1] = t1;
+      }
+      ++k;
+      return true;
+    });
+    counts[r] = k;
+  }
+  return 0;
+}
+
+int oracle_gen_rays(oracle_renderer* o, uint32_t frame, uint32_t sample, float* org, float* dir) {
+  if (!o->inited) return fail(o, "not initialized");
+  for (int py = 0; py < o->H; ++py)
+    for (int px = 0; px < o->W; ++px) {
+      const size_t i = size_t(px) + size_t(o->W) * py;
+      Rng rng(o->seed, uint32_t(i), frame, sample, 0u);
+      const float jit = rng();
+      const float su = (float(px) + jit) / float(o->W);
+      const float sv = (float(py) + jit) / float(o->H);
+      vec3f ro, rd;
+      generate_ray(o, su, sv, ro, rd);
+      org[3 * i] = ro.x; org[3 * i + 1] = ro.y; org[3 * i + 2] = ro.z;
+      dir[3 * i] = rd.x; dir[3 * i + 1] = rd.y; dir[3 * i + 2] = rd.z;
+    }
+  return 0;
+}
+
+int oracle_clip_rays(oracle_renderer* o, const float* org, const float* dir, int n, float* t01,
+                     int32_t* hit) {
+  for (int r = 0; r < n; ++r) {
+    float t0 = 0.f, t1 = 0.f;
+    hit[r] = clip_root(o, {org[3 * r], org[3 * r + 1], org[3 * r + 2]},
+                       {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]}, t0, t1) ? 1 : 0;
+    t01[2 * r] = t0;
+    t01[2 * r + 1] = t1;
+  }
+  return 0;
+}
+
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  philox10(ctr, key, out);
+}
+
+float oracle_logf(float x) { return plogf(x); }
+
+float oracle_sample(oracle_renderer* o, int ch, const float p[3]) {
+  return sample_volume(o, {p[0], p[1], p[2]}, ch);
+}
+
+void oracle_tf(oracle_renderer* o, int ch, float value, float out[4]) {
+  const vec4f v = transfer_function(o, value, ch);
+  out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+}
+
+}  // extern "C"

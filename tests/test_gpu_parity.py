@@ -1,0 +1,222 @@
+"""GPU parity: libmdwt.so on the MI355X against the CPU oracle on the same
+seeded inputs. Range grids, majorant grids, camera rays, root clips and DDA
+cell sequences must match bit for bit; with the shared counter-based RNG and
+portable log, whole accumulated images and the work counters must too."""
+import os
+
+import numpy as np
+import pytest
+
+from mdwtracker_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from mdwtracker_amd import capi
+    return capi
+
+
+@pytest.fixture(scope="module")
+def po():
+    from oracle import pyoracle
+    return pyoracle
+
+
+def _pair(capi, po, scene, mode):
+    g = capi.MdwtRenderer(0)
+    o = po.OracleRenderer()
+    scenes.load_scene(g, scene, mode)
+    scenes.load_scene(o, scene, mode)
+    return g, o
+
+
+SMALL = {}
+
+
+def _scene(name, scale=1):
+    key = (name, scale)
+    if key not in SMALL:
+        SMALL[key] = scenes.make_scene(name, seed=1234, scale=scale)
+    return SMALL[key]
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.mark.parametrize("name,scale", [("ndda_cpu_2ch64", 1), ("sync_4ch256", 1),
+                                        ("sync_4ch256", 4), ("large_10ch_f32", 8)])
+def test_ranges_and_majorants_bit_exact(capi, po, name, scale):
+    sc = _scene(name, scale)
+    print(f"[ranges] {name} scale={scale} dims={sc.volumes[0].shape} mc={sc.mc}", flush=True)
+    for mode in (2, 3):
+        g, o = _pair(capi, po, sc, mode)
+        np.testing.assert_array_equal(_bits(g.read_ranges()), _bits(o.read_ranges()))
+        np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o.read_majorants()))
+        g.set_mode(0)
+        o2 = po.OracleRenderer()
+        scenes.load_scene(o2, sc, 0)
+        np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o2.read_majorants()))
+        g.close()
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.float16, np.float32])
+def test_ranges_other_dtypes_and_bounds(capi, po, dtype):
+    rng = np.random.default_rng(17)
+    shape = (19, 26, 33)
+    if dtype == np.uint16:
+        vols = [rng.integers(0, 65535, shape, dtype=np.uint16) for _ in range(2)]
+    else:
+        vols = [(rng.standard_normal(shape) * 3).astype(dtype) for _ in range(2)]
+    tfs = [scenes.piecewise_tf(6, rng, c / 2.0, n_texels=97) for c in range(2)]
+    for r in (capi.MdwtRenderer(0), po.OracleRenderer()):
+        for v in vols:
+            r.push_channel(v)
+        r.set_bounds([1.3, 0.8, 2.2])
+        r.init(2, (7, 5, 3), True)
+        for c in range(2):
+            r.set_tf(c, tfs[c], 1.7, (-0.1, 0.9))
+        if isinstance(r, po.OracleRenderer):
+            o = r
+        else:
+            g = r
+    np.testing.assert_array_equal(_bits(g.read_ranges()), _bits(o.read_ranges()))
+    np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o.read_majorants()))
+    np.testing.assert_array_equal(g.get_domains(), o.get_domains())
+    assert g.get_dt() == o.get_dt()
+    g.set_frame(96, 64, 2, True, (0.1, 0.2, 0.3))
+    o.set_frame(96, 64, 2, True, (0.1, 0.2, 0.3))
+    g.set_camera([2.0, -1.0, 3.0], [0.6, 0.4, 1.1], [0, 0, 1], 38.0)
+    o.set_camera([2.0, -1.0, 3.0], [0.6, 0.4, 1.1], [0, 0, 1], 38.0)
+    g.render(3)
+    o.render(3, threads=THREADS)
+    np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+
+
+def test_camera_rays_and_clip_bit_exact(capi, po):
+    sc = _scene("ndda_cpu_2ch64")
+    g, o = _pair(capi, po, sc, 2)
+    for (w, h) in [(256, 256), (200, 120)]:
+        g.set_frame(w, h)
+        o.set_frame(w, h)
+        g.set_camera([-1.2, 0.8, 2.5], [0.5, 0.45, 0.55], [0, 1, 0], 50.0)
+        o.set_camera([-1.2, 0.8, 2.5], [0.5, 0.45, 0.55], [0, 1, 0], 50.0)
+        np.testing.assert_array_equal(_bits(g.get_camera()), _bits(o.get_camera()))
+        go, gd = g.gen_rays(5, 2)
+        oo, od = o.gen_rays(5, 2)
+        np.testing.assert_array_equal(_bits(go), _bits(oo))
+        np.testing.assert_array_equal(_bits(gd), _bits(od))
+        gt, gh = g.clip_rays(go, gd)
+        ot, oh = o.clip_rays(oo, od)
+        np.testing.assert_array_equal(gh, oh)
+        np.testing.assert_array_equal(_bits(gt), _bits(ot))
+        assert 0 < gh.sum() < gh.size
+
+
+def test_dda_cells_bit_exact_64k_rays(capi, po):
+    sc = _scene("sync_4ch256", 4)  # 64^3 volume, mc 4^3 -> use a finer grid below
+    rng = np.random.default_rng(23)
+    g = capi.MdwtRenderer(0)
+    o = po.OracleRenderer()
+    for r in (g, o):
+        for v in sc.volumes:
+            r.push_channel(v)
+        r.init(2, (16, 16, 16), True)
+        for c, t in enumerate(sc.tfs):
+            r.set_tf(c, t)
+        r.set_frame(1024, 1024)
+    # all centre-lattice camera rays of a 1024^2 frame at stride 4 ...
+    oo, od = o.gen_rays(0, 0)
+    lat = (np.arange(0, 1024, 4)[:, None] * 1024 + np.arange(0, 1024, 4)[None, :]).ravel()
+    org, d = oo[lat], od[lat]
+    # ... plus random rays from outside and inside the volume
+    n = 16384
+    ro = np.concatenate([0.5 + rng.normal(size=(n // 2, 3)) * 1.5, rng.uniform(0, 1, (n // 2, 3))])
+    rd = rng.normal(size=(n, 3))
+    rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    org = np.concatenate([org, ro]).astype(np.float32)
+    d = np.concatenate([d, rd]).astype(np.float32)
+    gc, gt, gn = g.trace_cells(org, d, 64)
+    oc, ot, on = o.trace_cells(org, d, 64)
+    print(f"[dda] rays={len(org)} mean cells={on.mean():.2f} max={on.max()}", flush=True)
+    np.testing.assert_array_equal(gn, on)
+    np.testing.assert_array_equal(gc, oc)
+    np.testing.assert_array_equal(_bits(gt), _bits(ot))
+    assert on.max() <= 48 and on.mean() > 5
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_images_bit_exact_small(capi, po, mode):
+    sc = _scene("ndda_cpu_2ch64")
+    g, o = _pair(capi, po, sc, mode)
+    print(f"[image] ndda_cpu_2ch64 mode={mode}", flush=True)
+    cg = g.count_frame()
+    g.render(16)
+    o.render(16, threads=THREADS)
+    a_g, m_g = g.read_accum(with_m2=True)
+    a_o, m_o = o.read_accum(with_m2=True)
+    np.testing.assert_array_equal(_bits(a_g), _bits(a_o))
+    np.testing.assert_array_equal(_bits(m_g), _bits(m_o))
+    np.testing.assert_array_equal(g.read_fb(), o.read_fb())
+    # counting pass of frame 0 equals the oracle's counters of frame 0
+    o1 = po.OracleRenderer()
+    scenes.load_scene(o1, sc, mode)
+    o1.render(1, threads=THREADS)
+    assert cg == o1.read_counters()
+    assert g.info()["frame_id"] == 16 and o.info()["frame_id"] == 16
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_images_bit_exact_4ch_quarter(capi, po, mode):
+    sc = _scene("sync_4ch256", 4)
+    g, o = _pair(capi, po, sc, mode)
+    g.render(4)
+    o.render(4, threads=THREADS)
+    np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+
+
+def test_image_bit_exact_full_4ch256_one_frame(capi, po):
+    sc = _scene("sync_4ch256")
+    g, o = _pair(capi, po, sc, 2)
+    print("[image] sync_4ch256 full, mode 2, 1 frame", flush=True)
+    cg = g.count_frame()
+    g.render(1)
+    o.render(1, threads=THREADS)
+    np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+    assert cg == o.read_counters()
+
+
+def test_tile_order_and_spp_and_no_accumulation(capi, po):
+    sc = _scene("ndda_cpu_2ch64")
+    g, o = _pair(capi, po, sc, 2)
+    for r in (g, o):
+        r.set_frame(130, 70, 3, False, (0.2, 0.3, 0.4))
+    g.set_tile_order(0)
+    g.render(2)
+    o.render(2, threads=THREADS)
+    np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+    g.set_tile_order(1)
+    g.reset_accumulation()
+    g.set_frame(130, 70, 3, True, (0.2, 0.3, 0.4))
+    o.set_frame(130, 70, 3, True, (0.2, 0.3, 0.4))
+    g.render(3)
+    o.render(3, threads=THREADS)
+    np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+
+
+def test_render_requires_every_transfer_function(capi):
+    g = capi.MdwtRenderer(0)
+    v = np.zeros((8, 8, 8), np.uint8)
+    g.push_channel(v)
+    g.push_channel(v)
+    g.init(2)
+    g.set_tf(0, np.ones((8, 4), np.float32))
+    with pytest.raises(capi.MdwtError) as e:
+        g.render(1)
+    assert e.value.code == 2
+    with pytest.raises(capi.MdwtError):
+        g.set_tf(2, np.ones((8, 4), np.float32))  # channel id out of range

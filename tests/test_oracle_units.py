@@ -1,0 +1,164 @@
+"""Pins the oracle's building blocks: Philox against its published known-answer
+vectors, the portable log against float64 log, and the texture lookups
+against fields they must reproduce exactly (constant, linear)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_philox_known_answers():
+    kat = json.loads((GOLDEN / "philox4x32_10_kat.json").read_text())
+    for v in kat["vectors"]:
+        ctr = [int(x, 16) for x in v["ctr"]]
+        key = [int(x, 16) for x in v["key"]]
+        out = po.philox4x32_10(ctr, key)
+        assert [f"{x:08x}" for x in out] == v["out"]
+
+
+def test_philox_counter_sensitivity():
+    # every counter word and key word changes all four outputs
+    base = po.philox4x32_10([1, 2, 3, 4], [5, 6])
+    for i in range(4):
+        c = [1, 2, 3, 4]
+        c[i] += 1
+        assert np.all(po.philox4x32_10(c, [5, 6]) != base)
+    for i in range(2):
+        k = [5, 6]
+        k[i] += 1
+        assert np.all(po.philox4x32_10([1, 2, 3, 4], k) != base)
+
+
+def test_log_accuracy_within_2ulp():
+    xs = np.concatenate([np.arange(1, 1 << 24, 251, dtype=np.float64) * 2.0 ** -24,
+                         [1.0, 2.0 ** -24, 0.5, 0.70710677, 0.7071068, 1.4142135, 1.4142137,
+                          3.0, 1e-30, 1e30, 7.5e5]]).astype(np.float32)
+    got = np.array([po.logf(float(x)) for x in xs], np.float64)
+    ref = np.log(xs.astype(np.float64))
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    err = np.abs(got - ref) / ulp
+    assert err.max() <= 2.0, (err.max(), xs[err.argmax()])
+    assert po.logf(1.0) == 0.0
+    # monotone on a fine sweep of (0, 1]: the free-flight sampler relies on it
+    sweep = np.array([po.logf(float(x)) for x in np.linspace(1e-6, 1.0, 20001, dtype=np.float32)])
+    assert np.all(np.diff(sweep) >= 0)
+
+
+def _renderer(vols, tfs, mode=2, mc=(2, 2, 2), ranges=None):
+    r = po.OracleRenderer()
+    for v in vols:
+        r.push_channel(v)
+    r.init(mode, mc, True)
+    for c, t in enumerate(tfs):
+        lo, hi = ranges[c] if ranges else (0.0, 1.0)
+        r.set_tf(c, t, 1.0, (lo, hi))
+    return r
+
+
+def _flat_tf(n=256, rgb=(1, 1, 1), a=1.0):
+    t = np.zeros((n, 4), np.float32)
+    t[:, :3] = rgb
+    t[:, 3] = a
+    return t
+
+
+def test_sample_constant_and_linear_field():
+    n = 16
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    lin = (0.5 * x + 0.25 * y + 2.0 * z).astype(np.float32)
+    const = np.full((n, n, n), 7, np.uint8)
+    r = _renderer([lin, const], [_flat_tf(), _flat_tf()])
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        p = rng.uniform(0.5 / n, 1 - 0.5 / n, 3).astype(np.float32)
+        xt = p.astype(np.float64) * n - 0.5  # texel coordinates (x, y, z)
+        expect = 0.5 * xt[0] + 0.25 * xt[1] + 2.0 * xt[2]
+        assert abs(r.sample(0, p) - expect) <= 1e-4 * max(1.0, abs(expect))
+        # u8 values are converted as v * 255 (rawFile.cpp:104)
+        assert r.sample(1, p) == 7.0 * 255.0
+    # clamp-to-edge: the half texel at each face holds the face value
+    assert r.sample(1, [0.0, 0.0, 0.0]) == 7.0 * 255.0
+    assert r.sample(0, [0.0, 0.0, 0.0]) == 0.0
+    # outside the unit cube the reference returns NaN (deviceCode.cu:125-126)
+    assert np.isnan(r.sample(1, [1.0001, 0.5, 0.5]))
+    assert np.isnan(r.sample(1, [0.5, -1e-7, 0.5]))
+
+
+def test_value_domains_follow_rawfile():
+    a = np.zeros((4, 4, 4), np.uint8)
+    a[1, 2, 3] = 200
+    b = np.full((4, 4, 4), -3.0, np.float32)
+    b[0, 0, 0] = 0.5
+    r = po.OracleRenderer()
+    r.push_channel(a)
+    r.push_channel(b.astype(np.float32))
+    r.init(2, (1, 1, 1), True)
+    d = r.get_domains()
+    # bounds_.w starts at [0, 1] and widens with the converted values
+    assert d[0].tolist() == [0.0, 200.0 * 255.0]
+    assert d[1].tolist() == [-3.0, 1.0]
+
+
+def test_tf_lookup_linear_ramp_and_domain():
+    n = 64
+    t = np.zeros((n, 4), np.float32)
+    t[:, 0] = np.arange(n) / (n - 1)  # red ramp (alpha is cubed on upload; rgb is not)
+    t[:, 3] = 1.0
+    vol = np.zeros((4, 4, 4), np.float32)
+    vol[0, 0, 0] = 10.0  # domain [0, 10]
+    r = _renderer([vol], [t], ranges=[(0.0, 1.0)])
+    for f in np.linspace(0.6, 9.4, 37):
+        out = r.tf(0, f)
+        xt = (f / 10.0) * n - 0.5
+        assert abs(out[0] - xt / (n - 1)) < 2e-6
+        assert out[3] == 1.0
+    # texel centres reproduce the texel exactly
+    for i in range(n):
+        f = (i + 0.5) / n * 10.0
+        assert abs(r.tf(0, f)[0] - t[i, 0]) < 1e-6
+    # outside the value domain: (1, 0, 1, 0) (deviceCode.cu:31-35)
+    assert r.tf(0, -0.1).tolist() == [1.0, 0.0, 1.0, 0.0]
+    assert r.tf(0, 10.1).tolist() == [1.0, 0.0, 1.0, 0.0]
+
+
+def test_alpha_is_cubed_and_opacity_scaled():
+    vol = np.zeros((4, 4, 4), np.uint8)
+    t = _flat_tf(a=0.5)
+    r = po.OracleRenderer()
+    r.push_channel(vol)
+    r.init(2, (1, 1, 1), True)
+    r.set_tf(0, t, 3.0, (0.0, 1.0))
+    assert r.tf(0, 0.5)[3] == np.float32(0.125) * np.float32(3.0)
+
+
+def test_reset_dt_and_mc_defaults():
+    r = po.OracleRenderer()
+    r.push_channel(np.zeros((12, 20, 40), np.uint8))
+    r.set_bounds([2.0, 1.0, 0.5])
+    r.init(0, (0, 0, 0), True)
+    info = r.info()
+    assert info["mc"] == (10, 5, 3)  # dims / 4 (CalculateMCGridDims(4))
+    # dt = 0.5 * min voxel side (renderer.cpp:1444-1453)
+    assert r.get_dt() == np.float32(0.5) * min(np.float32(2.0) / 40, np.float32(1.0) / 20,
+                                               np.float32(0.5) / 12)
+    r.set_dt(1e-9)
+    assert r.get_dt() == np.float32(1e-4)  # Setdt clamps (renderer.cpp:1462)
+
+
+def test_camera_default_closed_form():
+    r = po.OracleRenderer()
+    r.push_channel(np.zeros((4, 4, 4), np.uint8))
+    r.init(2, (1, 1, 1), True)
+    r.set_frame(200, 100)
+    org, llc, hor, ver = r.get_camera().astype(np.float64)
+    hh = np.tan(np.radians(45.0) / 2)
+    hw = 2.0 * hh
+    np.testing.assert_allclose(org, [0.5, 0.5, 3.0], atol=1e-6)
+    np.testing.assert_allclose(hor, [2 * hw * 10, 0, 0], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(ver, [0, 2 * hh * 10, 0], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(llc, [0.5 - hw * 10, 0.5 - hh * 10, 3.0 - 10.0], rtol=1e-5, atol=1e-4)

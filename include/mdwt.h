@@ -122,7 +122,8 @@ int mdwt_get_camera(mdwt_renderer* h, float out[12]);
 int mdwt_get_domains(mdwt_renderer* h, float* out);
 /* DDA over the majorant grid for n world rays (grid transform of the
  * closest-hit programs + dda.h:82-203): up to max_cells (cell id, t0, t1) per
- * ray, counts[r] = number of cells the DDA emits. */
+ * ray, counts[r] = number of cells the DDA emits; slots past a ray's count
+ * hold cell -1 and t0 = t1 = 0. */
 int mdwt_trace_cells(mdwt_renderer* h, const float* org, const float* dir, int n, int max_cells,
                      int32_t* cells, float* t01, int32_t* counts);
 /* camera rays (origin, direction) of every pixel for (frame, sample)

@@ -464,6 +464,12 @@ __global__ void trace_cells_kernel(VolumeParams vol, int mx, int my, int mz, con
       if (!d.advance()) break;
     }
   }
+  for (int j = k; j < max_cells; ++j) {  // unused slots: cell -1, t = 0
+    const size_t o = size_t(i) * max_cells + j;
+    cells[o] = -1;
+    t01[2 * o] = 0.f;
+    t01[2 * o + 1] = 0.f;
+  }
   counts[i] = k;
 }
 

@@ -1109,6 +1109,11 @@ int oracle_trace_cells(oracle_renderer* o, const float* org, const float* dir, i
       ++k;
       return true;
     });
+    for (int j = k; j < max_cells; ++j) {  // unused slots: cell -1, t = 0
+      cells[size_t(r) * max_cells + j] = -1;
+      t01[2 * (size_t(r) * max_cells + j)] = 0.f;
+      t01[2 * (size_t(r) * max_cells + j) + 1] = 0.f;
+    }
     counts[r] = k;
   }
   return 0;

@@ -146,7 +146,7 @@ def test_dda_cells_bit_exact_64k_rays(capi, po):
     np.testing.assert_array_equal(gn, on)
     np.testing.assert_array_equal(gc, oc)
     np.testing.assert_array_equal(_bits(gt), _bits(ot))
-    assert on.max() <= 48 and on.mean() > 5
+    assert on.max() <= 48 and (on > 0).mean() > 0.3 and on[on > 0].mean() > 10
 
 
 @pytest.mark.parametrize("mode", [0, 2, 3])

@@ -141,6 +141,12 @@ int mdwt_last_kernel_ms(mdwt_renderer* h, float* ms);
 /* Tile order of the frame kernel: 0 = row-major, 1 = Hilbert (default);
  * tile = 8, 16 or 32 pixels per side. Images do not depend on either. */
 int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile);
+/* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
+ * loops, a workgroup per 16x16 block (default); 1 = flat step loop in
+ * persistent waves that refill lanes from 8 interleaved pixel queues; 2 = flat
+ * step loop, a workgroup per 16x16 block. Images do not depend on it. No
+ * reference counterpart (OptiX schedules the reference's launch). */
+int mdwt_set_schedule(mdwt_renderer* h, int schedule);
 /* Read-dominated device copy of `bytes` bytes, `iters` times; GB/s of
  * (read + write) bytes. Measures the achievable HBM bandwidth. */
 int mdwt_stream_copy(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps);

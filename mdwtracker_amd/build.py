@@ -35,7 +35,7 @@ COMMON = [
     "-Wall",
 ]
 
-SOURCES = ["kernels_build.hip", "kernels_render.hip", "mdwt_api.cpp"]
+SOURCES = ["kernels_build.hip", "kernels_render.hip", "kernels_flat.hip", "mdwt_api.cpp"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:
@@ -51,21 +51,26 @@ def _run(cmd: list[str], verbose: bool) -> None:
         print("\n".join(lines), flush=True)
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP/C++ source and link libmdwt.so. Always recompiles."""
-    OBJ.mkdir(exist_ok=True)
+def build(verbose: bool = False, extra_flags: list[str] | None = None,
+          out: Path | None = None) -> Path:
+    """Compile every HIP/C++ source and link libmdwt.so (or `out`). Always
+    recompiles. extra_flags (e.g. tuning -D defines) go to every compile."""
+    out = Path(out) if out is not None else LIB
+    obj_dir = OBJ if out == LIB else out.parent / "_obj"
+    obj_dir.mkdir(parents=True, exist_ok=True)
+    extra = list(extra_flags or [])
     jobs = []
     for src in SOURCES:
-        obj = OBJ / (Path(src).stem + ".o")
+        obj = obj_dir / (Path(src).stem + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        jobs.append([HIPCC, *COMMON, *lang, "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append([HIPCC, *COMMON, *extra, *lang, "-c", str(CSRC / src), "-o", str(obj)])
     with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
-    objs = [str(OBJ / (Path(s).stem + ".o")) for s in SOURCES]
-    tmp = LIB.with_suffix(".so.tmp")
+    objs = [str(obj_dir / (Path(s).stem + ".o")) for s in SOURCES]
+    tmp = out.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *objs], verbose)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

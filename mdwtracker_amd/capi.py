@@ -63,6 +63,7 @@ SIGNATURES = {
     "mdwt_count_frame": [_h, _u64p],
     "mdwt_last_kernel_ms": [_h, _fp],
     "mdwt_set_tile_order": [_h, C.c_int, C.c_int],
+    "mdwt_set_schedule": [_h, C.c_int],
     "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_hilbert_d2xy": [C.c_int, C.c_uint32, _up, _up],
 }
@@ -202,6 +203,10 @@ class MdwtRenderer:
 
     def set_tile_order(self, order: int, tile: int = 16) -> None:
         self._chk(lib.mdwt_set_tile_order(self._h, order, tile))
+
+    def set_schedule(self, schedule: int) -> None:
+        """0: one thread per pixel; 1: persistent waves (default)."""
+        self._chk(lib.mdwt_set_schedule(self._h, schedule))
 
     # -- frames -------------------------------------------------------------
     def render(self, n_frames: int = 1) -> float:

@@ -97,6 +97,14 @@ struct mdwt_renderer {
   uint32_t* d_order = nullptr;
   int n_blocks = 0;
   int* d_win = nullptr;
+
+  // frame-kernel schedule (0: thread per pixel, 1: persistent, 2: flat)
+  int schedule = 0;
+  unsigned* d_queues = nullptr;  // 2 x 8 pixel-queue counters
+  uint32_t launch_seq = 0;
+  int cu_count = 0;
+  int grid_key = -1;  // (mode, count, lds) the cached grid was sized for
+  int grid = 0;
 };
 
 namespace {
@@ -414,7 +422,32 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.counters = h->d_counters;
   p.fr.block_order = h->d_order;
   p.fr.n_blocks = h->n_blocks;
+  p.fr.queues = h->d_queues;
+  p.fr.launch_seq = h->launch_seq;
+  p.fr.schedule = h->schedule;
+  p.fr.grid = h->grid;
   return p;
+}
+
+// One frame launch: sizes the persistent grid to the resident capacity
+// (cached per mode / counting / LDS size), launches, advances the queue set.
+int launch_frame(mdwt_renderer* h, bool count) {
+  mdwt::RenderParams p = render_params(h);
+  if (h->schedule == 1) {
+    const size_t lds = mdwt::render_lds_bytes(p);
+    const int key = int(lds) * 8 + h->mode * 2 + (count ? 1 : 0);
+    if (key != h->grid_key) {
+      int per_cu = 0;
+      HIPCHK(h, mdwt::persistent_blocks_per_cu(h->mode, h->dtype, count, lds, &per_cu));
+      if (per_cu < 1) per_cu = 1;
+      h->grid = per_cu * h->cu_count;
+      h->grid_key = key;
+    }
+    p.fr.grid = h->grid;
+  }
+  HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, count, h->stream));
+  h->launch_seq++;
+  return MDWT_OK;
 }
 
 int ready_to_render(mdwt_renderer* h) {
@@ -463,7 +496,7 @@ int mdwt_destroy(mdwt_renderer* h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_frame(h);
-    void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win};
+    void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win, h->d_queues};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -603,6 +636,10 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   HIPCHK(h, hipMalloc(&h->d_maj, n_cells * h->C * sizeof(float)));
   HIPCHK(h, hipMemsetAsync(h->d_maj, 0, n_cells * h->C * sizeof(float), h->stream));
   HIPCHK(h, hipMalloc(&h->d_counters, 4 * sizeof(unsigned long long)));
+  HIPCHK(h, hipMalloc(&h->d_queues, 16 * sizeof(unsigned)));
+  HIPCHK(h, hipMemsetAsync(h->d_queues, 0, 16 * sizeof(unsigned), h->stream));
+  HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device));
+  h->launch_seq = 0;
   mdwt::RangeParams rp{};
   rp.vol = h->d_vol;
   rp.chan_stride = h->chan_stride;
@@ -787,8 +824,8 @@ int mdwt_render(mdwt_renderer* h, int n_frames, float* ms_out) {
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int f = 0; f < n_frames; ++f) {
-    const mdwt::RenderParams p = render_params(h);
-    HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, false, h->stream));
+    rc = launch_frame(h, false);
+    if (rc != MDWT_OK) return rc;
     h->accum_id++;
     h->frame_id++;
   }
@@ -980,8 +1017,8 @@ int mdwt_count_frame(mdwt_renderer* h, uint64_t out[4]) {
   if (rc != MDWT_OK) return rc;
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
-  const mdwt::RenderParams p = render_params(h);
-  HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, true, h->stream));
+  rc = launch_frame(h, true);
+  if (rc != MDWT_OK) return rc;
   unsigned long long c[4];
   HIPCHK(h, hipMemcpyAsync(c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1018,6 +1055,14 @@ int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile) {
     (void)hipFree(m2);
     h->accum_id = keep;
   }
+  return MDWT_OK;
+}
+
+int mdwt_set_schedule(mdwt_renderer* h, int schedule) {
+  if (!h) return MDWT_ERR_ARG;
+  if (schedule < 0 || schedule > 2)
+    return fail(h, MDWT_ERR_ARG, "schedule must be 0 (thread per pixel), 1 (persistent) or 2 (flat)");
+  h->schedule = schedule;
   return MDWT_OK;
 }
 

@@ -33,7 +33,9 @@ __device__ __forceinline__ float qnan() { return __int_as_float(0x7fc00000); }
 
 // ------------------------------------------------------------- Philox ------
 // Philox4x32-10 (Salmon et al., SC'11). Stream key = (seed, pixel); counter =
-// (frame, sample, block, stream). Draw n of a stream is word n%4 of block n/4.
+// (frame, sample, block, stream). Each round's two 32x32->64 products are
+// written as 64-bit products so that each becomes one v_mad_u64_u32 (32-bit
+// integer multiplies are quarter rate on CDNA; the hi/lo pair would be two).
 __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                          uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
@@ -42,14 +44,14 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-    const uint32_t n0 = hi1 ^ c1 ^ k0;
-    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
     c0 = n0;
-    c1 = lo1;
+    c1 = uint32_t(p1);
     c2 = n2;
-    c3 = lo0;
+    c3 = uint32_t(p0);
   }
   out[0] = c0;
   out[1] = c1;
@@ -57,39 +59,45 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
   out[3] = c3;
 }
 
+// uniform in [0,1) with 24 bits: (w >> 8) * 2^-24, exact
+__device__ __forceinline__ float to_unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }
+
+// Word layout of a (pixel, frame, sample) stream: the jitter is word 0 (block
+// 0); free-flight step k owns words 4 + 2k (distance) and 5 + 2k (threshold),
+// i.e. block 1 + k/2, so one Philox block serves two consecutive steps.
 struct Rng {
-  uint32_t k0, k1, frame, sample, block, stream;
+  uint32_t k0, k1, frame, sample, stream;
+  uint32_t k, cur;
   uint32_t b0, b1, b2, b3;
-  int left;
   __device__ __forceinline__ void init(uint32_t seed, uint32_t pixel, uint32_t frame_, uint32_t sample_,
                                        uint32_t stream_) {
     k0 = seed;
     k1 = pixel;
     frame = frame_;
     sample = sample_;
-    block = 0;
     stream = stream_;
-    left = 0;
+    k = 0;
+    cur = 0;
   }
-  // uniform in [0,1) with 24 bits: (w >> 8) * 2^-24, exact
-  __device__ __forceinline__ float next() {
-    if (left == 0) {
+  __device__ __forceinline__ float jitter() const {
+    uint32_t o[4];
+    philox10(frame, sample, 0u, stream, k0, k1, o);
+    return to_unit(o[0]);
+  }
+  __device__ __forceinline__ float step() {
+    cur = k;
+    k += 1;
+    if ((cur & 1u) == 0u) {
       uint32_t o[4];
-      philox10(frame, sample, block, stream, k0, k1, o);
-      block += 1;
+      philox10(frame, sample, 1u + (cur >> 1), stream, k0, k1, o);
       b0 = o[0];
       b1 = o[1];
       b2 = o[2];
       b3 = o[3];
-      left = 4;
     }
-    const uint32_t w = b0;
-    b0 = b1;
-    b1 = b2;
-    b2 = b3;
-    left -= 1;
-    return float(w >> 8) * 0x1p-24f;
+    return to_unit((cur & 1u) ? b2 : b0);
   }
+  __device__ __forceinline__ float thr() const { return to_unit((cur & 1u) ? b3 : b1); }
 };
 
 // ------------------------------------------------------------ log ----------

@@ -7,9 +7,18 @@
 
 namespace mdwt {
 
+// host stub of a frame kernel
+using RenderFn = void (*)(RenderParams);
+// flat-loop frame kernels (kernels_flat.hip): refill = persistent schedule
+RenderFn select_flat_kernel(int mode, int dtype, bool count, bool refill);
+
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
 hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
+// dynamic LDS of a frame launch (transfer-function texels)
+size_t render_lds_bytes(const RenderParams& p);
+// resident workgroups per CU of the persistent frame kernel
+hipError_t persistent_blocks_per_cu(int mode, int dtype, bool count, size_t lds, int* out);
 hipError_t launch_gen_rays(const CameraParams& cam, int W, int H, uint32_t seed, uint32_t frame,
                            uint32_t sample, float* org, float* dir, hipStream_t s);
 hipError_t launch_clip(const VolumeParams& vol, const float* org, const float* dir, int n,

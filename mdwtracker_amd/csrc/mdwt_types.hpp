@@ -47,8 +47,14 @@ struct FrameParams {
   void* accum;  // float4 per pixel
   void* m2;     // float4 per pixel
   unsigned long long* counters;  // samples, visits, rejections, rays
-  const uint32_t* block_order;   // (bx | by << 16) per workgroup, in launch order
+  const uint32_t* block_order;   // (bx | by << 16) per 16x16 pixel block, Hilbert order
   int n_blocks;
+  // persistent schedule: two sets of 8 pixel-queue counters (one per XCD);
+  // launch k uses set k & 1 and clears set (k + 1) & 1 for the next launch
+  unsigned* queues;
+  uint32_t launch_seq;
+  int schedule;  // 0: one thread per pixel, 1: persistent waves
+  int grid;      // workgroups of the persistent launch
 };
 
 struct RenderParams {

@@ -1,0 +1,219 @@
+// Shared device pieces of the frame kernels: per-channel transfer functions
+// in LDS, volume sampling, work counters and the block schedule.
+#pragma once
+#include "mdwt_device.hpp"
+
+namespace mdwt {
+
+struct ChanTf {
+  float dlo, dhi, xlo, xhi, os;
+  int n;
+  int off;  // texel offset in LDS
+  int pad;
+};
+
+struct Counts {
+  uint32_t samples, visits, rejections, rays;
+};
+
+struct HitRec {
+  bool missed;
+  float r, g, b;
+  float tHit;
+  int channel;
+};
+
+template <int DT>
+struct Ctx {
+  const uint8_t* vol;
+  uint64_t chan_stride;
+  int nx, ny, nz;
+  float scale, offset;
+  const float4* tf;    // LDS
+  const ChanTf* chtf;  // LDS
+  const float* maj;
+  int C;
+  int mx, my, mz;
+  float dt;
+
+  __device__ __forceinline__ void init(const RenderParams& P, const float4* tf_lds,
+                                       const ChanTf* ch_lds) {
+    vol = P.vol.vol;
+    chan_stride = P.vol.chan_stride;
+    nx = P.vol.nx;
+    ny = P.vol.ny;
+    nz = P.vol.nz;
+    scale = P.vol.scale;
+    offset = P.vol.offset;
+    tf = tf_lds;
+    chtf = ch_lds;
+    maj = P.maj;
+    C = P.vol.channels;
+    mx = P.mc[0];
+    my = P.mc[1];
+    mz = P.mc[2];
+    dt = P.dt;
+  }
+
+  // sampleVolumeTexture (deviceCode.cu:122-132): NaN outside [0,1]^3, else
+  // clamp-to-edge trilinear at texel coordinate p*N - 0.5.
+  __device__ __forceinline__ float sample(V3 p, int c) const {
+    if (p.x < 0.0f || p.x > 1.0f || p.y < 0.0f || p.y > 1.0f || p.z < 0.0f || p.z > 1.0f)
+      return qnan();
+    const uint8_t* base = vol + size_t(c) * chan_stride;
+    const float x = p.x * float(nx) - 0.5f;
+    const float y = p.y * float(ny) - 0.5f;
+    const float z = p.z * float(nz) - 0.5f;
+    const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+    const float fx = x - flx, fy = y - fly, fz = z - flz;
+    const uint32_t x0 = uint32_t(clamp_index(flx, nx)), x1 = uint32_t(clamp_index_plus1(flx, nx));
+    const uint32_t y0 = uint32_t(clamp_index(fly, ny)), y1 = uint32_t(clamp_index_plus1(fly, ny));
+    const uint32_t z0 = uint32_t(clamp_index(flz, nz)), z1 = uint32_t(clamp_index_plus1(flz, nz));
+    const uint32_t sx = uint32_t(nx), sxy = uint32_t(nx) * uint32_t(ny);
+    const uint32_t r00 = y0 * sx + z0 * sxy, r10 = y1 * sx + z0 * sxy;
+    const uint32_t r01 = y0 * sx + z1 * sxy, r11 = y1 * sx + z1 * sxy;
+    const float v000 = fetch_raw<DT>(base, r00 + x0), v100 = fetch_raw<DT>(base, r00 + x1);
+    const float v010 = fetch_raw<DT>(base, r10 + x0), v110 = fetch_raw<DT>(base, r10 + x1);
+    const float v001 = fetch_raw<DT>(base, r01 + x0), v101 = fetch_raw<DT>(base, r01 + x1);
+    const float v011 = fetch_raw<DT>(base, r11 + x0), v111 = fetch_raw<DT>(base, r11 + x1);
+    const float c00 = v000 + fx * (v100 - v000);
+    const float c10 = v010 + fx * (v110 - v010);
+    const float c01 = v001 + fx * (v101 - v001);
+    const float c11 = v011 + fx * (v111 - v011);
+    const float c0 = c00 + fy * (c10 - c00);
+    const float c1 = c01 + fy * (c11 - c01);
+    const float r = c0 + fz * (c1 - c0);
+    return r * scale + offset;
+  }
+
+  // transferFunction (deviceCode.cu:27-42), opacity only; colour is looked
+  // up only for the accepted collision (same lerp, same texels).
+  __device__ __forceinline__ float tf_alpha(float f, int c, float& r2out) const {
+    const ChanTf t = chtf[c];
+    if (f < t.dlo || f > t.dhi) {
+      r2out = qnan();
+      return 0.0f;
+    }
+    const float r1 = (f - t.dlo) / (t.dhi - t.dlo);
+    const float r2 = (r1 - t.xlo) / (t.xhi - t.xlo);
+    r2out = r2;
+    return tf_lerp_alpha(tf + t.off, t.n, r2) * t.os;
+  }
+  __device__ __forceinline__ void tf_rgb(float r2, int c, HitRec& h) const {
+    const ChanTf t = chtf[c];
+    if (r2 != r2) {  // outside the value domain: (1, 0, 1, 0)
+      h.r = 1.f;
+      h.g = 0.f;
+      h.b = 1.f;
+      return;
+    }
+    const float4 v = tf_lerp(tf + t.off, t.n, r2);
+    h.r = v.x;
+    h.g = v.y;
+    h.b = v.z;
+  }
+};
+
+// Transfer-function texels and per-channel TF parameters -> LDS (the caller
+// synchronises the workgroup afterwards).
+__device__ __forceinline__ void load_tf_lds(const RenderParams& P, float4* s_tf, ChanTf* s_ch) {
+  const int C = P.vol.channels;
+  const float4* g = reinterpret_cast<const float4*>(P.tf.tab);
+  const int total = C * P.tf.stride;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) s_tf[i] = g[i];
+  if (int(threadIdx.x) < C) {
+    const int c = threadIdx.x;
+    ChanTf t;
+    t.dlo = P.tf.dom_lo[c];
+    t.dhi = P.tf.dom_hi[c];
+    t.xlo = P.tf.xf_lo[c];
+    t.xhi = P.tf.xf_hi[c];
+    t.os = P.tf.opacity[c];
+    t.n = P.tf.n[c];
+    t.off = c * P.tf.stride;
+    t.pad = 0;
+    s_ch[c] = t;
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ void flush_counts(const RenderParams& P, const Counts& cnt) {
+  const uint32_t s = wave_sum(cnt.samples), v = wave_sum(cnt.visits);
+  const uint32_t r = wave_sum(cnt.rejections), n = wave_sum(cnt.rays);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&P.fr.counters[0], (unsigned long long)s);
+    atomicAdd(&P.fr.counters[1], (unsigned long long)v);
+    atomicAdd(&P.fr.counters[2], (unsigned long long)r);
+    atomicAdd(&P.fr.counters[3], (unsigned long long)n);
+  }
+}
+
+// Workgroup -> 16x16 pixel block (index into the Hilbert block order).
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8), so the identity map deals consecutive Hilbert blocks to
+// different XCDs: every XCD gets an even share of the heavy image regions
+// (giving each XCD a contiguous run instead left half of them idle for half
+// the frame: the volume's footprint is not spread evenly along the curve),
+// and each XCD still works along the curve, every 8th block.
+__device__ __forceinline__ uint32_t block_slot(uint32_t b, uint32_t /*n*/) { return b; }
+
+// running mean of the colour and of its square (deviceCode.cu:463-469; the
+// second moment is an addition)
+__device__ __forceinline__ void accumulate_pixel(const RenderParams& P, uint32_t pixel, float fr,
+                                                 float fg, float fb, float fa) {
+  float4* acc = reinterpret_cast<float4*>(P.fr.accum) + pixel;
+  float4* sq = reinterpret_cast<float4*>(P.fr.m2) + pixel;
+  const float4 q = make_float4(fr * fr, fg * fg, fb * fb, fa * fa);
+  if (P.fr.accumulate) {
+    const float k = float(P.fr.accum_id);
+    const float k1 = float(P.fr.accum_id + 1);
+    const float4 a = *acc;
+    const float4 m = *sq;
+    *acc = make_float4((fr + k * a.x) / k1, (fg + k * a.y) / k1, (fb + k * a.z) / k1,
+                       (fa + k * a.w) / k1);
+    *sq = make_float4((q.x + k * m.x) / k1, (q.y + k * m.y) / k1, (q.z + k * m.z) / k1,
+                      (q.w + k * m.w) / k1);
+  } else {
+    *acc = make_float4(fr, fg, fb, fa);
+    *sq = q;
+  }
+}
+
+// shading with diffuse 1, specular 0, shadow 1 (deviceCode.cu:326-433) and
+// over(colour, (bg, 1)) (deviceCode.cu:44-59) added to the pixel sum
+__device__ __forceinline__ void composite_sample(const RenderParams& P, bool hit, float hr,
+                                                 float hg, float hb, float& fr, float& fg,
+                                                 float& fb) {
+  float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+  if (hit) {
+    const float li = 1.0f * 1.0f * P.fr.light_intensity;
+    const float sp = 0.0f * 1.0f * P.fr.light_intensity;
+    const float mr = hr * 0.75f + P.fr.spec[0] * 0.25f;
+    const float mg = hg * 0.75f + P.fr.spec[1] * 0.25f;
+    const float mb = hb * 0.75f + P.fr.spec[2] * 0.25f;
+    cr = li * hr + sp * mr + P.fr.ambient * mr;
+    cg = li * hg + sp * mg + P.fr.ambient * mg;
+    cb = li * hb + sp * mb + P.fr.ambient * mb;
+    ca = 1.0f;
+  }
+  fr = fr + (cr + P.fr.bg[0] * 1.f * (1.f - ca));
+  fg = fg + (cg + P.fr.bg[1] * 1.f * (1.f - ca));
+  fb = fb + (cb + P.fr.bg[2] * 1.f * (1.f - ca));
+}
+
+// Divides the pixel sums by spp; alpha is 1 + spp (each sample adds
+// ca + (1 - ca) * 1 = 1 exactly to the initial 1) over spp.
+__device__ __forceinline__ void finish_pixel(const RenderParams& P, uint32_t pixel, float fr,
+                                             float fg, float fb) {
+  float fa = 1.f;
+  for (int i = 0; i < P.fr.spp; ++i) fa = fa + 1.0f;
+  const float spp = float(P.fr.spp);
+  accumulate_pixel(P, pixel, fr / spp, fg / spp, fb / spp, fa / spp);
+}
+
+}  // namespace mdwt

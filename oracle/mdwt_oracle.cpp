@@ -79,22 +79,45 @@ inline void philox10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[
   for (int i = 0; i < 4; ++i) out[i] = c[i];
 }
 
-// One stream of uniforms in [0,1): draw n is word n%4 of block n/4.
+// One stream of uniforms in [0,1) per (pixel, frame, sample): word n is word
+// n%4 of Philox block n/4 (counter (frame, sample, n/4, stream), key (seed,
+// pixel)), converted as (w >> 8) * 2^-24. Fixed word layout: the pixel jitter
+// is word 0; free-flight step k of the ray (counted over the whole ray,
+// whichever cell or channel it is in) owns words 4 + 2k (distance) and
+// 5 + 2k (null-collision threshold, read only if the step lands inside the
+// cell). The reference draws the same quantities from one sequential LCG
+// stream (deviceCode.cu:296-298, 563, 583, 797, 830, 913, 931); fixed
+// positions let the GPU compute one Philox block per two steps.
 struct Rng {
   uint32_t key[2];
   uint32_t ctr[4];
   uint32_t buf[4];
-  int pos;
+  uint32_t cached;  // block held in buf (~0u: none)
+  uint32_t k;       // next free-flight step
+  uint32_t cur;     // current step
   Rng(uint32_t seed, uint32_t pixel, uint32_t frame, uint32_t sample, uint32_t stream) {
     key[0] = seed; key[1] = pixel;
     ctr[0] = frame; ctr[1] = sample; ctr[2] = 0; ctr[3] = stream;
-    pos = 4;
+    cached = ~0u;
+    k = 0;
+    cur = 0;
   }
-  float operator()() {
-    if (pos == 4) { philox10(ctr, key, buf); ctr[2] += 1; pos = 0; }
-    const uint32_t w = buf[pos++];
-    return float(w >> 8) * 0x1p-24f;  // exact: 24-bit integer times 2^-24
+  float word(uint32_t n) {
+    if (n / 4 != cached) {
+      ctr[2] = n / 4;
+      philox10(ctr, key, buf);
+      cached = n / 4;
+    }
+    return float(buf[n % 4] >> 8) * 0x1p-24f;  // exact: 24-bit integer times 2^-24
   }
+  float jitter() { return word(0); }
+  // distance uniform of the next free-flight step
+  float step() {
+    cur = k++;
+    return word(4 + 2 * cur);
+  }
+  // threshold uniform of the current step
+  float thr() { return word(5 + 2 * cur); }
 };
 
 // --------------------------------------------------------------- logf -------
@@ -613,7 +636,7 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf(1.0f - rng()) / majorant) * unit * g.worldToGridT;
+      t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       if (tMax < t) return false;
       const float tWorld = t * g.gridToWorldT;
@@ -622,7 +645,7 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
         prd.rejections++;
         return false;
       }
-      const float nullColThreshold = rng() * majorant;
+      const float nullColThreshold = rng.thr() * majorant;
       const float value = sample_volume(o, xTexture, curMesh);
       prd.samples++;
       if (std::isnan(value)) continue;
@@ -655,7 +678,7 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
       float t = t0;
       if (majorant == 0.0f) continue;
       while (true) {
-        t = t - (plogf(1.0f - rng()) / majorant) * unit * g.worldToGridT;
+        t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
         if (t >= minT) break;
         const float tWorld = t * g.gridToWorldT;
         const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -664,7 +687,7 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
           if (c == C - 1) return false;
           break;
         }
-        const float nullColThreshold = rng() * majorant;
+        const float nullColThreshold = rng.thr() * majorant;
         const float value = sample_volume(o, xTexture, c);
         prd.samples++;
         if (!std::isnan(value)) {
@@ -694,7 +717,7 @@ void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, 
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf(1.0f - rng()) / majorant) * unit * g.worldToGridT;
+      t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -702,7 +725,7 @@ void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, 
         prd.rejections++;
         return false;
       }
-      float nullColThreshold = rng() * majorant;
+      float nullColThreshold = rng.thr() * majorant;
       for (int c = 0; c < C; ++c) {
         const float value = sample_volume(o, xTexture, c);
         prd.samples++;
@@ -768,7 +791,7 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
   const uint32_t pixel = uint32_t(fbOfs);
   for (int sampleID = 0; sampleID < o->spp; ++sampleID) {
     Rng rng(o->seed, pixel, o->frame_id, uint32_t(sampleID), 0u);
-    const float jit = rng();
+    const float jit = rng.jitter();
     const float su = (float(px) + jit) / float(o->W);
     const float sv = (float(py) + jit) / float(o->H);
     vec3f org, dir;
@@ -1125,7 +1148,7 @@ int oracle_gen_rays(oracle_renderer* o, uint32_t frame, uint32_t sample, float* 
     for (int px = 0; px < o->W; ++px) {
       const size_t i = size_t(px) + size_t(o->W) * py;
       Rng rng(o->seed, uint32_t(i), frame, sample, 0u);
-      const float jit = rng();
+      const float jit = rng.jitter();
       const float su = (float(px) + jit) / float(o->W);
       const float sv = (float(py) + jit) / float(o->H);
       vec3f ro, rd;

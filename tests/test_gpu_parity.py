@@ -208,6 +208,28 @@ def test_tile_order_and_spp_and_no_accumulation(capi, po):
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
 
 
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_schedules_agree_bit_exact(capi, mode):
+    """The persistent schedule and the one-thread-per-pixel schedule compute
+    every pixel with the same operations: images and counters are identical."""
+    sc = _scene("sync_4ch256", 2)
+    out = []
+    for schedule in (0, 1):
+        g = capi.MdwtRenderer(0)
+        g.set_schedule(schedule)
+        scenes.load_scene(g, sc, mode)
+        g.set_frame(sc.width + 13, sc.height - 7, 2, True, (0.05, 0.1, 0.15))
+        cnt = g.count_frame()
+        g.render(3)
+        out.append((g.read_accum(with_m2=True), cnt))
+        g.close()
+    (a0, m0), c0 = out[0]
+    (a1, m1), c1 = out[1]
+    np.testing.assert_array_equal(_bits(a0), _bits(a1))
+    np.testing.assert_array_equal(_bits(m0), _bits(m1))
+    assert c0 == c1
+
+
 def test_render_requires_every_transfer_function(capi):
     g = capi.MdwtRenderer(0)
     v = np.zeros((8, 8, 8), np.uint8)

@@ -138,8 +138,11 @@ int mdwt_clip_rays(mdwt_renderer* h, const float* org, const float* dir, int n, 
 int mdwt_count_frame(mdwt_renderer* h, uint64_t out[4]);
 /* Average duration in ms of the last mdwt_render's frame kernel launches. */
 int mdwt_last_kernel_ms(mdwt_renderer* h, float* ms);
-/* Tile order of the frame kernel: 0 = row-major, 1 = Hilbert (default);
- * tile = 8, 16 or 32 pixels per side. Images do not depend on either. */
+/* Block order of the frame kernel (16x16-pixel blocks; tile must be 16):
+ * 0 = row-major, 1 = Hilbert curve (default), 2 = cost-sorted: starts as
+ * Hilbert, records each wave's wall-clock time and, at the next mdwt_render,
+ * issues the blocks heaviest first (one-thread-per-pixel schedule only).
+ * Images do not depend on the order. */
 int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile);
 /* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
  * loops, a workgroup per 16x16 block (default); 1 = flat step loop in

@@ -70,6 +70,11 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None,
     tmp = out.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *objs], verbose)
     os.replace(tmp, out)
+    if out == LIB:
+        # offline driver: plain C++ on the C ABI (no HIP code), rpath to this folder
+        _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+              str(CSRC / "mdwt_offline.cpp"), f"-L{PKG}", "-lmdwt", "-Wl,-rpath,$ORIGIN",
+              "-o", str(PKG / "mdwt_offline")], verbose)
     return out
 
 

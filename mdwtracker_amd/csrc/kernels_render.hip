@@ -180,6 +180,7 @@ template <int MODE, int DT, bool COUNT>
 __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
   extern __shared__ float4 s_tf[];
   __shared__ ChanTf s_ch[kMaxChannels];
+  const uint64_t t_begin = P.fr.wave_cost ? wall_clock64() : 0ull;
   load_tf_lds(P, s_tf, s_ch);
   __syncthreads();
 
@@ -222,6 +223,9 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
     if (!COUNT) finish_pixel(P, pixel, fr, fg, fb);
   }
   if (COUNT) flush_counts(P, cnt);
+  // per-wave duration for the cost-sorted block order of later frames
+  if (!COUNT && P.fr.wave_cost && lane == 0)
+    P.fr.wave_cost[4u * slot + uint32_t(wave)] = uint32_t(wall_clock64() - t_begin);
 }
 
 // ---------------------------------------------------------- test hooks -----

@@ -80,7 +80,7 @@ struct mdwt_renderer {
   uint32_t seed = 0x5eed1234u;
   uint32_t frame_id = 0, accum_id = 0;
   float dt = 0.05f;
-  int order_mode = 1;
+  int order_mode = 2;  // cost-sorted block order
   float last_kernel_ms = 0.f;
 
   // device memory
@@ -95,8 +95,12 @@ struct mdwt_renderer {
   size_t fb_pixels = 0;
   unsigned long long* d_counters = nullptr;
   uint32_t* d_order = nullptr;
+  std::vector<uint32_t> order;  // host copy of d_order
   int n_blocks = 0;
   int* d_win = nullptr;
+  uint32_t* d_wave_cost = nullptr;  // per-wave ticks of the last recorded frame
+  bool cost_ready = false;
+  bool need_resort = true;  // record costs and re-sort after the next frames
 
   // frame-kernel schedule (0: thread per pixel, 1: persistent, 2: flat)
   int schedule = 0;
@@ -112,6 +116,14 @@ namespace {
 int fail(mdwt_renderer* h, int code, const std::string& msg) {
   h->err = msg;
   return code;
+}
+
+// ResetAccumulation (renderer.cpp:1488-1495); every change that restarts the
+// accumulation may also change where the frame's work lies, so the
+// cost-sorted block order is re-measured after it.
+void reset_acc(mdwt_renderer* h) {
+  h->accum_id = 0;
+  h->need_resort = true;
 }
 
 #define HIPCHK(h, expr)                                                              \
@@ -195,7 +207,7 @@ void update_camera(mdwt_renderer* h) {
     dst[i][1] = all[i].y;
     dst[i][2] = all[i].z;
   }
-  h->accum_id = 0;
+  reset_acc(h);
 }
 
 // renderer.cu:387-391 project
@@ -283,9 +295,39 @@ void free_frame(mdwt_renderer* h) {
   if (h->d_accum) (void)hipFree(h->d_accum);
   if (h->d_m2) (void)hipFree(h->d_m2);
   if (h->d_order) (void)hipFree(h->d_order);
+  if (h->d_wave_cost) (void)hipFree(h->d_wave_cost);
   h->d_accum = h->d_m2 = nullptr;
   h->d_order = nullptr;
+  h->d_wave_cost = nullptr;
+  h->cost_ready = false;
   h->fb_pixels = 0;
+}
+
+// Cost-sorted block order (longest processing time first): blocks sorted by
+// the summed wall-clock ticks of their four waves in the last recorded frame,
+// heaviest first. Workgroup i runs on XCD i % 8, so the sorted list is dealt
+// round-robin over the XCDs and the slowest blocks start first instead of
+// trailing at the end of the frame.
+int resort_blocks(mdwt_renderer* h) {
+  const size_t nb = size_t(h->n_blocks);
+  std::vector<uint32_t> ticks(4 * nb);
+  HIPCHK(h, hipMemcpyAsync(ticks.data(), h->d_wave_cost, ticks.size() * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<std::pair<uint64_t, uint32_t>> cost(nb);
+  for (size_t s = 0; s < nb; ++s) {
+    const uint64_t c = uint64_t(ticks[4 * s]) + ticks[4 * s + 1] + ticks[4 * s + 2] + ticks[4 * s + 3];
+    cost[s] = {c, h->order[s]};
+  }
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
+                     return a.first > b.first;
+                   });
+  for (size_t s = 0; s < nb; ++s) h->order[s] = cost[s].second;
+  HIPCHK(h, hipMemcpyAsync(h->d_order, h->order.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           h->stream));
+  h->cost_ready = false;
+  return MDWT_OK;
 }
 
 // frame buffers and the workgroup -> pixel-block order of the frame kernel
@@ -301,7 +343,7 @@ int alloc_frame(mdwt_renderer* h) {
   const int by = (h->H + mdwt::kBlockPixels - 1) / mdwt::kBlockPixels;
   std::vector<uint32_t> order;
   order.reserve(size_t(bx) * by);
-  if (h->order_mode == 1) {
+  if (h->order_mode >= 1) {  // Hilbert (also the start of the cost-sorted order)
     int bits = 0;
     while ((1 << bits) < std::max(bx, by)) ++bits;
     const uint64_t total = uint64_t(1) << (2 * bits);
@@ -318,7 +360,10 @@ int alloc_frame(mdwt_renderer* h) {
   HIPCHK(h, hipMalloc(&h->d_order, order.size() * sizeof(uint32_t)));
   HIPCHK(h, hipMemcpyAsync(h->d_order, order.data(), order.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMalloc(&h->d_wave_cost, 4 * order.size() * sizeof(uint32_t)));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->order = std::move(order);
+  h->cost_ready = false;
   return MDWT_OK;
 }
 
@@ -426,6 +471,8 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.launch_seq = h->launch_seq;
   p.fr.schedule = h->schedule;
   p.fr.grid = h->grid;
+  p.fr.wave_cost =
+      (h->order_mode == 2 && h->schedule == 0 && h->need_resort) ? h->d_wave_cost : nullptr;
   return p;
 }
 
@@ -447,6 +494,7 @@ int launch_frame(mdwt_renderer* h, bool count) {
   }
   HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, count, h->stream));
   h->launch_seq++;
+  if (p.fr.wave_cost && !count) h->cost_ready = true;
   return MDWT_OK;
 }
 
@@ -665,7 +713,7 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   int rc = alloc_frame(h);
   if (rc != MDWT_OK) return rc;
   h->frame_id = 0;
-  h->accum_id = 0;
+  reset_acc(h);
   return mdwt_reset_dt(h);
 }
 
@@ -676,7 +724,7 @@ int mdwt_set_mode(mdwt_renderer* h, int mode) {
     return fail(h, MDWT_ERR_ARG, "mode must be 0 (N-DDA), 2 (syncDDA) or 3 (cumulative)");
   HIPCHK(h, hipSetDevice(h->device));
   h->mode = mode;
-  h->accum_id = 0;
+  reset_acc(h);
   return rebuild_majorants(h);
 }
 
@@ -698,7 +746,7 @@ int mdwt_set_xf_colormap(mdwt_renderer* h, int ch, const float* rgba, int n_texe
   }
   h->cm[ch] = std::move(t);
   h->cm_set[ch] = true;
-  h->accum_id = 0;
+  reset_acc(h);
   HIPCHK(h, hipSetDevice(h->device));
   return rebuild_majorants(h);
 }
@@ -710,7 +758,7 @@ int mdwt_set_xf_opacity_scale(mdwt_renderer* h, int ch, float opacity_scale) {
   if (!(opacity_scale >= 0.f && opacity_scale <= 1000.f))
     return fail(h, MDWT_ERR_ARG, "opacity scale must lie in [0, 1000]");
   h->os[ch] = opacity_scale;
-  h->accum_id = 0;
+  reset_acc(h);
   HIPCHK(h, hipSetDevice(h->device));
   return rebuild_majorants(h);
 }
@@ -723,7 +771,7 @@ int mdwt_set_xf_range(mdwt_renderer* h, int ch, float xf_lo, float xf_hi) {
     return fail(h, MDWT_ERR_ARG, "transfer-function range must be finite and non-empty");
   h->xlo[ch] = xf_lo;
   h->xhi[ch] = xf_hi;
-  h->accum_id = 0;
+  reset_acc(h);
   HIPCHK(h, hipSetDevice(h->device));
   return rebuild_majorants(h);
 }
@@ -759,7 +807,7 @@ int mdwt_set_dt(mdwt_renderer* h, float dt) {
   if (!h) return MDWT_ERR_ARG;
   if (!std::isfinite(dt)) return fail(h, MDWT_ERR_ARG, "non-finite dt");
   h->dt = std::fmax(dt, 1e-4f);  // renderer.cpp:1460-1466
-  h->accum_id = 0;
+  reset_acc(h);
   return MDWT_OK;
 }
 
@@ -788,7 +836,7 @@ int mdwt_set_frame(mdwt_renderer* h, int width, int height, int spp, int accumul
   h->accumulate = accumulate != 0;
   if (bg)
     for (int i = 0; i < 3; ++i) h->bg[i] = bg[i];
-  h->accum_id = 0;
+  reset_acc(h);
   if (h->inited) {
     update_camera(h);
     if (resize) {
@@ -805,14 +853,14 @@ int mdwt_set_lighting(mdwt_renderer* h, float light_intensity, float ambient) {
     return fail(h, MDWT_ERR_ARG, "non-finite lighting");
   h->light = light_intensity;
   h->ambient = ambient;
-  h->accum_id = 0;
+  reset_acc(h);
   return MDWT_OK;
 }
 
 int mdwt_set_seed(mdwt_renderer* h, uint32_t seed) {
   if (!h) return MDWT_ERR_ARG;
   h->seed = seed;
-  h->accum_id = 0;
+  reset_acc(h);
   return MDWT_OK;
 }
 
@@ -822,6 +870,11 @@ int mdwt_render(mdwt_renderer* h, int n_frames, float* ms_out) {
   if (rc != MDWT_OK) return rc;
   if (n_frames < 0) return fail(h, MDWT_ERR_ARG, "negative frame count");
   HIPCHK(h, hipSetDevice(h->device));
+  if (h->order_mode == 2 && h->need_resort && h->cost_ready) {
+    rc = resort_blocks(h);
+    if (rc != MDWT_OK) return rc;
+    h->need_resort = false;
+  }
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int f = 0; f < n_frames; ++f) {
     rc = launch_frame(h, false);
@@ -840,7 +893,7 @@ int mdwt_render(mdwt_renderer* h, int n_frames, float* ms_out) {
 
 int mdwt_reset_accumulation(mdwt_renderer* h) {
   if (!h) return MDWT_ERR_ARG;
-  h->accum_id = 0;
+  reset_acc(h);
   return MDWT_OK;
 }
 
@@ -1034,7 +1087,8 @@ int mdwt_last_kernel_ms(mdwt_renderer* h, float* ms) {
 
 int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile) {
   if (!h) return MDWT_ERR_ARG;
-  if (order != 0 && order != 1) return fail(h, MDWT_ERR_ARG, "order must be 0 (rows) or 1 (Hilbert)");
+  if (order < 0 || order > 2)
+    return fail(h, MDWT_ERR_ARG, "order must be 0 (rows), 1 (Hilbert) or 2 (cost-sorted)");
   if (tile != mdwt::kBlockPixels) return fail(h, MDWT_ERR_ARG, "tile must be 16");
   h->order_mode = order;
   if (h->inited) {

@@ -53,8 +53,11 @@ struct FrameParams {
   // launch k uses set k & 1 and clears set (k + 1) & 1 for the next launch
   unsigned* queues;
   uint32_t launch_seq;
-  int schedule;  // 0: one thread per pixel, 1: persistent waves
+  int schedule;  // 0: one thread per pixel, 1: persistent waves, 2: flat loop
   int grid;      // workgroups of the persistent launch
+  // cost-sorted order: wall-clock ticks (100 MHz) of each wave, indexed by
+  // 4 * (position in block_order) + wave; nullptr = not recorded
+  uint32_t* wave_cost;
 };
 
 struct RenderParams {

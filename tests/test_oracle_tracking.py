@@ -1,0 +1,140 @@
+"""Pins the oracle's Woodcock tracking (-m 0 baseLineDTCH, -m 2
+altMultiMajDTCH, -m 3 cummilativeDTCH), shading and accumulation with closed
+forms of homogeneous media and with the fact that all three methods estimate
+the same image:
+
+* one constant channel whose majorant equals its density: every tentative
+  collision is accepted, so P(hit) = 1 - exp(-mu L) with mu = sigma / dt per
+  world unit and L the ray's chord through the volume box;
+* two constant channels: the first hit comes from channel c with probability
+  mu_c / (mu_0 + mu_1) * (1 - exp(-(mu_0 + mu_1) L));
+* -m 0, -m 2 and -m 3 agree per pixel within their confidence intervals;
+* a transparent volume renders the background exactly.
+A dropped term (a missing dt or worldToGridT factor, a wrong channel order,
+a threshold drawn from the wrong majorant) moves the expected images away
+from these values by many standard errors."""
+import numpy as np
+import pytest
+
+from mdwtracker_amd import scenes
+from oracle import pyoracle as po
+
+W = H = 48
+FRAMES = 96
+
+
+def _const_volume(n=16, value=128):
+    return np.full((n, n, n), value, np.uint8)
+
+
+def _tf(rgb, alpha, n=64):
+    t = np.zeros((n, 4), np.float32)
+    t[:, :3] = rgb
+    t[:, 3] = alpha
+    return t
+
+
+def _chords(r, jit_samples=16):
+    """Expected chord length through [0,1]^3 per pixel, averaged over the
+    reference's scalar jitter (same offset on both axes), in float64."""
+    org, llc, hor, ver = r.get_camera().astype(np.float64)
+    js = (np.arange(jit_samples) + 0.5) / jit_samples
+    ys, xs = np.mgrid[0:H, 0:W]
+    L = np.zeros((jit_samples, H, W))
+    for k, j in enumerate(js):
+        p = llc + ((xs + j) / W)[..., None] * hor + ((ys + j) / H)[..., None] * ver
+        d = p - org
+        d /= np.linalg.norm(d, axis=-1, keepdims=True)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t_a = (0.0 - org) / d
+            t_b = (1.0 - org) / d
+        tn = np.nanmax(np.minimum(t_a, t_b), axis=-1)
+        tf = np.nanmin(np.maximum(t_a, t_b), axis=-1)
+        L[k] = np.where(tf > np.maximum(tn, 0), tf - np.maximum(tn, 0), 0.0)
+    return L
+
+
+def _render(vols, tfs, mode, frames=FRAMES, seed=7, bg=(0, 0, 0), mc=(4, 4, 4)):
+    r = po.OracleRenderer(threads=4)
+    for v in vols:
+        r.push_channel(v)
+    r.init(mode, mc, True)
+    for c, t in enumerate(tfs):
+        r.set_tf(c, t)
+    r.set_frame(W, H, 1, True, bg)
+    r.set_seed(seed)
+    r.render(frames, threads=4)
+    a, m2 = r.read_accum(with_m2=True)
+    return r, a, m2
+
+
+def _zsum(mean, expect, var_each, frames):
+    """z score of the summed per-pixel means against their expectation."""
+    return (mean.sum() - expect.sum()) / np.sqrt(var_each.sum() / frames)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_homogeneous_single_channel_hit_probability(mode):
+    alpha = 0.315  # alpha^3 ~ 1/32, so mu L ~ 1 for dt = 1/32
+    r, a, _ = _render([_const_volume()], [_tf((1, 1, 1), alpha)], mode)
+    sigma = np.float32(alpha) ** np.float32(3)
+    mu = float(sigma) / float(r.get_dt())
+    L = _chords(r)
+    p = (1.0 - np.exp(-mu * L)).mean(axis=0)
+    shade = 1.0 + 0.1 * (0.75 * 1.0 + 0.25 * 1.0)  # albedo + ambient * (0.75 albedo + 0.25 spec)
+    mean = a[..., 0] / shade
+    inside = p > 0.05
+    assert inside.sum() > 300
+    z = _zsum(mean[inside], p[inside], p[inside] * (1 - p[inside]), FRAMES)
+    assert abs(z) < 4.5, z
+    # outside the volume: background, exactly
+    assert np.all(a[..., :3][L.max(axis=0) == 0] == 0.0)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_homogeneous_two_channel_choice(mode):
+    a0, a1 = 0.27, 0.34
+    vols = [_const_volume(), _const_volume(value=200)]
+    r, a, _ = _render(vols, [_tf((1, 0, 0), a0), _tf((0, 1, 0), a1)], mode)
+    s0 = float(np.float32(a0) ** np.float32(3))
+    s1 = float(np.float32(a1) ** np.float32(3))
+    dt = float(r.get_dt())
+    L = _chords(r)
+    tot = 1.0 - np.exp(-(s0 + s1) / dt * L)
+    p0 = (s0 / (s0 + s1) * tot).mean(axis=0)
+    p1 = (s1 / (s0 + s1) * tot).mean(axis=0)
+    # red of a red hit 1.1, of a green hit 0.025 (ambient * 0.25 * white specular)
+    red = 1.1 * p0 + 0.025 * p1
+    green = 0.025 * p0 + 1.1 * p1
+    inside = (p0 + p1) > 0.05
+    for got, exp, hi_lo in ((a[..., 0], red, (1.1, 0.025)), (a[..., 1], green, (0.025, 1.1))):
+        var = (hi_lo[0] ** 2 * p0 + hi_lo[1] ** 2 * p1) - exp ** 2
+        z = _zsum(got[inside], exp[inside], var[inside], FRAMES)
+        assert abs(z) < 4.5, (mode, z)
+
+
+def test_modes_agree_on_a_scene():
+    sc = scenes.make_scene("ndda_cpu_2ch64", seed=3, scale=2)  # 32^3, 128^2
+    out = {}
+    for mode in (0, 2, 3):
+        r = po.OracleRenderer(threads=4)
+        scenes.load_scene(r, sc, mode)
+        r.set_frame(64, 64)
+        r.render(128, threads=4)
+        out[mode] = r.read_accum(with_m2=True)
+    for m in (2, 3):
+        a, q = out[0]
+        b, s = out[m]
+        va = np.maximum(q - a * a, 1e-12) / 128
+        vb = np.maximum(s - b * b, 1e-12) / 128
+        z = (a - b)[..., :3] / np.sqrt(va + vb)[..., :3]
+        assert np.mean(np.abs(z) > 3.29) < 0.01, (m, np.mean(np.abs(z) > 3.29))
+        assert abs(float((a - b)[..., :3].mean())) < 3e-3
+
+
+def test_transparent_volume_renders_background():
+    bg = (0.2, 0.3, 0.4)
+    for mode in (0, 2, 3):
+        r, a, _ = _render([_const_volume()], [_tf((1, 1, 1), 0.0)], mode, frames=2, bg=bg)
+        assert np.all(a[..., :3] == np.array(bg, np.float32))
+        assert r.read_counters()["samples"] == 0

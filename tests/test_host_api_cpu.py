@@ -1,0 +1,51 @@
+"""Host-side API checks that need no GPU: the RAW loader's file-name parsing and
+value conversions (rawFile.cpp:31-128), the C++ header compiling on its own,
+and the offline driver's argument handling (viewer.cpp:70-161, 204-224)."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_rawr_parses_name_and_converts(tmp_path):
+    from mdwtracker_amd.renderer import RawR
+    v = (np.arange(2 * 3 * 4) % 7).astype(np.int8).reshape(4, 3, 2)
+    p = tmp_path / "blob_2x3x4_int8.raw"
+    v.tofile(p)
+    r = RawR(p)
+    assert r.getDims() == (2, 3, 4) and r.getBytesPerVoxel() == 1
+    np.testing.assert_array_equal(r.pushable(), v.astype(np.float32) * 255 - 128)  # rawFile.cpp:102
+    u = np.arange(8, dtype=np.uint8).reshape(2, 2, 2)
+    q = tmp_path / "x_2x2x2_uint8.raw"
+    u.tofile(q)
+    ru = RawR(q)
+    assert ru.pushable().dtype == np.uint8  # u8 goes to the GPU as stored (x255 inside)
+    ru.reshapeBounds()
+    assert ru.bounds[1].tolist() == [2.0, 2.0, 2.0]
+    with pytest.raises(ValueError):
+        RawR(tmp_path / "nodims_uint8.raw")
+
+
+def test_cpp_header_compiles_standalone(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "mdwt_renderer.hpp"\nint main() { return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        f"-I{ROOT / 'include'}", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_offline_cli_argument_errors():
+    exe = ROOT / "mdwtracker_amd" / "mdwt_offline"
+    if not exe.exists():
+        pytest.skip("mdwt_offline not built")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "No data file given" in r.stderr
+    r = subprocess.run([str(exe), "-fr", "a_4x4x4_uint8.raw", "-mc", "0"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "positive" in r.stderr
+    r = subprocess.run([str(exe), "-fr", "a_4x4x4_uint8.raw", "-l", "1", "0", "0", "-1", "0"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "shadows" in r.stderr

@@ -20,22 +20,41 @@
 namespace mdwt {
 namespace {
 
-// -m 2, altMultiMajDTCH (deviceCode.cu:747-856)
+// Majorants of up to four channels of one macrocell: one 16-byte load for four
+// channels (the grid is cell-major, 16-byte aligned per cell when C = 4).
+__device__ __forceinline__ float4 load_cell_maj(const float* maj, int cid, int C) {
+  if (C == 4) return *reinterpret_cast<const float4*>(maj + size_t(cid) * 4);
+  const float* p = maj + size_t(cid) * C;
+  return make_float4(p[0], C > 1 ? p[1] : 0.f, C > 2 ? p[2] : 0.f, 0.f);
+}
+__device__ __forceinline__ float pick(const float4& m, int c) {
+  return c == 0 ? m.x : (c == 1 ? m.y : (c == 2 ? m.z : m.w));
+}
+
+// -m 2, altMultiMajDTCH (deviceCode.cu:747-856). With C <= 4 the next cell's
+// majorants are requested before the current cell's channels are tracked, so
+// the load latency overlaps the free-flight steps instead of preceding them.
 template <int DT>
 __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng& rng, HitRec& h,
                            Counts& cnt) {
   Dda d;
   if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
   const float step = X.dt;
+  const bool regs = X.C <= 4;
+  int cid = d.cx + d.cy * X.mx + d.cz * X.mx * X.my;
+  float4 mcur = regs ? load_cell_maj(X.maj, cid, X.C) : make_float4(0.f, 0.f, 0.f, 0.f);
   while (true) {
     cnt.visits++;
     float t0, t1;
     d.range(t0, t1);
-    const int cid = d.cx + d.cy * X.mx + d.cz * X.mx * X.my;
+    Dda dn = d;
+    const bool more = dn.advance();
+    const int ncid = dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my;
+    const float4 mnext = (regs && more) ? load_cell_maj(X.maj, ncid, X.C) : mcur;
     float minT = t1;
     bool terminate = false;
     for (int c = 0; c < X.C; ++c) {
-      const float majorant = X.maj[size_t(cid) * X.C + c];
+      const float majorant = regs ? pick(mcur, c) : X.maj[size_t(cid) * X.C + c];
       float t = t0;
       if (majorant == 0.0f) continue;
       while (true) {
@@ -67,7 +86,10 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
       if (terminate) break;
     }
     if (terminate || !h.missed) return;
-    if (!d.advance()) return;
+    if (!more) return;
+    d = dn;
+    cid = ncid;
+    mcur = mnext;
   }
 }
 
@@ -82,12 +104,16 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
   float tMaxHit = 1e20f;
   for (int c = 0; c < X.C; ++c) {
     Dda d = d0;
+    float majorant = X.maj[size_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my) * X.C + c];
     while (true) {
       cnt.visits++;
       float t0, t1;
       d.range(t0, t1);
-      const int cid = d.cx + d.cy * X.mx + d.cz * X.mx * X.my;
-      const float majorant = X.maj[size_t(cid) * X.C + c];
+      // next cell's majorant requested before this cell is tracked
+      Dda dn = d;
+      const bool more = dn.advance();
+      const float mnext =
+          more ? X.maj[size_t(dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my) * X.C + c] : 0.f;
       bool stop = false;
       if (majorant != 0.00f) {
         float t = t0;
@@ -124,7 +150,9 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
         }
       }
       if (stop) break;
-      if (!d.advance()) break;
+      if (!more) break;
+      d = dn;
+      majorant = mnext;
     }
   }
 }
@@ -136,12 +164,14 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
   Dda d;
   if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
   const float step = X.dt;
+  float majorant = X.maj[d.cx + d.cy * X.mx + d.cz * X.mx * X.my];
   while (true) {
     cnt.visits++;
     float t0, t1;
     d.range(t0, t1);
-    const int cid = d.cx + d.cy * X.mx + d.cz * X.mx * X.my;
-    const float majorant = X.maj[cid];
+    Dda dn = d;
+    const bool more = dn.advance();
+    const float mnext = more ? X.maj[dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my] : 0.f;
     if (majorant != 0.00f) {
       float t = t0;
       while (true) {
@@ -172,7 +202,9 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
         cnt.rejections++;
       }
     }
-    if (!d.advance()) return;
+    if (!more) return;
+    d = dn;
+    majorant = mnext;
   }
 }
 

@@ -1,0 +1,114 @@
+"""GPU checks of the host APIs above the C ABI and of full-size statistics:
+the Python Renderer mirror and the offline C++ driver produce the same images
+as direct ABI calls, and a homogeneous medium at 1024^2 matches its closed form
+(a size-independent property of -m 0/2/3)."""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from mdwtracker_amd import scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from mdwtracker_amd import capi
+    return capi
+
+
+def _write_tf(path, tf, opacity, rng=(0.0, 1.0)):
+    with open(path, "w") as f:
+        f.write(f"opacity {opacity}\nrange {rng[0]} {rng[1]}\n")
+        for row in tf:
+            f.write(" ".join(repr(float(x)) for x in row) + "\n")
+
+
+def test_renderer_mirror_and_cli_match_abi(tmp_path, capi):
+    from mdwtracker_amd.renderer import RawR, Renderer
+    sc = scenes.make_scene("ndda_cpu_2ch64", seed=11)
+    paths = []
+    for c, v in enumerate(sc.volumes):
+        p = tmp_path / f"ch{c}_64x64x64_uint8.raw"
+        v.tofile(p)
+        paths.append(p)
+    # direct ABI
+    g = capi.MdwtRenderer(0)
+    for v in sc.volumes:
+        g.push_channel(v)
+    g.init(2, (8, 8, 8), True)
+    for c, t in enumerate(sc.tfs):
+        g.set_tf(c, t, 0.05, (0.0, 1.0))
+    g.set_frame(160, 120, 1, True, (0.0, 0.0, 0.0))
+    g.render(6)
+    ref = g.read_fb()
+    # Python mirror of dtracker::Renderer
+    r = Renderer(0)
+    r.macrocellDims = [8, 8, 8]
+    r.fbSize = (160, 120)
+    for p in paths:
+        assert r.PushMesh(RawR(p))
+    r.Init(2)
+    for c, t in enumerate(sc.tfs):
+        r.SetXFColormap(t, c)
+        r.SetXFOpacityScale(0.05, c)
+        r.SetXFRange((0.0, 1.0), c)
+    r.SetXFColormap(sc.tfs[0], 7)  # unknown channel: ignored, like the reference
+    for _ in range(6):
+        r.Render()
+    np.testing.assert_array_equal(r.GetFrameBuffer(), ref)
+    # offline C++ driver (PNG is written bottom-up like the reference)
+    tfs = []
+    for c, t in enumerate(sc.tfs):
+        q = tmp_path / f"ch{c}.tf"
+        _write_tf(q, t, 0.05)
+        tfs.append(str(q))
+    exe = ROOT / "mdwtracker_amd" / "mdwt_offline"
+    cmd = [str(exe), "-fr", *map(str, paths), "-t", *tfs, "-m", "2", "-mc", "8", "-r", "160", "120",
+           "-n", "6", "-o", str(tmp_path / "out")]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "avg. fps" in out.stdout
+    png = tmp_path / "out_AMMB_w_6_shadows_off_frames.png"
+    assert png.exists() and png.stat().st_size > 160 * 120 * 4
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_homogeneous_medium_full_size(capi, mode):
+    n, alpha, frames = 64, 0.315, 32
+    g = capi.MdwtRenderer(0)
+    g.push_channel(np.full((n, n, n), 128, np.uint8))
+    g.init(mode, (8, 8, 8), True)
+    t = np.zeros((64, 4), np.float32)
+    t[:, :3] = 1.0
+    t[:, 3] = alpha
+    g.set_tf(0, t)
+    W = H = 1024
+    g.set_frame(W, H, 1, True, (0.0, 0.0, 0.0))
+    g.render(frames)
+    a = g.read_accum()
+    sigma = float(np.float32(alpha) ** np.float32(3))
+    mu = sigma / g.get_dt()
+    # chord of the jitter-free pixel centre rays (the pixel footprint is tiny
+    # against the chord's variation at 1024^2)
+    org, llc, hor, ver = g.get_camera().astype(np.float64)
+    ys, xs = np.mgrid[0:H, 0:W]
+    p = llc + ((xs + 0.5) / W)[..., None] * hor + ((ys + 0.5) / H)[..., None] * ver
+    d = p - org
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    with np.errstate(divide="ignore"):
+        ta, tb = (0.0 - org) / d, (1.0 - org) / d
+    tn = np.max(np.minimum(ta, tb), axis=-1)
+    tf = np.min(np.maximum(ta, tb), axis=-1)
+    L = np.where(tf > tn, tf - tn, 0.0)
+    prob = 1.0 - np.exp(-mu * L)
+    inside = (prob > 0.05) & (prob < 0.95)
+    mean = a[..., 0] / 1.1
+    z = (mean[inside].sum() - prob[inside].sum()) / np.sqrt(
+        (prob[inside] * (1 - prob[inside])).sum() / frames)
+    print(f"[homogeneous] mode {mode}: {inside.sum()} pixels, z = {z:.2f}", flush=True)
+    assert abs(z) < 6.0

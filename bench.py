@@ -84,7 +84,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--mode", type=int, default=2, choices=(0, 2, 3))
     ap.add_argument("--workload", default="sync_4ch256")
-    ap.add_argument("--cpu-frames", type=int, default=32)
+    ap.add_argument("--cpu-frames", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
     args = ap.parse_args()

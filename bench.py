@@ -49,7 +49,7 @@ def frame_counts(r, frames: int) -> dict:
 
 def algorithmic_bytes(counts: dict, scene, mode: int) -> float:
     bpv = scene.bytes_per_voxel()
-    k = scene.channels if mode == 2 else 1
+    k = scene.channels if mode in (1, 2) else 1
     return 8.0 * bpv * counts["samples"] + 4.0 * k * counts["visits"] + 32.0 * scene.width * scene.height
 
 
@@ -82,7 +82,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=16)
-    ap.add_argument("--mode", type=int, default=2, choices=(0, 2, 3))
+    ap.add_argument("--mode", type=int, default=2, choices=(0, 1, 2, 3))
     ap.add_argument("--workload", default="sync_4ch256")
     ap.add_argument("--cpu-frames", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -130,7 +130,7 @@ def main() -> int:
 
     extra_modes = {}
     if not args.no_extra_modes:
-        for m in (0, 2, 3):
+        for m in (0, 1, 2, 3):
             if m == args.mode:
                 continue
             r.set_mode(m)

@@ -38,8 +38,14 @@ enum mdwt_status {
  * u16 -> v*65535, f16/f32 as is) */
 enum mdwt_dtype { MDWT_U8 = 0, MDWT_U16 = 1, MDWT_F16 = 2, MDWT_F32 = 3 };
 
-/* -m values (renderer.h:24-36 Mode): 0 N-DDA, 2 syncDDA, 3 cumulative majorant */
-enum mdwt_mode { MDWT_MODE_NDDA = 0, MDWT_MODE_SYNCDDA = 2, MDWT_MODE_CUMULATIVE = 3 };
+/* -m values (renderer.h:24-36 Mode): 0 N-DDA, 1 multi-majorant (one
+ * exponential clock per channel), 2 syncDDA, 3 cumulative majorant */
+enum mdwt_mode {
+  MDWT_MODE_NDDA = 0,
+  MDWT_MODE_MULTI = 1,
+  MDWT_MODE_SYNCDDA = 2,
+  MDWT_MODE_CUMULATIVE = 3
+};
 
 typedef struct mdwt_renderer mdwt_renderer;
 
@@ -111,7 +117,7 @@ int mdwt_read_fb(mdwt_renderer* h, uint32_t* rgba8);
 /* ---- inspection hooks (tests, bench) ---- */
 /* value-range grid: float2 (min, max) per (cell, channel), index cell*C + ch */
 int mdwt_read_ranges(mdwt_renderer* h, float* out);
-/* majorants: float per (cell, channel) for modes 0/2, float per cell for 3 */
+/* majorants: float per (cell, channel) for modes 0-2, float per cell for 3 */
 int mdwt_read_majorants(mdwt_renderer* h, float* out);
 /* out = {mc[3], dims[3], channels, mode, frame_id, accum_id} */
 int mdwt_get_info(mdwt_renderer* h, int out[10]);
@@ -147,8 +153,9 @@ int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile);
 /* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
  * loops, a workgroup per 16x16 block (default); 1 = flat step loop in
  * persistent waves that refill lanes from 8 interleaved pixel queues; 2 = flat
- * step loop, a workgroup per 16x16 block. Images do not depend on it. No
- * reference counterpart (OptiX schedules the reference's launch). */
+ * step loop, a workgroup per 16x16 block. Images do not depend on it. -m 1
+ * always runs schedule 0. No reference counterpart (OptiX schedules the
+ * reference's launch). */
 int mdwt_set_schedule(mdwt_renderer* h, int schedule);
 /* Read-dominated device copy of `bytes` bytes, `iters` times; GB/s of
  * (read + write) bytes. Measures the achievable HBM bandwidth. */

@@ -1,13 +1,13 @@
 // Kernels (b)+(c), one-thread-per-pixel schedule: per-pixel ray generation,
 // root clip, 3D-DDA over the majorant grid and the per-channel Woodcock
-// tracking loop for -m 0 (N-DDA), -m 2 (syncDDA) and -m 3 (cumulative
-// majorant), plus accumulation; also the test-hook kernels and the frame
+// tracking loop for -m 0 (N-DDA), -m 1 (multi-majorant), -m 2 (syncDDA) and
+// -m 3 (cumulative majorant), plus accumulation; also the test-hook kernels and the frame
 // launcher (which also dispatches the flat schedules of kernels_flat.hip).
 //
 // Reference: mainRG (deviceCode.cu:283-479) launched as one OptiX thread per
 // pixel, volumeIntersection (deviceCode.cu:1676-1726), the closest-hit
-// programs baseLineDTCH (858-970), altMultiMajDTCH (747-856) and
-// cummilativeDTCH (513-617) running dda::dda3 (dda.h:82-203) with a Woodcock
+// programs baseLineDTCH (858-970), multiMajDTCH (619-745), altMultiMajDTCH
+// (747-856) and cummilativeDTCH (513-617) running dda::dda3 (dda.h:82-203) with a Woodcock
 // lambda. Here OptiX is gone: a workgroup of four wave64 owns a 16x16 pixel
 // block, each wave an 8x8 quad (rays of a wave start close together and walk
 // the same macrocells); blocks are issued in Hilbert order and spread over the
@@ -208,6 +208,87 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
   }
 }
 
+// -m 1, multiMajDTCH (deviceCode.cu:619-745): per cell one exponential clock
+// per channel, the earliest fires; only the fired channel's clock advances.
+// Clocks and majorants live in registers: the loops run over kMaxChannels
+// with constant indices and stop at X.C, so no private-memory arrays appear.
+template <int DT>
+__device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng& rng, HitRec& h,
+                            Counts& cnt) {
+  Dda d;
+  if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
+  const float step = X.dt;
+  float maj[kMaxChannels], ts[kMaxChannels];
+  while (true) {
+    cnt.visits++;
+    float t0, t1;
+    d.range(t0, t1);
+    const int cid = d.cx + d.cy * X.mx + d.cz * X.mx * X.my;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kMaxChannels; ++i) {
+      if (i < X.C) {
+        maj[i] = X.maj[size_t(cid) * X.C + i];
+        sum += maj[i];
+      } else {
+        maj[i] = 0.0f;
+      }
+      ts[i] = t0;
+    }
+    if (!(sum <= 0.0000001f)) {
+#pragma unroll
+      for (int i = 0; i < kMaxChannels; ++i) {
+        if (i < X.C) {
+          if (maj[i] > 0.0f)
+            ts[i] = ts[i] - (plogf(1.0f - rng.step()) / maj[i]) * step * g.worldToGridT;
+          else
+            ts[i] = 1e20f;
+        }
+      }
+      while (true) {
+        float minT = ts[0], mj = maj[0];
+        int sel = 0;
+        const float rand = rng.step();
+#pragma unroll
+        for (int i = 1; i < kMaxChannels; ++i) {
+          if (i < X.C && ts[i] < minT && maj[i] > 0.f) {
+            sel = i;
+            minT = ts[i];
+            mj = maj[i];
+          }
+        }
+        if (minT >= t1) break;
+        const float tWorld = minT * g.gridToWorldT;
+        const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+        if (tWorld >= prd_t1) {
+          cnt.rejections++;
+          return;
+        }
+        const float thr = rng.thr() * mj;
+        const float value = X.sample(xT, sel);
+        cnt.samples++;
+        if (!(value != value)) {
+          float r2;
+          const float a = X.tf_alpha(value, sel, r2);
+          if (a > 0.0f && thr < a) {
+            h.missed = false;
+            h.tHit = tWorld;
+            h.channel = sel;
+            X.tf_rgb(r2, sel, h);
+            return;
+          }
+        }
+        cnt.rejections++;
+        const float nt = minT - (plogf(1.0f - rand) / mj) * step * g.worldToGridT;
+#pragma unroll
+        for (int i = 0; i < kMaxChannels; ++i)
+          if (i == sel) ts[i] = nt;
+      }
+    }
+    if (!d.advance()) return;
+  }
+}
+
 template <int MODE, int DT, bool COUNT>
 __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
   extern __shared__ float4 s_tf[];
@@ -244,6 +325,8 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
         const GridRay g = make_grid_ray(org, dir, blo, bhi, X.mx, X.my, X.mz);
         if (MODE == 0)
           trace_ndda<DT>(X, g, t1, rng, h, cnt);
+        else if (MODE == 1)
+          trace_multi<DT>(X, g, t1, rng, h, cnt);
         else if (MODE == 2)
           trace_sync<DT>(X, g, t1, rng, h, cnt);
         else
@@ -359,6 +442,7 @@ RenderFn static_fn_dt(int dtype) {
 }
 
 RenderFn select_render(int mode, int dtype, bool count, int schedule) {
+  if (mode == 1) return count ? static_fn_dt<1, true>(dtype) : static_fn_dt<1, false>(dtype);
   if (schedule == 1 || schedule == 2) return select_flat_kernel(mode, dtype, count, schedule == 1);
   if (mode == 0) return count ? static_fn_dt<0, true>(dtype) : static_fn_dt<0, false>(dtype);
   if (mode == 2) return count ? static_fn_dt<2, true>(dtype) : static_fn_dt<2, false>(dtype);

@@ -441,6 +441,9 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
   return v;
 }
 
+// -m 1 has only the thread-per-pixel kernel; the flat schedules cover 0/2/3.
+int eff_schedule(const mdwt_renderer* h) { return h->mode == 1 ? 0 : h->schedule; }
+
 mdwt::RenderParams render_params(const mdwt_renderer* h) {
   mdwt::RenderParams p{};
   p.vol = volume_params(h);
@@ -469,10 +472,10 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.n_blocks = h->n_blocks;
   p.fr.queues = h->d_queues;
   p.fr.launch_seq = h->launch_seq;
-  p.fr.schedule = h->schedule;
+  p.fr.schedule = eff_schedule(h);
   p.fr.grid = h->grid;
   p.fr.wave_cost =
-      (h->order_mode == 2 && h->schedule == 0 && h->need_resort) ? h->d_wave_cost : nullptr;
+      (h->order_mode == 2 && eff_schedule(h) == 0 && h->need_resort) ? h->d_wave_cost : nullptr;
   return p;
 }
 
@@ -480,7 +483,7 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
 // (cached per mode / counting / LDS size), launches, advances the queue set.
 int launch_frame(mdwt_renderer* h, bool count) {
   mdwt::RenderParams p = render_params(h);
-  if (h->schedule == 1) {
+  if (eff_schedule(h) == 1) {
     const size_t lds = mdwt::render_lds_bytes(p);
     const int key = int(lds) * 8 + h->mode * 2 + (count ? 1 : 0);
     if (key != h->grid_key) {
@@ -633,8 +636,9 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   if (!h) return MDWT_ERR_ARG;
   if (h->inited) return fail(h, MDWT_ERR_STATE, "already initialized");
   if (h->C == 0) return fail(h, MDWT_ERR_STATE, "no data source (push a channel first)");
-  if (mode != 0 && mode != 2 && mode != 3)
-    return fail(h, MDWT_ERR_ARG, "mode must be 0 (N-DDA), 2 (syncDDA) or 3 (cumulative)");
+  if (mode < 0 || mode > 3)
+    return fail(h, MDWT_ERR_ARG,
+                "mode must be 0 (N-DDA), 1 (multi-majorant), 2 (syncDDA) or 3 (cumulative)");
   int m[3];
   for (int i = 0; i < 3; ++i) {
     const int v = mc ? mc[i] : 0;
@@ -720,8 +724,9 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
 int mdwt_set_mode(mdwt_renderer* h, int mode) {
   if (!h) return MDWT_ERR_ARG;
   if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
-  if (mode != 0 && mode != 2 && mode != 3)
-    return fail(h, MDWT_ERR_ARG, "mode must be 0 (N-DDA), 2 (syncDDA) or 3 (cumulative)");
+  if (mode < 0 || mode > 3)
+    return fail(h, MDWT_ERR_ARG,
+                "mode must be 0 (N-DDA), 1 (multi-majorant), 2 (syncDDA) or 3 (cumulative)");
   HIPCHK(h, hipSetDevice(h->device));
   h->mode = mode;
   reset_acc(h);

@@ -665,6 +665,68 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
   }
 }
 
+// deviceCode.cu:619-745 multiMajDTCH (-m 1): per cell one exponential clock
+// per channel; the earliest clock fires, its sample is tested against that
+// channel's majorant, and only that channel's clock is advanced. The draw
+// order follows the reference: one draw per channel with a positive
+// majorant on cell entry, then per event the next-step draw first and the
+// threshold second (one Rng step per draw pair, see Rng).
+void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
+  const int C = int(o->ch.size());
+  const float unit = o->dt;
+  float majorants[kMaxChannels];
+  float ts[kMaxChannels];
+  auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
+    prd.visits++;
+    const int cid = cell_id(o, cellIdx);
+    float majorantSum = 0.0f;
+    for (int i = 0; i < C; ++i) {
+      majorants[i] = o->maj[size_t(cid) * C + i];
+      majorantSum += majorants[i];
+      ts[i] = t0;
+    }
+    if (majorantSum <= 0.0000001f) return true;
+    for (int i = 0; i < C; ++i) {
+      if (majorants[i] > 0.0f)
+        ts[i] = ts[i] - (plogf(1.0f - rng.step()) / majorants[i]) * unit * g.worldToGridT;
+      else
+        ts[i] = 1e20f;
+    }
+    while (true) {
+      float minT = ts[0];
+      int sel = 0;
+      const float rand = rng.step();
+      for (int i = 1; i < C; ++i) {
+        if (ts[i] < minT && majorants[i] > 0.f) {
+          sel = i;
+          minT = ts[i];
+        }
+      }
+      if (minT >= t1) break;
+      const float tWorld = minT * g.gridToWorldT;
+      const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+      if (tWorld >= prd_t1) {
+        prd.rejections++;
+        return false;
+      }
+      const float nullColThreshold = rng.thr() * majorants[sel];
+      const float value = sample_volume(o, xTexture, sel);
+      prd.samples++;
+      if (!std::isnan(value)) {
+        const vec4f s = transfer_function(o, value, sel);
+        if (s.w > 0.0f && nullColThreshold < s.w) {
+          record_hit(prd, tWorld, s, sel);
+          return false;
+        }
+      }
+      prd.rejections++;
+      ts[sel] = ts[sel] - (plogf(1.0f - rand) / majorants[sel]) * unit * g.worldToGridT;
+    }
+    return true;
+  };
+  dda3(g.org, g.dir, 1e20f, o->mc, lambda);
+}
+
 // deviceCode.cu:747-856 altMultiMajDTCH (-m 2, syncDDA)
 void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
   const int C = int(o->ch.size());
@@ -803,6 +865,8 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
       const GridRay g = grid_ray(o, org, dir);
       if (o->mode == 0)
         trace_ndda(o, g, t1, rng, prd);
+      else if (o->mode == 1)
+        trace_multi(o, g, t1, rng, prd);
       else if (o->mode == 2)
         trace_sync(o, g, t1, rng, prd);
       else
@@ -900,7 +964,7 @@ int oracle_set_bounds(oracle_renderer* o, const float remap[3]) {
 // renderer.cpp:324-942 Init (RAW path 638-903, camera 856-863, dt 935)
 int oracle_init(oracle_renderer* o, int mode, const int mc[3], int auto_camera) {
   if (o->ch.empty()) return fail(o, "no channels");
-  if (mode != 0 && mode != 2 && mode != 3) return fail(o, "unsupported mode");
+  if (mode < 0 || mode > 3) return fail(o, "unsupported mode");
   o->mode = mode;
   // CalculateMCGridDims(4) (renderer.cpp:1342-1355) for any zero entry
   for (int i = 0; i < 3; ++i) {

@@ -77,7 +77,7 @@ def test_renderer_mirror_and_cli_match_abi(tmp_path, capi):
     assert png.exists() and png.stat().st_size > 160 * 120 * 4
 
 
-@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_homogeneous_medium_full_size(capi, mode):
     n, alpha, frames = 64, 0.315, 32
     g = capi.MdwtRenderer(0)

@@ -57,10 +57,11 @@ def test_ranges_and_majorants_bit_exact(capi, po, name, scale):
         g, o = _pair(capi, po, sc, mode)
         np.testing.assert_array_equal(_bits(g.read_ranges()), _bits(o.read_ranges()))
         np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o.read_majorants()))
-        g.set_mode(0)
-        o2 = po.OracleRenderer()
-        scenes.load_scene(o2, sc, 0)
-        np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o2.read_majorants()))
+        for m in (0, 1):
+            g.set_mode(m)
+            o2 = po.OracleRenderer()
+            scenes.load_scene(o2, sc, m)
+            np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o2.read_majorants()))
         g.close()
 
 
@@ -149,7 +150,7 @@ def test_dda_cells_bit_exact_64k_rays(capi, po):
     assert on.max() <= 48 and (on > 0).mean() > 0.3 and on[on > 0].mean() > 10
 
 
-@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_images_bit_exact_small(capi, po, mode):
     sc = _scene("ndda_cpu_2ch64")
     g, o = _pair(capi, po, sc, mode)
@@ -170,7 +171,7 @@ def test_images_bit_exact_small(capi, po, mode):
     assert g.info()["frame_id"] == 16 and o.info()["frame_id"] == 16
 
 
-@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_images_bit_exact_4ch_quarter(capi, po, mode):
     sc = _scene("sync_4ch256", 4)
     g, o = _pair(capi, po, sc, mode)
@@ -208,7 +209,7 @@ def test_tile_order_and_spp_and_no_accumulation(capi, po):
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
 
 
-@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_schedules_agree_bit_exact(capi, mode):
     """The persistent schedule and the one-thread-per-pixel schedule compute
     every pixel with the same operations: images and counters are identical."""

@@ -1,5 +1,5 @@
-"""Pins the oracle's Woodcock tracking (-m 0 baseLineDTCH, -m 2
-altMultiMajDTCH, -m 3 cummilativeDTCH), shading and accumulation with closed
+"""Pins the oracle's Woodcock tracking (-m 0 baseLineDTCH, -m 1
+multiMajDTCH, -m 2 altMultiMajDTCH, -m 3 cummilativeDTCH), shading and accumulation with closed
 forms of homogeneous media and with the fact that all three methods estimate
 the same image:
 
@@ -8,7 +8,7 @@ the same image:
   world unit and L the ray's chord through the volume box;
 * two constant channels: the first hit comes from channel c with probability
   mu_c / (mu_0 + mu_1) * (1 - exp(-(mu_0 + mu_1) L));
-* -m 0, -m 2 and -m 3 agree per pixel within their confidence intervals;
+* -m 0, -m 1, -m 2 and -m 3 agree per pixel within their confidence intervals;
 * a transparent volume renders the background exactly.
 A dropped term (a missing dt or worldToGridT factor, a wrong channel order,
 a threshold drawn from the wrong majorant) moves the expected images away
@@ -73,7 +73,7 @@ def _zsum(mean, expect, var_each, frames):
     return (mean.sum() - expect.sum()) / np.sqrt(var_each.sum() / frames)
 
 
-@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_homogeneous_single_channel_hit_probability(mode):
     alpha = 0.315  # alpha^3 ~ 1/32, so mu L ~ 1 for dt = 1/32
     r, a, _ = _render([_const_volume()], [_tf((1, 1, 1), alpha)], mode)
@@ -91,7 +91,7 @@ def test_homogeneous_single_channel_hit_probability(mode):
     assert np.all(a[..., :3][L.max(axis=0) == 0] == 0.0)
 
 
-@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_homogeneous_two_channel_choice(mode):
     a0, a1 = 0.27, 0.34
     vols = [_const_volume(), _const_volume(value=200)]
@@ -113,16 +113,66 @@ def test_homogeneous_two_channel_choice(mode):
         assert abs(z) < 4.5, (mode, z)
 
 
+def _step_tf(rgb, a_lo, a_hi, n=256):
+    t = np.zeros((n, 4), np.float32)
+    t[:, :3] = rgb
+    t[: n // 2, 3] = a_lo
+    t[n // 2:, 3] = a_hi
+    return t
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_two_channel_choice_with_null_collisions(mode):
+    """Constant volumes looked up half-way between two TF texels (transfer
+    range (0, 2) maps the volume's value 1.0 to u = 0.5, texel coordinate
+    127.5): the density is the mean of texels 127 and 128 while the majorant
+    is texel 128's, so tentative collisions are rejected with probability
+    1 - sigma/majorant and every method must re-advance its clocks at the
+    right rate for the expected image to hold. One macrocell, so that clocks
+    run over the whole chord instead of restarting at every cell."""
+    lo_hi = ((0.25, 0.4), (0.15, 0.33))
+    r = po.OracleRenderer(threads=4)
+    for v in (_const_volume(), _const_volume(value=200)):
+        r.push_channel(v)
+    r.init(mode, (1, 1, 1), True)
+    rgbs = ((1, 0, 0), (0, 1, 0))
+    for c in range(2):
+        r.set_tf(c, _step_tf(rgbs[c], *lo_hi[c]), 1.0, (0.0, 2.0))
+    r.set_frame(W, H, 1, True, (0, 0, 0))
+    r.set_seed(11)
+    r.render(FRAMES, threads=4)
+    a = r.read_accum()
+    sig = []
+    for lo, hi in lo_hi:
+        l3, h3 = np.float32(lo) ** np.float32(3), np.float32(hi) ** np.float32(3)
+        sig.append(float(l3 + np.float32(0.5) * (h3 - l3)))
+    maj = r.read_majorants()
+    want = [0.4 ** 3, 0.33 ** 3] if mode < 3 else [0.4 ** 3 + 0.33 ** 3]
+    assert np.allclose(maj.reshape(-1), want, rtol=1e-5)
+    dt = float(r.get_dt())
+    L = _chords(r)
+    tot = 1.0 - np.exp(-(sig[0] + sig[1]) / dt * L)
+    p0 = (sig[0] / (sig[0] + sig[1]) * tot).mean(axis=0)
+    p1 = (sig[1] / (sig[0] + sig[1]) * tot).mean(axis=0)
+    red = 1.1 * p0 + 0.025 * p1
+    green = 0.025 * p0 + 1.1 * p1
+    inside = (p0 + p1) > 0.05
+    for got, exp, hl in ((a[..., 0], red, (1.1, 0.025)), (a[..., 1], green, (0.025, 1.1))):
+        var = (hl[0] ** 2 * p0 + hl[1] ** 2 * p1) - exp ** 2
+        z = _zsum(got[inside], exp[inside], var[inside], FRAMES)
+        assert abs(z) < 4.5, (mode, z)
+
+
 def test_modes_agree_on_a_scene():
     sc = scenes.make_scene("ndda_cpu_2ch64", seed=3, scale=2)  # 32^3, 128^2
     out = {}
-    for mode in (0, 2, 3):
+    for mode in (0, 1, 2, 3):
         r = po.OracleRenderer(threads=4)
         scenes.load_scene(r, sc, mode)
         r.set_frame(64, 64)
         r.render(128, threads=4)
         out[mode] = r.read_accum(with_m2=True)
-    for m in (2, 3):
+    for m in (1, 2, 3):
         a, q = out[0]
         b, s = out[m]
         va = np.maximum(q - a * a, 1e-12) / 128
@@ -134,7 +184,7 @@ def test_modes_agree_on_a_scene():
 
 def test_transparent_volume_renders_background():
     bg = (0.2, 0.3, 0.4)
-    for mode in (0, 2, 3):
+    for mode in (0, 1, 2, 3):
         r, a, _ = _render([_const_volume()], [_tf((1, 1, 1), 0.0)], mode, frames=2, bg=bg)
         assert np.all(a[..., :3] == np.array(bg, np.float32))
         assert r.read_counters()["samples"] == 0

@@ -61,6 +61,32 @@ def read_traffic(workload: str, mode: int):
     return d.get("bytes_per_launch"), str(p.relative_to(ROOT))
 
 
+def time_to_rmse(r, frames: int, target: float, ms_per_frame: float) -> dict:
+    """Frames (and seconds at the measured frame time) that plain accumulation
+    needs for an expected RMSE of `target` against the converged image.
+
+    The per-pixel variance of one frame comes from the accumulated first and
+    second moments of a `frames`-frame render (unbiased, averaged over pixels
+    and RGB): sigma^2. The mean of k independent frames has expected RMSE
+    sqrt(sigma^2 / k) against the true image, so k = ceil(sigma^2 / target^2).
+    The `frames`-frame render is the reference image; its own expected RMSE
+    is reported beside it."""
+    import numpy as np
+
+    r.reset_accumulation()
+    r.render(frames)
+    a, m2 = r.read_accum(with_m2=True)
+    mean = a[..., :3].astype(np.float64)
+    var = np.maximum(m2[..., :3].astype(np.float64) - mean * mean, 0.0) * frames / (frames - 1)
+    sigma2 = float(var.mean())
+    k = max(1, int(np.ceil(sigma2 / (target * target))))
+    return {"target_rmse": target, "per_frame_variance": sigma2,
+            "plain": {"frames": k, "seconds": round(k * ms_per_frame * 1e-3, 3)},
+            "denoised": None,
+            "reference_frames": frames,
+            "reference_rmse": round(float(np.sqrt(sigma2 / frames)), 6)}
+
+
 def cpu_baseline(scene, mode: int, frames: int) -> dict:
     from oracle import pyoracle
     from mdwtracker_amd import scenes
@@ -82,11 +108,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=16)
-    ap.add_argument("--mode", type=int, default=2, choices=(0, 1, 2, 3))
+    ap.add_argument("--mode", type=int, default=2, choices=range(10))
     ap.add_argument("--workload", default="sync_4ch256")
     ap.add_argument("--cpu-frames", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
+    ap.add_argument("--rmse-frames", type=int, default=4096,
+                    help="frames of the reference render for time-to-RMSE (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -128,6 +156,11 @@ def main() -> int:
     if traffic_src:
         roofline["traffic_source"] = traffic_src
 
+    ttr = None
+    if args.rmse_frames > 1:
+        ttr = time_to_rmse(r, args.rmse_frames, 1e-3, ms_per_step)
+        log(f"[bench] time-to-RMSE 1e-3: {ttr}")
+
     extra_modes = {}
     if not args.no_extra_modes:
         for m in (0, 1, 2, 3):
@@ -162,7 +195,7 @@ def main() -> int:
         "samples_per_s": round(counts["samples"] / (kernel_ms * 1e-3)),
         "cell_visits_per_s": round(counts["visits"] / (kernel_ms * 1e-3)),
         "work_per_frame": {k: round(v) for k, v in counts.items()},
-        "time_to_rmse": None,
+        "time_to_rmse": ttr,
         "roofline": roofline,
     }
     line.update(extra_modes)

@@ -39,12 +39,21 @@ enum mdwt_status {
 enum mdwt_dtype { MDWT_U8 = 0, MDWT_U16 = 1, MDWT_F16 = 2, MDWT_F32 = 3 };
 
 /* -m values (renderer.h:24-36 Mode): 0 N-DDA, 1 multi-majorant (one
- * exponential clock per channel), 2 syncDDA, 3 cumulative majorant */
+ * exponential clock per channel), 2 syncDDA, 3 cumulative majorant; 4/5
+ * Woodcock tracking of the max-opacity / mixed blend of the channels on the
+ * summed majorant; 6/7/8 ray marchers over the density / max / mix blend;
+ * 9 per-channel ray marcher composited by opacity */
 enum mdwt_mode {
   MDWT_MODE_NDDA = 0,
   MDWT_MODE_MULTI = 1,
   MDWT_MODE_SYNCDDA = 2,
-  MDWT_MODE_CUMULATIVE = 3
+  MDWT_MODE_CUMULATIVE = 3,
+  MDWT_MODE_MAX = 4,
+  MDWT_MODE_MIX = 5,
+  MDWT_MODE_MARCHER_MULTI = 6,
+  MDWT_MODE_MARCHER_MAX = 7,
+  MDWT_MODE_MARCHER_MIX = 8,
+  MDWT_MODE_MARCHER_COMP = 9
 };
 
 typedef struct mdwt_renderer mdwt_renderer;
@@ -117,7 +126,8 @@ int mdwt_read_fb(mdwt_renderer* h, uint32_t* rgba8);
 /* ---- inspection hooks (tests, bench) ---- */
 /* value-range grid: float2 (min, max) per (cell, channel), index cell*C + ch */
 int mdwt_read_ranges(mdwt_renderer* h, float* out);
-/* majorants: float per (cell, channel) for modes 0-2, float per cell for 3 */
+/* majorants: float per (cell, channel) for modes 0-2, float per cell (the
+ * channel sum) for modes 3-9 */
 int mdwt_read_majorants(mdwt_renderer* h, float* out);
 /* out = {mc[3], dims[3], channels, mode, frame_id, accum_id} */
 int mdwt_get_info(mdwt_renderer* h, int out[10]);
@@ -153,8 +163,8 @@ int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile);
 /* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
  * loops, a workgroup per 16x16 block (default); 1 = flat step loop in
  * persistent waves that refill lanes from 8 interleaved pixel queues; 2 = flat
- * step loop, a workgroup per 16x16 block. Images do not depend on it. -m 1
- * always runs schedule 0. No reference counterpart (OptiX schedules the
+ * step loop, a workgroup per 16x16 block. Images do not depend on it. Modes
+ * other than 0, 2 and 3 always run schedule 0. No reference counterpart (OptiX schedules the
  * reference's launch). */
 int mdwt_set_schedule(mdwt_renderer* h, int schedule);
 /* Read-dominated device copy of `bytes` bytes, `iters` times; GB/s of

@@ -291,12 +291,16 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
     // ------------------------ finish a sample / start the next sample --
     while (st == S_FINISH || st == S_START) {
       if (st == S_FINISH) {
-        composite_sample(P, L.hit, L.hr, L.hg, L.hb, L.ar, L.ag, L.ab);
+        // a Woodcock hit is opaque, so each sample adds exactly 1 to alpha
+        float fa_unused = 0.f;
+        composite_sample(P, L.hit, L.hr, L.hg, L.hb, 1.f, L.ar, L.ag, L.ab, fa_unused);
         L.s += 1;
         if (L.s < spp) {
           st = S_START;
         } else {
-          if (!COUNT) finish_pixel(P, L.pixel, L.ar, L.ag, L.ab);
+          float fa = 1.f;
+          for (int i = 0; i < spp; ++i) fa = fa + 1.0f;
+          if (!COUNT) finish_pixel(P, L.pixel, L.ar, L.ag, L.ab, fa);
           st = S_IDLE;
         }
       } else {

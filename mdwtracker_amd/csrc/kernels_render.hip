@@ -289,6 +289,182 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
   }
 }
 
+// blendChannels (deviceCode.cu:222-241): maxOpacityBlend (165-179) for -m 4
+// and 7, mixBlend (182-201) for -m 5 and 8, densityBlend (204-220) for -m 6;
+// (0,0,0,0) outside the unit cube.
+template <int MODE, int DT>
+__device__ float4 blend_channels(const Ctx<DT>& X, V3 p) {
+  if (p.x < 0.0f || p.x > 1.0f || p.y < 0.0f || p.y > 1.0f || p.z < 0.0f || p.z > 1.0f)
+    return make_float4(0.f, 0.f, 0.f, 0.f);
+  if (MODE == 6) {
+    float sum = 0.0f;
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = 0; c < X.C; ++c) {
+      const float4 s = X.tf_rgba(X.sample(p, c), c);
+      col.x = col.x + s.w * s.x;
+      col.y = col.y + s.w * s.y;
+      col.z = col.z + s.w * s.z;
+      col.w = col.w + s.w * s.w;
+      sum += s.w;
+    }
+    if (sum <= 0.0f) return make_float4(0.f, 0.f, 0.f, 0.f);
+    return make_float4(col.x / sum, col.y / sum, col.z / sum, col.w / sum);
+  }
+  if (MODE == 4 || MODE == 7) {
+    float4 best = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = 0; c < X.C; ++c) {
+      const float4 s = X.tf_rgba(X.sample(p, c), c);
+      if (s.w > best.w) best = s;
+    }
+    return best;
+  }
+  float cr = 0.f, cg = 0.f, cb = 0.f, mo = 0.0f;  // -m 5 and 8
+  for (int c = 0; c < X.C; ++c) {
+    const float4 s = X.tf_rgba(X.sample(p, c), c);
+    cr = cr + s.x * s.w;
+    cg = cg + s.y * s.w;
+    cb = cb + s.z * s.w;
+    if (s.w > mo) mo = s.w;
+  }
+  if (mo <= 0.0f) return make_float4(0.f, 0.f, 0.f, 0.f);
+  return make_float4(cr / mo, cg / mo, cb / mo, mo);
+}
+
+// -m 4 / 5, blendDTCH (deviceCode.cu:972-1065): Woodcock tracking on the
+// summed majorant with a blended sample; the hit keeps channel 0 (the
+// reference never sets the payload's channel in this program).
+template <int MODE, int DT>
+__device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng& rng, HitRec& h,
+                            Counts& cnt) {
+  Dda d;
+  if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
+  const float step = X.dt;
+  float majorant = X.maj[d.cx + d.cy * X.mx + d.cz * X.mx * X.my];
+  while (true) {
+    cnt.visits++;
+    float t0, t1;
+    d.range(t0, t1);
+    Dda dn = d;
+    const bool more = dn.advance();
+    const float mnext = more ? X.maj[dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my] : 0.f;
+    if (majorant != 0.00f) {
+      float t = t0;
+      while (true) {
+        t = t - (plogf(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        if (t >= t1) break;
+        const float tWorld = t * g.gridToWorldT;
+        const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+        if (tWorld >= prd_t1) {
+          cnt.rejections++;
+          return;
+        }
+        const float4 s = blend_channels<MODE, DT>(X, xT);
+        cnt.samples++;
+        if (s.w > 0.0f && rng.thr() * majorant < s.w) {
+          h.missed = false;
+          h.tHit = tWorld;
+          h.channel = 0;
+          h.r = s.x;
+          h.g = s.y;
+          h.b = s.z;
+          return;
+        }
+        cnt.rejections++;
+      }
+    }
+    if (!more) return;
+    d = dn;
+    majorant = mnext;
+  }
+}
+
+// -m 6 / 7 / 8, rayMarcherCH (deviceCode.cu:1067-1157) with shadows off:
+// front-to-back compositing of blended samples every dt from the root-box
+// entry; the sample position trails t by one step and starts at the ray
+// origin (the reference moves pos after sampling).
+template <int MODE, int DT>
+__device__ void march_blend(const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u, float prd_t0, float prd_t1,
+                            HitRec& h, Counts& cnt) {
+  const float dt = X.dt;
+  const V3 dir = normalize3(dir0);
+  V3 pos = org;
+  float alpha = 0.0f;
+  float cr = 0.f, cg = 0.f, cb = 0.f;
+  for (float t = prd_t0; (t < prd_t1) && (alpha < 0.99f); t += dt) {
+    const V3 posTex = pos * w2u;
+    const float4 b = blend_channels<MODE, DT>(X, posTex);
+    cnt.samples += uint32_t(X.C);
+    const float k = 1.f - alpha;
+    cr = cr + b.x * b.w * k;
+    cg = cg + b.y * b.w * k;
+    cb = cb + b.z * b.w * k;
+    alpha = alpha + (1.f - alpha) * b.w;
+    pos = org + dir * t;
+  }
+  h.missed = false;
+  h.r = cr;
+  h.g = cg;
+  h.b = cb;
+  h.a = alpha;
+}
+
+// -m 9, compRayMarcherCH (deviceCode.cu:1159-1276): each channel marched and
+// composited on its own, then mixed by opacity. The sample position is not
+// reset between channels (channel n > 0 starts where channel n-1 stopped), as
+// in the reference.
+template <int DT>
+__device__ void march_comp(const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u, float prd_t0, float prd_t1,
+                           HitRec& h, Counts& cnt) {
+  const float dt = X.dt;
+  const V3 dir = normalize3(dir0);
+  V3 pos = org;
+  float asum = 0.0f, fr = 0.f, fg = 0.f, fb = 0.f;
+  float al[kMaxChannels], rr[kMaxChannels], gg[kMaxChannels], bb[kMaxChannels];
+#pragma unroll
+  for (int n = 0; n < kMaxChannels; ++n) {
+    float a = 0.0f, r = 0.f, g = 0.f, b = 0.f;
+    if (n < X.C) {
+      for (float t = prd_t0; (t < prd_t1) && (a < 0.99f); t += dt) {
+        const V3 posTex = pos * w2u;
+        const float v = X.sample(posTex, n);
+        float4 sc = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!(v != v)) {
+          sc = X.tf_rgba(v, n);
+          cnt.samples++;
+        }
+        const float k = 1.f - a;
+        r = r + sc.x * sc.w * k;
+        g = g + sc.y * sc.w * k;
+        b = b + sc.z * sc.w * k;
+        a = a + (1.f - a) * sc.w;
+        pos = org + dir * t;
+      }
+    }
+    al[n] = a;
+    rr[n] = r;
+    gg[n] = g;
+    bb[n] = b;
+  }
+#pragma unroll
+  for (int n = 0; n < kMaxChannels; ++n)
+    if (n < X.C) asum += al[n];
+  if (asum > 0.0f) {
+#pragma unroll
+    for (int n = 0; n < kMaxChannels; ++n) {
+      if (n < X.C) {
+        fr = fr + rr[n] * al[n] / asum;
+        fg = fg + gg[n] * al[n] / asum;
+        fb = fb + bb[n] * al[n] / asum;
+      }
+    }
+  }
+  h.missed = false;
+  h.r = fr;
+  h.g = fg;
+  h.b = fb;
+  h.a = asum / float(X.C);
+}
+
 template <int MODE, int DT, bool COUNT>
 __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
   extern __shared__ float4 s_tf[];
@@ -311,7 +487,7 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
   if (px < W && py < H) {
     const uint32_t pixel = uint32_t(px) + uint32_t(W) * uint32_t(py);
     const V3 blo = load3(P.vol.blo), bhi = load3(P.vol.bhi);
-    float fr = 0.f, fg = 0.f, fb = 0.f;
+    float fr = 0.f, fg = 0.f, fb = 0.f, fa = 1.f;
     for (int s = 0; s < P.fr.spp; ++s) {
       Rng rng;
       rng.init(P.fr.seed, pixel, P.fr.frame_id, uint32_t(s), 0u);
@@ -320,6 +496,7 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
       camera_ray(P.cam, px, py, W, H, jit, org, dir);
       HitRec h;
       h.missed = true;
+      h.a = 1.f;
       float t0, t1;
       if (clip_root(org, dir, blo, bhi, t0, t1)) {
         const GridRay g = make_grid_ray(org, dir, blo, bhi, X.mx, X.my, X.mz);
@@ -329,13 +506,19 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
           trace_multi<DT>(X, g, t1, rng, h, cnt);
         else if (MODE == 2)
           trace_sync<DT>(X, g, t1, rng, h, cnt);
-        else
+        else if (MODE == 3)
           trace_cumulative<DT>(X, g, t1, rng, h, cnt);
+        else if (MODE <= 5)
+          trace_blend<MODE, DT>(X, g, t1, rng, h, cnt);
+        else if (MODE <= 8)
+          march_blend<MODE, DT>(X, org, dir, g.worldToUnit, t0, t1, h, cnt);
+        else
+          march_comp<DT>(X, org, dir, g.worldToUnit, t0, t1, h, cnt);
       }
       cnt.rays++;
-      composite_sample(P, !h.missed, h.r, h.g, h.b, fr, fg, fb);
+      composite_sample(P, !h.missed, h.r, h.g, h.b, h.a, fr, fg, fb, fa);
     }
-    if (!COUNT) finish_pixel(P, pixel, fr, fg, fb);
+    if (!COUNT) finish_pixel(P, pixel, fr, fg, fb, fa);
   }
   if (COUNT) flush_counts(P, cnt);
   // per-wave duration for the cost-sorted block order of later frames
@@ -441,13 +624,24 @@ RenderFn static_fn_dt(int dtype) {
   }
 }
 
+#define MDWT_STATIC(M) (count ? static_fn_dt<M, true>(dtype) : static_fn_dt<M, false>(dtype))
 RenderFn select_render(int mode, int dtype, bool count, int schedule) {
-  if (mode == 1) return count ? static_fn_dt<1, true>(dtype) : static_fn_dt<1, false>(dtype);
+  switch (mode) {  // modes with the thread-per-pixel kernel only
+    case 1: return MDWT_STATIC(1);
+    case 4: return MDWT_STATIC(4);
+    case 5: return MDWT_STATIC(5);
+    case 6: return MDWT_STATIC(6);
+    case 7: return MDWT_STATIC(7);
+    case 8: return MDWT_STATIC(8);
+    case 9: return MDWT_STATIC(9);
+    default: break;
+  }
   if (schedule == 1 || schedule == 2) return select_flat_kernel(mode, dtype, count, schedule == 1);
-  if (mode == 0) return count ? static_fn_dt<0, true>(dtype) : static_fn_dt<0, false>(dtype);
-  if (mode == 2) return count ? static_fn_dt<2, true>(dtype) : static_fn_dt<2, false>(dtype);
-  return count ? static_fn_dt<3, true>(dtype) : static_fn_dt<3, false>(dtype);
+  if (mode == 0) return MDWT_STATIC(0);
+  if (mode == 2) return MDWT_STATIC(2);
+  return MDWT_STATIC(3);
 }
+#undef MDWT_STATIC
 
 }  // namespace
 

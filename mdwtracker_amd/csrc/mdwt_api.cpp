@@ -441,8 +441,10 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
   return v;
 }
 
-// -m 1 has only the thread-per-pixel kernel; the flat schedules cover 0/2/3.
-int eff_schedule(const mdwt_renderer* h) { return h->mode == 1 ? 0 : h->schedule; }
+// The flat schedules cover -m 0/2/3; every other mode runs the thread-per-pixel kernel.
+int eff_schedule(const mdwt_renderer* h) {
+  return (h->mode == 0 || h->mode == 2 || h->mode == 3) ? h->schedule : 0;
+}
 
 mdwt::RenderParams render_params(const mdwt_renderer* h) {
   mdwt::RenderParams p{};
@@ -636,9 +638,10 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   if (!h) return MDWT_ERR_ARG;
   if (h->inited) return fail(h, MDWT_ERR_STATE, "already initialized");
   if (h->C == 0) return fail(h, MDWT_ERR_STATE, "no data source (push a channel first)");
-  if (mode < 0 || mode > 3)
+  if (mode < 0 || mode > 9)
     return fail(h, MDWT_ERR_ARG,
-                "mode must be 0 (N-DDA), 1 (multi-majorant), 2 (syncDDA) or 3 (cumulative)");
+                "mode must be 0-9 (0 N-DDA, 1 multi-majorant, 2 syncDDA, 3 cumulative, "
+                "4/5 max/mix blend tracking, 6/7/8 ray marchers, 9 composite marcher)");
   int m[3];
   for (int i = 0; i < 3; ++i) {
     const int v = mc ? mc[i] : 0;
@@ -724,9 +727,10 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
 int mdwt_set_mode(mdwt_renderer* h, int mode) {
   if (!h) return MDWT_ERR_ARG;
   if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
-  if (mode < 0 || mode > 3)
+  if (mode < 0 || mode > 9)
     return fail(h, MDWT_ERR_ARG,
-                "mode must be 0 (N-DDA), 1 (multi-majorant), 2 (syncDDA) or 3 (cumulative)");
+                "mode must be 0-9 (0 N-DDA, 1 multi-majorant, 2 syncDDA, 3 cumulative, "
+                "4/5 max/mix blend tracking, 6/7/8 ray marchers, 9 composite marcher)");
   HIPCHK(h, hipSetDevice(h->device));
   h->mode = mode;
   reset_acc(h);

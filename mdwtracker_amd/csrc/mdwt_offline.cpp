@@ -142,13 +142,19 @@ const char* mode_string(int m) {  // viewer.cpp:238-270
     case 1: return "MMB";
     case 2: return "AMMB";
     case 3: return "CMB";
+    case 4: return "MAX";
+    case 5: return "MIX";
+    case 6: return "MM_RM";
+    case 7: return "MAX_RM";
+    case 8: return "MIX_RM";
+    case 9: return "COMP";
     default: return "?";
   }
 }
 
 int usage(const char* msg) {
   std::fprintf(stderr,
-               "%s\nusage: mdwt_offline -fr FILE... [-t TF...] [-m 0|1|2|3] [-mc M | -mc X Y Z]\n"
+               "%s\nusage: mdwt_offline -fr FILE... [-t TF...] [-m 0..9] [-mc M | -mc X Y Z]\n"
                "  [-r W H] [-c px py pz ax ay az ux uy uz fovy] [-bg r g b] [-dt dt] [-cb [sx sy sz]]\n"
                "  [-spp N] [-acc 0|1] [-l shadows gradient dx dy dz [intensity [ambient]]]\n"
                "  [-n frames] [-wu warmup] [-nt N] [-o name] [-seed S]\n",

@@ -19,6 +19,7 @@ struct Counts {
 struct HitRec {
   bool missed;
   float r, g, b;
+  float a;  // transparency: 1 for a Woodcock hit, the composited alpha for the marchers
   float tHit;
   int channel;
 };
@@ -98,6 +99,17 @@ struct Ctx {
     const float r2 = (r1 - t.xlo) / (t.xhi - t.xlo);
     r2out = r2;
     return tf_lerp_alpha(tf + t.off, t.n, r2) * t.os;
+  }
+  // full transferFunction (deviceCode.cu:27-42) for the blend and marcher
+  // modes; f is never NaN here (callers test the unit cube or the sample)
+  __device__ __forceinline__ float4 tf_rgba(float f, int c) const {
+    const ChanTf t = chtf[c];
+    if (f < t.dlo || f > t.dhi) return make_float4(1.f, 0.f, 1.f, 0.f);
+    const float r1 = (f - t.dlo) / (t.dhi - t.dlo);
+    const float r2 = (r1 - t.xlo) / (t.xhi - t.xlo);
+    float4 v = tf_lerp(tf + t.off, t.n, r2);
+    v.w = v.w * t.os;
+    return v;
   }
   __device__ __forceinline__ void tf_rgb(float r2, int c, HitRec& h) const {
     const ChanTf t = chtf[c];
@@ -187,8 +199,8 @@ __device__ __forceinline__ void accumulate_pixel(const RenderParams& P, uint32_t
 // shading with diffuse 1, specular 0, shadow 1 (deviceCode.cu:326-433) and
 // over(colour, (bg, 1)) (deviceCode.cu:44-59) added to the pixel sum
 __device__ __forceinline__ void composite_sample(const RenderParams& P, bool hit, float hr,
-                                                 float hg, float hb, float& fr, float& fg,
-                                                 float& fb) {
+                                                 float hg, float hb, float ha, float& fr,
+                                                 float& fg, float& fb, float& fa) {
   float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
   if (hit) {
     const float li = 1.0f * 1.0f * P.fr.light_intensity;
@@ -199,19 +211,18 @@ __device__ __forceinline__ void composite_sample(const RenderParams& P, bool hit
     cr = li * hr + sp * mr + P.fr.ambient * mr;
     cg = li * hg + sp * mg + P.fr.ambient * mg;
     cb = li * hb + sp * mb + P.fr.ambient * mb;
-    ca = 1.0f;
+    ca = ha;
   }
+  // over(color, (bg, 1)) (deviceCode.cu:54-59, 433)
   fr = fr + (cr + P.fr.bg[0] * 1.f * (1.f - ca));
   fg = fg + (cg + P.fr.bg[1] * 1.f * (1.f - ca));
   fb = fb + (cb + P.fr.bg[2] * 1.f * (1.f - ca));
+  fa = fa + (ca + (1.f - ca) * 1.f);
 }
 
-// Divides the pixel sums by spp; alpha is 1 + spp (each sample adds
-// ca + (1 - ca) * 1 = 1 exactly to the initial 1) over spp.
+// Divides the pixel sums (alpha started at 1) by spp and accumulates.
 __device__ __forceinline__ void finish_pixel(const RenderParams& P, uint32_t pixel, float fr,
-                                             float fg, float fb) {
-  float fa = 1.f;
-  for (int i = 0; i < P.fr.spp; ++i) fa = fa + 1.0f;
+                                             float fg, float fb, float fa) {
   const float spp = float(P.fr.spp);
   accumulate_pixel(P, pixel, fr / spp, fg / spp, fb / spp, fa / spp);
 }

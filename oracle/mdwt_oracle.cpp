@@ -37,6 +37,7 @@ inline vec3f operator*(vec3f a, vec3f b) { return {a.x * b.x, a.y * b.y, a.z * b
 inline vec3f operator*(float s, vec3f a) { return {s * a.x, s * a.y, s * a.z}; }
 inline vec3f operator*(vec3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 inline vec3f operator/(float s, vec3f a) { return {s / a.x, s / a.y, s / a.z}; }
+inline vec3f operator/(vec3f a, float s) { return {a.x / s, a.y / s, a.z / s}; }
 inline vec3f operator-(vec3f a) { return {-a.x, -a.y, -a.z}; }
 inline float dot(vec3f a, vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline float length(vec3f a) { return sqrtf(dot(a, a)); }
@@ -806,6 +807,146 @@ void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, 
   dda3(g.org, g.dir, 1e20f, o->mc, lambda);
 }
 
+// deviceCode.cu:222-241 blendChannels with maxOpacityBlend (165-179, modes 4
+// and 7), mixBlend (182-201, modes 5 and 8) and densityBlend (204-220, mode
+// 6); (0,0,0,0) outside the unit cube.
+vec4f blend_channels(const oracle_renderer* o, vec3f p) {
+  if (p.x < 0.0f || p.x > 1.0f || p.y < 0.0f || p.y > 1.0f || p.z < 0.0f || p.z > 1.0f)
+    return {0.f, 0.f, 0.f, 0.f};
+  const int C = int(o->ch.size());
+  if (o->mode == 6) {
+    float densitySum = 0.0f;
+    vec4f color = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < C; ++i) {
+      const vec4f s = transfer_function(o, sample_volume(o, p, i), i);
+      color = color + s.w * s;
+      densitySum += s.w;
+    }
+    if (densitySum <= 0.0f) return {0.f, 0.f, 0.f, 0.f};
+    return color / densitySum;
+  }
+  if (o->mode == 4 || o->mode == 7) {
+    vec4f best = {0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < C; ++c) {
+      const vec4f s = transfer_function(o, sample_volume(o, p, c), c);
+      if (s.w > best.w) best = s;
+    }
+    return best;
+  }
+  vec3f color = {0.f, 0.f, 0.f};  // modes 5 and 8
+  float maxOpacity = 0.0f;
+  for (int c = 0; c < C; ++c) {
+    const vec4f s = transfer_function(o, sample_volume(o, p, c), c);
+    color = color + vec3f{s.x, s.y, s.z} * s.w;
+    if (s.w > maxOpacity) maxOpacity = s.w;
+  }
+  if (maxOpacity <= 0.0f) return {0.f, 0.f, 0.f, 0.f};
+  color = {color.x / maxOpacity, color.y / maxOpacity, color.z / maxOpacity};
+  return {color.x, color.y, color.z, maxOpacity};
+}
+
+// deviceCode.cu:972-1065 blendDTCH (-m 4 MAX, -m 5 MIX): Woodcock tracking on
+// the summed majorant with a blended sample. The reference leaves the
+// payload's channel at 0 for these modes.
+void trace_blend(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
+  const float unit = o->dt;
+  auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
+    prd.visits++;
+    const float majorant = o->maj[size_t(cell_id(o, cellIdx))];
+    if (majorant == 0.00f) return true;
+    float t = t0;
+    while (true) {
+      t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      if (t >= t1) break;
+      const float tWorld = t * g.gridToWorldT;
+      const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
+      if (tWorld >= prd_t1) {
+        prd.rejections++;
+        return false;
+      }
+      const vec4f s = blend_channels(o, xTexture);
+      prd.samples++;
+      if (s.w > 0.0f && rng.thr() * majorant < s.w) {
+        record_hit(prd, tWorld, s, 0);
+        return false;
+      }
+      prd.rejections++;
+    }
+    return true;
+  };
+  dda3(g.org, g.dir, 1e20f, o->mc, lambda);
+}
+
+inline vec3f over3(vec3f Cin, vec3f Cx, float Ain, float Ax) {  // deviceCode.cu:44-47
+  return Cin + Cx * Ax * (1.f - Ain);
+}
+inline float over1(float Ain, float Ax) { return Ain + (1.f - Ain) * Ax; }  // deviceCode.cu:49-52
+
+// deviceCode.cu:1067-1157 rayMarcherCH (-m 6/7/8) with shadows off: front to
+// back compositing of blended samples every dt from the root-box entry. The
+// sample position trails t by one step and starts at the ray origin (the
+// reference updates pos after sampling), which is reproduced.
+void march_blend(const oracle_renderer* o, vec3f org, vec3f dir0, float prd_t0, float prd_t1,
+                 Hit& prd) {
+  const vec3f worldToUnit = 1.f / (o->bhi - o->blo);
+  const float dt = o->dt;
+  const vec3f dir = normalize(dir0);
+  vec3f pos = org;
+  float alpha = 0.0f;
+  vec3f color = {0.f, 0.f, 0.f};
+  const int C = int(o->ch.size());
+  for (float t = prd_t0; (t < prd_t1) && (alpha < 0.99f); t += dt) {
+    const vec3f posTex = pos * worldToUnit;
+    const vec4f b = blend_channels(o, posTex);
+    prd.samples += uint64_t(C);
+    color = over3(color, vec3f{b.x, b.y, b.z}, alpha, b.w);
+    alpha = over1(alpha, b.w);
+    pos = org + dir * t;
+  }
+  prd.missed = false;
+  prd.rgba = {color.x, color.y, color.z, alpha};
+}
+
+// deviceCode.cu:1159-1276 compRayMarcherCH (-m 9): each channel marched and
+// composited on its own, then mixed by opacity. The reference does not reset
+// the sample position between channels, so channel n > 0 starts where
+// channel n-1 stopped; reproduced.
+void march_comp(const oracle_renderer* o, vec3f org, vec3f dir0, float prd_t0, float prd_t1,
+                Hit& prd) {
+  const vec3f worldToUnit = 1.f / (o->bhi - o->blo);
+  const float dt = o->dt;
+  const vec3f dir = normalize(dir0);
+  vec3f pos = org;
+  const int C = int(o->ch.size());
+  float alphas[kMaxChannels];
+  vec3f colors[kMaxChannels];
+  for (int i = 0; i < C; ++i) {
+    alphas[i] = 0.0f;
+    colors[i] = {0.f, 0.f, 0.f};
+  }
+  for (int n = 0; n < C; ++n) {
+    for (float t = prd_t0; (t < prd_t1) && (alphas[n] < 0.99f); t += dt) {
+      const vec3f posTex = pos * worldToUnit;
+      const float v = sample_volume(o, posTex, n);
+      vec4f sc = {0.f, 0.f, 0.f, 0.f};
+      if (!std::isnan(v)) {
+        sc = transfer_function(o, v, n);
+        prd.samples++;
+      }
+      colors[n] = over3(colors[n], vec3f{sc.x, sc.y, sc.z}, alphas[n], sc.w);
+      alphas[n] = over1(alphas[n], sc.w);
+      pos = org + dir * t;
+    }
+  }
+  float alphaSum = 0.0f;
+  for (int i = 0; i < C; ++i) alphaSum += alphas[i];
+  vec3f fin = {0.f, 0.f, 0.f};
+  if (alphaSum > 0.0f)
+    for (int i = 0; i < C; ++i) fin = fin + colors[i] * alphas[i] / alphaSum;
+  prd.missed = false;
+  prd.rgba = {fin.x, fin.y, fin.z, alphaSum / float(C)};
+}
+
 // deviceCode.cu:61-76 generateRay
 inline void generate_ray(const oracle_renderer* o, float su, float sv, vec3f& org, vec3f& dir) {
   org = o->c_org;
@@ -869,8 +1010,14 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
         trace_multi(o, g, t1, rng, prd);
       else if (o->mode == 2)
         trace_sync(o, g, t1, rng, prd);
-      else
+      else if (o->mode == 3)
         trace_cumulative(o, g, t1, rng, prd);
+      else if (o->mode <= 5)
+        trace_blend(o, g, t1, rng, prd);
+      else if (o->mode <= 8)
+        march_blend(o, org, dir, t0, t1, prd);
+      else
+        march_comp(o, org, dir, t0, t1, prd);
       cnt[0] += prd.samples;
       cnt[1] += prd.visits;
       cnt[2] += prd.rejections;
@@ -964,7 +1111,7 @@ int oracle_set_bounds(oracle_renderer* o, const float remap[3]) {
 // renderer.cpp:324-942 Init (RAW path 638-903, camera 856-863, dt 935)
 int oracle_init(oracle_renderer* o, int mode, const int mc[3], int auto_camera) {
   if (o->ch.empty()) return fail(o, "no channels");
-  if (mode < 0 || mode > 3) return fail(o, "unsupported mode");
+  if (mode < 0 || mode > 9) return fail(o, "unsupported mode");
   o->mode = mode;
   // CalculateMCGridDims(4) (renderer.cpp:1342-1355) for any zero entry
   for (int i = 0; i < 3; ++i) {

@@ -150,7 +150,7 @@ def test_dda_cells_bit_exact_64k_rays(capi, po):
     assert on.max() <= 48 and (on > 0).mean() > 0.3 and on[on > 0].mean() > 10
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", list(range(10)))
 def test_images_bit_exact_small(capi, po, mode):
     sc = _scene("ndda_cpu_2ch64")
     g, o = _pair(capi, po, sc, mode)
@@ -171,7 +171,7 @@ def test_images_bit_exact_small(capi, po, mode):
     assert g.info()["frame_id"] == 16 and o.info()["frame_id"] == 16
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", list(range(10)))
 def test_images_bit_exact_4ch_quarter(capi, po, mode):
     sc = _scene("sync_4ch256", 4)
     g, o = _pair(capi, po, sc, mode)

@@ -107,6 +107,23 @@ int mdwt_set_frame(mdwt_renderer* h, int width, int height, int spp, int accumul
                    const float bg[3]);
 /* SetLightIntensity / SetAmbient (renderer.cpp:1476-1486) */
 int mdwt_set_lighting(mdwt_renderer* h, float light_intensity, float ambient);
+/* SetLightDirection (renderer.cpp:1468-1474) and the enableShadows /
+ * enableGradientShading fields (renderer.h:121-124; the -l flag,
+ * viewer.cpp:271-281). Shadows trace one ray per hit towards -light_dir
+ * through the same method (deviceCode.cu:383-403; the ray marchers march
+ * their own, 1117-1154); gradient shading is Blinn-Phong on the TF-opacity
+ * gradient of the hit channel (deviceCode.cu:326-381). Resets accumulation. */
+int mdwt_set_light(mdwt_renderer* h, const float light_dir[3], int shadows, int gradient_shading);
+/* SetMaterialDiffuse / Specular / Shininess / GradientStep / SpecularColor
+ * (renderer.cpp:1195-1248) for one channel (0..MDWT_MAX_CHANNELS-1; the
+ * reference ignores unknown ids, this returns MDWT_ERR_ARG). Resets
+ * accumulation. */
+int mdwt_set_material(mdwt_renderer* h, int channel, float diffuse, float specular,
+                      float shininess, float gradient_step, const float specular_color[3]);
+/* ApplyMaterialPreset's table (renderer.cpp:1250-1301): out = {diffuse,
+ * specular, shininess, gradient_step, specular r, g, b} for "Plastic",
+ * "Iron", "Chrome", "Emerald"; any other name gives the default material. */
+int mdwt_material_preset(const char* name, float out[7]);
 /* seed of the counter-based RNG (replaces the LCG seeding, deviceCode.cu:296-297) */
 int mdwt_set_seed(mdwt_renderer* h, uint32_t seed);
 

@@ -124,6 +124,14 @@ class Renderer {
   bool enableAccumulation = true;
   std::array<float, 3> bgColor{0.f, 0.f, 0.f};
   float lightIntensity = 1.0f, ambient = 0.1f;
+  bool enableShadows = false, enableGradientShading = false;   // renderer.h:121-122
+  std::array<float, 3> lightDir{0.0001f, -1.f, 0.0001f};       // renderer.h:124
+  // renderer.h:134-141, one per pushed channel
+  struct MaterialData {
+    float diffuse = 0.8f, specular = 0.3f, shininess = 8.0f, gradientStep = 0.01f;
+    std::array<float, 3> specularColor{1.f, 1.f, 1.f};
+  };
+  std::vector<MaterialData> materials;
   float dt = 0.05f;
   int accumID = 0, frameID = 0;
   float avgTime = 0.f, minTime = std::numeric_limits<float>::max(), totalTime = 0.f;
@@ -131,6 +139,7 @@ class Renderer {
   bool PushMesh(std::shared_ptr<RawR> raw) {
     if (raws_.size() >= MDWT_MAX_CHANNELS) return false;
     raws_.push_back(std::move(raw));
+    materials.push_back(MaterialData());
     return true;
   }
 
@@ -147,6 +156,7 @@ class Renderer {
     macrocellDims = {unsigned(info[0]), unsigned(info[1]), unsigned(info[2])};
     Update();
     check(mdwt_set_lighting(h_, lightIntensity, ambient), h_);
+    for (size_t c = 0; c < materials.size(); ++c) PushMaterial(c);
     if (camera.given) UpdateCamera();
     check(mdwt_get_dt(h_, &dt), h_);
     inited_ = true;
@@ -192,13 +202,44 @@ class Renderer {
     check(mdwt_set_lighting(h_, lightIntensity, ambient), h_);
     ResetAccumulation();
   }
+  // renderer.cpp:1468-1474
+  void SetLightDirection(const std::array<float, 3>& d) {
+    lightDir = d;
+    PushLight();
+    ResetAccumulation();
+  }
+  // renderer.cpp:1195-1248 (unknown channel ids are ignored, as there)
+  void SetMaterialDiffuse(float v, size_t ch = 0) { SetMaterial(ch, [&](MaterialData& m) { m.diffuse = v; }); }
+  void SetMaterialSpecular(float v, size_t ch = 0) { SetMaterial(ch, [&](MaterialData& m) { m.specular = v; }); }
+  void SetMaterialShininess(float v, size_t ch = 0) { SetMaterial(ch, [&](MaterialData& m) { m.shininess = v; }); }
+  void SetMaterialGradientStep(float v, size_t ch = 0) {
+    SetMaterial(ch, [&](MaterialData& m) { m.gradientStep = v; });
+  }
+  void SetMaterialSpecularColor(const std::array<float, 3>& v, size_t ch = 0) {
+    SetMaterial(ch, [&](MaterialData& m) { m.specularColor = v; });
+  }
+  // renderer.cpp:1250-1301 (table in the library: mdwt_material_preset)
+  void ApplyMaterialPreset(const std::string& name, size_t ch = 0) {
+    float v[7];
+    check(mdwt_material_preset(name.c_str(), v), h_);
+    SetMaterial(ch, [&](MaterialData& m) {
+      m.diffuse = v[0];
+      m.specular = v[1];
+      m.shininess = v[2];
+      m.gradientStep = v[3];
+      m.specularColor = {v[4], v[5], v[6]};
+    });
+  }
   void Resize(const std::array<int, 2>& newSize) {
     fbSize = newSize;
     Update();
   }
-  // renderer.cpp:975-1008: spp, accumulation, background
+  // renderer.cpp:975-1008: spp, accumulation, background, light switches
   void Update() {
     check(mdwt_set_frame(h_, fbSize[0], fbSize[1], spp, enableAccumulation ? 1 : 0, bgColor.data()), h_);
+    if (lightDir != pushedDir_ || enableShadows != pushedShadows_ ||
+        enableGradientShading != pushedGradient_)
+      PushLight();
   }
   // renderer.cpp:944-973
   void Render(short /*heatMapMode*/ = 0) {
@@ -236,6 +277,27 @@ class Renderer {
   mdwt_renderer* handle() { return h_; }
 
  private:
+  void PushLight() {
+    check(mdwt_set_light(h_, lightDir.data(), enableShadows ? 1 : 0, enableGradientShading ? 1 : 0), h_);
+    pushedDir_ = lightDir;
+    pushedShadows_ = enableShadows;
+    pushedGradient_ = enableGradientShading;
+  }
+  void PushMaterial(size_t c) {
+    const MaterialData& m = materials[c];
+    check(mdwt_set_material(h_, int(c), m.diffuse, m.specular, m.shininess, m.gradientStep,
+                            m.specularColor.data()),
+          h_);
+  }
+  template <class F>
+  void SetMaterial(size_t ch, F&& f) {
+    if (ch >= materials.size()) return;
+    f(materials[ch]);
+    if (inited_) PushMaterial(ch);
+    ResetAccumulation();
+  }
+  std::array<float, 3> pushedDir_{0.0001f, -1.f, 0.0001f};
+  bool pushedShadows_ = false, pushedGradient_ = false;
   mdwt_renderer* h_ = nullptr;
   std::vector<std::shared_ptr<RawR>> raws_;
   bool inited_ = false;

@@ -46,6 +46,9 @@ SIGNATURES = {
     "mdwt_reset_dt": [_h],
     "mdwt_set_frame": [_h, C.c_int, C.c_int, C.c_int, C.c_int, _fp],
     "mdwt_set_lighting": [_h, C.c_float, C.c_float],
+    "mdwt_set_light": [_h, _fp, C.c_int, C.c_int],
+    "mdwt_set_material": [_h, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _fp],
+    "mdwt_material_preset": [C.c_char_p, _fp],
     "mdwt_set_seed": [_h, C.c_uint32],
     "mdwt_render": [_h, C.c_int, _fp],
     "mdwt_reset_accumulation": [_h],
@@ -102,6 +105,16 @@ def _i(a) -> C.POINTER(C.c_int32):
 
 def _vec3(v) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(3))
+
+
+def material_preset(name: str) -> dict:
+    """ApplyMaterialPreset's values (renderer.cpp:1250-1301) for `name`."""
+    out = np.zeros(7, np.float32)
+    rc = lib.mdwt_material_preset(name.encode(), _f(out))
+    if rc:
+        raise MdwtError(rc, "material_preset")
+    return {"diffuse": float(out[0]), "specular": float(out[1]), "shininess": float(out[2]),
+            "gradient_step": float(out[3]), "spec": tuple(float(x) for x in out[4:7])}
 
 
 def hilbert_d2xy(bits: int, d: int) -> tuple[int, int]:
@@ -197,6 +210,15 @@ class MdwtRenderer:
 
     def set_lighting(self, light_intensity: float, ambient: float) -> None:
         self._chk(lib.mdwt_set_lighting(self._h, light_intensity, ambient))
+
+    def set_light(self, direction, shadows: bool = False, gradient: bool = False) -> None:
+        self._chk(lib.mdwt_set_light(self._h, _f(_vec3(direction)), int(shadows), int(gradient)))
+
+    def set_material(self, ch: int, diffuse: float = 0.8, specular: float = 0.3,
+                     shininess: float = 8.0, gradient_step: float = 0.01,
+                     spec=(1.0, 1.0, 1.0)) -> None:
+        self._chk(lib.mdwt_set_material(self._h, ch, diffuse, specular, shininess, gradient_step,
+                                        _f(_vec3(spec))))
 
     def set_seed(self, seed: int) -> None:
         self._chk(lib.mdwt_set_seed(self._h, seed & 0xFFFFFFFF))

@@ -42,6 +42,7 @@ struct Lane {
   uint32_t mask;
   float m0, m1, m2, m3;  // majorants of the current cell (at most 4 channels)
   bool hit;
+  int hc;  // channel of the hit
   float hr, hg, hb;
 };
 
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
                 HitRec h;
                 X.tf_rgb(r2, L.c, h);
                 L.hit = true;
+                L.hc = L.c;
                 L.hr = h.r;
                 L.hg = h.g;
                 L.hb = h.b;
@@ -227,6 +229,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
                 HitRec h;
                 X.tf_rgb(r2, L.c, h);
                 L.hit = true;
+                L.hc = L.c;
                 L.hr = h.r;
                 L.hg = h.g;
                 L.hb = h.b;
@@ -260,6 +263,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
                 HitRec h;
                 X.tf_rgb(r2, c, h);
                 L.hit = true;
+                L.hc = c;
                 L.hr = h.r;
                 L.hg = h.g;
                 L.hb = h.b;
@@ -293,7 +297,8 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
       if (st == S_FINISH) {
         // a Woodcock hit is opaque, so each sample adds exactly 1 to alpha
         float fa_unused = 0.f;
-        composite_sample(P, L.hit, L.hr, L.hg, L.hb, 1.f, L.ar, L.ag, L.ab, fa_unused);
+        composite_sample(P, L.hit, L.hr, L.hg, L.hb, 1.f, 1.f, 0.f, 1.f, L.hc, L.ar, L.ag, L.ab,
+                         fa_unused);
         L.s += 1;
         if (L.s < spp) {
           st = S_START;
@@ -306,6 +311,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
       } else {
         L.k = 0;
         L.hit = false;
+        L.hc = 0;
         cnt.rays++;
         uint32_t o[4];
         philox10(frame, uint32_t(L.s), 0u, 0u, seed, L.pixel, o);

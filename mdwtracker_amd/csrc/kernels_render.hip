@@ -383,13 +383,15 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
 // entry; the sample position trails t by one step and starts at the ray
 // origin (the reference moves pos after sampling).
 template <int MODE, int DT>
-__device__ void march_blend(const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u, float prd_t0, float prd_t1,
-                            HitRec& h, Counts& cnt) {
+__device__ void march_blend(const RenderParams& P, const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u,
+                            float prd_t0, float prd_t1, HitRec& h, Counts& cnt) {
   const float dt = X.dt;
   const V3 dir = normalize3(dir0);
   V3 pos = org;
   float alpha = 0.0f;
   float cr = 0.f, cg = 0.f, cb = 0.f;
+  const bool shadows = P.fr.sh.shadows != 0;
+  float hiC = 0.0f, hiT = 0.0f;
   for (float t = prd_t0; (t < prd_t1) && (alpha < 0.99f); t += dt) {
     const V3 posTex = pos * w2u;
     const float4 b = blend_channels<MODE, DT>(X, posTex);
@@ -399,12 +401,44 @@ __device__ void march_blend(const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u, float prd
     cg = cg + b.y * b.w * k;
     cb = cb + b.z * b.w * k;
     alpha = alpha + (1.f - alpha) * b.w;
+    if (shadows && hiC < b.w) {
+      hiC = b.w;
+      hiT = t;
+    }
     pos = org + dir * t;
   }
+  // deviceCode.cu:1117-1154: one marched shadow ray from the most opaque
+  // sample towards the light; its first sample sits one world unit from that
+  // point (posSh = orgSh + dirSh), as in the reference.
+  float sh = 1.0f;
+  if (shadows && hiC > 0.0f) {
+    const V3 orgSh = org + dir * hiT;
+    const V3 dirSh = v3(-P.fr.sh.light_dir[0], -P.fr.sh.light_dir[1], -P.fr.sh.light_dir[2]);
+    float alphaSh = 0.f;
+    V3 posSh = orgSh + dirSh;
+    const V3 df = v3(1.f / dirSh.x, 1.f / dirSh.y, 1.f / dirSh.z);
+    const float t1 = (P.vol.blo[0] - orgSh.x) * df.x, t2 = (P.vol.bhi[0] - orgSh.x) * df.x;
+    const float t3 = (P.vol.blo[1] - orgSh.y) * df.y, t4 = (P.vol.bhi[1] - orgSh.y) * df.y;
+    const float t5 = (P.vol.blo[2] - orgSh.z) * df.z, t6 = (P.vol.bhi[2] - orgSh.z) * df.z;
+    float thit0 = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+    const float thit1 = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+    if ((thit1 >= 0) && (thit0 < thit1)) {
+      thit0 = fmaxf(thit0, 0.00001f);
+      if (sqrtf(dot3(dirSh, dirSh)) < 1e-5f) thit0 = thit1;
+      for (float tSh = thit0; (tSh < thit1) && (alphaSh < 0.99f); tSh += dt) {
+        const V3 posTexSh = posSh * w2u;
+        const float opacitySh = blend_channels<MODE, DT>(X, posTexSh).w;
+        alphaSh = alphaSh + (1.f - alphaSh) * opacitySh;
+        posSh = orgSh + dirSh * tSh;
+      }
+      cnt.samples += uint32_t(X.C);
+    }
+    sh = (1.f - P.fr.ambient) * (1.f - alphaSh) + P.fr.ambient;
+  }
   h.missed = false;
-  h.r = cr;
-  h.g = cg;
-  h.b = cb;
+  h.r = cr * sh;
+  h.g = cg * sh;
+  h.b = cb * sh;
   h.a = alpha;
 }
 
@@ -465,6 +499,53 @@ __device__ void march_comp(const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u, float prd_
   h.a = asum / float(X.C);
 }
 
+// The mode's closest-hit program for the Woodcock modes (0-5).
+template <int MODE, int DT>
+__device__ __forceinline__ void trace_mode(const Ctx<DT>& X, const GridRay& g, float t1, Rng& rng,
+                                           HitRec& h, Counts& cnt) {
+  if (MODE == 0)
+    trace_ndda<DT>(X, g, t1, rng, h, cnt);
+  else if (MODE == 1)
+    trace_multi<DT>(X, g, t1, rng, h, cnt);
+  else if (MODE == 2)
+    trace_sync<DT>(X, g, t1, rng, h, cnt);
+  else if (MODE == 3)
+    trace_cumulative<DT>(X, g, t1, rng, h, cnt);
+  else
+    trace_blend<MODE, DT>(X, g, t1, rng, h, cnt);
+}
+
+// Gradient (Blinn-Phong) shading of a Woodcock hit, deviceCode.cu:326-381,
+// with the hit channel's material. pow is exp(y log x) (ppowf).
+template <int DT>
+__device__ void shade_gradient(const RenderParams& P, const Ctx<DT>& X, V3 org, V3 dir,
+                               const HitRec& h, float& diffuse, float& specular) {
+  if (!(sqrtf(h.r * h.r + h.g * h.g + h.b * h.b) > 0.001f)) return;
+  const int m = h.channel;
+  const V3 blo = load3(P.vol.blo), bhi = load3(P.vol.bhi);
+  const V3 w2u = v3(1.f / (bhi.x - blo.x), 1.f / (bhi.y - blo.y), 1.f / (bhi.z - blo.z));
+  const V3 tex = (org + h.tHit * dir) * w2u;
+  const V3 grad = X.gradient(tex, P.fr.sh.grad_step[m], m);
+  if (sqrtf(dot3(grad, grad)) > 1e-6f) {
+    const V3 N = normalize3(v3(-grad.x, -grad.y, -grad.z));
+    const V3 L = normalize3(load3(P.fr.sh.light_dir));
+    const V3 V = normalize3(v3(-dir.x, -dir.y, -dir.z));
+    const V3 Hh = normalize3(L + V);
+    const float NdotL = fmaxf(dot3(N, L), 0.0f);
+    const float NdotH = fmaxf(dot3(N, Hh), 0.0f);
+    diffuse = P.fr.sh.diffuse[m] * NdotL;
+    if (NdotL > 0.0f && NdotH > 0.001f) {
+      const float normalization = (P.fr.sh.shininess[m] + 8.0f) / (8.0f * 3.14159265f);
+      specular = P.fr.sh.specular[m] * normalization * ppowf(NdotH, P.fr.sh.shininess[m]);
+    } else {
+      specular = 0.0f;
+    }
+  } else {
+    diffuse = P.fr.sh.diffuse[m];
+    specular = 0.0f;
+  }
+}
+
 template <int MODE, int DT, bool COUNT>
 __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
   extern __shared__ float4 s_tf[];
@@ -497,26 +578,47 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
       HitRec h;
       h.missed = true;
       h.a = 1.f;
-      float t0, t1;
-      if (clip_root(org, dir, blo, bhi, t0, t1)) {
-        const GridRay g = make_grid_ray(org, dir, blo, bhi, X.mx, X.my, X.mz);
-        if (MODE == 0)
-          trace_ndda<DT>(X, g, t1, rng, h, cnt);
-        else if (MODE == 1)
-          trace_multi<DT>(X, g, t1, rng, h, cnt);
-        else if (MODE == 2)
-          trace_sync<DT>(X, g, t1, rng, h, cnt);
-        else if (MODE == 3)
-          trace_cumulative<DT>(X, g, t1, rng, h, cnt);
-        else if (MODE <= 5)
-          trace_blend<MODE, DT>(X, g, t1, rng, h, cnt);
-        else if (MODE <= 8)
-          march_blend<MODE, DT>(X, org, dir, g.worldToUnit, t0, t1, h, cnt);
-        else
-          march_comp<DT>(X, org, dir, g.worldToUnit, t0, t1, h, cnt);
+      h.channel = 0;
+      float diffuse = 1.0f, specular = 0.0f, shadow = 1.0f;
+      if (MODE <= 5) {
+        // primary ray, then (when shadows are on and it hit) one shadow ray
+        // towards the light through the same tracking code, own RNG stream
+        // (deviceCode.cu:321, 383-403)
+        V3 ro = org, rd = dir;
+        for (int pass = 0; pass < 2; ++pass) {
+          HitRec hp;
+          hp.missed = true;
+          hp.a = 1.f;
+          hp.channel = 0;
+          float t0, t1;
+          if (clip_root(ro, rd, blo, bhi, t0, t1)) {
+            const GridRay g = make_grid_ray(ro, rd, blo, bhi, X.mx, X.my, X.mz);
+            trace_mode<MODE, DT>(X, g, t1, rng, hp, cnt);
+          }
+          if (pass == 1) {
+            shadow = (1.f - P.fr.ambient) * (1.f - (hp.missed ? 0.0f : 1.0f));
+            break;
+          }
+          h = hp;
+          if (h.missed || !P.fr.sh.shadows) break;
+          ro = org + h.tHit * dir;
+          rd = v3(-P.fr.sh.light_dir[0], -P.fr.sh.light_dir[1], -P.fr.sh.light_dir[2]);
+          rng.init(P.fr.seed, pixel, P.fr.frame_id, uint32_t(s), 1u);
+        }
+        if (!h.missed && P.fr.sh.gradient) shade_gradient(P, X, org, dir, h, diffuse, specular);
+      } else {
+        float t0, t1;
+        if (clip_root(org, dir, blo, bhi, t0, t1)) {
+          const V3 w2u = v3(1.f / (bhi.x - blo.x), 1.f / (bhi.y - blo.y), 1.f / (bhi.z - blo.z));
+          if (MODE <= 8)
+            march_blend<MODE, DT>(P, X, org, dir, w2u, t0, t1, h, cnt);
+          else
+            march_comp<DT>(X, org, dir, w2u, t0, t1, h, cnt);
+        }
       }
       cnt.rays++;
-      composite_sample(P, !h.missed, h.r, h.g, h.b, h.a, fr, fg, fb, fa);
+      composite_sample(P, !h.missed, h.r, h.g, h.b, h.a, diffuse, specular, shadow, h.channel, fr,
+                       fg, fb, fa);
     }
     if (!COUNT) finish_pixel(P, pixel, fr, fg, fb, fa);
   }

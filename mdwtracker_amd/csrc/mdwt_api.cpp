@@ -40,6 +40,23 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+// renderer.h:121-142 defaults: shadows and gradient shading off, light
+// from above, MaterialData per channel
+inline mdwt::ShadeParams default_shade() {
+  mdwt::ShadeParams s{};
+  s.light_dir[0] = 0.0001f;
+  s.light_dir[1] = -1.f;
+  s.light_dir[2] = 0.0001f;
+  for (int c = 0; c < mdwt::kMaxChannels; ++c) {
+    s.diffuse[c] = 0.8f;
+    s.specular[c] = 0.3f;
+    s.shininess[c] = 8.0f;
+    s.grad_step[c] = 0.01f;
+    s.spec[c][0] = s.spec[c][1] = s.spec[c][2] = 1.f;
+  }
+  return s;
+}
+
 struct mdwt_renderer {
   int device = 0;
   std::string err;
@@ -76,7 +93,7 @@ struct mdwt_renderer {
   bool accumulate = true;
   float bg[3] = {0.f, 0.f, 0.f};
   float light = 1.0f, ambient = 0.1f;   // renderer.h:124-126
-  float spec[3] = {1.f, 1.f, 1.f};      // renderer.h:159 material specularColor
+  mdwt::ShadeParams sh = default_shade();
   uint32_t seed = 0x5eed1234u;
   uint32_t frame_id = 0, accum_id = 0;
   float dt = 0.05f;
@@ -442,7 +459,9 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
 }
 
 // The flat schedules cover -m 0/2/3; every other mode runs the thread-per-pixel kernel.
+// Shadow rays and gradient shading run in the thread-per-pixel kernel only.
 int eff_schedule(const mdwt_renderer* h) {
+  if (h->sh.shadows || h->sh.gradient) return 0;
   return (h->mode == 0 || h->mode == 2 || h->mode == 3) ? h->schedule : 0;
 }
 
@@ -458,10 +477,8 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.height = h->H;
   p.fr.spp = h->spp;
   p.fr.accumulate = h->accumulate ? 1 : 0;
-  for (int i = 0; i < 3; ++i) {
-    p.fr.bg[i] = h->bg[i];
-    p.fr.spec[i] = h->spec[i];
-  }
+  for (int i = 0; i < 3; ++i) p.fr.bg[i] = h->bg[i];
+  p.fr.sh = h->sh;
   p.fr.light_intensity = h->light;
   p.fr.ambient = h->ambient;
   p.fr.seed = h->seed;
@@ -863,6 +880,57 @@ int mdwt_set_lighting(mdwt_renderer* h, float light_intensity, float ambient) {
   h->light = light_intensity;
   h->ambient = ambient;
   reset_acc(h);
+  return MDWT_OK;
+}
+
+int mdwt_set_light(mdwt_renderer* h, const float light_dir[3], int shadows, int gradient_shading) {
+  if (!h || !light_dir) return MDWT_ERR_ARG;
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(light_dir[i])) return fail(h, MDWT_ERR_ARG, "non-finite light direction");
+  for (int i = 0; i < 3; ++i) h->sh.light_dir[i] = light_dir[i];
+  h->sh.shadows = shadows ? 1 : 0;
+  h->sh.gradient = gradient_shading ? 1 : 0;
+  reset_acc(h);
+  return MDWT_OK;
+}
+
+int mdwt_set_material(mdwt_renderer* h, int channel, float diffuse, float specular,
+                      float shininess, float gradient_step, const float specular_color[3]) {
+  if (!h || !specular_color) return MDWT_ERR_ARG;
+  if (channel < 0 || channel >= mdwt::kMaxChannels)
+    return fail(h, MDWT_ERR_ARG, "channel out of range");
+  const float v[7] = {diffuse, specular, shininess, gradient_step, specular_color[0],
+                      specular_color[1], specular_color[2]};
+  for (float x : v)
+    if (!std::isfinite(x)) return fail(h, MDWT_ERR_ARG, "non-finite material value");
+  if (!(gradient_step > 0.f)) return fail(h, MDWT_ERR_ARG, "gradient step must be positive");
+  h->sh.diffuse[channel] = diffuse;
+  h->sh.specular[channel] = specular;
+  h->sh.shininess[channel] = shininess;
+  h->sh.grad_step[channel] = gradient_step;
+  for (int i = 0; i < 3; ++i) h->sh.spec[channel][i] = specular_color[i];
+  reset_acc(h);
+  return MDWT_OK;
+}
+
+int mdwt_material_preset(const char* name, float out[7]) {
+  if (!name || !out) return MDWT_ERR_ARG;
+  // renderer.cpp:1250-1301 ApplyMaterialPreset (unknown names: the default)
+  struct Preset {
+    const char* name;
+    float v[7];
+  };
+  static const Preset kPresets[] = {
+      {"Plastic", {0.1f, 0.50f, 32.0f, 0.01f, 0.5f, 0.5f, 0.5f}},
+      {"Iron", {0.51f, 0.51f, 51.0f, 0.01f, 0.51f, 0.51f, 0.51f}},
+      {"Chrome", {0.4f, 0.77f, 77.0f, 0.01f, 0.77f, 0.77f, 0.77f}},
+      {"Emerald", {0.255f, 0.66f, 77.0f, 0.005f, 0.63f, 0.73f, 0.63f}},
+  };
+  static const float kDefault[7] = {0.8f, 0.3f, 8.0f, 0.01f, 1.0f, 1.0f, 1.0f};
+  const float* v = kDefault;
+  for (const Preset& p : kPresets)
+    if (std::strcmp(p.name, name) == 0) v = p.v;
+  for (int i = 0; i < 7; ++i) out[i] = v[i];
   return MDWT_OK;
 }
 

@@ -121,6 +121,23 @@ __device__ __forceinline__ float plogf(float x) {
   return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
 }
 
+// fp32 exp: x = n ln2 + r (Cody-Waite, the ln2 split of plogf), e^r by its
+// Taylor polynomial to r^6 (|r| <= ln2/2), times 2^n from exponent bits.
+// 0 below -87.3 (no denormals), +inf above 88. Same operations as the host.
+__device__ __forceinline__ float pexpf(float x) {
+  if (!(x > -87.3f)) return 0.0f;
+  if (x > 88.0f) return __int_as_float(0x7f800000);
+  const float n = floorf(x * 1.44269504f + 0.5f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.42860677e-06f;
+  const float p =
+      1.0f + r * (1.0f + r * (0.5f + r * (0.16666667f + r * (0.041666668f + r * (0.008333334f +
+                                                                           r * 0.0013888889f)))));
+  const int ni = int(n);
+  return p * __uint_as_float(uint32_t(ni + 127) << 23);
+}
+// pow(x, y) for x > 0 (Blinn-Phong lobe) as exp(y log x)
+__device__ __forceinline__ float ppowf(float x, float y) { return pexpf(y * plogf(x)); }
+
 // --------------------------------------------------------- index clamps ----
 __device__ __forceinline__ int clamp_index(float fl, int n) {
   const float c = fminf(fmaxf(fl, -1.0f), float(n));

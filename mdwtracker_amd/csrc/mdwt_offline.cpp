@@ -10,8 +10,7 @@
 //   optional lines "opacity <scale>" and "range <lo> <hi>", then one "r g b a" line per
 //   texel (alpha in [0,1]; it is cubed on upload as in SetXFColormap).
 // Without -t every channel gets a white ramp with alpha = value.
-// Lighting (-l): shadows and gradient shading are not part of this path; the flags must be
-// 0 0 (light direction, intensity and ambient are honoured).
+// Lighting (-l shadows gradient dx dy dz [intensity [ambient]]) as viewer.cpp:271-281.
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -210,9 +209,10 @@ int main(int argc, char** argv) {
     if (has("-acc")) r.enableAccumulation = u("-acc", 0) != 0;
     if (has("-l")) {
       auto& l = args["-l"];
-      if (l.size() < 5) return usage("-l takes 5 to 7 values");
-      if (std::stof(l[0]) != 0.f || std::stof(l[1]) != 0.f)
-        return usage("shadows / gradient shading are not part of this renderer (use -l 0 0 ...)");
+      if (l.size() < 5 || l.size() > 7) return usage("-l takes 5 to 7 values");
+      r.enableShadows = std::stof(l[0]) != 0.f;
+      r.enableGradientShading = std::stof(l[1]) != 0.f;
+      r.lightDir = {std::stof(l[2]), std::stof(l[3]), std::stof(l[4])};
       if (l.size() > 5) r.lightIntensity = std::stof(l[5]);
       if (l.size() > 6) r.ambient = std::stof(l[6]);
     }
@@ -244,7 +244,7 @@ int main(int argc, char** argv) {
       if (unsigned(r.frameID) <= wu) r.ResetAccumulation();
       r.Render();
       if (unsigned(r.frameID) > wu && nth != 0 && (r.frameID - wu) % nth == 0 && !out.empty())
-        write_png(out + tag + std::to_string(r.frameID - wu) + "_shadows_off_frames.png", r.FrameBuffer(),
+        write_png(out + tag + std::to_string(r.frameID - wu) + std::string("_shadows_") + (r.enableShadows ? "on" : "off") + "_frames.png", r.FrameBuffer(),
                   r.fbSize[0], r.fbSize[1]);
     }
     std::printf("============ Scene Info ============\n");
@@ -255,7 +255,7 @@ int main(int argc, char** argv) {
     std::printf("avg. fps: %.3f (%0.4f sec)\n", 1.0f / r.avgTime, r.avgTime);
     std::printf("best. fps: %.3f (%0.4f sec)\n", 1.0f / r.minTime, r.minTime);
     if (!out.empty())
-      write_png(out + tag + std::to_string(r.frameID - int(wu)) + "_shadows_off_frames.png", r.FrameBuffer(),
+      write_png(out + tag + std::to_string(r.frameID - int(wu)) + std::string("_shadows_") + (r.enableShadows ? "on" : "off") + "_frames.png", r.FrameBuffer(),
                 r.fbSize[0], r.fbSize[1]);
   } catch (const mdwt::Error& e) {
     std::fprintf(stderr, "error (%d): %s\n", e.status, e.what());

@@ -38,11 +38,21 @@ struct TfParams {
   float dom_lo[kMaxChannels], dom_hi[kMaxChannels];
 };
 
+// Lighting and per-channel materials (LaunchParams lightDir, enableShadows,
+// enableGradientShading, material[], deviceCode.h:72-79; renderer.h:121-142).
+struct ShadeParams {
+  float light_dir[3];
+  int shadows, gradient;
+  float diffuse[kMaxChannels], specular[kMaxChannels];
+  float shininess[kMaxChannels], grad_step[kMaxChannels];
+  float spec[kMaxChannels][3];
+};
+
 struct FrameParams {
   int width, height, spp, accumulate;
   float bg[3];
   float light_intensity, ambient;
-  float spec[3];
+  ShadeParams sh;
   uint32_t seed, frame_id, accum_id;
   void* accum;  // float4 per pixel
   void* m2;     // float4 per pixel

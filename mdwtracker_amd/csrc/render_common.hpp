@@ -111,6 +111,35 @@ struct Ctx {
     v.w = v.w * t.os;
     return v;
   }
+  // calculateGradient (deviceCode.cu:243-281): central differences of the
+  // channel's TF opacity, one-sided at the faces of the unit cube
+  __device__ V3 gradient(V3 p, float eps, int c) const {
+    const float pc[3] = {p.x, p.y, p.z};
+    float g[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      float p1[3] = {pc[0], pc[1], pc[2]}, p2[3] = {pc[0], pc[1], pc[2]};
+      p1[i] -= eps;
+      p2[i] += eps;
+      float div;
+      if (p1[i] < 0.0f) {
+        p1[i] = pc[i];
+        p2[i] = pc[i] + eps;
+        div = eps;
+      } else if (p2[i] > 1.0f) {
+        p1[i] = pc[i] - eps;
+        p2[i] = pc[i];
+        div = eps;
+      } else {
+        div = 2.0f * eps;
+      }
+      float r;
+      const float v1 = tf_alpha(sample(v3(p1[0], p1[1], p1[2]), c), c, r);
+      const float v2 = tf_alpha(sample(v3(p2[0], p2[1], p2[2]), c), c, r);
+      g[i] = (v2 - v1) / div;
+    }
+    return v3(g[0], g[1], g[2]);
+  }
   __device__ __forceinline__ void tf_rgb(float r2, int c, HitRec& h) const {
     const ChanTf t = chtf[c];
     if (r2 != r2) {  // outside the value domain: (1, 0, 1, 0)
@@ -198,19 +227,24 @@ __device__ __forceinline__ void accumulate_pixel(const RenderParams& P, uint32_t
 
 // shading with diffuse 1, specular 0, shadow 1 (deviceCode.cu:326-433) and
 // over(colour, (bg, 1)) (deviceCode.cu:44-59) added to the pixel sum
+// Shading of one sample (deviceCode.cu:411-433): diffuse, specular and shadow
+// factors from the caller (1, 0, 1 without gradient shading and shadows),
+// the hit channel's material specular colour, then over the background.
 __device__ __forceinline__ void composite_sample(const RenderParams& P, bool hit, float hr,
-                                                 float hg, float hb, float ha, float& fr,
+                                                 float hg, float hb, float ha, float diffuse,
+                                                 float specular, float shadow, int ch, float& fr,
                                                  float& fg, float& fb, float& fa) {
   float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
   if (hit) {
-    const float li = 1.0f * 1.0f * P.fr.light_intensity;
-    const float sp = 0.0f * 1.0f * P.fr.light_intensity;
-    const float mr = hr * 0.75f + P.fr.spec[0] * 0.25f;
-    const float mg = hg * 0.75f + P.fr.spec[1] * 0.25f;
-    const float mb = hb * 0.75f + P.fr.spec[2] * 0.25f;
-    cr = li * hr + sp * mr + P.fr.ambient * mr;
-    cg = li * hg + sp * mg + P.fr.ambient * mg;
-    cb = li * hb + sp * mb + P.fr.ambient * mb;
+    const float ld = diffuse * shadow * P.fr.light_intensity;
+    const float ls = specular * shadow * P.fr.light_intensity;
+    const float* sc = P.fr.sh.spec[ch];
+    const float mr = hr * 0.75f + sc[0] * 0.25f;
+    const float mg = hg * 0.75f + sc[1] * 0.25f;
+    const float mb = hb * 0.75f + sc[2] * 0.25f;
+    cr = ld * hr + ls * mr + P.fr.ambient * mr;
+    cg = ld * hg + ls * mg + P.fr.ambient * mg;
+    cb = ld * hb + ls * mb + P.fr.ambient * mb;
     ca = ha;
   }
   // over(color, (bg, 1)) (deviceCode.cu:54-59, 433)

@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .capi import MAX_CHANNELS, MdwtError, MdwtRenderer
+from .capi import MAX_CHANNELS, MdwtError, MdwtRenderer, material_preset
 
 # rawFile.cpp:63-82 dtype names -> numpy dtype
 _RAW_DTYPES = {"int8": np.int8, "int16": np.int16, "int32": np.int32, "uint8": np.uint8,
@@ -103,6 +103,16 @@ class TFData:
     opacityScale: float = 1.0
 
 
+@dataclass
+class MaterialData:
+    """renderer.h:134-141 per-channel material."""
+    diffuse: float = 0.8
+    specular: float = 0.3
+    shininess: float = 8.0
+    gradientStep: float = 0.01
+    specularColor: tuple = (1.0, 1.0, 1.0)
+
+
 class Renderer:
     """dtracker::Renderer (RAW path) backed by libmdwt."""
 
@@ -117,6 +127,11 @@ class Renderer:
         self.bgColor = (0.0, 0.0, 0.0)   # renderer.h:128
         self.lightIntensity = 1.0        # renderer.h:125
         self.ambient = 0.1               # renderer.h:126
+        self.enableShadows = False       # renderer.h:121
+        self.enableGradientShading = False  # renderer.h:122
+        self.lightDir = (0.0001, -1.0, 0.0001)  # renderer.h:124
+        self.materials: list[MaterialData] = []  # renderer.h:133-142
+        self._pushed_light = None
         self.dt = 0.05                   # renderer.h:119
         self.mode = 0
         self.tfdatas: list[TFData] = []
@@ -132,6 +147,7 @@ class Renderer:
         if len(self.rawPtrs) >= MAX_CHANNELS:
             return False
         self.rawPtrs.append(raw)
+        self.materials.append(MaterialData())
         return True
 
     # renderer.cpp:324-942
@@ -150,6 +166,9 @@ class Renderer:
         self._r.set_frame(self.fbSize[0], self.fbSize[1], self.spp, self.enableAccumulation,
                           self.bgColor)
         self._r.set_lighting(self.lightIntensity, self.ambient)
+        self._push_light()
+        for c in range(len(self.materials)):
+            self._push_material(c)
         if self.camera.given:
             self.UpdateCamera()
         self.dt = self._r.get_dt()
@@ -206,6 +225,55 @@ class Renderer:
         self._r.set_lighting(self.lightIntensity, self.ambient)
         self.ResetAccumulation()
 
+    # renderer.cpp:1468-1474
+    def SetLightDirection(self, lightDir) -> None:
+        self.lightDir = tuple(float(x) for x in lightDir)
+        if self._inited:
+            self._push_light()
+        self.ResetAccumulation()
+
+    # renderer.cpp:1195-1248 (unknown channel ids are ignored, as there)
+    def SetMaterialDiffuse(self, v: float, channelID: int = 0) -> None:
+        self._set_material(channelID, diffuse=float(v))
+
+    def SetMaterialSpecular(self, v: float, channelID: int = 0) -> None:
+        self._set_material(channelID, specular=float(v))
+
+    def SetMaterialShininess(self, v: float, channelID: int = 0) -> None:
+        self._set_material(channelID, shininess=float(v))
+
+    def SetMaterialGradientStep(self, v: float, channelID: int = 0) -> None:
+        self._set_material(channelID, gradientStep=float(v))
+
+    def SetMaterialSpecularColor(self, v, channelID: int = 0) -> None:
+        self._set_material(channelID, specularColor=tuple(float(x) for x in v))
+
+    # renderer.cpp:1250-1301 (the table lives in the library)
+    def ApplyMaterialPreset(self, presetName: str, channelID: int = 0) -> None:
+        p = material_preset(presetName)
+        self._set_material(channelID, diffuse=p["diffuse"], specular=p["specular"],
+                           shininess=p["shininess"], gradientStep=p["gradient_step"],
+                           specularColor=p["spec"])
+
+    def _set_material(self, ch: int, **kw) -> None:
+        if ch >= len(self.materials):
+            return
+        for k, v in kw.items():
+            setattr(self.materials[ch], k, v)
+        if self._inited:
+            self._push_material(ch)
+        self.ResetAccumulation()
+
+    def _push_material(self, ch: int) -> None:
+        m = self.materials[ch]
+        self._r.set_material(ch, m.diffuse, m.specular, m.shininess, m.gradientStep,
+                             m.specularColor)
+
+    def _push_light(self) -> None:
+        cur = (tuple(self.lightDir), bool(self.enableShadows), bool(self.enableGradientShading))
+        self._r.set_light(cur[0], cur[1], cur[2])
+        self._pushed_light = cur
+
     # renderer.cpp:1046-1056
     def Resize(self, newSize) -> None:
         self.fbSize = (int(newSize[0]), int(newSize[1]))
@@ -216,6 +284,9 @@ class Renderer:
         if self._inited:
             self._r.set_frame(self.fbSize[0], self.fbSize[1], self.spp, self.enableAccumulation,
                               self.bgColor)
+            cur = (tuple(self.lightDir), bool(self.enableShadows), bool(self.enableGradientShading))
+            if cur != self._pushed_light:
+                self._push_light()
 
     # renderer.cpp:944-973: one frame, timed (avg over the accumulation, min)
     def Render(self, heatMapMode: int = 0) -> None:

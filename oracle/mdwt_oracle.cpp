@@ -144,6 +144,29 @@ inline float plogf(float x) {
   return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
 }
 
+// Portable fp32 exp: x = n ln2 + r (Cody-Waite, same ln2 split as plogf),
+// e^r by its Taylor polynomial to r^6 (|r| <= ln2/2, truncation < 2^-24
+// relative), scaled by 2^n built from the exponent bits. Flushes to 0 below
+// -87.3 (no denormal results) and saturates to +inf above 88 (so that 2^n
+// stays a normal number).
+inline float pexpf(float x) {
+  if (!(x > -87.3f)) return 0.0f;
+  if (x > 88.0f) return INFINITY;
+  const float n = floorf(x * 1.44269504f + 0.5f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.42860677e-06f;
+  const float p =
+      1.0f + r * (1.0f + r * (0.5f + r * (0.16666667f + r * (0.041666668f + r * (0.008333334f +
+                                                                           r * 0.0013888889f)))));
+  const int ni = int(n);
+  const uint32_t b = uint32_t(ni + 127) << 23;
+  float scale;
+  std::memcpy(&scale, &b, 4);
+  return p * scale;
+}
+// pow(x, y) for x > 0 as exp(y log x) with the portable pair above; replaces
+// pow() at deviceCode.cu:383 so that CPU and GPU round identically.
+inline float ppowf(float x, float y) { return pexpf(y * plogf(x)); }
+
 // ------------------------------------------------------------- textures ----
 // Clamp-to-edge linear lookup of a 1D float4 table at normalized u: texel
 // coordinate x = u*N - 0.5, lerp a + f*(b-a) (renderer.cpp:1140-1152 texture
@@ -220,6 +243,12 @@ struct Channel {
   float value_at(size_t i) const { return raw_at(i) * scale + offset; }
 };
 
+// renderer.h:134-141 MaterialData defaults
+struct Material {
+  float diffuse = 0.8f, specular = 0.3f, shininess = 8.0f, gradient_step = 0.01f;
+  vec3f spec{1.f, 1.f, 1.f};
+};
+
 struct Hit {
   bool missed = true;
   vec4f rgba{0, 0, 0, 0};
@@ -254,7 +283,9 @@ struct oracle_renderer {
   bool accumulate = true;
   vec3f bg{0, 0, 0};
   float light_intensity = 1.0f, ambient = 0.1f;   // renderer.h:124-126
-  vec3f spec_color{1.f, 1.f, 1.f};                 // renderer.h:159
+  vec3f light_dir{0.0001f, -1.f, 0.0001f};         // renderer.h:124
+  bool enable_shadows = false, enable_gradient = false;  // renderer.h:121-122
+  Material mat[kMaxChannels];                      // renderer.h:133-142
   uint32_t seed = 0x5eed1234u;
   uint32_t frame_id = 0, accum_id = 0;
   std::vector<vec4f> accum, m2;
@@ -895,16 +926,51 @@ void march_blend(const oracle_renderer* o, vec3f org, vec3f dir0, float prd_t0, 
   float alpha = 0.0f;
   vec3f color = {0.f, 0.f, 0.f};
   const int C = int(o->ch.size());
+  float highestContribution = 0.0f, highestContribDistance = 0.0f;
   for (float t = prd_t0; (t < prd_t1) && (alpha < 0.99f); t += dt) {
     const vec3f posTex = pos * worldToUnit;
     const vec4f b = blend_channels(o, posTex);
     prd.samples += uint64_t(C);
     color = over3(color, vec3f{b.x, b.y, b.z}, alpha, b.w);
     alpha = over1(alpha, b.w);
+    if (o->enable_shadows && highestContribution < b.w) {
+      highestContribution = b.w;
+      highestContribDistance = t;
+    }
     pos = org + dir * t;
   }
+  // deviceCode.cu:1117-1154: one marched shadow ray from the point of
+  // highest opacity towards the light. Its first sample sits one world unit
+  // away from that point (posSh = orgSh + dirSh), as in the reference.
+  vec3f shadow = {1.f, 1.f, 1.f};
+  if (o->enable_shadows && highestContribution > 0.0f) {
+    const vec3f orgSh = org + dir * highestContribDistance;
+    const vec3f dirSh = -o->light_dir;
+    float alphaSh = 0.f;
+    vec3f posSh = orgSh + dirSh;
+    const vec3f dirfrac = 1.f / dirSh;
+    const float t1 = (o->blo.x - orgSh.x) * dirfrac.x, t2 = (o->bhi.x - orgSh.x) * dirfrac.x;
+    const float t3 = (o->blo.y - orgSh.y) * dirfrac.y, t4 = (o->bhi.y - orgSh.y) * dirfrac.y;
+    const float t5 = (o->blo.z - orgSh.z) * dirfrac.z, t6 = (o->bhi.z - orgSh.z) * dirfrac.z;
+    float thit0 = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+    const float thit1 = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+    if ((thit1 >= 0) && (thit0 < thit1)) {
+      thit0 = fmaxf(thit0, 0.00001f);
+      if (length(dirSh) < 1e-5f) thit0 = thit1;
+      for (float tSh = thit0; (tSh < thit1) && (alphaSh < 0.99f); tSh += dt) {
+        const vec3f posTexSh = posSh * worldToUnit;
+        const float opacitySh = blend_channels(o, posTexSh).w;
+        alphaSh = over1(alphaSh, opacitySh);
+        posSh = orgSh + dirSh * tSh;
+      }
+      prd.samples += uint64_t(C);
+    }
+    const float sh = (1.f - o->ambient) * (1.f - alphaSh) + o->ambient;
+    shadow = {sh, sh, sh};
+  }
   prd.missed = false;
-  prd.rgba = {color.x, color.y, color.z, alpha};
+  const vec3f cs = color * shadow;
+  prd.rgba = {cs.x, cs.y, cs.z, alpha};
 }
 
 // deviceCode.cu:1159-1276 compRayMarcherCH (-m 9): each channel marched and
@@ -947,6 +1013,34 @@ void march_comp(const oracle_renderer* o, vec3f org, vec3f dir0, float prd_t0, f
   prd.rgba = {fin.x, fin.y, fin.z, alphaSum / float(C)};
 }
 
+// deviceCode.cu:243-281 calculateGradient: central differences of the
+// channel's TF opacity, one-sided at the faces of the unit cube.
+vec3f calc_gradient(const oracle_renderer* o, vec3f p, float eps, int c) {
+  float g[3];
+  const float pc[3] = {p.x, p.y, p.z};
+  for (int i = 0; i < 3; ++i) {
+    float p1[3] = {pc[0], pc[1], pc[2]}, p2[3] = {pc[0], pc[1], pc[2]};
+    p1[i] -= eps;
+    p2[i] += eps;
+    float div;
+    if (p1[i] < 0.0f) {
+      p1[i] = pc[i];
+      p2[i] = pc[i] + eps;
+      div = eps;
+    } else if (p2[i] > 1.0f) {
+      p1[i] = pc[i] - eps;
+      p2[i] = pc[i];
+      div = eps;
+    } else {
+      div = 2.0f * eps;
+    }
+    const float v1 = transfer_function(o, sample_volume(o, {p1[0], p1[1], p1[2]}, c), c).w;
+    const float v2 = transfer_function(o, sample_volume(o, {p2[0], p2[1], p2[2]}, c), c).w;
+    g[i] = (v2 - v1) / div;
+  }
+  return {g[0], g[1], g[2]};
+}
+
 // deviceCode.cu:61-76 generateRay
 inline void generate_ray(const oracle_renderer* o, float su, float sv, vec3f& org, vec3f& dir) {
   org = o->c_org;
@@ -986,8 +1080,25 @@ inline uint32_t make_rgba(vec4f c) {
   return make_8bit(c.x) | (make_8bit(c.y) << 8) | (make_8bit(c.z) << 16) | (make_8bit(c.w) << 24);
 }
 
-// deviceCode.cu:283-479 mainRG for one pixel and one frame (shadows and
-// gradient shading off, renderer.h:121-122; no surfaces, tMax = 1e20).
+// Root clip + the mode's closest-hit program for a Woodcock mode (0-5):
+// deviceCode.cu:321 (primary) and 394 (shadow ray) traceRay.
+bool trace_woodcock(const oracle_renderer* o, vec3f org, vec3f dir, Rng& rng, Hit& prd) {
+  float t0, t1;
+  if (!clip_root(o, org, dir, t0, t1)) return false;
+  const GridRay g = grid_ray(o, org, dir);
+  switch (o->mode) {
+    case 0: trace_ndda(o, g, t1, rng, prd); break;
+    case 1: trace_multi(o, g, t1, rng, prd); break;
+    case 2: trace_sync(o, g, t1, rng, prd); break;
+    case 3: trace_cumulative(o, g, t1, rng, prd); break;
+    default: trace_blend(o, g, t1, rng, prd); break;
+  }
+  return true;
+}
+
+// deviceCode.cu:283-479 mainRG for one pixel and one frame (no surfaces,
+// tMax = 1e20), with the gradient shading and shadow rays of 326-408 when
+// enabled.
 void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
   const size_t fbOfs = size_t(px) + size_t(o->W) * py;
   vec4f finalColor = {0.f, 0.f, 0.f, 1.f};
@@ -1000,35 +1111,61 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
     vec3f org, dir;
     generate_ray(o, su, sv, org, dir);
     vec4f color = {0.f, 0.f, 0.f, 0.f};
-    float t0, t1;
     Hit prd;
-    if (clip_root(o, org, dir, t0, t1)) {
-      const GridRay g = grid_ray(o, org, dir);
-      if (o->mode == 0)
-        trace_ndda(o, g, t1, rng, prd);
-      else if (o->mode == 1)
-        trace_multi(o, g, t1, rng, prd);
-      else if (o->mode == 2)
-        trace_sync(o, g, t1, rng, prd);
-      else if (o->mode == 3)
-        trace_cumulative(o, g, t1, rng, prd);
-      else if (o->mode <= 5)
-        trace_blend(o, g, t1, rng, prd);
-      else if (o->mode <= 8)
-        march_blend(o, org, dir, t0, t1, prd);
-      else
-        march_comp(o, org, dir, t0, t1, prd);
-      cnt[0] += prd.samples;
-      cnt[1] += prd.visits;
-      cnt[2] += prd.rejections;
+    if (o->mode <= 5) {
+      trace_woodcock(o, org, dir, rng, prd);
+    } else {
+      float t0, t1;
+      if (clip_root(o, org, dir, t0, t1)) {
+        if (o->mode <= 8)
+          march_blend(o, org, dir, t0, t1, prd);
+        else
+          march_comp(o, org, dir, t0, t1, prd);
+      }
     }
-    cnt[3] += 1;
     if (!prd.missed) {
-      // deviceCode.cu:326-433 with diffuse = 1, specular = 0, shadow = 1
+      // deviceCode.cu:322-433
       const vec3f albedo = {prd.rgba.x, prd.rgba.y, prd.rgba.z};
       const float transparency = prd.rgba.w;
-      const float diffuse = 1.0f, specular = 0.0f, shadow = 1.0f;
-      const vec3f sc = o->spec_color;
+      const Material& m = o->mat[prd.channel];
+      const bool calculateShading = o->enable_gradient && (o->mode < 6) && length(albedo) > 0.001f;
+      const bool calculateShadows = o->enable_shadows && (o->mode < 6);
+      float diffuse = 1.0f, specular = 0.0f, shadow = 1.0f;
+      if (calculateShading) {
+        const vec3f worldToUnit = 1.f / (o->bhi - o->blo);
+        const vec3f texSpacePos = (org + prd.tHit * dir) * worldToUnit;
+        const vec3f gradient = calc_gradient(o, texSpacePos, m.gradient_step, prd.channel);
+        if (length(gradient) > 1e-6f) {
+          const vec3f N = normalize(-gradient);
+          const vec3f L = normalize(o->light_dir);
+          const vec3f V = normalize(-dir);
+          const vec3f Hh = normalize(L + V);
+          const float NdotL = fmaxf(dot(N, L), 0.0f);
+          const float NdotH = fmaxf(dot(N, Hh), 0.0f);
+          diffuse = m.diffuse * NdotL;
+          if (NdotL > 0.0f && NdotH > 0.001f) {
+            const float normalization = (m.shininess + 8.0f) / (8.0f * 3.14159265f);
+            specular = m.specular * normalization * ppowf(NdotH, m.shininess);
+          } else {
+            specular = 0.0f;
+          }
+        } else {
+          diffuse = m.diffuse;
+          specular = 0.0f;
+        }
+      }
+      if (calculateShadows) {
+        // shadow ray from the hit towards the light, own RNG stream
+        Rng srng(o->seed, pixel, o->frame_id, uint32_t(sampleID), 1u);
+        Hit sp;
+        trace_woodcock(o, org + prd.tHit * dir, -o->light_dir, srng, sp);
+        const float sw = sp.missed ? 0.0f : sp.rgba.w;
+        shadow = (1.f - o->ambient) * (1.f - sw);
+        prd.samples += sp.samples;
+        prd.rejections += sp.rejections;
+        prd.visits += sp.visits;
+      }
+      const vec3f sc = m.spec;
       const vec3f mixedSpecularColor = albedo * 0.75f + sc * 0.25f;
       const vec3f ambientColor = albedo * 0.75f + sc * 0.25f;
       const vec3f diffuseContrib = diffuse * shadow * o->light_intensity * albedo;
@@ -1037,6 +1174,10 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
       const vec3f shaded = diffuseContrib + specularContrib + ambientContrib;
       color = {shaded.x, shaded.y, shaded.z, transparency};
     }
+    cnt[0] += prd.samples;
+    cnt[1] += prd.visits;
+    cnt[2] += prd.rejections;
+    cnt[3] += 1;
     // over(color, vec4f(bg, 1)) (deviceCode.cu:44-59, 433)
     const float Ain = color.w, Ax = 1.f;
     const vec3f Cin = {color.x, color.y, color.z};
@@ -1237,6 +1378,31 @@ int oracle_set_lighting(oracle_renderer* o, float light_intensity, float ambient
   return 0;
 }
 
+// SetLightDirection (renderer.cpp:1468-1474) and the enableShadows /
+// enableGradientShading fields (renderer.h:121-124, the -l flag)
+int oracle_set_light(oracle_renderer* o, const float dir[3], int shadows, int gradient) {
+  o->light_dir = {dir[0], dir[1], dir[2]};
+  o->enable_shadows = shadows != 0;
+  o->enable_gradient = gradient != 0;
+  o->accum_id = 0;
+  return 0;
+}
+
+// SetMaterialDiffuse/Specular/Shininess/GradientStep/SpecularColor
+// (renderer.cpp:1195-1248)
+int oracle_set_material(oracle_renderer* o, int ch, float diffuse, float specular,
+                        float shininess, float gradient_step, const float spec[3]) {
+  if (ch < 0 || ch >= kMaxChannels) return fail(o, "bad channel");
+  Material& m = o->mat[ch];
+  m.diffuse = diffuse;
+  m.specular = specular;
+  m.shininess = shininess;
+  m.gradient_step = gradient_step;
+  m.spec = {spec[0], spec[1], spec[2]};
+  o->accum_id = 0;
+  return 0;
+}
+
 // renderer.cpp:944-973 Render: frame k of an accumulation uses accumID = k
 int oracle_render(oracle_renderer* o, int n_frames, int y0, int y1, int nthreads) {
   if (!o->inited) return fail(o, "render before init");
@@ -1387,6 +1553,8 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
 }
 
 float oracle_logf(float x) { return plogf(x); }
+float oracle_expf(float x) { return pexpf(x); }
+float oracle_powf(float x, float y) { return ppowf(x, y); }
 
 float oracle_sample(oracle_renderer* o, int ch, const float p[3]) {
   return sample_volume(o, {p[0], p[1], p[2]}, ch);

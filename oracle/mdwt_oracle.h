@@ -52,6 +52,9 @@ int oracle_reset_dt(oracle_renderer* o);
 int oracle_set_frame(oracle_renderer* o, int w, int h, int spp, int accumulate, const float bg[3]);
 int oracle_set_seed(oracle_renderer* o, uint32_t seed);
 int oracle_set_lighting(oracle_renderer* o, float light_intensity, float ambient);
+int oracle_set_light(oracle_renderer* o, const float dir[3], int shadows, int gradient);
+int oracle_set_material(oracle_renderer* o, int ch, float diffuse, float specular,
+                        float shininess, float gradient_step, const float spec[3]);
 /* OpenMP threads for the range-grid build (result does not depend on it) */
 int oracle_set_threads(oracle_renderer* o, int n);
 /* Render n_frames frames; rows [y0, y1) only (y1 <= 0 means all rows).
@@ -86,6 +89,8 @@ int oracle_clip_rays(oracle_renderer* o, const float* org, const float* dir, int
 /* unit hooks */
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float oracle_logf(float x);
+float oracle_expf(float x);
+float oracle_powf(float x, float y);
 float oracle_sample(oracle_renderer* o, int ch, const float p[3]);
 void oracle_tf(oracle_renderer* o, int ch, float value, float out[4]);
 

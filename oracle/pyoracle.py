@@ -37,6 +37,8 @@ _SIG = {
     "oracle_set_frame": ([_h, C.c_int, C.c_int, C.c_int, C.c_int, _fp], C.c_int),
     "oracle_set_seed": ([_h, C.c_uint32], C.c_int),
     "oracle_set_lighting": ([_h, C.c_float, C.c_float], C.c_int),
+    "oracle_set_light": ([_h, _fp, C.c_int, C.c_int], C.c_int),
+    "oracle_set_material": ([_h, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _fp], C.c_int),
     "oracle_set_threads": ([_h, C.c_int], C.c_int),
     "oracle_render": ([_h, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "oracle_reset_accumulation": ([_h], C.c_int),
@@ -54,6 +56,8 @@ _SIG = {
     "oracle_clip_rays": ([_h, _fp, _fp, C.c_int, _fp, _ip], C.c_int),
     "oracle_philox4x32_10": ([_up, _up, _up], None),
     "oracle_logf": ([C.c_float], C.c_float),
+    "oracle_expf": ([C.c_float], C.c_float),
+    "oracle_powf": ([C.c_float, C.c_float], C.c_float),
     "oracle_sample": ([_h, C.c_int, _fp], C.c_float),
     "oracle_tf": ([_h, C.c_int, C.c_float, _fp], None),
 }
@@ -104,6 +108,14 @@ def philox4x32_10(ctr, key) -> np.ndarray:
 
 def logf(x: float) -> float:
     return lib.oracle_logf(float(x))
+
+
+def expf(x: float) -> float:
+    return lib.oracle_expf(float(x))
+
+
+def powf(x: float, y: float) -> float:
+    return lib.oracle_powf(float(x), float(y))
 
 
 class OracleRenderer:
@@ -164,6 +176,14 @@ class OracleRenderer:
 
     def set_lighting(self, light_intensity, ambient):
         self._chk(lib.oracle_set_lighting(self._h, light_intensity, ambient))
+
+    def set_light(self, direction, shadows=False, gradient=False):
+        self._chk(lib.oracle_set_light(self._h, _f(_vec3(direction)), int(shadows), int(gradient)))
+
+    def set_material(self, ch, diffuse=0.8, specular=0.3, shininess=8.0, gradient_step=0.01,
+                     spec=(1.0, 1.0, 1.0)):
+        self._chk(lib.oracle_set_material(self._h, ch, diffuse, specular, shininess, gradient_step,
+                                          _f(_vec3(spec))))
 
     def render(self, n_frames=1, rows=None, threads=1):
         y0, y1 = (0, 0) if rows is None else rows

@@ -112,3 +112,51 @@ def test_homogeneous_medium_full_size(capi, mode):
         (prob[inside] * (1 - prob[inside])).sum() / frames)
     print(f"[homogeneous] mode {mode}: {inside.sum()} pixels, z = {z:.2f}", flush=True)
     assert abs(z) < 6.0
+
+
+def test_renderer_mirror_lighting_matches_abi(tmp_path, capi):
+    """enableShadows / enableGradientShading / lightDir set as public fields
+    before Init, a material preset and a specular colour: the mirror pushes
+    the same state as direct ABI calls."""
+    from mdwtracker_amd.renderer import RawR, Renderer
+    sc = scenes.make_scene("ndda_cpu_2ch64", seed=21)
+    paths = []
+    for c, v in enumerate(sc.volumes):
+        p = tmp_path / f"ch{c}_64x64x64_uint8.raw"
+        v.tofile(p)
+        paths.append(p)
+    chrome = capi.material_preset("Chrome")
+    g = capi.MdwtRenderer(0)
+    for v in sc.volumes:
+        g.push_channel(v)
+    g.init(2, (8, 8, 8), True)
+    for c, t in enumerate(sc.tfs):
+        g.set_tf(c, t, 0.05, (0.0, 1.0))
+    g.set_frame(96, 80, 1, True, (0.0, 0.0, 0.0))
+    g.set_light((0.3, -1.0, 0.5), True, True)
+    g.set_material(0, chrome["diffuse"], chrome["specular"], chrome["shininess"],
+                   chrome["gradient_step"], chrome["spec"])
+    g.set_material(1, spec=(0.2, 0.9, 0.4))
+    g.render(4)
+    ref = g.read_accum()
+    r = Renderer(0)
+    r.macrocellDims = [8, 8, 8]
+    r.fbSize = (96, 80)
+    r.enableShadows = True
+    r.enableGradientShading = True
+    r.lightDir = (0.3, -1.0, 0.5)
+    for p in paths:
+        assert r.PushMesh(RawR(p))
+    r.Init(2)
+    for c, t in enumerate(sc.tfs):
+        r.SetXFColormap(t, c)
+        r.SetXFOpacityScale(0.05, c)
+        r.SetXFRange((0.0, 1.0), c)
+    r.ApplyMaterialPreset("Chrome", 0)
+    r.SetMaterialSpecularColor((0.2, 0.9, 0.4), 1)
+    for _ in range(4):
+        r.Render()
+    np.testing.assert_array_equal(r._r.read_accum(), ref)
+    assert chrome == {"diffuse": pytest.approx(0.4), "specular": pytest.approx(0.77),
+                      "shininess": 77.0, "gradient_step": pytest.approx(0.01),
+                      "spec": (pytest.approx(0.77),) * 3}

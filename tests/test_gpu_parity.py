@@ -243,3 +243,30 @@ def test_render_requires_every_transfer_function(capi):
     assert e.value.code == 2
     with pytest.raises(capi.MdwtError):
         g.set_tf(2, np.ones((8, 4), np.float32))  # channel id out of range
+
+
+@pytest.mark.parametrize("mode", list(range(10)))
+def test_lighting_bit_exact(capi, po, mode):
+    """Shadow rays, gradient shading and per-channel materials (-m 0-5), the
+    marchers' own shadow march (-m 6-8); -m 9 ignores both."""
+    sc = _scene("ndda_cpu_2ch64")
+    g, o = _pair(capi, po, sc, mode)
+    print(f"[lighting] ndda_cpu_2ch64 mode={mode}", flush=True)
+    for r in (g, o):
+        r.set_light((0.4, -1.0, 0.3), shadows=True, gradient=True)
+        r.set_material(1, 0.6, 0.5, 24.0, 0.02, (0.8, 0.9, 1.0))
+        r.set_lighting(1.2, 0.15)
+    cg = g.count_frame()
+    g.render(3)
+    o.render(3, threads=THREADS)
+    a_g, m_g = g.read_accum(with_m2=True)
+    a_o, m_o = o.read_accum(with_m2=True)
+    np.testing.assert_array_equal(_bits(a_g), _bits(a_o))
+    np.testing.assert_array_equal(_bits(m_g), _bits(m_o))
+    o1 = po.OracleRenderer()
+    scenes.load_scene(o1, sc, mode)
+    o1.set_light((0.4, -1.0, 0.3), shadows=True, gradient=True)
+    o1.set_material(1, 0.6, 0.5, 24.0, 0.02, (0.8, 0.9, 1.0))
+    o1.set_lighting(1.2, 0.15)
+    o1.render(1, threads=THREADS)
+    assert cg == o1.read_counters()

@@ -46,6 +46,6 @@ def test_offline_cli_argument_errors():
     r = subprocess.run([str(exe), "-fr", "a_4x4x4_uint8.raw", "-mc", "0"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2 and "positive" in r.stderr
-    r = subprocess.run([str(exe), "-fr", "a_4x4x4_uint8.raw", "-l", "1", "0", "0", "-1", "0"],
+    r = subprocess.run([str(exe), "-fr", "a_4x4x4_uint8.raw", "-l", "1", "0", "0"],
                        capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "shadows" in r.stderr
+    assert r.returncode == 2 and "-l takes 5 to 7 values" in r.stderr

@@ -382,7 +382,7 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
 // front-to-back compositing of blended samples every dt from the root-box
 // entry; the sample position trails t by one step and starts at the ray
 // origin (the reference moves pos after sampling).
-template <int MODE, int DT>
+template <int MODE, int DT, bool LIT>
 __device__ void march_blend(const RenderParams& P, const Ctx<DT>& X, V3 org, V3 dir0, V3 w2u,
                             float prd_t0, float prd_t1, HitRec& h, Counts& cnt) {
   const float dt = X.dt;
@@ -390,7 +390,7 @@ __device__ void march_blend(const RenderParams& P, const Ctx<DT>& X, V3 org, V3 
   V3 pos = org;
   float alpha = 0.0f;
   float cr = 0.f, cg = 0.f, cb = 0.f;
-  const bool shadows = P.fr.sh.shadows != 0;
+  const bool shadows = LIT && P.fr.sh.shadows != 0;
   float hiC = 0.0f, hiT = 0.0f;
   for (float t = prd_t0; (t < prd_t1) && (alpha < 0.99f); t += dt) {
     const V3 posTex = pos * w2u;
@@ -546,7 +546,10 @@ __device__ void shade_gradient(const RenderParams& P, const Ctx<DT>& X, V3 org, 
   }
 }
 
-template <int MODE, int DT, bool COUNT>
+// LIT: shadow rays / gradient shading compiled in. The unlit instances keep
+// the register budget of the plain method (the lit code needs ~15 more VGPRs,
+// which would cost a wave per SIMD).
+template <int MODE, int DT, bool COUNT, bool LIT>
 __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
   extern __shared__ float4 s_tf[];
   __shared__ ChanTf s_ch[kMaxChannels];
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
         // towards the light through the same tracking code, own RNG stream
         // (deviceCode.cu:321, 383-403)
         V3 ro = org, rd = dir;
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < (LIT ? 2 : 1); ++pass) {
           HitRec hp;
           hp.missed = true;
           hp.a = 1.f;
@@ -600,18 +603,18 @@ __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
             break;
           }
           h = hp;
-          if (h.missed || !P.fr.sh.shadows) break;
+          if (!LIT || h.missed || !P.fr.sh.shadows) break;
           ro = org + h.tHit * dir;
           rd = v3(-P.fr.sh.light_dir[0], -P.fr.sh.light_dir[1], -P.fr.sh.light_dir[2]);
           rng.init(P.fr.seed, pixel, P.fr.frame_id, uint32_t(s), 1u);
         }
-        if (!h.missed && P.fr.sh.gradient) shade_gradient(P, X, org, dir, h, diffuse, specular);
+        if (LIT && !h.missed && P.fr.sh.gradient) shade_gradient(P, X, org, dir, h, diffuse, specular);
       } else {
         float t0, t1;
         if (clip_root(org, dir, blo, bhi, t0, t1)) {
           const V3 w2u = v3(1.f / (bhi.x - blo.x), 1.f / (bhi.y - blo.y), 1.f / (bhi.z - blo.z));
           if (MODE <= 8)
-            march_blend<MODE, DT>(P, X, org, dir, w2u, t0, t1, h, cnt);
+            march_blend<MODE, DT, LIT>(P, X, org, dir, w2u, t0, t1, h, cnt);
           else
             march_comp<DT>(X, org, dir, w2u, t0, t1, h, cnt);
         }
@@ -711,23 +714,25 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ sr
   for (; i < n; i += stride) dst[i] = src[i];
 }
 
-template <int MODE, int DT, bool COUNT>
+template <int MODE, int DT, bool COUNT, bool LIT>
 RenderFn static_fn() {
-  return render_kernel<MODE, DT, COUNT>;
+  return render_kernel<MODE, DT, COUNT, LIT>;
 }
-
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool LIT>
 RenderFn static_fn_dt(int dtype) {
   switch (dtype) {
-    case MDWT_DT_U8: return static_fn<MODE, MDWT_DT_U8, COUNT>();
-    case MDWT_DT_U16: return static_fn<MODE, MDWT_DT_U16, COUNT>();
-    case MDWT_DT_F16: return static_fn<MODE, MDWT_DT_F16, COUNT>();
-    default: return static_fn<MODE, MDWT_DT_F32, COUNT>();
+    case MDWT_DT_U8: return static_fn<MODE, MDWT_DT_U8, COUNT, LIT>();
+    case MDWT_DT_U16: return static_fn<MODE, MDWT_DT_U16, COUNT, LIT>();
+    case MDWT_DT_F16: return static_fn<MODE, MDWT_DT_F16, COUNT, LIT>();
+    default: return static_fn<MODE, MDWT_DT_F32, COUNT, LIT>();
   }
 }
 
-#define MDWT_STATIC(M) (count ? static_fn_dt<M, true>(dtype) : static_fn_dt<M, false>(dtype))
-RenderFn select_render(int mode, int dtype, bool count, int schedule) {
+#define MDWT_STATIC(M)                                                                 \
+  (count ? (lit ? static_fn_dt<M, true, true>(dtype) : static_fn_dt<M, true, false>(dtype)) \
+         : (lit ? static_fn_dt<M, false, true>(dtype) : static_fn_dt<M, false, false>(dtype)))
+// -m 9 has no lighting (its shadow code is commented out in the reference).
+RenderFn select_render(int mode, int dtype, bool count, int schedule, bool lit) {
   switch (mode) {  // modes with the thread-per-pixel kernel only
     case 1: return MDWT_STATIC(1);
     case 4: return MDWT_STATIC(4);
@@ -735,10 +740,11 @@ RenderFn select_render(int mode, int dtype, bool count, int schedule) {
     case 6: return MDWT_STATIC(6);
     case 7: return MDWT_STATIC(7);
     case 8: return MDWT_STATIC(8);
-    case 9: return MDWT_STATIC(9);
+    case 9: return count ? static_fn_dt<9, true, false>(dtype) : static_fn_dt<9, false, false>(dtype);
     default: break;
   }
-  if (schedule == 1 || schedule == 2) return select_flat_kernel(mode, dtype, count, schedule == 1);
+  if (!lit && (schedule == 1 || schedule == 2))
+    return select_flat_kernel(mode, dtype, count, schedule == 1);
   if (mode == 0) return MDWT_STATIC(0);
   if (mode == 2) return MDWT_STATIC(2);
   return MDWT_STATIC(3);
@@ -752,7 +758,7 @@ size_t render_lds_bytes(const RenderParams& p) {
 }
 
 hipError_t persistent_blocks_per_cu(int mode, int dtype, bool count, size_t lds, int* out) {
-  const RenderFn f = select_render(mode, dtype, count, 1);
+  const RenderFn f = select_render(mode, dtype, count, 1, false);
   hipError_t e = hipSuccess;
   if (lds > 65536 &&
       (e = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
@@ -763,7 +769,8 @@ hipError_t persistent_blocks_per_cu(int mode, int dtype, bool count, size_t lds,
 
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s) {
   const size_t lds = render_lds_bytes(p);
-  const RenderFn f = select_render(mode, dtype, count, p.fr.schedule);
+  const bool lit = p.fr.sh.shadows != 0 || p.fr.sh.gradient != 0;
+  const RenderFn f = select_render(mode, dtype, count, p.fr.schedule, lit);
   if (lds > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));

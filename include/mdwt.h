@@ -171,12 +171,61 @@ int mdwt_clip_rays(mdwt_renderer* h, const float* org, const float* dir, int n, 
 int mdwt_count_frame(mdwt_renderer* h, uint64_t out[4]);
 /* Average duration in ms of the last mdwt_render's frame kernel launches. */
 int mdwt_last_kernel_ms(mdwt_renderer* h, float* ms);
-/* Block order of the frame kernel (16x16-pixel blocks; tile must be 16):
- * 0 = row-major, 1 = Hilbert curve (default), 2 = cost-sorted: starts as
- * Hilbert, records each wave's wall-clock time and, at the next mdwt_render,
- * issues the blocks heaviest first (one-thread-per-pixel schedule only).
- * Images do not depend on the order. */
+/* Block order of the frame kernel: 0 = row-major, 1 = Hilbert curve over
+ * tile x tile-pixel tiles (tile 8, 16 or 32; blocks row-major inside a tile),
+ * 2 = cost-sorted (default): starts as the Hilbert order, records each wave's
+ * wall-clock time and, at the next mdwt_render, issues the blocks heaviest
+ * first (one-thread-per-pixel schedule only). Images do not depend on the
+ * order. No reference counterpart (the reference sorts umesh elements along a
+ * Hilbert curve, renderer.cu:1364-1411; hilbert.h). */
 int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile);
+/* Workgroup of the thread-per-pixel frame kernel: 64 (8x8 pixels), 128 (16x8)
+ * or 256 (16x16, default) threads, one 8x8-pixel quad per wave. Images do not
+ * depend on it. No reference counterpart (OptiX picks the launch shape). */
+int mdwt_set_block(mdwt_renderer* h, int threads);
+/* Rebuilds the range and majorant grids for a new macrocell count per side
+ * (the -mc value, viewer.cpp:204-224) on the resident volume, without a new
+ * Init; resets accumulation (also of the batched views). */
+int mdwt_set_macrocells(mdwt_renderer* h, const int mc[3]);
+
+/* ---- batched views (no reference counterpart: the reference renders one
+ * camera per launch). Many views of uneven size and spp rendered by one
+ * persistent launch whose workgroups pull (view, pixel block) items from a
+ * device counter; the launch sequence is captured once in a HIP graph and
+ * replayed every frame. A view's image equals a single-view render with the
+ * same camera, size, spp and seed (the RNG is keyed by pixel), so views never
+ * depend on the batch they are in. */
+typedef struct mdwt_view {
+  float from[3], at[3], up[3]; /* camera.h setOrientation arguments */
+  float fovy;                  /* degrees */
+  int width, height, spp;
+  uint32_t seed;
+} mdwt_view;
+/* Replaces the batch (1..4096 views); each view gets its own accumulators. */
+int mdwt_batch_set_views(mdwt_renderer* h, const mdwt_view* views, int n);
+/* Renders n_frames frames of every view (ms = total GPU time). The first
+ * frame after mdwt_batch_set_views runs eagerly and times every item; later
+ * frames issue items heaviest first. */
+int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms);
+/* Counting pass over the batch: {samples, cell visits, rejections, rays}. */
+int mdwt_batch_count(mdwt_renderer* h, uint64_t out[4]);
+/* Restarts the views' accumulation (frame ids keep counting). */
+int mdwt_batch_reset(mdwt_renderer* h);
+/* width * height float4 running means (and second moments) of one view. */
+int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2);
+/* Per view: completion time in microseconds of its last item in the last
+ * batched frame, from the first item's start (100 MHz device clock). */
+int mdwt_batch_view_times(mdwt_renderer* h, float* us);
+/* Autotuner: times `frames` frames (after one warm-up frame) for every
+ * combination of macrocell size s (voxels per macrocell side; grid = dims /
+ * s), workgroup threads (64/128/256) and Hilbert tile (8/16/32), on the batch
+ * if one is set, else on the single view, with the tile order running (no
+ * cost sort). table gets 6 floats per combination: s, threads, tile,
+ * ms/frame, samples/s, cell visits/s. Leaves the fastest combination set
+ * (best = {s, threads, tile}) and accumulation reset. */
+int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* threads,
+                  int n_threads, const int* tiles, int n_tiles, int frames, float* table,
+                  int best[3]);
 /* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
  * loops, a workgroup per 16x16 block (default); 1 = flat step loop in
  * persistent waves that refill lanes from 8 interleaved pixel queues; 2 = flat

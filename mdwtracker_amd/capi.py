@@ -69,7 +69,23 @@ SIGNATURES = {
     "mdwt_set_schedule": [_h, C.c_int],
     "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_hilbert_d2xy": [C.c_int, C.c_uint32, _up, _up],
+    "mdwt_set_block": [_h, C.c_int],
+    "mdwt_set_macrocells": [_h, _ip],
+    "mdwt_batch_set_views": [_h, C.c_void_p, C.c_int],
+    "mdwt_batch_render": [_h, C.c_int, _fp],
+    "mdwt_batch_count": [_h, _u64p],
+    "mdwt_batch_reset": [_h],
+    "mdwt_batch_read_view": [_h, C.c_int, _fp, _fp],
+    "mdwt_batch_view_times": [_h, _fp],
+    "mdwt_autotune": [_h, _ip, C.c_int, _ip, C.c_int, _ip, C.c_int, C.c_int, _fp, _ip],
 }
+
+
+class View(C.Structure):
+    """mdwt_view: one camera of a batched launch."""
+    _fields_ = [("frm", C.c_float * 3), ("at", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float), ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int),
+                ("seed", C.c_uint32)]
 _RESTYPES = {"mdwt_last_error": C.c_char_p}
 
 
@@ -225,6 +241,67 @@ class MdwtRenderer:
 
     def set_tile_order(self, order: int, tile: int = 16) -> None:
         self._chk(lib.mdwt_set_tile_order(self._h, order, tile))
+
+    def set_block(self, threads: int) -> None:
+        """Workgroup of the frame kernel: 64 (8x8), 128 (16x8) or 256 (16x16) threads."""
+        self._chk(lib.mdwt_set_block(self._h, threads))
+
+    def set_macrocells(self, mc) -> None:
+        m = np.ascontiguousarray(np.asarray(mc, np.int32).reshape(3))
+        self._chk(lib.mdwt_set_macrocells(self._h, _i(m)))
+
+    # -- batched views --------------------------------------------------------
+    def batch_set_views(self, views) -> None:
+        """views: dicts with from, at, up, fovy, width, height, spp, seed."""
+        arr = (View * len(views))()
+        for i, v in enumerate(views):
+            arr[i].frm[:] = [float(x) for x in v["from"]]
+            arr[i].at[:] = [float(x) for x in v["at"]]
+            arr[i].up[:] = [float(x) for x in v.get("up", (0.0, 1.0, 0.0))]
+            arr[i].fovy = float(v.get("fovy", 45.0))
+            arr[i].width, arr[i].height = int(v["width"]), int(v["height"])
+            arr[i].spp = int(v.get("spp", 1))
+            arr[i].seed = int(v.get("seed", 0)) & 0xFFFFFFFF
+        self._views = [(int(v["width"]), int(v["height"])) for v in views]
+        self._chk(lib.mdwt_batch_set_views(self._h, C.cast(arr, C.c_void_p), len(views)))
+
+    def batch_render(self, n_frames: int = 1) -> float:
+        ms = C.c_float()
+        self._chk(lib.mdwt_batch_render(self._h, n_frames, C.byref(ms)))
+        return ms.value
+
+    def batch_count(self) -> dict:
+        o = np.zeros(4, np.uint64)
+        self._chk(lib.mdwt_batch_count(self._h, o.ctypes.data_as(_u64p)))
+        return {"samples": int(o[0]), "visits": int(o[1]), "rejections": int(o[2]),
+                "rays": int(o[3])}
+
+    def batch_reset(self) -> None:
+        self._chk(lib.mdwt_batch_reset(self._h))
+
+    def batch_read_view(self, i: int, with_m2: bool = False):
+        w, h = self._views[i]
+        a = np.empty((h, w, 4), np.float32)
+        m = np.empty_like(a) if with_m2 else None
+        self._chk(lib.mdwt_batch_read_view(self._h, i, _f(a), _f(m) if with_m2 else None))
+        return (a, m) if with_m2 else a
+
+    def batch_view_times(self) -> np.ndarray:
+        us = np.zeros(len(self._views), np.float32)
+        self._chk(lib.mdwt_batch_view_times(self._h, _f(us)))
+        return us
+
+    def autotune(self, mc_sizes, threads=(64, 128, 256), tiles=(8, 16, 32), frames: int = 4):
+        """Times every (macrocell size, workgroup, tile) combination; returns
+        (rows of (s, threads, tile, ms, samples/s, visits/s), best (s, threads, tile))."""
+        a = np.ascontiguousarray(mc_sizes, np.int32)
+        b = np.ascontiguousarray(threads, np.int32)
+        c = np.ascontiguousarray(tiles, np.int32)
+        table = np.zeros((len(a) * len(b) * len(c), 6), np.float32)
+        best = np.zeros(3, np.int32)
+        self._chk(lib.mdwt_autotune(self._h, _i(a), len(a), _i(b), len(b), _i(c), len(c), frames,
+                                    _f(table), _i(best)))
+        return table, tuple(int(x) for x in best)
 
     def set_schedule(self, schedule: int) -> None:
         """0: one thread per pixel; 1: persistent waves (default)."""

@@ -1,0 +1,134 @@
+// Batched views: one persistent launch renders many views of uneven size and
+// spp. Its workgroups pull work items (view, pixel block) from a device
+// counter until the list is exhausted, so small and large views share the
+// machine without a launch per view, and the heaviest items (measured on the
+// first frame) go first. The counter reset and the frame / accumulation id
+// advance are kernels, not memsets, so the three-launch sequence is captured
+// once in a HIP graph and replayed every frame (mdwt_api.cpp). No reference
+// counterpart: the reference renders one camera per owlLaunch2D
+// (renderer.cpp:949).
+#include "mdwt_internal.hpp"
+#include "render_paths.hpp"
+
+namespace mdwt {
+namespace {
+
+template <int MODE, int DT, bool COUNT, bool LIT>
+__global__ __launch_bounds__(256) void render_batch_kernel(RenderParams P) {
+  extern __shared__ float4 s_tf[];
+  __shared__ ChanTf s_ch[kMaxChannels];
+  __shared__ uint32_t s_item;
+  load_tf_lds(P, s_tf, s_ch);
+  __syncthreads();
+
+  Ctx<DT> X;
+  X.init(P, s_tf, s_ch);
+  Counts cnt = {0, 0, 0, 0};
+  const uint32_t fid = P.fr.work[1], aid = P.fr.work[2];
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&P.fr.work[0], 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();
+    if (it >= uint32_t(P.fr.n_items)) break;
+    const uint64_t t_item = P.fr.item_ticks ? wall_clock64() : 0ull;
+    const uint32_t code = P.fr.items[it];
+    const ViewParams* vp = P.fr.views + (code >> 20);
+    ViewCtx V;
+    V.cam = &vp->cam;
+    V.W = vp->width;
+    V.H = vp->height;
+    V.spp = vp->spp;
+    V.seed = vp->seed;
+    V.frame_id = fid;
+    V.accum_id = aid;
+    V.accum = vp->accum;
+    V.m2 = vp->m2;
+    render_block<MODE, DT, COUNT, LIT>(P, X, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu),
+                                       cnt);
+    if (P.fr.item_ticks) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        P.fr.item_ticks[2 * it] = t_item;
+        P.fr.item_ticks[2 * it + 1] = wall_clock64();
+      }
+    }
+  }
+  if (COUNT) flush_counts(P, cnt);
+}
+
+__global__ void batch_begin_kernel(unsigned* work) {
+  if (threadIdx.x == 0) work[0] = 0u;
+}
+__global__ void batch_end_kernel(unsigned* work, int accumulate) {
+  if (threadIdx.x == 0) {
+    work[1] = work[1] + 1u;
+    work[2] = accumulate ? work[2] + 1u : 0u;
+  }
+}
+
+template <int MODE, bool COUNT, bool LIT>
+RenderFn batch_fn_dt(int dtype) {
+  switch (dtype) {
+    case MDWT_DT_U8: return render_batch_kernel<MODE, MDWT_DT_U8, COUNT, LIT>;
+    case MDWT_DT_U16: return render_batch_kernel<MODE, MDWT_DT_U16, COUNT, LIT>;
+    case MDWT_DT_F16: return render_batch_kernel<MODE, MDWT_DT_F16, COUNT, LIT>;
+    default: return render_batch_kernel<MODE, MDWT_DT_F32, COUNT, LIT>;
+  }
+}
+
+#define MDWT_BATCH(M)                                                                 \
+  (count ? (lit ? batch_fn_dt<M, true, true>(dtype) : batch_fn_dt<M, true, false>(dtype)) \
+         : (lit ? batch_fn_dt<M, false, true>(dtype) : batch_fn_dt<M, false, false>(dtype)))
+RenderFn select_batch(int mode, int dtype, bool count, bool lit) {
+  switch (mode) {
+    case 0: return MDWT_BATCH(0);
+    case 1: return MDWT_BATCH(1);
+    case 2: return MDWT_BATCH(2);
+    case 3: return MDWT_BATCH(3);
+    case 4: return MDWT_BATCH(4);
+    case 5: return MDWT_BATCH(5);
+    case 6: return MDWT_BATCH(6);
+    case 7: return MDWT_BATCH(7);
+    case 8: return MDWT_BATCH(8);
+    default: return count ? batch_fn_dt<9, true, false>(dtype) : batch_fn_dt<9, false, false>(dtype);
+  }
+}
+#undef MDWT_BATCH
+
+bool is_lit(const RenderParams& p) { return p.fr.sh.shadows != 0 || p.fr.sh.gradient != 0; }
+
+hipError_t prepare(RenderFn f, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(f),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+}
+
+}  // namespace
+
+hipError_t batch_blocks_per_cu(const RenderParams& p, int mode, int dtype, bool count, int* out) {
+  const size_t lds = render_lds_bytes(p);
+  const RenderFn f = select_batch(mode, dtype, count, is_lit(p));
+  const hipError_t e = prepare(f, lds);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, f, p.fr.bw * p.fr.bh, lds);
+}
+
+hipError_t launch_batch(const RenderParams& p, int mode, int dtype, bool count, int grid,
+                        hipStream_t s) {
+  const size_t lds = render_lds_bytes(p);
+  const RenderFn f = select_batch(mode, dtype, count, is_lit(p));
+  hipError_t e = prepare(f, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(64), 0, s, p.fr.work);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(f, dim3(unsigned(grid)), dim3(unsigned(p.fr.bw * p.fr.bh)), lds, s, p);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (!count) {
+    hipLaunchKernelGGL(batch_end_kernel, dim3(1), dim3(64), 0, s, p.fr.work, p.fr.accumulate);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+}  // namespace mdwt

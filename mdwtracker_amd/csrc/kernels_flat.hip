@@ -305,7 +305,8 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
         } else {
           float fa = 1.f;
           for (int i = 0; i < spp; ++i) fa = fa + 1.0f;
-          if (!COUNT) finish_pixel(P, L.pixel, L.ar, L.ag, L.ab, fa);
+          if (!COUNT)
+            finish_pixel(P, P.fr.accum, P.fr.m2, P.fr.accum_id, spp, L.pixel, L.ar, L.ag, L.ab, fa);
           st = S_IDLE;
         }
       } else {

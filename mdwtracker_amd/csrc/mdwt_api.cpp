@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -126,9 +127,32 @@ struct mdwt_renderer {
   int cu_count = 0;
   int grid_key = -1;  // (mode, count, lds) the cached grid was sized for
   int grid = 0;
+
+  // workgroup pixel block (64: 8x8, 128: 16x8, 256: 16x16 threads/pixels) and
+  // the Hilbert tile (pixels) of the block order
+  int bw = 16, bh = 16, tile = 16;
+
+  // batched views
+  std::vector<mdwt::ViewParams> views;   // device accumulators in accum / m2
+  mdwt::ViewParams* d_views = nullptr;
+  std::vector<uint32_t> items;           // (view << 20) | (by << 10) | bx, issue order
+  uint32_t* d_items = nullptr;
+  unsigned* d_work = nullptr;            // item counter, frame id, accumulation id
+  unsigned long long* d_item_ticks = nullptr;
+  bool batch_sorted = false;             // items re-sorted by measured cost
+  bool batch_resort = true;              // sort after the first recorded frame
+  uint32_t batch_frame = 0, batch_accum = 0;
+  int batch_grid = 0, batch_grid_key = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_valid = false;
+  mdwt::RenderParams graph_params;  // what the captured graph launches
 };
 
 namespace {
+
+void free_batch(mdwt_renderer* h);
+int reorder(mdwt_renderer* h);
 
 int fail(mdwt_renderer* h, int code, const std::string& msg) {
   h->err = msg;
@@ -321,19 +345,21 @@ void free_frame(mdwt_renderer* h) {
 }
 
 // Cost-sorted block order (longest processing time first): blocks sorted by
-// the summed wall-clock ticks of their four waves in the last recorded frame,
+// the summed wall-clock ticks of their waves in the last recorded frame,
 // heaviest first. Workgroup i runs on XCD i % 8, so the sorted list is dealt
 // round-robin over the XCDs and the slowest blocks start first instead of
 // trailing at the end of the frame.
 int resort_blocks(mdwt_renderer* h) {
   const size_t nb = size_t(h->n_blocks);
-  std::vector<uint32_t> ticks(4 * nb);
+  const size_t wpb = size_t(h->bw * h->bh / 64);  // waves per block
+  std::vector<uint32_t> ticks(wpb * nb);
   HIPCHK(h, hipMemcpyAsync(ticks.data(), h->d_wave_cost, ticks.size() * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   std::vector<std::pair<uint64_t, uint32_t>> cost(nb);
   for (size_t s = 0; s < nb; ++s) {
-    const uint64_t c = uint64_t(ticks[4 * s]) + ticks[4 * s + 1] + ticks[4 * s + 2] + ticks[4 * s + 3];
+    uint64_t c = 0;
+    for (size_t w = 0; w < wpb; ++w) c += ticks[wpb * s + w];
     cost[s] = {c, h->order[s]};
   }
   std::stable_sort(cost.begin(), cost.end(),
@@ -347,6 +373,35 @@ int resort_blocks(mdwt_renderer* h) {
   return MDWT_OK;
 }
 
+// Block order of one image (nbx x nby blocks of bw x bh pixels): row-major,
+// or a Hilbert curve over tile x tile-pixel super-tiles (blocks row-major
+// inside a super-tile). Entries are bx | by << 16.
+std::vector<uint32_t> block_order(int nbx, int nby, int bw, int bh, int tile, bool hilbert) {
+  std::vector<uint32_t> order;
+  order.reserve(size_t(nbx) * nby);
+  if (!hilbert) {
+    for (int y = 0; y < nby; ++y)
+      for (int x = 0; x < nbx; ++x) order.push_back(uint32_t(x) | (uint32_t(y) << 16));
+    return order;
+  }
+  const int tx = std::max(1, tile / bw), ty = std::max(1, tile / bh);
+  const int sx = (nbx + tx - 1) / tx, sy = (nby + ty - 1) / ty;
+  int bits = 0;
+  while ((1 << bits) < std::max(sx, sy)) ++bits;
+  const uint64_t total = uint64_t(1) << (2 * bits);
+  for (uint64_t d = 0; d < total; ++d) {
+    uint32_t x, y;
+    hilbert_d2xy(bits, uint32_t(d), x, y);
+    if (int(x) >= sx || int(y) >= sy) continue;
+    for (int j = 0; j < ty; ++j)
+      for (int i = 0; i < tx; ++i) {
+        const int bx = int(x) * tx + i, by = int(y) * ty + j;
+        if (bx < nbx && by < nby) order.push_back(uint32_t(bx) | (uint32_t(by) << 16));
+      }
+  }
+  return order;
+}
+
 // frame buffers and the workgroup -> pixel-block order of the frame kernel
 int alloc_frame(mdwt_renderer* h) {
   free_frame(h);
@@ -356,23 +411,11 @@ int alloc_frame(mdwt_renderer* h) {
   HIPCHK(h, hipMemsetAsync(h->d_accum, 0, n * sizeof(Float4), h->stream));
   HIPCHK(h, hipMemsetAsync(h->d_m2, 0, n * sizeof(Float4), h->stream));
   h->fb_pixels = n;
-  const int bx = (h->W + mdwt::kBlockPixels - 1) / mdwt::kBlockPixels;
-  const int by = (h->H + mdwt::kBlockPixels - 1) / mdwt::kBlockPixels;
-  std::vector<uint32_t> order;
-  order.reserve(size_t(bx) * by);
-  if (h->order_mode >= 1) {  // Hilbert (also the start of the cost-sorted order)
-    int bits = 0;
-    while ((1 << bits) < std::max(bx, by)) ++bits;
-    const uint64_t total = uint64_t(1) << (2 * bits);
-    for (uint64_t d = 0; d < total; ++d) {
-      uint32_t x, y;
-      hilbert_d2xy(bits, uint32_t(d), x, y);
-      if (int(x) < bx && int(y) < by) order.push_back(x | (y << 16));
-    }
-  } else {
-    for (int y = 0; y < by; ++y)
-      for (int x = 0; x < bx; ++x) order.push_back(uint32_t(x) | (uint32_t(y) << 16));
-  }
+  // (the flat / persistent schedules only run with 16x16 blocks: mdwt_set_block
+  // and mdwt_set_schedule keep bw = bh = 16 whenever schedule != 0)
+  const int bw = h->bw, bh = h->bh;
+  const int nbx = (h->W + bw - 1) / bw, nby = (h->H + bh - 1) / bh;
+  std::vector<uint32_t> order = block_order(nbx, nby, bw, bh, h->tile, h->order_mode >= 1);
   h->n_blocks = int(order.size());
   HIPCHK(h, hipMalloc(&h->d_order, order.size() * sizeof(uint32_t)));
   HIPCHK(h, hipMemcpyAsync(h->d_order, order.data(), order.size() * sizeof(uint32_t),
@@ -381,6 +424,7 @@ int alloc_frame(mdwt_renderer* h) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->order = std::move(order);
   h->cost_ready = false;
+  h->need_resort = true;
   return MDWT_OK;
 }
 
@@ -466,7 +510,8 @@ int eff_schedule(const mdwt_renderer* h) {
 }
 
 mdwt::RenderParams render_params(const mdwt_renderer* h) {
-  mdwt::RenderParams p{};
+  mdwt::RenderParams p;
+  std::memset(&p, 0, sizeof(p));  // padding too: batched launches compare params bytewise
   p.vol = volume_params(h);
   p.tf = tf_params(h);
   p.cam = h->cam;
@@ -495,6 +540,13 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.grid = h->grid;
   p.fr.wave_cost =
       (h->order_mode == 2 && eff_schedule(h) == 0 && h->need_resort) ? h->d_wave_cost : nullptr;
+  p.fr.bw = h->bw;
+  p.fr.bh = h->bh;
+  p.fr.views = nullptr;
+  p.fr.items = nullptr;
+  p.fr.n_items = 0;
+  p.fr.work = nullptr;
+  p.fr.item_ticks = nullptr;
   return p;
 }
 
@@ -517,6 +569,45 @@ int launch_frame(mdwt_renderer* h, bool count) {
   HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, count, h->stream));
   h->launch_seq++;
   if (p.fr.wave_cost && !count) h->cost_ready = true;
+  return MDWT_OK;
+}
+
+// (Re)allocates the value-range and majorant grids for h->mc and runs the
+// range kernel over the resident volume (renderer.cu:1116-1179).
+int build_range_grid(mdwt_renderer* h, const std::vector<int>& wx, const std::vector<int>& wy,
+                     const std::vector<int>& wz) {
+  const size_t n_cells = size_t(h->mc[0]) * h->mc[1] * h->mc[2];
+  void* old[] = {h->d_win, h->d_ranges, h->d_maj};
+  for (void* q : old)
+    if (q) (void)hipFree(q);
+  h->d_win = nullptr;
+  h->d_ranges = nullptr;
+  h->d_maj = nullptr;
+  std::vector<int> win;
+  win.insert(win.end(), wx.begin(), wx.end());
+  win.insert(win.end(), wy.begin(), wy.end());
+  win.insert(win.end(), wz.begin(), wz.end());
+  HIPCHK(h, hipMalloc(&h->d_win, win.size() * sizeof(int)));
+  HIPCHK(h, hipMemcpyAsync(h->d_win, win.data(), win.size() * sizeof(int), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMalloc(&h->d_ranges, n_cells * h->C * 2 * sizeof(float)));
+  HIPCHK(h, hipMalloc(&h->d_maj, n_cells * h->C * sizeof(float)));
+  HIPCHK(h, hipMemsetAsync(h->d_maj, 0, n_cells * h->C * sizeof(float), h->stream));
+  mdwt::RangeParams rp{};
+  rp.vol = h->d_vol;
+  rp.chan_stride = h->chan_stride;
+  rp.nx = h->dims[0];
+  rp.ny = h->dims[1];
+  rp.nz = h->dims[2];
+  rp.channels = h->C;
+  rp.scale = value_scale(h->dtype);
+  rp.offset = 0.f;
+  for (int i = 0; i < 3; ++i) rp.mc[i] = h->mc[i];
+  rp.win = h->d_win;
+  rp.ranges = h->d_ranges;
+  HIPCHK(h, mdwt::launch_ranges(rp, h->dtype, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->maj_valid = false;
   return MDWT_OK;
 }
 
@@ -566,6 +657,7 @@ int mdwt_destroy(mdwt_renderer* h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_frame(h);
+    free_batch(h);
     void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win, h->d_queues};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
@@ -697,35 +789,13 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   for (int c = 0; c < h->C; ++c)
     HIPCHK(h, hipMemcpyAsync(h->d_vol + size_t(c) * h->chan_stride, h->staging[c].data(), chan_bytes,
                              hipMemcpyHostToDevice, h->stream));
-  std::vector<int> win;
-  win.insert(win.end(), wx.begin(), wx.end());
-  win.insert(win.end(), wy.begin(), wy.end());
-  win.insert(win.end(), wz.begin(), wz.end());
-  HIPCHK(h, hipMalloc(&h->d_win, win.size() * sizeof(int)));
-  HIPCHK(h, hipMemcpyAsync(h->d_win, win.data(), win.size() * sizeof(int), hipMemcpyHostToDevice,
-                           h->stream));
-  HIPCHK(h, hipMalloc(&h->d_ranges, n_cells * h->C * 2 * sizeof(float)));
-  HIPCHK(h, hipMalloc(&h->d_maj, n_cells * h->C * sizeof(float)));
-  HIPCHK(h, hipMemsetAsync(h->d_maj, 0, n_cells * h->C * sizeof(float), h->stream));
+  int rc0 = build_range_grid(h, wx, wy, wz);
+  if (rc0 != MDWT_OK) return rc0;
   HIPCHK(h, hipMalloc(&h->d_counters, 4 * sizeof(unsigned long long)));
   HIPCHK(h, hipMalloc(&h->d_queues, 16 * sizeof(unsigned)));
   HIPCHK(h, hipMemsetAsync(h->d_queues, 0, 16 * sizeof(unsigned), h->stream));
   HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device));
   h->launch_seq = 0;
-  mdwt::RangeParams rp{};
-  rp.vol = h->d_vol;
-  rp.chan_stride = h->chan_stride;
-  rp.nx = h->dims[0];
-  rp.ny = h->dims[1];
-  rp.nz = h->dims[2];
-  rp.channels = h->C;
-  rp.scale = value_scale(h->dtype);
-  rp.offset = 0.f;
-  for (int i = 0; i < 3; ++i) rp.mc[i] = m[i];
-  rp.win = h->d_win;
-  rp.ranges = h->d_ranges;
-  HIPCHK(h, mdwt::launch_ranges(rp, h->dtype, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   // the reference drops its scalar buffers after the build (renderer.cpp:738-739)
   std::vector<std::vector<uint8_t>>().swap(h->staging);
   h->cm.assign(h->C, std::vector<Float4>());
@@ -1166,33 +1236,18 @@ int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile) {
   if (!h) return MDWT_ERR_ARG;
   if (order < 0 || order > 2)
     return fail(h, MDWT_ERR_ARG, "order must be 0 (rows), 1 (Hilbert) or 2 (cost-sorted)");
-  if (tile != mdwt::kBlockPixels) return fail(h, MDWT_ERR_ARG, "tile must be 16");
+  if (tile != 8 && tile != 16 && tile != 32) return fail(h, MDWT_ERR_ARG, "tile must be 8, 16 or 32");
+  h->tile = tile;
   h->order_mode = order;
-  if (h->inited) {
-    HIPCHK(h, hipSetDevice(h->device));
-    const uint32_t keep = h->accum_id;
-    // keep the accumulated image: only the block order changes
-    Float4* acc = h->d_accum;
-    Float4* m2 = h->d_m2;
-    h->d_accum = nullptr;
-    h->d_m2 = nullptr;
-    if (h->d_order) (void)hipFree(h->d_order);
-    h->d_order = nullptr;
-    int rc = alloc_frame(h);
-    if (rc != MDWT_OK) return rc;
-    HIPCHK(h, hipMemcpy(h->d_accum, acc, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
-    HIPCHK(h, hipMemcpy(h->d_m2, m2, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
-    (void)hipFree(acc);
-    (void)hipFree(m2);
-    h->accum_id = keep;
-  }
-  return MDWT_OK;
+  return reorder(h);
 }
 
 int mdwt_set_schedule(mdwt_renderer* h, int schedule) {
   if (!h) return MDWT_ERR_ARG;
   if (schedule < 0 || schedule > 2)
     return fail(h, MDWT_ERR_ARG, "schedule must be 0 (thread per pixel), 1 (persistent) or 2 (flat)");
+  if (schedule != 0 && (h->bw != 16 || h->bh != 16))
+    return fail(h, MDWT_ERR_ARG, "the flat schedules need 256-thread (16x16) blocks");
   h->schedule = schedule;
   return MDWT_OK;
 }
@@ -1234,6 +1289,428 @@ int mdwt_hilbert_d2xy(int bits, uint32_t d, uint32_t* x, uint32_t* y) {
   if (bits < 0 || bits > 15 || !x || !y) return MDWT_ERR_ARG;
   if (bits < 16 && uint64_t(d) >= (uint64_t(1) << (2 * bits))) return MDWT_ERR_ARG;
   hilbert_d2xy(bits, d, *x, *y);
+  return MDWT_OK;
+}
+
+}  // extern "C"
+
+// ===================================================== batched views ======
+namespace {
+
+void free_batch(mdwt_renderer* h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  h->graph_valid = false;
+  for (auto& v : h->views) {
+    if (v.accum) (void)hipFree(v.accum);
+    if (v.m2) (void)hipFree(v.m2);
+  }
+  h->views.clear();
+  void* ptrs[] = {h->d_views, h->d_items, h->d_work, h->d_item_ticks};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  h->d_views = nullptr;
+  h->d_items = nullptr;
+  h->d_work = nullptr;
+  h->d_item_ticks = nullptr;
+  h->items.clear();
+}
+
+// Work items of all views: views by decreasing work (pixels x spp), each
+// view's blocks in the Hilbert-tile order of the current block shape.
+int build_items(mdwt_renderer* h) {
+  std::vector<int> idx(h->views.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = int(i);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    const auto& va = h->views[size_t(a)];
+    const auto& vb = h->views[size_t(b)];
+    return uint64_t(va.width) * va.height * va.spp > uint64_t(vb.width) * vb.height * vb.spp;
+  });
+  std::vector<uint32_t> items;
+  for (int v : idx) {
+    const auto& vw = h->views[size_t(v)];
+    const int nbx = (vw.width + h->bw - 1) / h->bw, nby = (vw.height + h->bh - 1) / h->bh;
+    if (nbx > 1024 || nby > 1024) return fail(h, MDWT_ERR_LIMIT, "view has more than 1024 blocks per side");
+    for (uint32_t o : block_order(nbx, nby, h->bw, h->bh, h->tile, true))
+      items.push_back((uint32_t(v) << 20) | ((o >> 16) << 10) | (o & 0xffffu));
+  }
+  if (h->d_items) (void)hipFree(h->d_items);
+  if (h->d_item_ticks) (void)hipFree(h->d_item_ticks);
+  h->d_items = nullptr;
+  h->d_item_ticks = nullptr;
+  HIPCHK(h, hipMalloc(&h->d_items, items.size() * sizeof(uint32_t)));
+  HIPCHK(h, hipMalloc(&h->d_item_ticks, 2 * items.size() * sizeof(unsigned long long)));
+  HIPCHK(h, hipMemcpy(h->d_items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(h->d_item_ticks, 0, 2 * items.size() * sizeof(unsigned long long)));
+  h->items = std::move(items);
+  h->batch_sorted = false;
+  return MDWT_OK;
+}
+
+mdwt::RenderParams batch_params(const mdwt_renderer* h) {
+  mdwt::RenderParams p = render_params(h);
+  p.fr.schedule = 0;
+  p.fr.wave_cost = nullptr;
+  p.fr.views = h->d_views;
+  p.fr.items = h->d_items;
+  p.fr.n_items = int(h->items.size());
+  p.fr.work = h->d_work;
+  p.fr.item_ticks = h->d_item_ticks;
+  return p;
+}
+
+// resident workgroups of the persistent batched launch (cached per kernel
+// choice and LDS size)
+int batch_grid(mdwt_renderer* h, const mdwt::RenderParams& p, bool count, int* grid) {
+  const bool lit = p.fr.sh.shadows || p.fr.sh.gradient;
+  const int key = int(mdwt::render_lds_bytes(p)) * 64 + h->mode * 16 + h->bw * h->bh / 64 * 2 +
+                  (count ? 1 : 0) + (lit ? 1 << 30 : 0);
+  if (key != h->batch_grid_key) {
+    int per_cu = 0;
+    HIPCHK(h, mdwt::batch_blocks_per_cu(p, h->mode, h->dtype, count, &per_cu));
+    h->batch_grid = std::max(1, per_cu) * h->cu_count;
+    h->batch_grid_key = key;
+  }
+  *grid = h->batch_grid;
+  return MDWT_OK;
+}
+
+// Items sorted by their measured duration (heaviest first), as the single
+// view's blocks are.
+int resort_items(mdwt_renderer* h) {
+  const size_t n = h->items.size();
+  std::vector<unsigned long long> t(2 * n);
+  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+  std::vector<std::pair<uint64_t, uint32_t>> cost(n);
+  for (size_t i = 0; i < n; ++i) cost[i] = {t[2 * i + 1] - t[2 * i], h->items[i]};
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
+                     return a.first > b.first;
+                   });
+  for (size_t i = 0; i < n; ++i) h->items[i] = cost[i].second;
+  HIPCHK(h, hipMemcpy(h->d_items, h->items.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  h->batch_sorted = true;
+  return MDWT_OK;
+}
+
+int batch_ready(mdwt_renderer* h) {
+  int rc = ready_to_render(h);
+  if (rc != MDWT_OK) return rc;
+  if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views (mdwt_batch_set_views)");
+  HIPCHK(h, hipSetDevice(h->device));
+  return MDWT_OK;
+}
+
+mdwt::CameraParams view_camera(const mdwt_view& v) {
+  mdwt_renderer tmp;
+  tmp.W = v.width;
+  tmp.H = v.height;
+  set_orientation(&tmp, F3{v.from[0], v.from[1], v.from[2]}, F3{v.at[0], v.at[1], v.at[2]},
+                  F3{v.up[0], v.up[1], v.up[2]}, v.fovy);
+  update_camera(&tmp);
+  return tmp.cam;
+}
+
+// Block shape / tile changes: new single-view block order (the accumulated
+// image is kept) and new batch items.
+int reorder(mdwt_renderer* h) {
+  if (h->inited) {
+    HIPCHK(h, hipSetDevice(h->device));
+    const uint32_t keep = h->accum_id;
+    Float4* acc = h->d_accum;
+    Float4* m2 = h->d_m2;
+    h->d_accum = nullptr;
+    h->d_m2 = nullptr;
+    int rc = alloc_frame(h);
+    if (rc != MDWT_OK) return rc;
+    HIPCHK(h, hipMemcpy(h->d_accum, acc, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
+    HIPCHK(h, hipMemcpy(h->d_m2, m2, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
+    (void)hipFree(acc);
+    (void)hipFree(m2);
+    h->accum_id = keep;
+  }
+  if (!h->views.empty()) return build_items(h);
+  return MDWT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdwt_set_block(mdwt_renderer* h, int threads) {
+  if (!h) return MDWT_ERR_ARG;
+  int bw, bh;
+  switch (threads) {
+    case 64: bw = 8; bh = 8; break;
+    case 128: bw = 16; bh = 8; break;
+    case 256: bw = 16; bh = 16; break;
+    default: return fail(h, MDWT_ERR_ARG, "workgroup size must be 64, 128 or 256");
+  }
+  if (h->schedule != 0 && threads != 256)
+    return fail(h, MDWT_ERR_ARG, "the flat schedules need 256-thread (16x16) blocks");
+  if (bw == h->bw && bh == h->bh) return MDWT_OK;
+  h->bw = bw;
+  h->bh = bh;
+  return reorder(h);
+}
+
+int mdwt_set_macrocells(mdwt_renderer* h, const int mc[3]) {
+  if (!h || !mc) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  for (int i = 0; i < 3; ++i)
+    if (mc[i] < 1) return fail(h, MDWT_ERR_ARG, "macrocell count must be positive");
+  if (uint64_t(mc[0]) * mc[1] * mc[2] * uint64_t(h->C) > (1ull << 28))
+    return fail(h, MDWT_ERR_LIMIT, "macrocell grid too large");
+  std::vector<int> wx, wy, wz;
+  if (!axis_windows(h->dims[0], mc[0], 0.f, h->remap.x, wx) ||
+      !axis_windows(h->dims[1], mc[1], 0.f, h->remap.y, wy) ||
+      !axis_windows(h->dims[2], mc[2], 0.f, h->remap.z, wz))
+    return fail(h, MDWT_ERR_ARG, "macrocell projection windows are not intervals");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < 3; ++i) h->mc[i] = mc[i];
+  int rc = build_range_grid(h, wx, wy, wz);
+  if (rc != MDWT_OK) return rc;
+  reset_acc(h);
+  if (!h->views.empty() && (rc = mdwt_batch_reset(h)) != MDWT_OK) return rc;
+  return rebuild_majorants(h);
+}
+
+int mdwt_batch_set_views(mdwt_renderer* h, const mdwt_view* views, int n) {
+  if (!h || !views) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (n < 1 || n > 4096) return fail(h, MDWT_ERR_ARG, "1 to 4096 views");
+  for (int i = 0; i < n; ++i) {
+    const mdwt_view& v = views[i];
+    if (v.width < 1 || v.height < 1 || v.width > 8192 || v.height > 8192)
+      return fail(h, MDWT_ERR_ARG, "view size must be 1..8192");
+    if (v.spp < 1 || v.spp > 1024) return fail(h, MDWT_ERR_ARG, "view spp must be 1..1024");
+    if (!finite3(v.from) || !finite3(v.at) || !finite3(v.up) || !std::isfinite(v.fovy))
+      return fail(h, MDWT_ERR_ARG, "non-finite view camera");
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  free_batch(h);
+  for (int i = 0; i < n; ++i) {
+    const mdwt_view& v = views[i];
+    mdwt::ViewParams vp{};
+    vp.cam = view_camera(v);
+    vp.width = v.width;
+    vp.height = v.height;
+    vp.spp = v.spp;
+    vp.seed = v.seed;
+    const size_t bytes = size_t(v.width) * v.height * sizeof(Float4);
+    if (hipMalloc(&vp.accum, bytes) != hipSuccess || hipMalloc(&vp.m2, bytes) != hipSuccess) {
+      h->views.push_back(vp);
+      free_batch(h);
+      return fail(h, MDWT_ERR_OOM, "view accumulators");
+    }
+    h->views.push_back(vp);
+    HIPCHK(h, hipMemset(vp.accum, 0, bytes));
+    HIPCHK(h, hipMemset(vp.m2, 0, bytes));
+  }
+  HIPCHK(h, hipMalloc(&h->d_views, h->views.size() * sizeof(mdwt::ViewParams)));
+  HIPCHK(h, hipMemcpy(h->d_views, h->views.data(), h->views.size() * sizeof(mdwt::ViewParams),
+                      hipMemcpyHostToDevice));
+  HIPCHK(h, hipMalloc(&h->d_work, 4 * sizeof(unsigned)));
+  const unsigned zero[4] = {0, 0, 0, 0};
+  HIPCHK(h, hipMemcpy(h->d_work, zero, sizeof(zero), hipMemcpyHostToDevice));
+  h->batch_frame = h->batch_accum = 0;
+  return build_items(h);
+}
+
+int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms_out) {
+  if (!h) return MDWT_ERR_ARG;
+  int rc = batch_ready(h);
+  if (rc != MDWT_OK) return rc;
+  if (n_frames < 0) return fail(h, MDWT_ERR_ARG, "negative frame count");
+  const mdwt::RenderParams p = batch_params(h);
+  int grid = 0;
+  if ((rc = batch_grid(h, p, false, &grid)) != MDWT_OK) return rc;
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  int done = 0;
+  if (n_frames > 0 && !h->batch_sorted && h->batch_resort) {
+    // first frame eagerly; its per-item durations order the next frames
+    HIPCHK(h, mdwt::launch_batch(p, h->mode, h->dtype, false, grid, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    rc = resort_items(h);
+    if (rc != MDWT_OK) return rc;
+    done = 1;
+  }
+  if (n_frames > done) {
+    if (!h->graph_valid || std::memcmp(&p, &h->graph_params, sizeof(p)) != 0) {
+      if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+      if (h->graph) (void)hipGraphDestroy(h->graph);
+      h->graph_exec = nullptr;
+      h->graph = nullptr;
+      h->graph_valid = false;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t le = mdwt::launch_batch(p, h->mode, h->dtype, false, grid, h->stream);
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(h, MDWT_ERR_HIP, std::string("graph capture: ") +
+                                         hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      h->graph = g;
+      HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+      h->graph_params = p;
+      h->graph_valid = true;
+    }
+    for (int f = done; f < n_frames; ++f) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->batch_frame += uint32_t(n_frames);
+  h->batch_accum = h->accumulate ? h->batch_accum + uint32_t(n_frames) : 0u;
+  h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
+  if (ms_out) *ms_out = ms;
+  return MDWT_OK;
+}
+
+int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
+  if (!h || !counts) return MDWT_ERR_ARG;
+  int rc = batch_ready(h);
+  if (rc != MDWT_OK) return rc;
+  mdwt::RenderParams p = batch_params(h);
+  p.fr.item_ticks = nullptr;
+  int grid = 0;
+  if ((rc = batch_grid(h, p, true, &grid)) != MDWT_OK) return rc;
+  HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
+  HIPCHK(h, mdwt::launch_batch(p, h->mode, h->dtype, true, grid, h->stream));
+  HIPCHK(h, hipMemcpyAsync(counts, h->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MDWT_OK;
+}
+
+int mdwt_batch_reset(mdwt_renderer* h) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const unsigned w[3] = {0u, h->batch_frame, 0u};
+  HIPCHK(h, hipMemcpy(h->d_work, w, sizeof(w), hipMemcpyHostToDevice));
+  h->batch_accum = 0;
+  return MDWT_OK;
+}
+
+int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2) {
+  if (!h) return MDWT_ERR_ARG;
+  if (view < 0 || size_t(view) >= h->views.size()) return fail(h, MDWT_ERR_ARG, "no such view");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const auto& v = h->views[size_t(view)];
+  const size_t bytes = size_t(v.width) * v.height * sizeof(Float4);
+  if (rgba) HIPCHK(h, hipMemcpy(rgba, v.accum, bytes, hipMemcpyDeviceToHost));
+  if (m2) HIPCHK(h, hipMemcpy(m2, v.m2, bytes, hipMemcpyDeviceToHost));
+  return MDWT_OK;
+}
+
+int mdwt_batch_view_times(mdwt_renderer* h, float* us) {
+  if (!h || !us) return MDWT_ERR_ARG;
+  if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const size_t n = h->items.size();
+  std::vector<unsigned long long> t(2 * n);
+  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+  unsigned long long t0 = ~0ull;
+  for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[2 * i]);
+  std::vector<unsigned long long> end(h->views.size(), 0ull);
+  for (size_t i = 0; i < n; ++i) {
+    const size_t v = h->items[i] >> 20;
+    end[v] = std::max(end[v], t[2 * i + 1]);
+  }
+  for (size_t v = 0; v < end.size(); ++v)
+    us[v] = end[v] >= t0 ? float(double(end[v] - t0) / 100.0) : 0.f;  // 100 MHz clock
+  return MDWT_OK;
+}
+
+int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* threads, int n_threads,
+                  const int* tiles, int n_tiles, int frames, float* table, int best[3]) {
+  if (!h || !mc_sizes || !threads || !tiles || n_mc < 1 || n_threads < 1 || n_tiles < 1)
+    return MDWT_ERR_ARG;
+  if (frames < 1) return fail(h, MDWT_ERR_ARG, "frames must be positive");
+  const bool batch = !h->views.empty();
+  int rc = batch ? batch_ready(h) : ready_to_render(h);
+  if (rc != MDWT_OK) return rc;
+  if (h->schedule != 0) return fail(h, MDWT_ERR_STATE, "autotuning runs the thread-per-pixel schedule");
+  const int keep_order = h->order_mode;
+  const bool keep_resort = h->batch_resort;
+  h->order_mode = 1;  // the tile order is what runs while timing
+  h->batch_resort = false;
+  float best_ms = std::numeric_limits<float>::max();
+  int bi[3] = {mc_sizes[0], threads[0], tiles[0]};
+  int row = 0;
+  for (int a = 0; a < n_mc && rc == MDWT_OK; ++a) {
+    const int s = mc_sizes[a];
+    if (s < 1) {
+      rc = fail(h, MDWT_ERR_ARG, "macrocell size must be positive");
+      break;
+    }
+    const int mc[3] = {std::max(1, h->dims[0] / s), std::max(1, h->dims[1] / s),
+                       std::max(1, h->dims[2] / s)};
+    if ((rc = mdwt_set_macrocells(h, mc)) != MDWT_OK) break;
+    uint64_t cnt[4] = {0, 0, 0, 0};
+    for (int b = 0; b < n_threads && rc == MDWT_OK; ++b) {
+      if ((rc = mdwt_set_block(h, threads[b])) != MDWT_OK) break;
+      for (int c = 0; c < n_tiles && rc == MDWT_OK; ++c) {
+        if ((rc = mdwt_set_tile_order(h, 1, tiles[c])) != MDWT_OK) break;
+        if (batch) {
+          if (b == 0 && c == 0 && (rc = mdwt_batch_count(h, cnt)) != MDWT_OK) break;
+          if ((rc = mdwt_batch_render(h, 1, nullptr)) != MDWT_OK) break;
+          if ((rc = mdwt_batch_render(h, frames, nullptr)) != MDWT_OK) break;
+        } else {
+          if (b == 0 && c == 0 && (rc = mdwt_count_frame(h, cnt)) != MDWT_OK) break;
+          if ((rc = mdwt_render(h, 1, nullptr)) != MDWT_OK) break;
+          if ((rc = mdwt_render(h, frames, nullptr)) != MDWT_OK) break;
+        }
+        const float ms = h->last_kernel_ms;
+        if (table) {
+          float* r = table + 6 * row;
+          r[0] = float(s);
+          r[1] = float(threads[b]);
+          r[2] = float(tiles[c]);
+          r[3] = ms;
+          r[4] = ms > 0.f ? float(double(cnt[0]) / (double(ms) * 1e-3)) : 0.f;
+          r[5] = ms > 0.f ? float(double(cnt[1]) / (double(ms) * 1e-3)) : 0.f;
+        }
+        ++row;
+        if (ms < best_ms) {
+          best_ms = ms;
+          bi[0] = s;
+          bi[1] = threads[b];
+          bi[2] = tiles[c];
+        }
+      }
+    }
+  }
+  h->order_mode = keep_order;
+  h->batch_resort = keep_resort;
+  if (rc != MDWT_OK) return rc;
+  const int mc[3] = {std::max(1, h->dims[0] / bi[0]), std::max(1, h->dims[1] / bi[0]),
+                     std::max(1, h->dims[2] / bi[0])};
+  if ((rc = mdwt_set_macrocells(h, mc)) != MDWT_OK) return rc;
+  if ((rc = mdwt_set_block(h, bi[1])) != MDWT_OK) return rc;
+  if ((rc = mdwt_set_tile_order(h, keep_order, bi[2])) != MDWT_OK) return rc;
+  if (batch) {
+    h->batch_sorted = false;
+    if ((rc = mdwt_batch_reset(h)) != MDWT_OK) return rc;
+  }
+  reset_acc(h);
+  if (best) {
+    best[0] = bi[0];
+    best[1] = bi[1];
+    best[2] = bi[2];
+  }
   return MDWT_OK;
 }
 

@@ -15,6 +15,12 @@ RenderFn select_flat_kernel(int mode, int dtype, bool count, bool refill);
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
 hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
+// batched views: item-counter reset, persistent frame kernel over p.fr.items
+// (grid workgroups), frame / accumulation id advance (skipped when counting)
+hipError_t launch_batch(const RenderParams& p, int mode, int dtype, bool count, int grid,
+                        hipStream_t s);
+// resident workgroups per CU of the batched frame kernel
+hipError_t batch_blocks_per_cu(const RenderParams& p, int mode, int dtype, bool count, int* out);
 // dynamic LDS of a frame launch (transfer-function texels)
 size_t render_lds_bytes(const RenderParams& p);
 // resident workgroups per CU of the persistent frame kernel

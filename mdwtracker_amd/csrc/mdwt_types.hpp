@@ -48,6 +48,17 @@ struct ShadeParams {
   float spec[kMaxChannels][3];
 };
 
+// One view of a batched launch: camera, size, samples, RNG seed and its own
+// accumulators (a batched view's image equals a single-view render with the
+// same camera, size, spp and seed).
+struct ViewParams {
+  CameraParams cam;
+  int width, height, spp;
+  uint32_t seed;
+  void* accum;  // float4 per pixel
+  void* m2;     // float4 per pixel
+};
+
 struct FrameParams {
   int width, height, spp, accumulate;
   float bg[3];
@@ -66,8 +77,18 @@ struct FrameParams {
   int schedule;  // 0: one thread per pixel, 1: persistent waves, 2: flat loop
   int grid;      // workgroups of the persistent launch
   // cost-sorted order: wall-clock ticks (100 MHz) of each wave, indexed by
-  // 4 * (position in block_order) + wave; nullptr = not recorded
+  // waves_per_block * (position in block_order) + wave; nullptr = not recorded
   uint32_t* wave_cost;
+  int bw, bh;  // pixel block of a workgroup: 8x8, 16x8 or 16x16 (bw*bh/64 waves of 8x8 pixels)
+  // batched views (views == nullptr: the single view described above). The
+  // workgroups of one launch pull items (view << 20 | by << 10 | bx) from
+  // work[0]; work[1] / work[2] hold the frame and accumulation ids, advanced
+  // on the device so that a captured graph can be replayed frame after frame.
+  const ViewParams* views;
+  const uint32_t* items;
+  int n_items;
+  unsigned* work;
+  unsigned long long* item_ticks;  // start / end wall clock per item, nullptr = off
 };
 
 struct RenderParams {

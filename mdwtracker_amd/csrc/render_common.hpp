@@ -205,14 +205,15 @@ __device__ __forceinline__ uint32_t block_slot(uint32_t b, uint32_t /*n*/) { ret
 
 // running mean of the colour and of its square (deviceCode.cu:463-469; the
 // second moment is an addition)
-__device__ __forceinline__ void accumulate_pixel(const RenderParams& P, uint32_t pixel, float fr,
+__device__ __forceinline__ void accumulate_pixel(const RenderParams& P, void* accum, void* m2,
+                                                 uint32_t accum_id, uint32_t pixel, float fr,
                                                  float fg, float fb, float fa) {
-  float4* acc = reinterpret_cast<float4*>(P.fr.accum) + pixel;
-  float4* sq = reinterpret_cast<float4*>(P.fr.m2) + pixel;
+  float4* acc = reinterpret_cast<float4*>(accum) + pixel;
+  float4* sq = reinterpret_cast<float4*>(m2) + pixel;
   const float4 q = make_float4(fr * fr, fg * fg, fb * fb, fa * fa);
   if (P.fr.accumulate) {
-    const float k = float(P.fr.accum_id);
-    const float k1 = float(P.fr.accum_id + 1);
+    const float k = float(accum_id);
+    const float k1 = float(accum_id + 1);
     const float4 a = *acc;
     const float4 m = *sq;
     *acc = make_float4((fr + k * a.x) / k1, (fg + k * a.y) / k1, (fb + k * a.z) / k1,
@@ -225,8 +226,6 @@ __device__ __forceinline__ void accumulate_pixel(const RenderParams& P, uint32_t
   }
 }
 
-// shading with diffuse 1, specular 0, shadow 1 (deviceCode.cu:326-433) and
-// over(colour, (bg, 1)) (deviceCode.cu:44-59) added to the pixel sum
 // Shading of one sample (deviceCode.cu:411-433): diffuse, specular and shadow
 // factors from the caller (1, 0, 1 without gradient shading and shadows),
 // the hit channel's material specular colour, then over the background.
@@ -255,10 +254,11 @@ __device__ __forceinline__ void composite_sample(const RenderParams& P, bool hit
 }
 
 // Divides the pixel sums (alpha started at 1) by spp and accumulates.
-__device__ __forceinline__ void finish_pixel(const RenderParams& P, uint32_t pixel, float fr,
-                                             float fg, float fb, float fa) {
-  const float spp = float(P.fr.spp);
-  accumulate_pixel(P, pixel, fr / spp, fg / spp, fb / spp, fa / spp);
+__device__ __forceinline__ void finish_pixel(const RenderParams& P, void* accum, void* m2,
+                                             uint32_t accum_id, int spp_i, uint32_t pixel,
+                                             float fr, float fg, float fb, float fa) {
+  const float spp = float(spp_i);
+  accumulate_pixel(P, accum, m2, accum_id, pixel, fr / spp, fg / spp, fb / spp, fa / spp);
 }
 
 }  // namespace mdwt

@@ -201,7 +201,9 @@ typedef struct mdwt_view {
   int width, height, spp;
   uint32_t seed;
 } mdwt_view;
-/* Replaces the batch (1..4096 views); each view gets its own accumulators. */
+/* Replaces the batch (1..4096 views); each view gets its own accumulators.
+ * Currently returns MDWT_ERR_STATE: the batched launch is switched off until
+ * a hang seen on its first GPU run is explained. */
 int mdwt_batch_set_views(mdwt_renderer* h, const mdwt_view* views, int n);
 /* Renders n_frames frames of every view (ms = total GPU time). The first
  * frame after mdwt_batch_set_views runs eagerly and times every item; later

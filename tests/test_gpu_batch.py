@@ -25,6 +25,11 @@ def scene():
     return scenes.make_scene("ndda_cpu_2ch64", seed=5)
 
 
+BATCH_OFF = pytest.mark.skip(
+    reason="batched launch switched off: its first GPU run hung (killed after 180 s without "
+           "output) and no cause has been shown yet; mdwt_batch_set_views returns an error")
+
+
 def _bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
 
@@ -55,6 +60,7 @@ def _single(capi, scene, mode, v, frames):
     return a, cnt
 
 
+@BATCH_OFF
 @pytest.mark.parametrize("mode", [2, 0, 7])
 def test_batched_views_equal_single_view_renders(capi, scene, mode):
     views = _views(6, seed=mode)
@@ -86,6 +92,7 @@ def test_batched_views_equal_single_view_renders(capi, scene, mode):
     g.close()
 
 
+@BATCH_OFF
 def test_batch_with_lighting_equals_single(capi, scene):
     views = _views(3, seed=11)
     g = capi.MdwtRenderer(0)
@@ -148,6 +155,7 @@ def test_set_macrocells_matches_fresh_init(capi, scene):
     b.close()
 
 
+@BATCH_OFF
 def test_autotune_batch(capi, scene):
     g = capi.MdwtRenderer(0)
     scenes.load_scene(g, scene, 2)
@@ -162,4 +170,29 @@ def test_autotune_batch(capi, scene):
     assert g.info()["mc"] == (64 // best[0],) * 3
     g.batch_render(2)
     assert np.all(np.isfinite(g.batch_read_view(0)))
+    g.close()
+
+
+def test_batch_switched_off_reports_an_error(capi, scene):
+    g = capi.MdwtRenderer(0)
+    scenes.load_scene(g, scene, 2)
+    with pytest.raises(capi.MdwtError, match="disabled"):
+        g.batch_set_views(_views(2))
+    g.close()
+
+
+def test_autotune_single_view(capi, scene):
+    g = capi.MdwtRenderer(0)
+    scenes.load_scene(g, scene, 2)
+    g.set_frame(96, 96, 1, True, (0.0, 0.0, 0.0))
+    print("[autotune] single view 96x96, s in (8, 16), threads (64, 256), tiles (8, 16)", flush=True)
+    table, best = g.autotune((8, 16), (64, 256), (8, 16), frames=2)
+    print(f"[autotune] best={best}\n{table}", flush=True)
+    assert table.shape == (8, 6)
+    assert np.all(table[:, 3] > 0) and np.all(table[:, 4] > 0)
+    row = table[(table[:, 0] == best[0]) & (table[:, 1] == best[1]) & (table[:, 2] == best[2])]
+    assert row[0, 3] == table[:, 3].min()
+    assert g.info()["mc"] == (64 // best[0],) * 3
+    g.render(2)
+    assert np.all(np.isfinite(g.read_accum()))
     g.close()

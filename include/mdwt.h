@@ -188,6 +188,14 @@ int mdwt_set_block(mdwt_renderer* h, int threads);
  * Init; resets accumulation (also of the batched views). */
 int mdwt_set_macrocells(mdwt_renderer* h, const int mc[3]);
 
+/* Auxiliary buffers for the denoiser (no reference counterpart): when on,
+ * every frame also accumulates per pixel the running means of the first-hit
+ * distance (0 on a miss), the transmittance (1 - hit fraction) and the hit
+ * fractions of channels 0..3. Switching restarts the accumulation. */
+int mdwt_set_aov(mdwt_renderer* h, int on);
+/* width * height * 6 floats in that order */
+int mdwt_read_aov(mdwt_renderer* h, float* out);
+
 /* ---- batched views (no reference counterpart: the reference renders one
  * camera per launch). Many views of uneven size and spp rendered by one
  * persistent launch whose workgroups pull (view, pixel block) items from a

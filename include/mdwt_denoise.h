@@ -1,0 +1,51 @@
+/* C ABI of the learned denoiser beside plain accumulation (libmdwt.so).
+ *
+ * No reference counterpart: the reference accumulates plain samples only
+ * (deviceCode.cu:463-469). The denoiser is a 3-level U-Net of 3x3
+ * convolutions (32 / 64 / 96 channels, 2x2 average pooling, nearest
+ * upsampling with skip concatenation, ReLU) with BF16 weights and
+ * activations and FP32 accumulation on the matrix cores; it predicts a
+ * correction that is added to the input colour. Input per pixel, 9 floats:
+ * r, g, b (the accumulated image), first-hit depth, transmittance (1 - hit
+ * fraction) and the hit fractions of channels 0..3 (mdwt_set_aov /
+ * mdwt_read_aov). Images are row-major, pixel-interleaved float arrays.
+ * Width and height must be multiples of 4 (two pooling levels). One HIP
+ * stream per handle; not thread-safe. Status codes are those of mdwt.h. */
+#ifndef MDWT_DENOISE_H
+#define MDWT_DENOISE_H
+
+#include <stdint.h>
+
+#include "mdwt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdwt_denoiser mdwt_denoiser;
+
+/* A denoiser for width x height images on HIP device hip_device; weights
+ * initialised from seed (He initialisation, output layer near zero). */
+int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, uint32_t seed);
+int mdwt_dn_destroy(mdwt_denoiser* d);
+const char* mdwt_dn_last_error(const mdwt_denoiser* d);
+/* number of trainable parameters (weights [9][cout][cin] then bias, per layer) */
+int mdwt_dn_num_params(const mdwt_denoiser* d, int* n);
+int mdwt_dn_get_params(mdwt_denoiser* d, float* out);
+int mdwt_dn_set_params(mdwt_denoiser* d, const float* in);
+/* Denoise one image: input [h][w][9] -> output [h][w][3]; ms = GPU time of
+ * the network (uploads and downloads excluded). */
+int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms);
+/* Gradient of the mean squared error against target [h][w][3] for one
+ * image, in the parameter order of mdwt_dn_get_params; loss = that error. */
+int mdwt_dn_gradient(mdwt_denoiser* d, const float* input, const float* target, float* grad,
+                     float* loss);
+/* One Adam step (beta 0.9 / 0.999) on count (input, target) pairs stored
+ * back to back; loss = their mean squared error before the step. */
+int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targets, int count,
+                       float lr, float* loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDWT_DENOISE_H */

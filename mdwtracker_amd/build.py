@@ -35,7 +35,8 @@ COMMON = [
     "-Wall",
 ]
 
-SOURCES = ["kernels_build.hip", "kernels_render.hip", "kernels_batch.hip", "kernels_flat.hip", "mdwt_api.cpp"]
+SOURCES = ["kernels_build.hip", "kernels_render.hip", "kernels_batch.hip", "kernels_flat.hip",
+           "kernels_denoise.hip", "mdwt_api.cpp", "denoise_api.cpp"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

@@ -78,6 +78,17 @@ SIGNATURES = {
     "mdwt_batch_read_view": [_h, C.c_int, _fp, _fp],
     "mdwt_batch_view_times": [_h, _fp],
     "mdwt_autotune": [_h, _ip, C.c_int, _ip, C.c_int, _ip, C.c_int, C.c_int, _fp, _ip],
+    "mdwt_set_aov": [_h, C.c_int],
+    "mdwt_read_aov": [_h, _fp],
+    "mdwt_dn_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_uint32],
+    "mdwt_dn_destroy": [_h],
+    "mdwt_dn_last_error": [_h],
+    "mdwt_dn_num_params": [_h, _ip],
+    "mdwt_dn_get_params": [_h, _fp],
+    "mdwt_dn_set_params": [_h, _fp],
+    "mdwt_dn_infer": [_h, _fp, _fp, _fp],
+    "mdwt_dn_gradient": [_h, _fp, _fp, _fp, _fp],
+    "mdwt_dn_train_step": [_h, _fp, _fp, C.c_int, C.c_float, _fp],
 }
 
 
@@ -86,7 +97,7 @@ class View(C.Structure):
     _fields_ = [("frm", C.c_float * 3), ("at", C.c_float * 3), ("up", C.c_float * 3),
                 ("fovy", C.c_float), ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int),
                 ("seed", C.c_uint32)]
-_RESTYPES = {"mdwt_last_error": C.c_char_p}
+_RESTYPES = {"mdwt_last_error": C.c_char_p, "mdwt_dn_last_error": C.c_char_p}
 
 
 def _load() -> C.CDLL:
@@ -302,6 +313,16 @@ class MdwtRenderer:
         self._chk(lib.mdwt_autotune(self._h, _i(a), len(a), _i(b), len(b), _i(c), len(c), frames,
                                     _f(table), _i(best)))
         return table, tuple(int(x) for x in best)
+
+    def set_aov(self, on: bool = True) -> None:
+        """Accumulate the denoiser's auxiliary inputs with every frame."""
+        self._chk(lib.mdwt_set_aov(self._h, int(on)))
+
+    def read_aov(self) -> np.ndarray:
+        """[h][w][6]: first-hit distance, transmittance, hit fractions of channels 0..3."""
+        a = np.empty((self.height, self.width, 6), np.float32)
+        self._chk(lib.mdwt_read_aov(self._h, _f(a)))
+        return a
 
     def set_schedule(self, schedule: int) -> None:
         """0: one thread per pixel; 1: persistent waves (default)."""

@@ -1,0 +1,44 @@
+// Host-side launchers of the denoiser kernels (kernels_denoise.hip), used by
+// denoise_api.cpp. Tensors are NHWC: element (pixel p, channel c) of a tensor
+// with channel stride s and channel offset o is at p * s + o + c. bf16 values
+// are stored as uint16_t.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace mdwt {
+namespace dn {
+
+// 3x3 convolution (zero padding) as an implicit GEMM on the matrix cores.
+// w: bf16 [9][cout_pad][cin]; cin % 8 == 0, cout_pad % 32 == 0, xs and xo
+// multiples of 4. Writes channels < cout of y16 (bf16) and/or y32 (fp32,
+// added when accum).
+hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, const uint16_t* w,
+                   const float* bias, int cout, int cout_pad, uint16_t* y16, float* y32, int ys,
+                   int yo, bool relu, bool accum, hipStream_t s);
+// dw [9][cout][cin] += sum over pixels of g[p][co] * x[p + tap][ci]
+hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
+                 int cin, int cout, float* dw, hipStream_t s);
+// gm = bf16(gy * relu'(y)) [npix][cout_pad]; db += per-channel sums
+hipError_t relu_grad(const float* gy, int gs, const uint16_t* y, int ys, int yo, int npix, int cout,
+                     int cout_pad, bool relu, uint16_t* gm, float* db, hipStream_t s);
+hipError_t pool(const uint16_t* x, int H, int W, int C, uint16_t* y, hipStream_t s);
+hipError_t pool_grad(const float* gy, int H, int W, int C, float* gx, hipStream_t s);
+hipError_t up(const uint16_t* x, int H, int W, int C, uint16_t* y, int ys, int yo, hipStream_t s);
+hipError_t up_grad(const float* gy, int H, int W, int ys, int yo, int C, float* gx, hipStream_t s);
+hipError_t copy_slice(const uint16_t* x, int xs, int xo, int n, int C, uint16_t* y, int ys, int yo,
+                      hipStream_t s);
+hipError_t add_slice(const float* gy, int ys, int yo, int n, int C, float* gx, hipStream_t s);
+hipError_t pack_input(const float* in, int n, uint16_t* x, hipStream_t s);
+hipError_t residual(const float* in, const float* out, int n, float* pred, hipStream_t s);
+hipError_t mse(const float* pred, const float* target, int n, float inv_count, float* gout,
+               float* err, hipStream_t s);
+hipError_t adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
+                float c1, float c2, float eps, uint16_t* w16, hipStream_t s);
+hipError_t weight_layouts(const float* w, int cin, int cout, int cin_pad, int cout_pad,
+                          uint16_t* wf, int cin_pad32, int cout_pad8, uint16_t* wd, hipStream_t s);
+
+}  // namespace dn
+}  // namespace mdwt

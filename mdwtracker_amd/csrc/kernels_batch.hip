@@ -91,12 +91,15 @@ RenderFn select_batch(int mode, int dtype, bool count, bool lit) {
     case 6: return MDWT_BATCH(6);
     case 7: return MDWT_BATCH(7);
     case 8: return MDWT_BATCH(8);
-    default: return count ? batch_fn_dt<9, true, false>(dtype) : batch_fn_dt<9, false, false>(dtype);
+    default: return MDWT_BATCH(9);
   }
 }
 #undef MDWT_BATCH
 
-bool is_lit(const RenderParams& p) { return p.fr.sh.shadows != 0 || p.fr.sh.gradient != 0; }
+// the extended kernel: shadow rays, gradient shading or denoiser inputs
+bool is_lit(const RenderParams& p) {
+  return p.fr.sh.shadows != 0 || p.fr.sh.gradient != 0 || p.fr.aov != nullptr;
+}
 
 hipError_t prepare(RenderFn f, size_t lds) {
   if (lds <= 65536) return hipSuccess;

@@ -1,0 +1,372 @@
+// Kernels of the learned denoiser (a 3-level U-Net of 3x3 convolutions): BF16
+// activations and weights, FP32 accumulation on the matrix cores
+// (v_mfma_f32_32x32x8bf16_1k), FP32 gradients and Adam state. No reference
+// counterpart (the reference renders plain accumulation only).
+//
+// Tensors are NHWC with a channel stride (so that a convolution can read or
+// write a slice of a concatenated tensor). A 3x3 convolution is an implicit
+// GEMM: M = pixels, N = output channels, K = 9 taps x input channels. One
+// wave computes a 32-pixel x 32-channel tile; the MFMA lane layout is
+//   A (32 x 8):  lane l holds A[l % 32][(l / 32) * 4 + j], j = 0..3
+//   B (8 x 32):  lane l holds B[(l / 32) * 4 + j][l % 32]
+//   D (32 x 32): lane l, register v holds D[(v / 4) * 8 + (l / 32) * 4 + v % 4][l % 32]
+// (checked on the device with known matrices).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "denoise_internal.hpp"
+
+namespace mdwt {
+namespace dn {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (finite inputs)
+  const uint32_t u = __float_as_uint(f);
+  return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// 4 consecutive bf16 (8 bytes, 8-byte aligned) or zeros
+__device__ __forceinline__ s4 load4(const uint16_t* p, bool ok) {
+  if (!ok) return s4{0, 0, 0, 0};
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  return s4{short(v.x & 0xffffu), short(v.x >> 16), short(v.y & 0xffffu), short(v.y >> 16)};
+}
+
+// y = conv3x3(x) (+ bias, ReLU): x [H*W][xs] (channels xo .. xo+cin), weights
+// w [9][cout_pad][cin] (tap-major, K contiguous), output channel n of pixel p
+// to y16[p * ys + yo + n] (bf16) and/or y32[p * ys + yo + n] (fp32, added to
+// the existing value when accum). cin is a multiple of 8, cout_pad of 32;
+// channels n >= cout are not written. Grid: (ceil(H*W / 32), cout_pad / 32),
+// one wave per block.
+__global__ __launch_bounds__(64) void conv3x3_kernel(const uint16_t* __restrict__ x, int H, int W,
+                                                     int xs, int xo, int cin,
+                                                     const uint16_t* __restrict__ w,
+                                                     const float* __restrict__ bias, int cout,
+                                                     int cout_pad, uint16_t* y16, float* y32,
+                                                     int ys, int yo, int relu, int accum) {
+  const int l = threadIdx.x;
+  const int npix = H * W;
+  const int p0 = blockIdx.x * 32;
+  const int n0 = blockIdx.y * 32;
+  const int pa = p0 + (l & 31);  // pixel of this lane's A row
+  const int pay = pa / W, pax = pa - pay * W;
+  const int kq = (l >> 5) * 4;  // this lane's 4 K values inside a chunk of 8
+  f16v acc = {};
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int sy = pay + dy, sx = pax + dx;
+    const bool in = pa < npix && sy >= 0 && sy < H && sx >= 0 && sx < W;
+    const uint16_t* xa = x + (size_t(in ? sy * W + sx : 0) * xs + xo + kq);
+    const uint16_t* wb = w + (size_t(tap) * cout_pad + n0 + (l & 31)) * cin + kq;
+    for (int kc = 0; kc < cin; kc += 8) {
+      const s4 a = load4(xa + kc, in);
+      const s4 b = load4(wb + kc, true);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+  }
+  const int n = n0 + (l & 31);
+  if (n >= cout) return;
+  const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int p = p0 + (v >> 2) * 8 + (l >> 5) * 4 + (v & 3);
+    if (p >= npix) continue;
+    float r = acc[v] + bn;
+    if (relu) r = fmaxf(r, 0.f);
+    const size_t o = size_t(p) * ys + yo + n;
+    if (y16) y16[o] = f2bf(r);
+    if (y32) y32[o] = accum ? y32[o] + r : r;
+  }
+}
+
+// dW[tap][co][ci] += sum_p g[p][co] * x[p + tap][ci] over a chunk of pixels.
+// g: bf16 [H*W][gs] (the masked output gradient), x: bf16 [H*W][xs] channels
+// xo .. xo+cin. One wave per (tap, co tile, ci tile, pixel chunk); atomics
+// reduce the chunks. dw is fp32 [9][cout][cin].
+__global__ __launch_bounds__(64) void wgrad_kernel(const uint16_t* __restrict__ g, int gs,
+                                                   const uint16_t* __restrict__ x, int xs, int xo,
+                                                   int H, int W, int cin, int cout, int chunk,
+                                                   float* dw) {
+  const int l = threadIdx.x;
+  const int tap = blockIdx.z % 9, pc = blockIdx.z / 9;
+  const int co0 = blockIdx.y * 32, ci0 = blockIdx.x * 32;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const int npix = H * W;
+  const int pbeg = pc * chunk, pend = min(npix, pbeg + chunk);
+  const int co = co0 + (l & 31), ci = ci0 + (l & 31);
+  const int kq = (l >> 5) * 4;
+  f16v acc = {};
+  for (int pb = pbeg; pb < pend; pb += 8) {
+    s4 a, b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = pb + kq + j;
+      uint16_t av = 0, bv = 0;
+      if (p < pend) {
+        if (co < cout) av = g[size_t(p) * gs + co];
+        const int py = p / W, px = p - py * W;
+        const int sy = py + dy, sx = px + dx;
+        if (ci < cin && sy >= 0 && sy < H && sx >= 0 && sx < W)
+          bv = x[size_t(sy * W + sx) * xs + xo + ci];
+      }
+      a[j] = short(av);
+      b[j] = short(bv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  const int cc = ci0 + (l & 31);
+  if (cc >= cin) return;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int r = co0 + (v >> 2) * 8 + (l >> 5) * 4 + (v & 3);
+    if (r < cout) atomicAdd(&dw[(size_t(tap) * cout + r) * cin + cc], acc[v]);
+  }
+}
+
+// gm = bf16(gy * (y > 0)) for ReLU layers (gy * 1 otherwise).
+// gy: fp32 [n][gs]; y: bf16 [n][ys] at offset yo; gm: bf16 [n][cout_pad]
+// (channels >= cout zeroed). One thread per (pixel, channel).
+__global__ void relu_grad_kernel(const float* __restrict__ gy, int gs,
+                                 const uint16_t* __restrict__ y, int ys, int yo, int npix,
+                                 int cout, int cout_pad, int relu, uint16_t* gm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix * cout_pad) return;
+  const int p = i / cout_pad, c = i - p * cout_pad;
+  float v = 0.f;
+  if (c < cout) {
+    v = gy[size_t(p) * gs + c];
+    if (relu && !(bf2f(y[size_t(p) * ys + yo + c]) > 0.f)) v = 0.f;
+  }
+  gm[size_t(p) * cout_pad + c] = f2bf(v);
+}
+
+// db[c] += sum over a chunk of pixels of gm[p][c]: block (c, chunk), 256
+// threads, one atomic per block
+__global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ gm,
+                                                        int cout_pad, int npix, int chunk,
+                                                        float* db) {
+  __shared__ float part[4];
+  const int c = blockIdx.x;
+  const int p0 = blockIdx.y * chunk, p1 = min(npix, p0 + chunk);
+  float sum = 0.f;
+  for (int p = p0 + int(threadIdx.x); p < p1; p += blockDim.x) sum += bf2f(gm[size_t(p) * cout_pad + c]);
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&db[c], part[0] + part[1] + part[2] + part[3]);
+}
+
+// 2x2 average pool: y[(py, px)] = mean of x over the 2x2 block (bf16, C channels)
+__global__ void pool_kernel(const uint16_t* x, int H, int W, int C, uint16_t* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = H / 2, w = W / 2;
+  if (i >= h * w * C) return;
+  const int c = i % C, q = i / C, qy = q / w, qx = q - qy * w;
+  const size_t b = (size_t(2 * qy) * W + 2 * qx) * C + c;
+  const float s = bf2f(x[b]) + bf2f(x[b + C]) + bf2f(x[b + size_t(W) * C]) + bf2f(x[b + size_t(W) * C + C]);
+  y[i] = f2bf(0.25f * s);
+}
+// gx[p] += gy[p / 2] / 4 (fp32; gx channel stride C)
+__global__ void pool_grad_kernel(const float* gy, int H, int W, int C, float* gx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= H * W * C) return;
+  const int c = i % C, p = i / C, py = p / W, px = p - py * W;
+  gx[i] += 0.25f * gy[(size_t(py / 2) * (W / 2) + px / 2) * C + c];
+}
+// nearest upsample into a channel slice: y[(p)][yo + c] = x[p / 2][c]
+__global__ void up_kernel(const uint16_t* x, int H, int W, int C, uint16_t* y, int ys, int yo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= H * W * C) return;
+  const int c = i % C, p = i / C, py = p / W, px = p - py * W;
+  y[size_t(p) * ys + yo + c] = x[(size_t(py / 2) * (W / 2) + px / 2) * C + c];
+}
+// gx[q][c] = sum of gy over the 2x2 block of q (gy: slice yo of stride ys)
+__global__ void up_grad_kernel(const float* gy, int H, int W, int ys, int yo, int C, float* gx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = H / 2, w = W / 2;
+  if (i >= h * w * C) return;
+  const int c = i % C, q = i / C, qy = q / w, qx = q - qy * w;
+  const size_t b = (size_t(2 * qy) * W + 2 * qx) * ys + yo + c;
+  gx[i] = gy[b] + gy[b + ys] + gy[b + size_t(W) * ys] + gy[b + size_t(W) * ys + ys];
+}
+// copy a channel slice (bf16): y[p][yo + c] = x[p][xo + c]
+__global__ void copy_slice_kernel(const uint16_t* x, int xs, int xo, int n, int C, uint16_t* y,
+                                  int ys, int yo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * C) return;
+  const int c = i % C, p = i / C;
+  y[size_t(p) * ys + yo + c] = x[size_t(p) * xs + xo + c];
+}
+// gx[p][c] += gy[p][yo + c] (fp32 slices)
+__global__ void add_slice_kernel(const float* gy, int ys, int yo, int n, int C, float* gx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * C) return;
+  const int c = i % C, p = i / C;
+  gx[i] += gy[size_t(p) * ys + yo + c];
+}
+
+// network input: in (fp32 [n][9]: rgb, depth, transmittance, 4 channel
+// fractions) -> bf16 [n][16] (channels 9..15 zero)
+__global__ void pack_input_kernel(const float* in, int n, uint16_t* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * 16) return;
+  const int p = i / 16, c = i - p * 16;
+  x[i] = c < 9 ? f2bf(in[size_t(p) * 9 + c]) : uint16_t(0);
+}
+// prediction = input rgb + network output (fp32 [n][32], first 3 channels)
+__global__ void residual_kernel(const float* in, const float* out, int n, float* pred) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * 3) return;
+  const int p = i / 3, c = i - p * 3;
+  pred[i] = in[size_t(p) * 9 + c] + out[size_t(p) * 32 + c];
+}
+// squared error: g = 2 (pred - target) / count into gout [n][32] (rgb
+// channels), err accumulates sum (pred - target)^2
+__global__ void mse_kernel(const float* pred, const float* target, int n, float inv_count,
+                           float* gout, float* err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float e = 0.f;
+  if (i < n * 3) {
+    const int p = i / 3, c = i - p * 3;
+    const float d = pred[i] - target[i];
+    gout[size_t(p) * 32 + c] = 2.f * d * inv_count;
+    e = d * d;
+  }
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(err, e);
+}
+
+// Adam step on fp32 masters (Kingma & Ba 2015), then the bf16 forward copy.
+__global__ void adam_kernel(float* w, const float* g, float* m, float* v, int n, float lr,
+                            float b1, float b2, float c1, float c2, float eps, uint16_t* w16) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (w16) w16[i] = f2bf(wi);
+}
+
+// forward weights [9][cout_pad][cin_pad] (bf16) and dgrad weights
+// [9][cin_pad32][cout_pad8] = w[8 - tap][co][ci] from the fp32 master
+// [9][cout][cin]; padding zero.
+__global__ void weight_layouts_kernel(const float* w, int cin, int cout, int cin_pad, int cout_pad,
+                                      uint16_t* wf, int cin_pad32, int cout_pad8, uint16_t* wd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nf = 9 * cout_pad * cin_pad, nd = 9 * cin_pad32 * cout_pad8;
+  if (i < nf) {
+    const int ci = i % cin_pad, r = i / cin_pad, co = r % cout_pad, tap = r / cout_pad;
+    wf[i] = (ci < cin && co < cout) ? f2bf(w[(size_t(tap) * cout + co) * cin + ci]) : uint16_t(0);
+  }
+  if (i < nd) {
+    const int co = i % cout_pad8, r = i / cout_pad8, ci = r % cin_pad32, tap = r / cin_pad32;
+    wd[i] = (ci < cin && co < cout) ? f2bf(w[(size_t(8 - tap) * cout + co) * cin + ci])
+                                    : uint16_t(0);
+  }
+}
+
+inline unsigned nb(size_t n, unsigned t = 256) { return unsigned((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, const uint16_t* w,
+                   const float* bias, int cout, int cout_pad, uint16_t* y16, float* y32, int ys,
+                   int yo, bool relu, bool accum, hipStream_t s) {
+  const dim3 grid(nb(size_t(H) * W, 32), unsigned(cout_pad / 32));
+  hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(64), 0, s, x, H, W, xs, xo, cin, w, bias, cout,
+                     cout_pad, y16, y32, ys, yo, relu ? 1 : 0, accum ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
+                 int cin, int cout, float* dw, hipStream_t s) {
+  const int chunk = 2048;
+  const int npix = H * W;
+  const dim3 grid(unsigned((cin + 31) / 32), unsigned((cout + 31) / 32),
+                  unsigned(9 * ((npix + chunk - 1) / chunk)));
+  hipLaunchKernelGGL(wgrad_kernel, grid, dim3(64), 0, s, g, gs, x, xs, xo, H, W, cin, cout, chunk,
+                     dw);
+  return hipGetLastError();
+}
+
+hipError_t relu_grad(const float* gy, int gs, const uint16_t* y, int ys, int yo, int npix, int cout,
+                     int cout_pad, bool relu, uint16_t* gm, float* db, hipStream_t s) {
+  hipLaunchKernelGGL(relu_grad_kernel, dim3(nb(size_t(npix) * cout_pad)), dim3(256), 0, s, gy, gs,
+                     y, ys, yo, npix, cout, cout_pad, relu ? 1 : 0, gm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int chunk = 4096;
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(unsigned(cout), unsigned((npix + chunk - 1) / chunk)),
+                     dim3(256), 0, s, gm, cout_pad, npix, chunk, db);
+  return hipGetLastError();
+}
+
+hipError_t pool(const uint16_t* x, int H, int W, int C, uint16_t* y, hipStream_t s) {
+  hipLaunchKernelGGL(pool_kernel, dim3(nb(size_t(H / 2) * (W / 2) * C)), dim3(256), 0, s, x, H, W,
+                     C, y);
+  return hipGetLastError();
+}
+hipError_t pool_grad(const float* gy, int H, int W, int C, float* gx, hipStream_t s) {
+  hipLaunchKernelGGL(pool_grad_kernel, dim3(nb(size_t(H) * W * C)), dim3(256), 0, s, gy, H, W, C,
+                     gx);
+  return hipGetLastError();
+}
+hipError_t up(const uint16_t* x, int H, int W, int C, uint16_t* y, int ys, int yo, hipStream_t s) {
+  hipLaunchKernelGGL(up_kernel, dim3(nb(size_t(H) * W * C)), dim3(256), 0, s, x, H, W, C, y, ys,
+                     yo);
+  return hipGetLastError();
+}
+hipError_t up_grad(const float* gy, int H, int W, int ys, int yo, int C, float* gx, hipStream_t s) {
+  hipLaunchKernelGGL(up_grad_kernel, dim3(nb(size_t(H / 2) * (W / 2) * C)), dim3(256), 0, s, gy, H,
+                     W, ys, yo, C, gx);
+  return hipGetLastError();
+}
+hipError_t copy_slice(const uint16_t* x, int xs, int xo, int n, int C, uint16_t* y, int ys, int yo,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(copy_slice_kernel, dim3(nb(size_t(n) * C)), dim3(256), 0, s, x, xs, xo, n, C,
+                     y, ys, yo);
+  return hipGetLastError();
+}
+hipError_t add_slice(const float* gy, int ys, int yo, int n, int C, float* gx, hipStream_t s) {
+  hipLaunchKernelGGL(add_slice_kernel, dim3(nb(size_t(n) * C)), dim3(256), 0, s, gy, ys, yo, n, C,
+                     gx);
+  return hipGetLastError();
+}
+hipError_t pack_input(const float* in, int n, uint16_t* x, hipStream_t s) {
+  hipLaunchKernelGGL(pack_input_kernel, dim3(nb(size_t(n) * 16)), dim3(256), 0, s, in, n, x);
+  return hipGetLastError();
+}
+hipError_t residual(const float* in, const float* out, int n, float* pred, hipStream_t s) {
+  hipLaunchKernelGGL(residual_kernel, dim3(nb(size_t(n) * 3)), dim3(256), 0, s, in, out, n, pred);
+  return hipGetLastError();
+}
+hipError_t mse(const float* pred, const float* target, int n, float inv_count, float* gout,
+               float* err, hipStream_t s) {
+  hipLaunchKernelGGL(mse_kernel, dim3(nb(size_t(n) * 3)), dim3(256), 0, s, pred, target, n,
+                     inv_count, gout, err);
+  return hipGetLastError();
+}
+hipError_t adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
+                float c1, float c2, float eps, uint16_t* w16, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3(nb(size_t(n))), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2,
+                     c1, c2, eps, w16);
+  return hipGetLastError();
+}
+hipError_t weight_layouts(const float* w, int cin, int cout, int cin_pad, int cout_pad,
+                          uint16_t* wf, int cin_pad32, int cout_pad8, uint16_t* wd, hipStream_t s) {
+  const size_t n = std::max(size_t(9) * cout_pad * cin_pad, size_t(9) * cin_pad32 * cout_pad8);
+  hipLaunchKernelGGL(weight_layouts_kernel, dim3(nb(n)), dim3(256), 0, s, w, cin, cout, cin_pad,
+                     cout_pad, wf, cin_pad32, cout_pad8, wd);
+  return hipGetLastError();
+}
+
+}  // namespace dn
+}  // namespace mdwt

@@ -17,9 +17,10 @@
 namespace mdwt {
 namespace {
 
-// LIT: shadow rays / gradient shading compiled in. The unlit instances keep
-// the register budget of the plain method (the lit code needs ~15 more VGPRs,
-// which would cost a wave per SIMD). Workgroup i renders pixel block
+// LIT: shadow rays, gradient shading and the denoiser's auxiliary outputs
+// compiled in. The other instances keep the register budget of the plain
+// method (the extra code needs ~15 more VGPRs, which would cost a wave per
+// SIMD). Workgroup i renders pixel block
 // block_order[i] of the single view.
 template <int MODE, int DT, bool COUNT, bool LIT>
 __global__ __launch_bounds__(256) void render_kernel(RenderParams P) {
@@ -152,7 +153,8 @@ RenderFn static_fn_dt(int dtype) {
 #define MDWT_STATIC(M)                                                                 \
   (count ? (lit ? static_fn_dt<M, true, true>(dtype) : static_fn_dt<M, true, false>(dtype)) \
          : (lit ? static_fn_dt<M, false, true>(dtype) : static_fn_dt<M, false, false>(dtype)))
-// -m 9 has no lighting (its shadow code is commented out in the reference).
+// (-m 9 has no lighting: its shadow code is commented out in the reference;
+// its LIT instance only adds the auxiliary outputs.)
 RenderFn select_render(int mode, int dtype, bool count, int schedule, bool lit) {
   switch (mode) {  // modes with the thread-per-pixel kernel only
     case 1: return MDWT_STATIC(1);
@@ -161,7 +163,7 @@ RenderFn select_render(int mode, int dtype, bool count, int schedule, bool lit) 
     case 6: return MDWT_STATIC(6);
     case 7: return MDWT_STATIC(7);
     case 8: return MDWT_STATIC(8);
-    case 9: return count ? static_fn_dt<9, true, false>(dtype) : static_fn_dt<9, false, false>(dtype);
+    case 9: return MDWT_STATIC(9);
     default: break;
   }
   if (!lit && (schedule == 1 || schedule == 2))
@@ -189,7 +191,10 @@ hipError_t persistent_blocks_per_cu(int mode, int dtype, bool count, size_t lds,
 }
 
 namespace {
-bool is_lit(const RenderParams& p) { return p.fr.sh.shadows != 0 || p.fr.sh.gradient != 0; }
+// the extended kernel: shadow rays, gradient shading or denoiser inputs
+bool is_lit(const RenderParams& p) {
+  return p.fr.sh.shadows != 0 || p.fr.sh.gradient != 0 || p.fr.aov != nullptr;
+}
 hipError_t prepare(RenderFn f, size_t lds) {
   if (lds <= 65536) return hipSuccess;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(f),

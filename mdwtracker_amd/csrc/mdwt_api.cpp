@@ -117,6 +117,8 @@ struct mdwt_renderer {
   int n_blocks = 0;
   int* d_win = nullptr;
   uint32_t* d_wave_cost = nullptr;  // per-wave ticks of the last recorded frame
+  bool aov_on = false;
+  Float4* d_aov = nullptr;          // 2 per pixel when aov_on
   bool cost_ready = false;
   bool need_resort = true;  // record costs and re-sort after the next frames
 
@@ -333,6 +335,8 @@ void hilbert_d2xy(int bits, uint32_t d, uint32_t& ox, uint32_t& oy) {
 }
 
 void free_frame(mdwt_renderer* h) {
+  if (h->d_aov) (void)hipFree(h->d_aov);
+  h->d_aov = nullptr;
   if (h->d_accum) (void)hipFree(h->d_accum);
   if (h->d_m2) (void)hipFree(h->d_m2);
   if (h->d_order) (void)hipFree(h->d_order);
@@ -411,6 +415,10 @@ int alloc_frame(mdwt_renderer* h) {
   HIPCHK(h, hipMemsetAsync(h->d_accum, 0, n * sizeof(Float4), h->stream));
   HIPCHK(h, hipMemsetAsync(h->d_m2, 0, n * sizeof(Float4), h->stream));
   h->fb_pixels = n;
+  if (h->aov_on) {
+    HIPCHK(h, hipMalloc(&h->d_aov, 2 * n * sizeof(Float4)));
+    HIPCHK(h, hipMemsetAsync(h->d_aov, 0, 2 * n * sizeof(Float4), h->stream));
+  }
   // (the flat / persistent schedules only run with 16x16 blocks: mdwt_set_block
   // and mdwt_set_schedule keep bw = bh = 16 whenever schedule != 0)
   const int bw = h->bw, bh = h->bh;
@@ -503,9 +511,10 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
 }
 
 // The flat schedules cover -m 0/2/3; every other mode runs the thread-per-pixel kernel.
-// Shadow rays and gradient shading run in the thread-per-pixel kernel only.
+// Shadow rays, gradient shading and auxiliary outputs run in the
+// thread-per-pixel kernel only.
 int eff_schedule(const mdwt_renderer* h) {
-  if (h->sh.shadows || h->sh.gradient) return 0;
+  if (h->sh.shadows || h->sh.gradient || h->aov_on) return 0;
   return (h->mode == 0 || h->mode == 2 || h->mode == 3) ? h->schedule : 0;
 }
 
@@ -547,6 +556,7 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.n_items = 0;
   p.fr.work = nullptr;
   p.fr.item_ticks = nullptr;
+  p.fr.aov = h->d_aov;
   return p;
 }
 
@@ -1366,7 +1376,7 @@ mdwt::RenderParams batch_params(const mdwt_renderer* h) {
 // resident workgroups of the persistent batched launch (cached per kernel
 // choice and LDS size)
 int batch_grid(mdwt_renderer* h, const mdwt::RenderParams& p, bool count, int* grid) {
-  const bool lit = p.fr.sh.shadows || p.fr.sh.gradient;
+  const bool lit = p.fr.sh.shadows || p.fr.sh.gradient || p.fr.aov != nullptr;
   const int key = int(mdwt::render_lds_bytes(p)) * 64 + h->mode * 16 + h->bw * h->bh / 64 * 2 +
                   (count ? 1 : 0) + (lit ? 1 << 30 : 0);
   if (key != h->batch_grid_key) {
@@ -1716,6 +1726,42 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
     best[0] = bi[0];
     best[1] = bi[1];
     best[2] = bi[2];
+  }
+  return MDWT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mdwt_set_aov(mdwt_renderer* h, int on) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if ((on != 0) == h->aov_on) return MDWT_OK;
+  h->aov_on = on != 0;
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int rc = alloc_frame(h);  // (re)allocates the buffers; accumulation restarts
+  reset_acc(h);
+  return rc;
+}
+
+int mdwt_read_aov(mdwt_renderer* h, float* out) {
+  if (!h || !out) return MDWT_ERR_ARG;
+  if (!h->aov_on || !h->d_aov) return fail(h, MDWT_ERR_STATE, "auxiliary buffers are off");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<Float4> tmp(2 * h->fb_pixels);
+  HIPCHK(h, hipMemcpy(tmp.data(), h->d_aov, tmp.size() * sizeof(Float4), hipMemcpyDeviceToHost));
+  for (size_t p = 0; p < h->fb_pixels; ++p) {
+    const Float4 a = tmp[2 * p], b = tmp[2 * p + 1];
+    float* o = out + 6 * p;
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    o[3] = a.w;
+    o[4] = b.x;
+    o[5] = b.y;
   }
   return MDWT_OK;
 }

@@ -548,6 +548,25 @@ struct ViewCtx {
   void* m2;
 };
 
+// Denoiser inputs of one sample folded into the pixel's running means:
+// first-hit distance (0 on a miss), transmittance (1 on a miss, 0 on a hit)
+// and a one-hot of the hit channel (channels 0..3).
+__device__ __forceinline__ void aov_sample(const RenderParams& P, const ViewCtx& V, uint32_t pixel,
+                                           int s, const HitRec& h) {
+  float4* a = reinterpret_cast<float4*>(P.fr.aov) + 2 * size_t(pixel);
+  const uint32_t n = (P.fr.accumulate ? V.accum_id * uint32_t(V.spp) : 0u) + uint32_t(s);
+  const float k = float(n), k1 = float(n + 1u);
+  const bool hit = !h.missed;
+  const float d = hit ? h.tHit : 0.f, T = hit ? 0.f : 1.f;
+  const float c0 = hit && h.channel == 0 ? 1.f : 0.f, c1 = hit && h.channel == 1 ? 1.f : 0.f;
+  const float c2 = hit && h.channel == 2 ? 1.f : 0.f, c3 = hit && h.channel == 3 ? 1.f : 0.f;
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 a0 = n ? a[0] : z, a1 = n ? a[1] : z;
+  a[0] = make_float4((d + k * a0.x) / k1, (T + k * a0.y) / k1, (c0 + k * a0.z) / k1,
+                     (c1 + k * a0.w) / k1);
+  a[1] = make_float4((c2 + k * a1.x) / k1, (c3 + k * a1.y) / k1, 0.f, 0.f);
+}
+
 // One workgroup's pixel block (bw x bh pixels, one 8x8 quad per wave): every
 // pixel's samples, shading and accumulation.
 template <int MODE, int DT, bool COUNT, bool LIT>
@@ -571,6 +590,7 @@ __device__ __forceinline__ void render_block(const RenderParams& P, const Ctx<DT
     h.missed = true;
     h.a = 1.f;
     h.channel = 0;
+    h.tHit = 0.f;
     float diffuse = 1.0f, specular = 0.0f, shadow = 1.0f;
     if (MODE <= 5) {
       // primary ray, then (when shadows are on and it hit) one shadow ray
@@ -611,6 +631,7 @@ __device__ __forceinline__ void render_block(const RenderParams& P, const Ctx<DT
     cnt.rays++;
     composite_sample(P, !h.missed, h.r, h.g, h.b, h.a, diffuse, specular, shadow, h.channel, fr,
                      fg, fb, fa);
+    if (LIT && !COUNT && P.fr.aov) aov_sample(P, V, pixel, s, h);
   }
   if (!COUNT) finish_pixel(P, V.accum, V.m2, V.accum_id, V.spp, pixel, fr, fg, fb, fa);
 }

@@ -1,0 +1,181 @@
+"""GPU checks of the learned denoiser and of the auxiliary buffers it reads.
+
+The BF16 U-Net's forward pass and its gradient are compared with a float32
+PyTorch restatement of the same network on the CPU that rounds weights and
+stored activations to bfloat16 at the same places the kernels do (products
+are exact in fp32 either way; only the summation order differs), with the
+tolerances written in each test. Training must reduce the loss on a fixed
+set. The auxiliary buffers must not change the image, and per pixel the
+transmittance plus the channel fractions is 1 (every sample either misses
+or hits exactly one channel)."""
+import numpy as np
+import pytest
+
+from mdwtracker_amd import scenes
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+LAYERS = [(9, 32, True), (32, 32, True), (32, 64, True), (64, 64, True), (64, 96, True),
+          (96, 96, True), (160, 64, True), (64, 64, True), (96, 32, True), (32, 32, True),
+          (32, 3, False)]
+
+
+@pytest.fixture(scope="module")
+def dnmod():
+    from mdwtracker_amd import denoise
+    return denoise
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def _unpack(p):
+    out, o = [], 0
+    for cin, cout, _ in LAYERS:
+        w = p[o:o + 9 * cout * cin].reshape(9, cout, cin)
+        o += 9 * cout * cin
+        b = p[o:o + cout]
+        o += cout
+        out.append((w, b))
+    assert o == p.size
+    return out
+
+
+def _torch_forward(params, x9):
+    """float32 restatement of the U-Net with the kernels' bf16 rounding."""
+    F = torch.nn.functional
+    layers = []
+    for (w, b), (cin, cout, relu) in zip(_unpack(params), LAYERS):
+        wt = torch.tensor(w).reshape(3, 3, cout, cin).permute(2, 3, 0, 1).contiguous()
+        layers.append((_bf(wt).requires_grad_(True), torch.tensor(b).requires_grad_(True), relu))
+
+    def conv(i, x, store_bf16=True):
+        w, b, relu = layers[i]
+        y = F.conv2d(x, w, b, padding=1)
+        if relu:
+            y = F.relu(y)
+        return _bf(y) if store_bf16 else y
+
+    x = torch.tensor(x9).permute(2, 0, 1)[None]
+    x0 = _bf(x)
+    a1 = conv(1, conv(0, x0))
+    p1 = _bf(F.avg_pool2d(a1, 2))
+    a3 = conv(3, conv(2, p1))
+    p2 = _bf(F.avg_pool2d(a3, 2))
+    a5 = conv(5, conv(4, p2))
+    u2 = torch.cat([F.interpolate(a5, scale_factor=2, mode="nearest"), a3], 1)
+    a7 = conv(7, conv(6, u2))
+    u1 = torch.cat([F.interpolate(a7, scale_factor=2, mode="nearest"), a1], 1)
+    a9 = conv(9, conv(8, u1))
+    out = conv(10, a9, store_bf16=False)
+    pred = x[:, :3] + out
+    return pred[0].permute(1, 2, 0), layers
+
+
+def _rand_input(rng, h, w):
+    x = np.zeros((h, w, 9), np.float32)
+    x[..., :3] = rng.uniform(0, 1.2, (h, w, 3))
+    x[..., 3] = rng.uniform(0, 1, (h, w))
+    frac = rng.dirichlet(np.ones(5), size=(h, w)).astype(np.float32)
+    x[..., 4:9] = frac
+    return x
+
+
+def test_forward_matches_torch_reference(dnmod):
+    rng = np.random.default_rng(0)
+    h, w = 32, 48
+    dn = dnmod.Denoiser(w, h, seed=3)
+    p = dn.get_params()
+    # a non-trivial output layer (the default starts near zero)
+    p = p + rng.normal(0, 0.02, p.size).astype(np.float32)
+    dn.set_params(p)
+    x = _rand_input(rng, h, w)
+    got, ms = dn.infer(x)
+    with torch.no_grad():
+        want, _ = _torch_forward(p, x)
+    want = want.numpy()
+    err = np.abs(got - want)
+    scale = np.abs(want - x[..., :3]).max()
+    print(f"[denoise] forward: max |err| {err.max():.3e}, correction scale {scale:.3e}, {ms:.3f} ms", flush=True)
+    assert scale > 1e-2
+    # bf16 storage of activations: a stored value can round the other way
+    # after a different fp32 summation order (1 bf16 ulp, 2^-8 relative)
+    assert err.max() < 2e-2 * scale
+    assert np.mean(err) < 2e-3 * scale
+    dn.close()
+
+
+def test_gradient_matches_torch_reference(dnmod):
+    rng = np.random.default_rng(1)
+    h, w = 16, 24
+    dn = dnmod.Denoiser(w, h, seed=5)
+    p = dn.get_params() + rng.normal(0, 0.02, dn.num_params()).astype(np.float32)
+    dn.set_params(p)
+    x = _rand_input(rng, h, w)
+    t = np.clip(x[..., :3] + rng.normal(0, 0.05, (h, w, 3)), 0, None).astype(np.float32)
+    g, loss = dn.gradient(x, t)
+    pred, layers = _torch_forward(p, x)
+    tl = torch.mean((pred - torch.tensor(t)) ** 2)
+    tl.backward()
+    assert abs(loss - tl.item()) < 1e-2 * tl.item()
+    o = 0
+    for i, ((cin, cout, _), (wt, bt, _)) in enumerate(zip(LAYERS, layers)):
+        gw = g[o:o + 9 * cout * cin].reshape(3, 3, cout, cin).transpose(2, 3, 0, 1)
+        o += 9 * cout * cin
+        gb = g[o:o + cout]
+        o += cout
+        rw = wt.grad.numpy()
+        rb = bt.grad.numpy()
+        cos = float(np.dot(gw.ravel(), rw.ravel()) / (np.linalg.norm(gw) * np.linalg.norm(rw) + 1e-30))
+        rel = float(np.linalg.norm(gw - rw) / (np.linalg.norm(rw) + 1e-30))
+        relb = float(np.linalg.norm(gb - rb) / (np.linalg.norm(rb) + 1e-30))
+        print(f"[denoise] layer {i}: cos {cos:.5f} rel {rel:.3e} bias rel {relb:.3e}", flush=True)
+        # masked gradients are rounded to bf16 before the matrix products
+        assert cos > 0.999 and rel < 5e-2 and relb < 5e-2, i
+    dn.close()
+
+
+def test_training_reduces_loss(dnmod):
+    rng = np.random.default_rng(2)
+    h = w = 32
+    n = 8
+    yy, xx = np.mgrid[0:h, 0:w] / 32.0
+    ts, xs = [], []
+    for i in range(n):
+        f = rng.uniform(1, 4)
+        clean = np.stack([0.5 + 0.4 * np.sin(f * xx + i), 0.5 + 0.4 * np.cos(f * yy),
+                          0.5 + 0.3 * np.sin(f * (xx + yy))], -1).astype(np.float32)
+        x = _rand_input(rng, h, w)
+        x[..., :3] = clean + rng.normal(0, 0.15, clean.shape)
+        xs.append(x)
+        ts.append(clean)
+    xs, ts = np.stack(xs), np.stack(ts)
+    dn = dnmod.Denoiser(w, h, seed=9)
+    losses = dnmod.train(dn, xs, ts, steps=120, batch=4, lr=2e-3)
+    print(f"[denoise] loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert np.mean(losses[-10:]) < 0.5 * losses[0]
+    dn.close()
+
+
+def test_aov_buffers():
+    from mdwtracker_amd import capi
+    sc = scenes.make_scene("ndda_cpu_2ch64", seed=4)
+    a = capi.MdwtRenderer(0)
+    b = capi.MdwtRenderer(0)
+    for r in (a, b):
+        scenes.load_scene(r, sc, 2)
+        r.set_frame(64, 48, 2, True, (0.0, 0.0, 0.0))
+    b.set_aov(True)
+    a.render(3)
+    b.render(3)
+    np.testing.assert_array_equal(a.read_accum(), b.read_accum())
+    aov = b.read_aov()
+    s = aov[..., 1] + aov[..., 2:6].sum(-1)
+    np.testing.assert_allclose(s, 1.0, atol=1e-5)
+    hit = 1.0 - aov[..., 1]
+    assert hit.max() > 0.3 and np.all(aov[..., 0][hit == 0] == 0)
+    assert np.all(aov[..., 4:6] == 0)  # two channels only
+    a.close()
+    b.close()

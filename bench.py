@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -61,7 +62,10 @@ def read_traffic(workload: str, mode: int):
     return d.get("bytes_per_launch"), str(p.relative_to(ROOT))
 
 
-def time_to_rmse(r, frames: int, target: float, ms_per_frame: float) -> dict:
+REF_SEED = 0x13579BDF  # seed of the plain reference render; other renders use other seeds
+
+
+def time_to_rmse(r, frames: int, target: float, ms_per_frame: float):
     """Frames (and seconds at the measured frame time) that plain accumulation
     needs for an expected RMSE of `target` against the converged image.
 
@@ -70,21 +74,90 @@ def time_to_rmse(r, frames: int, target: float, ms_per_frame: float) -> dict:
     and RGB): sigma^2. The mean of k independent frames has expected RMSE
     sqrt(sigma^2 / k) against the true image, so k = ceil(sigma^2 / target^2).
     The `frames`-frame render is the reference image; its own expected RMSE
-    is reported beside it."""
+    is reported beside it. Returns (result, reference mean, variance of the
+    reference mean per pixel and channel)."""
     import numpy as np
+    from mdwtracker_amd import denoise
 
-    r.reset_accumulation()
-    r.render(frames)
-    a, m2 = r.read_accum(with_m2=True)
-    mean = a[..., :3].astype(np.float64)
-    var = np.maximum(m2[..., :3].astype(np.float64) - mean * mean, 0.0) * frames / (frames - 1)
-    sigma2 = float(var.mean())
+    ref, ref_var = denoise.render_mean(r, frames, REF_SEED)
+    sigma2 = float(ref_var.mean() * frames)
     k = max(1, int(np.ceil(sigma2 / (target * target))))
-    return {"target_rmse": target, "per_frame_variance": sigma2,
-            "plain": {"frames": k, "seconds": round(k * ms_per_frame * 1e-3, 3)},
-            "denoised": None,
-            "reference_frames": frames,
-            "reference_rmse": round(float(np.sqrt(sigma2 / frames)), 6)}
+    res = {"target_rmse": target, "per_frame_variance": sigma2,
+           "plain": {"frames": k, "seconds": round(k * ms_per_frame * 1e-3, 3)},
+           "denoised": None,
+           "reference_frames": frames,
+           "reference_rmse": round(float(np.sqrt(sigma2 / frames)), 6)}
+    return res, ref, ref_var
+
+
+SWEEP_LEVELS = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768)
+
+
+def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float, args) -> dict:
+    """Train the denoiser on held-out scenes of the workload family, then run
+    one accumulation of the workload (with the auxiliary buffers) read at
+    SWEEP_LEVELS frames, denoise every read and estimate the error of the
+    plain and the denoised image against the converged image with the
+    reference's own variance removed (denoise.noise_corrected_mse). The
+    denoised time to `target` is the interpolated frame count at which its
+    error reaches it, times the frame time with auxiliary buffers, plus one
+    network pass."""
+    import numpy as np
+    from mdwtracker_amd import denoise
+
+    target = ttr["target_rmse"]
+    t0 = time.perf_counter()
+    xs, ts, ks = denoise.make_training_set(n_scenes=args.dn_scenes, crops=args.dn_crops,
+                                           size=args.dn_size, seed=args.dn_seed,
+                                           workload=scene.name, log=log)
+    t1 = time.perf_counter()
+    small = denoise.Denoiser(args.dn_size, args.dn_size, seed=1)
+    losses = denoise.train(small, xs, ts, args.dn_steps, batch=8, lr=args.dn_lr,
+                           weights=denoise.level_weights(ks), log=log)
+    params = small.get_params()
+    small.close()
+    t2 = time.perf_counter()
+    n_pairs = len(xs)
+    del xs, ts
+    log(f"[bench] denoiser: {n_pairs} pairs in {t1 - t0:.1f} s, {args.dn_steps} steps in {t2 - t1:.1f} s")
+
+    dn = denoise.Denoiser(scene.width, scene.height, seed=1)
+    dn.set_params(params)
+    r.set_aov(True)
+    r.render(16)
+    r.reset_accumulation()
+    ta = time.perf_counter()
+    r.render(256)
+    ms_aov = (time.perf_counter() - ta) * 1000.0 / 256
+    sweep, infer_ms = [], []
+    for k, x in zip(SWEEP_LEVELS, denoise.accumulate_inputs(r, SWEEP_LEVELS, REF_SEED ^ 0x2468ACE)):
+        d, ms = dn.infer(x)
+        infer_ms.append(ms)
+        pm, pse = denoise.noise_corrected_mse(x[..., :3], ref, ref_var)
+        dm, dse = denoise.noise_corrected_mse(d, ref, ref_var)
+        sweep.append({"frames": k, "plain_mse": pm, "plain_se": pse, "denoised_mse": dm,
+                      "denoised_se": dse, "plain_rmse_vs_ref": denoise.rmse(x[..., :3], ref),
+                      "denoised_rmse_vs_ref": denoise.rmse(d, ref)})
+        log(f"[bench] sweep {k:6d} frames: plain rmse {np.sqrt(max(pm, 0)):.3e} "
+            f"(model {np.sqrt(ttr['per_frame_variance'] / k):.3e}), denoised {np.sqrt(max(dm, 0)):.3e}")
+    r.set_aov(False)
+    dn.close()
+    ms_net = float(np.median(infer_ms[1:] if len(infer_ms) > 1 else infer_ms))
+    kd = denoise.frames_to_target(SWEEP_LEVELS, [s["denoised_mse"] for s in sweep], target * target)
+    out = {"frame_ms_with_aov": round(ms_aov, 4), "network_ms": round(ms_net, 3),
+           "train": {"pairs": n_pairs, "size": args.dn_size, "steps": args.dn_steps,
+                     "scenes": args.dn_scenes, "levels": list(denoise.TRAIN_LEVELS),
+                     "final_loss": float(np.mean(losses[-20:])),
+                     "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1)},
+           "sweep": [{k: (round(v, 9) if isinstance(v, float) else v) for k, v in s.items()}
+                     for s in sweep]}
+    if kd is None:
+        out.update({"frames": None, "seconds": None, "speedup": None})
+    else:
+        sec = kd * ms_aov * 1e-3 + ms_net * 1e-3
+        out.update({"frames": int(math.ceil(kd)), "seconds": round(sec, 3),
+                    "speedup": round(ttr["plain"]["seconds"] / sec, 2)})
+    return out
 
 
 def cpu_baseline(scene, mode: int, frames: int) -> dict:
@@ -115,6 +188,13 @@ def main() -> int:
     ap.add_argument("--no-extra-modes", action="store_true")
     ap.add_argument("--rmse-frames", type=int, default=4096,
                     help="frames of the reference render for time-to-RMSE (0: skip)")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoised time-to-RMSE")
+    ap.add_argument("--dn-scenes", type=int, default=4)
+    ap.add_argument("--dn-crops", type=int, default=6)
+    ap.add_argument("--dn-size", type=int, default=256)
+    ap.add_argument("--dn-steps", type=int, default=1500)
+    ap.add_argument("--dn-lr", type=float, default=1e-3)
+    ap.add_argument("--dn-seed", type=int, default=77)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -158,8 +238,13 @@ def main() -> int:
 
     ttr = None
     if args.rmse_frames > 1:
-        ttr = time_to_rmse(r, args.rmse_frames, 1e-3, ms_per_step)
-        log(f"[bench] time-to-RMSE 1e-3: {ttr}")
+        ttr, ref, ref_var = time_to_rmse(r, args.rmse_frames, 1e-3, ms_per_step)
+        log(f"[bench] plain time-to-RMSE 1e-3: {ttr}")
+        if not args.no_denoise:
+            ttr["denoised"] = denoised_time_to_rmse(r, scene, ttr, ref, ref_var, ms_per_step, args)
+            log(f"[bench] denoised time-to-RMSE 1e-3: "
+                f"{ {k: v for k, v in ttr['denoised'].items() if k != 'sweep'} }")
+        del ref, ref_var
 
     extra_modes = {}
     if not args.no_extra_modes:

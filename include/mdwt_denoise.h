@@ -5,10 +5,13 @@
  * convolutions (32 / 64 / 96 channels, 2x2 average pooling, nearest
  * upsampling with skip concatenation, ReLU) with BF16 weights and
  * activations and FP32 accumulation on the matrix cores; it predicts a
- * correction that is added to the input colour. Input per pixel, 9 floats:
+ * correction that is added to the input colour. Input per pixel, 11 floats:
  * r, g, b (the accumulated image), first-hit depth, transmittance (1 - hit
- * fraction) and the hit fractions of channels 0..3 (mdwt_set_aov /
- * mdwt_read_aov). Images are row-major, pixel-interleaved float arrays.
+ * fraction), the hit fractions of channels 0..3 (mdwt_set_aov /
+ * mdwt_read_aov), the standard error of the pixel mean (from the second
+ * moment of mdwt_read_accum) and log2 of the accumulated frame count, the
+ * last two in the scaling the caller trained with. Images are row-major,
+ * pixel-interleaved float arrays.
  * Width and height must be multiples of 4 (two pooling levels). One HIP
  * stream per handle; not thread-safe. Status codes are those of mdwt.h. */
 #ifndef MDWT_DENOISE_H
@@ -33,7 +36,7 @@ const char* mdwt_dn_last_error(const mdwt_denoiser* d);
 int mdwt_dn_num_params(const mdwt_denoiser* d, int* n);
 int mdwt_dn_get_params(mdwt_denoiser* d, float* out);
 int mdwt_dn_set_params(mdwt_denoiser* d, const float* in);
-/* Denoise one image: input [h][w][9] -> output [h][w][3]; ms = GPU time of
+/* Denoise one image: input [h][w][11] -> output [h][w][3]; ms = GPU time of
  * the network (uploads and downloads excluded). */
 int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms);
 /* Gradient of the mean squared error against target [h][w][3] for one
@@ -41,9 +44,11 @@ int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms
 int mdwt_dn_gradient(mdwt_denoiser* d, const float* input, const float* target, float* grad,
                      float* loss);
 /* One Adam step (beta 0.9 / 0.999) on count (input, target) pairs stored
- * back to back; loss = their mean squared error before the step. */
-int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targets, int count,
-                       float lr, float* loss);
+ * back to back. The gradient is that of sum_i weights[i] * MSE_i / count
+ * (weights may be NULL: all 1; each finite and >= 0); loss = the unweighted
+ * mean squared error of the pairs before the step. */
+int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targets,
+                       const float* weights, int count, float lr, float* loss);
 
 #ifdef __cplusplus
 }

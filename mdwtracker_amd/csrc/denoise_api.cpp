@@ -3,8 +3,10 @@
 // upsampling with skip concatenation, ReLU), BF16 weights and activations,
 // FP32 accumulation on the matrix cores, trained in place with Adam on pairs
 // of low- and high-sample renders. It predicts a correction added to the
-// input colour. Input per pixel (9 values): rgb, first-hit depth,
-// transmittance, the four channel fractions of the hits (mdwt_set_aov).
+// input colour. Input per pixel (kInputs = 11 values): rgb, first-hit
+// depth, transmittance, the four channel fractions of the hits
+// (mdwt_set_aov), the standard error of the pixel mean and log2 of the frame
+// count.
 // No reference counterpart; C ABI in include/mdwt_denoise.h.
 #include "mdwt_denoise.h"
 
@@ -76,10 +78,10 @@ int alloc(mdwt_denoiser* d, T** p, size_t n) {
   return MDWT_OK;
 }
 
-// layer table: (cin, cout, relu); c0 reads the 9-channel input padded to 16
-const int kLayers[11][3] = {{9, 32, 1},   {32, 32, 1}, {32, 64, 1}, {64, 64, 1},
-                            {64, 96, 1},  {96, 96, 1}, {160, 64, 1}, {64, 64, 1},
-                            {96, 32, 1},  {32, 32, 1}, {32, 3, 0}};
+// layer table: (cin, cout, relu); c0 reads the input padded to 16 channels
+const int kLayers[11][3] = {{mdwt::dn::kInputs, 32, 1}, {32, 32, 1}, {32, 64, 1},
+                            {64, 64, 1},  {64, 96, 1}, {96, 96, 1}, {160, 64, 1},
+                            {64, 64, 1},  {96, 32, 1}, {32, 32, 1}, {32, 3, 0}};
 
 int setup(mdwt_denoiser* d, uint32_t seed) {
   std::mt19937 gen(seed);
@@ -119,7 +121,7 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
       (rc = alloc(d, &d->u1, n * 96)) || (rc = alloc(d, &d->a8, n * 32)) ||
       (rc = alloc(d, &d->a9, n * 32)) || (rc = alloc(d, &d->gm, n * 32)))
     return rc;
-  if ((rc = alloc(d, &d->in32, n * 9)) || (rc = alloc(d, &d->tgt, n * 3)) ||
+  if ((rc = alloc(d, &d->in32, n * mdwt::dn::kInputs)) || (rc = alloc(d, &d->tgt, n * 3)) ||
       (rc = alloc(d, &d->out32, n * 32)) || (rc = alloc(d, &d->pred, n * 3)) ||
       (rc = alloc(d, &d->err_sum, 1)) || (rc = alloc(d, &d->gout, n * 32)) ||
       (rc = alloc(d, &d->g9, n * 32)) || (rc = alloc(d, &d->g8, n * 32)) ||
@@ -229,7 +231,8 @@ int zero_grads(mdwt_denoiser* d) {
 // upload one (input, target) pair and accumulate its loss gradient
 int one_pair(mdwt_denoiser* d, const float* in, const float* tgt, float inv_count) {
   const size_t n = size_t(d->H) * d->W;
-  DNCHK(d, hipMemcpyAsync(d->in32, in, n * 9 * sizeof(float), hipMemcpyHostToDevice, d->stream));
+  DNCHK(d, hipMemcpyAsync(d->in32, in, n * mdwt::dn::kInputs * sizeof(float),
+                          hipMemcpyHostToDevice, d->stream));
   DNCHK(d, hipMemcpyAsync(d->tgt, tgt, n * 3 * sizeof(float), hipMemcpyHostToDevice, d->stream));
   int rc = forward(d);
   if (rc) return rc;
@@ -323,7 +326,8 @@ int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms
   if (!d || !input || !output) return MDWT_ERR_ARG;
   DNCHK(d, hipSetDevice(d->device));
   const size_t n = size_t(d->H) * d->W;
-  DNCHK(d, hipMemcpyAsync(d->in32, input, n * 9 * sizeof(float), hipMemcpyHostToDevice, d->stream));
+  DNCHK(d, hipMemcpyAsync(d->in32, input, n * mdwt::dn::kInputs * sizeof(float),
+                          hipMemcpyHostToDevice, d->stream));
   DNCHK(d, hipEventRecord(d->ev0, d->stream));
   int rc = forward(d);
   if (rc) return rc;
@@ -360,9 +364,11 @@ int mdwt_dn_gradient(mdwt_denoiser* d, const float* input, const float* target, 
   return MDWT_OK;
 }
 
-int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targets, int count,
-                       float lr, float* loss) {
+int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targets,
+                       const float* weights, int count, float lr, float* loss) {
   if (!d || !inputs || !targets || count < 1 || !(lr > 0.f)) return MDWT_ERR_ARG;
+  for (int i = 0; weights && i < count; ++i)
+    if (!(weights[i] >= 0.f) || std::isinf(weights[i])) return MDWT_ERR_ARG;
   DNCHK(d, hipSetDevice(d->device));
   const size_t n = size_t(d->H) * d->W;
   int rc = zero_grads(d);
@@ -370,7 +376,9 @@ int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targe
   DNCHK(d, hipMemsetAsync(d->err_sum, 0, sizeof(float), d->stream));
   const float inv = 1.f / float(n * 3 * size_t(count));
   for (int i = 0; i < count; ++i) {
-    if ((rc = one_pair(d, inputs + size_t(i) * n * 9, targets + size_t(i) * n * 3, inv))) return rc;
+    const float* xi = inputs + size_t(i) * n * mdwt::dn::kInputs;
+    const float wi = weights ? inv * weights[i] : inv;
+    if ((rc = one_pair(d, xi, targets + size_t(i) * n * 3, wi))) return rc;
     DNCHK(d, hipStreamSynchronize(d->stream));  // host inputs are reused by the caller
   }
   ++d->step;

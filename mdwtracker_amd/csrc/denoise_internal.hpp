@@ -11,6 +11,11 @@
 namespace mdwt {
 namespace dn {
 
+// values per pixel of the network input: rgb, first-hit depth, transmittance,
+// the four channel hit fractions, the standard error of the pixel mean and
+// log2 of the frame count (both scaled)
+constexpr int kInputs = 11;
+
 // 3x3 convolution (zero padding) as an implicit GEMM on the matrix cores.
 // w: bf16 [9][cout_pad][cin]; cin % 8 == 0, cout_pad % 32 == 0, xs and xo
 // multiples of 4. Writes channels < cout of y16 (bf16) and/or y32 (fp32,

@@ -209,20 +209,20 @@ __global__ void add_slice_kernel(const float* gy, int ys, int yo, int n, int C, 
   gx[i] += gy[size_t(p) * ys + yo + c];
 }
 
-// network input: in (fp32 [n][9]: rgb, depth, transmittance, 4 channel
-// fractions) -> bf16 [n][16] (channels 9..15 zero)
+// network input: in (fp32 [n][kInputs]) -> bf16 [n][16] (channels from
+// kInputs up zero)
 __global__ void pack_input_kernel(const float* in, int n, uint16_t* x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * 16) return;
   const int p = i / 16, c = i - p * 16;
-  x[i] = c < 9 ? f2bf(in[size_t(p) * 9 + c]) : uint16_t(0);
+  x[i] = c < kInputs ? f2bf(in[size_t(p) * kInputs + c]) : uint16_t(0);
 }
 // prediction = input rgb + network output (fp32 [n][32], first 3 channels)
 __global__ void residual_kernel(const float* in, const float* out, int n, float* pred) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * 3) return;
   const int p = i / 3, c = i - p * 3;
-  pred[i] = in[size_t(p) * 9 + c] + out[size_t(p) * 32 + c];
+  pred[i] = in[size_t(p) * kInputs + c] + out[size_t(p) * 32 + c];
 }
 // squared error: g = 2 (pred - target) / count into gout [n][32] (rgb
 // channels), err accumulates sum (pred - target)^2

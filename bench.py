@@ -100,14 +100,15 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     plain and the denoised image against the converged image with the
     reference's own variance removed (denoise.noise_corrected_mse). The
     denoised time to `target` is the interpolated frame count at which its
-    error reaches it, times the frame time with auxiliary buffers, plus one
-    network pass."""
+    error reaches it, priced at the frame time with auxiliary buffers for
+    the first denoise.AOV_FRAMES frames and the plain frame time after them,
+    plus one network pass."""
     import numpy as np
     from mdwtracker_amd import denoise
 
     target = ttr["target_rmse"]
     t0 = time.perf_counter()
-    xs, ts, ks = denoise.make_training_set(n_scenes=args.dn_scenes, crops=args.dn_crops,
+    xs, ts, ks = denoise.make_training_set(n_scenes=args.dn_scenes, views=args.dn_views,
                                            size=args.dn_size, seed=args.dn_seed,
                                            workload=scene.name, log=log)
     t1 = time.perf_counter()
@@ -124,11 +125,13 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     dn = denoise.Denoiser(scene.width, scene.height, seed=1)
     dn.set_params(params)
     r.set_aov(True)
+    r.set_aov_frames(0)
     r.render(16)
     r.reset_accumulation()
     ta = time.perf_counter()
     r.render(256)
     ms_aov = (time.perf_counter() - ta) * 1000.0 / 256
+    n_aov = denoise.AOV_FRAMES
     sweep, infer_ms = [], []
     for k, x in zip(SWEEP_LEVELS, denoise.accumulate_inputs(r, SWEEP_LEVELS, REF_SEED ^ 0x2468ACE)):
         d, ms = dn.infer(x)
@@ -140,13 +143,15 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
                       "denoised_rmse_vs_ref": denoise.rmse(d, ref)})
         log(f"[bench] sweep {k:6d} frames: plain rmse {np.sqrt(max(pm, 0)):.3e} "
             f"(model {np.sqrt(ttr['per_frame_variance'] / k):.3e}), denoised {np.sqrt(max(dm, 0)):.3e}")
+    r.set_aov_frames(0)
     r.set_aov(False)
     dn.close()
     ms_net = float(np.median(infer_ms[1:] if len(infer_ms) > 1 else infer_ms))
     kd = denoise.frames_to_target(SWEEP_LEVELS, [s["denoised_mse"] for s in sweep], target * target)
-    out = {"frame_ms_with_aov": round(ms_aov, 4), "network_ms": round(ms_net, 3),
+    out = {"frame_ms_with_aov": round(ms_aov, 4), "aov_frames": n_aov,
+           "network_ms": round(ms_net, 3),
            "train": {"pairs": n_pairs, "size": args.dn_size, "steps": args.dn_steps,
-                     "scenes": args.dn_scenes, "levels": list(denoise.TRAIN_LEVELS),
+                     "scenes": args.dn_scenes, "views": args.dn_views, "levels": list(denoise.TRAIN_LEVELS),
                      "final_loss": float(np.mean(losses[-20:])),
                      "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1)},
            "sweep": [{k: (round(v, 9) if isinstance(v, float) else v) for k, v in s.items()}
@@ -154,7 +159,8 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     if kd is None:
         out.update({"frames": None, "seconds": None, "speedup": None})
     else:
-        sec = kd * ms_aov * 1e-3 + ms_net * 1e-3
+        # the first n_aov frames carry the auxiliary buffers, the rest are plain
+        sec = (min(kd, n_aov) * ms_aov + max(0.0, kd - n_aov) * ms_per_frame + ms_net) * 1e-3
         out.update({"frames": int(math.ceil(kd)), "seconds": round(sec, 3),
                     "speedup": round(ttr["plain"]["seconds"] / sec, 2)})
     return out
@@ -189,10 +195,10 @@ def main() -> int:
     ap.add_argument("--rmse-frames", type=int, default=4096,
                     help="frames of the reference render for time-to-RMSE (0: skip)")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoised time-to-RMSE")
-    ap.add_argument("--dn-scenes", type=int, default=4)
-    ap.add_argument("--dn-crops", type=int, default=6)
+    ap.add_argument("--dn-scenes", type=int, default=3)
+    ap.add_argument("--dn-views", type=int, default=2)
     ap.add_argument("--dn-size", type=int, default=256)
-    ap.add_argument("--dn-steps", type=int, default=1500)
+    ap.add_argument("--dn-steps", type=int, default=3000)
     ap.add_argument("--dn-lr", type=float, default=1e-3)
     ap.add_argument("--dn-seed", type=int, default=77)
     args = ap.parse_args()

@@ -4,8 +4,11 @@
  * (deviceCode.cu:463-469). The denoiser is a 3-level U-Net of 3x3
  * convolutions (32 / 64 / 96 channels, 2x2 average pooling, nearest
  * upsampling with skip concatenation, ReLU) with BF16 weights and
- * activations and FP32 accumulation on the matrix cores; it predicts a
- * correction that is added to the input colour. Input per pixel, 11 floats:
+ * activations and FP32 accumulation on the matrix cores. Its output is
+ * kernel-predicting (Bako et al. 2017): per pixel, the softmax of 49 logits
+ * weighs the input colour of the 7x7 neighbourhood (coordinates clamped to
+ * the image), so every output is a convex combination of input colours; the
+ * untrained network applies a Gaussian of 1.5 pixels. Input per pixel, 11 floats:
  * r, g, b (the accumulated image), first-hit depth, transmittance (1 - hit
  * fraction), the hit fractions of channels 0..3 (mdwt_set_aov /
  * mdwt_read_aov), the standard error of the pixel mean (from the second
@@ -28,7 +31,8 @@ extern "C" {
 typedef struct mdwt_denoiser mdwt_denoiser;
 
 /* A denoiser for width x height images on HIP device hip_device; weights
- * initialised from seed (He initialisation, output layer near zero). */
+ * initialised from seed (He initialisation; output layer near zero with the
+ * Gaussian's log-weights as bias). */
 int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, uint32_t seed);
 int mdwt_dn_destroy(mdwt_denoiser* d);
 const char* mdwt_dn_last_error(const mdwt_denoiser* d);

@@ -79,6 +79,7 @@ SIGNATURES = {
     "mdwt_batch_view_times": [_h, _fp],
     "mdwt_autotune": [_h, _ip, C.c_int, _ip, C.c_int, _ip, C.c_int, C.c_int, _fp, _ip],
     "mdwt_set_aov": [_h, C.c_int],
+    "mdwt_set_aov_frames": [_h, C.c_int],
     "mdwt_read_aov": [_h, _fp],
     "mdwt_dn_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_uint32],
     "mdwt_dn_destroy": [_h],
@@ -317,6 +318,10 @@ class MdwtRenderer:
     def set_aov(self, on: bool = True) -> None:
         """Accumulate the denoiser's auxiliary inputs with every frame."""
         self._chk(lib.mdwt_set_aov(self._h, int(on)))
+
+    def set_aov_frames(self, frames: int) -> None:
+        """Accumulate the auxiliary buffers over the first `frames` frames only (0: all)."""
+        self._chk(lib.mdwt_set_aov_frames(self._h, frames))
 
     def read_aov(self) -> np.ndarray:
         """[h][w][6]: first-hit distance, transmittance, hit fractions of channels 0..3."""

@@ -2,8 +2,9 @@
 // convolutions, 32 / 64 / 96 channels, 2x2 average pooling, nearest
 // upsampling with skip concatenation, ReLU), BF16 weights and activations,
 // FP32 accumulation on the matrix cores, trained in place with Adam on pairs
-// of low- and high-sample renders. It predicts a correction added to the
-// input colour. Input per pixel (kInputs = 11 values): rgb, first-hit
+// of low- and high-sample renders. Its output is kernel-predicting (Bako et
+// al. 2017): per pixel, softmax weights of a 7x7 filter applied to the input
+// colour. Input per pixel (kInputs = 11 values): rgb, first-hit
 // depth, transmittance, the four channel fractions of the hits
 // (mdwt_set_aov), the standard error of the pixel mean and log2 of the frame
 // count.
@@ -78,10 +79,14 @@ int alloc(mdwt_denoiser* d, T** p, size_t n) {
   return MDWT_OK;
 }
 
-// layer table: (cin, cout, relu); c0 reads the input padded to 16 channels
+// layer table: (cin, cout, relu); c0 reads the input padded to 16 channels,
+// c10 writes the kTaps filter logits
 const int kLayers[11][3] = {{mdwt::dn::kInputs, 32, 1}, {32, 32, 1}, {32, 64, 1},
                             {64, 64, 1},  {64, 96, 1}, {96, 96, 1}, {160, 64, 1},
-                            {64, 64, 1},  {96, 32, 1}, {32, 32, 1}, {32, 3, 0}};
+                            {64, 64, 1},  {96, 32, 1}, {32, 32, 1}, {32, mdwt::dn::kTaps, 0}};
+
+// width in pixels of the Gaussian filter the untrained network applies
+constexpr float kInitSigma = 1.5f;
 
 int setup(mdwt_denoiser* d, uint32_t seed) {
   std::mt19937 gen(seed);
@@ -102,13 +107,23 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
         (rc = alloc(d, &L.wf, size_t(9) * L.cout_pad * L.cin_pad)) ||
         (rc = alloc(d, &L.wd, size_t(9) * L.cin_pad32 * L.cout_pad)))
       return rc;
-    // He initialisation for ReLU layers; the output layer starts small so
-    // that the untrained network passes the input colour through
+    // He initialisation for ReLU layers; the output layer starts small
+    // with Gaussian log-weights as bias, so that the untrained network
+    // filters with a Gaussian of kInitSigma pixels
     const float sd = L.relu ? std::sqrt(2.f / float(9 * L.cin)) : 1e-3f;
     std::normal_distribution<float> nd(0.f, sd);
     std::vector<float> hw(L.nw);
     for (auto& v : hw) v = nd(gen);
     DNCHK(d, hipMemcpy(L.w, hw.data(), L.nw * sizeof(float), hipMemcpyHostToDevice));
+    if (!L.relu) {
+      std::vector<float> hb(L.cout);
+      for (int j = 0; j < L.cout; ++j) {
+        const float dy = float(j / mdwt::dn::kKernel - mdwt::dn::kKernelRadius);
+        const float dx = float(j % mdwt::dn::kKernel - mdwt::dn::kKernelRadius);
+        hb[j] = -(dx * dx + dy * dy) / (2.f * kInitSigma * kInitSigma);
+      }
+      DNCHK(d, hipMemcpy(L.b, hb.data(), hb.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
   }
   const size_t n = size_t(d->H) * d->W, n4 = n / 4, n16 = n / 16;
   int rc;
@@ -119,11 +134,11 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
       (rc = alloc(d, &d->a5, n16 * 96)) || (rc = alloc(d, &d->u2, n4 * 160)) ||
       (rc = alloc(d, &d->a6, n4 * 64)) || (rc = alloc(d, &d->a7, n4 * 64)) ||
       (rc = alloc(d, &d->u1, n * 96)) || (rc = alloc(d, &d->a8, n * 32)) ||
-      (rc = alloc(d, &d->a9, n * 32)) || (rc = alloc(d, &d->gm, n * 32)))
+      (rc = alloc(d, &d->a9, n * 32)) || (rc = alloc(d, &d->gm, n * mdwt::dn::kLogitStride)))
     return rc;
   if ((rc = alloc(d, &d->in32, n * mdwt::dn::kInputs)) || (rc = alloc(d, &d->tgt, n * 3)) ||
-      (rc = alloc(d, &d->out32, n * 32)) || (rc = alloc(d, &d->pred, n * 3)) ||
-      (rc = alloc(d, &d->err_sum, 1)) || (rc = alloc(d, &d->gout, n * 32)) ||
+      (rc = alloc(d, &d->out32, n * mdwt::dn::kLogitStride)) || (rc = alloc(d, &d->pred, n * 3)) ||
+      (rc = alloc(d, &d->err_sum, 1)) || (rc = alloc(d, &d->gout, n * mdwt::dn::kLogitStride)) ||
       (rc = alloc(d, &d->g9, n * 32)) || (rc = alloc(d, &d->g8, n * 32)) ||
       (rc = alloc(d, &d->gu1, n * 96)) || (rc = alloc(d, &d->g7, n4 * 64)) ||
       (rc = alloc(d, &d->g6, n4 * 64)) || (rc = alloc(d, &d->gu2, n4 * 160)) ||
@@ -173,8 +188,8 @@ int forward(mdwt_denoiser* d) {
   DNCHK(d, mdwt::dn::copy_slice(d->a1, 32, 0, n, 32, d->u1, 96, 64, s));
   if ((rc = fwd(d, 8, d->u1, H, W, 96, 0, d->a8, nullptr, 32))) return rc;
   if ((rc = fwd(d, 9, d->a8, H, W, 32, 0, d->a9, nullptr, 32))) return rc;
-  if ((rc = fwd(d, 10, d->a9, H, W, 32, 0, nullptr, d->out32, 32))) return rc;
-  DNCHK(d, mdwt::dn::residual(d->in32, d->out32, n, d->pred, s));
+  if ((rc = fwd(d, 10, d->a9, H, W, 32, 0, nullptr, d->out32, mdwt::dn::kLogitStride))) return rc;
+  DNCHK(d, mdwt::dn::kpn_apply(d->in32, d->out32, H, W, d->pred, s));
   return MDWT_OK;
 }
 
@@ -200,7 +215,9 @@ int backward(mdwt_denoiser* d) {
   int rc;
   DNCHK(d, hipMemsetAsync(d->g1, 0, size_t(n) * 32 * sizeof(float), s));
   DNCHK(d, hipMemsetAsync(d->g3, 0, size_t(H2) * W2 * 64 * sizeof(float), s));
-  if ((rc = bwd(d, 10, d->gout, 32, nullptr, 32, d->a9, 32, 0, H, W, d->g9, 32))) return rc;
+  if ((rc = bwd(d, 10, d->gout, mdwt::dn::kLogitStride, nullptr, 0, d->a9, 32, 0, H, W, d->g9,
+                32)))
+    return rc;
   if ((rc = bwd(d, 9, d->g9, 32, d->a9, 32, d->a8, 32, 0, H, W, d->g8, 32))) return rc;
   if ((rc = bwd(d, 8, d->g8, 32, d->a8, 32, d->u1, 96, 0, H, W, d->gu1, 96))) return rc;
   DNCHK(d, mdwt::dn::up_grad(d->gu1, H, W, 96, 0, 64, d->g7, s));
@@ -236,8 +253,8 @@ int one_pair(mdwt_denoiser* d, const float* in, const float* tgt, float inv_coun
   DNCHK(d, hipMemcpyAsync(d->tgt, tgt, n * 3 * sizeof(float), hipMemcpyHostToDevice, d->stream));
   int rc = forward(d);
   if (rc) return rc;
-  DNCHK(d, hipMemsetAsync(d->gout, 0, n * 32 * sizeof(float), d->stream));
-  DNCHK(d, mdwt::dn::mse(d->pred, d->tgt, int(n), inv_count, d->gout, d->err_sum, d->stream));
+  DNCHK(d, mdwt::dn::kpn_loss_grad(d->in32, d->out32, d->pred, d->tgt, d->H, d->W, inv_count,
+                                   d->gout, d->err_sum, d->stream));
   return backward(d);
 }
 

@@ -15,6 +15,12 @@ namespace dn {
 // the four channel hit fractions, the standard error of the pixel mean and
 // log2 of the frame count (both scaled)
 constexpr int kInputs = 11;
+// kernel-predicting output: kKernel x kKernel filter weights per pixel
+constexpr int kKernelRadius = 3;
+constexpr int kKernel = 2 * kKernelRadius + 1;
+constexpr int kTaps = kKernel * kKernel;
+constexpr int kLogitStride = 64;  // logits per pixel, padded to the conv's N tiles
+static_assert(kTaps <= kLogitStride, "logit stride");
 
 // 3x3 convolution (zero padding) as an implicit GEMM on the matrix cores.
 // w: bf16 [9][cout_pad][cin]; cin % 8 == 0, cout_pad % 32 == 0, xs and xo
@@ -37,9 +43,14 @@ hipError_t copy_slice(const uint16_t* x, int xs, int xo, int n, int C, uint16_t*
                       hipStream_t s);
 hipError_t add_slice(const float* gy, int ys, int yo, int n, int C, float* gx, hipStream_t s);
 hipError_t pack_input(const float* in, int n, uint16_t* x, hipStream_t s);
-hipError_t residual(const float* in, const float* out, int n, float* pred, hipStream_t s);
-hipError_t mse(const float* pred, const float* target, int n, float inv_count, float* gout,
-               float* err, hipStream_t s);
+// pred [H*W][3] = softmax(logits) over the kTaps neighbourhood of the input
+// colour (in: fp32 [H*W][kInputs]; logits: fp32 [H*W][kLogitStride])
+hipError_t kpn_apply(const float* in, const float* logits, int H, int W, float* pred,
+                     hipStream_t s);
+// err += sum (pred - target)^2; gz = d(inv_count * that sum) / d logits
+hipError_t kpn_loss_grad(const float* in, const float* logits, const float* pred,
+                         const float* target, int H, int W, float inv_count, float* gz, float* err,
+                         hipStream_t s);
 hipError_t adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
                 float c1, float c2, float eps, uint16_t* w16, hipStream_t s);
 hipError_t weight_layouts(const float* w, int cin, int cout, int cin_pad, int cout_pad,

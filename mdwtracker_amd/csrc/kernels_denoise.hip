@@ -217,24 +217,74 @@ __global__ void pack_input_kernel(const float* in, int n, uint16_t* x) {
   const int p = i / 16, c = i - p * 16;
   x[i] = c < kInputs ? f2bf(in[size_t(p) * kInputs + c]) : uint16_t(0);
 }
-// prediction = input rgb + network output (fp32 [n][32], first 3 channels)
-__global__ void residual_kernel(const float* in, const float* out, int n, float* pred) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * 3) return;
-  const int p = i / 3, c = i - p * 3;
-  pred[i] = in[size_t(p) * kInputs + c] + out[size_t(p) * 32 + c];
+// Kernel-predicting output (Bako et al. 2017): per pixel, the softmax of
+// the kTaps logits (fp32 [n][kLogitStride]) weighs the input colour of the
+// kKernel x kKernel neighbourhood (coordinates clamped to the image).
+__device__ __forceinline__ int kpn_neighbour(int py, int px, int j, int H, int W) {
+  const int sy = min(max(py + j / kKernel - kKernelRadius, 0), H - 1);
+  const int sx = min(max(px + j % kKernel - kKernelRadius, 0), W - 1);
+  return sy * W + sx;
 }
-// squared error: g = 2 (pred - target) / count into gout [n][32] (rgb
-// channels), err accumulates sum (pred - target)^2
-__global__ void mse_kernel(const float* pred, const float* target, int n, float inv_count,
-                           float* gout, float* err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// softmax weights of pixel p into w (unnormalised) and their sum
+__device__ __forceinline__ float kpn_weights(const float* logits, int p, float* w) {
+  const float* z = logits + size_t(p) * kLogitStride;
+  float m = z[0];
+#pragma unroll
+  for (int j = 1; j < kTaps; ++j) m = fmaxf(m, z[j]);
+  float sum = 0.f;
+#pragma unroll
+  for (int j = 0; j < kTaps; ++j) {
+    w[j] = __expf(z[j] - m);
+    sum += w[j];
+  }
+  return sum;
+}
+// pred [n][3] = sum_j softmax(z)_j * rgb(neighbour j)
+__global__ void kpn_apply_kernel(const float* __restrict__ in, const float* __restrict__ logits,
+                                 int H, int W, float* __restrict__ pred) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= H * W) return;
+  const int py = p / W, px = p - py * W;
+  float w[kTaps];
+  const float inv = 1.f / kpn_weights(logits, p, w);
+  float r = 0.f, g = 0.f, b = 0.f;
+#pragma unroll
+  for (int j = 0; j < kTaps; ++j) {
+    const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * kInputs;
+    r += w[j] * c[0];
+    g += w[j] * c[1];
+    b += w[j] * c[2];
+  }
+  pred[size_t(p) * 3 + 0] = r * inv;
+  pred[size_t(p) * 3 + 1] = g * inv;
+  pred[size_t(p) * 3 + 2] = b * inv;
+}
+// Squared error of pred against target: err += sum (pred - target)^2, and
+// the gradient of inv_count * that sum with respect to the logits,
+// dz_j = w_j * (c_j - pred) . gpred with gpred = 2 inv_count (pred - target),
+// into gz [n][kLogitStride] (channels < kTaps).
+__global__ void kpn_loss_grad_kernel(const float* __restrict__ in, const float* __restrict__ logits,
+                                     const float* __restrict__ pred, const float* __restrict__ target,
+                                     int H, int W, float inv_count, float* __restrict__ gz,
+                                     float* err) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float e = 0.f;
-  if (i < n * 3) {
-    const int p = i / 3, c = i - p * 3;
-    const float d = pred[i] - target[i];
-    gout[size_t(p) * 32 + c] = 2.f * d * inv_count;
-    e = d * d;
+  if (p < H * W) {
+    const int py = p / W, px = p - py * W;
+    float w[kTaps];
+    const float inv = 1.f / kpn_weights(logits, p, w);
+    const float pr = pred[size_t(p) * 3], pg = pred[size_t(p) * 3 + 1], pb = pred[size_t(p) * 3 + 2];
+    const float dr = pr - target[size_t(p) * 3], dg = pg - target[size_t(p) * 3 + 1],
+                db = pb - target[size_t(p) * 3 + 2];
+    e = dr * dr + dg * dg + db * db;
+    const float s = 2.f * inv_count;
+    const float gr = s * dr, gg = s * dg, gb = s * db;
+    const float base = pr * gr + pg * gg + pb * gb;
+#pragma unroll
+    for (int j = 0; j < kTaps; ++j) {
+      const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * kInputs;
+      gz[size_t(p) * kLogitStride + j] = w[j] * inv * (c[0] * gr + c[1] * gg + c[2] * gb - base);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
   if ((threadIdx.x & 63) == 0) atomicAdd(err, e);
@@ -344,14 +394,17 @@ hipError_t pack_input(const float* in, int n, uint16_t* x, hipStream_t s) {
   hipLaunchKernelGGL(pack_input_kernel, dim3(nb(size_t(n) * 16)), dim3(256), 0, s, in, n, x);
   return hipGetLastError();
 }
-hipError_t residual(const float* in, const float* out, int n, float* pred, hipStream_t s) {
-  hipLaunchKernelGGL(residual_kernel, dim3(nb(size_t(n) * 3)), dim3(256), 0, s, in, out, n, pred);
+hipError_t kpn_apply(const float* in, const float* logits, int H, int W, float* pred,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(kpn_apply_kernel, dim3(nb(size_t(H) * W)), dim3(256), 0, s, in, logits, H, W,
+                     pred);
   return hipGetLastError();
 }
-hipError_t mse(const float* pred, const float* target, int n, float inv_count, float* gout,
-               float* err, hipStream_t s) {
-  hipLaunchKernelGGL(mse_kernel, dim3(nb(size_t(n) * 3)), dim3(256), 0, s, pred, target, n,
-                     inv_count, gout, err);
+hipError_t kpn_loss_grad(const float* in, const float* logits, const float* pred,
+                         const float* target, int H, int W, float inv_count, float* gz, float* err,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(kpn_loss_grad_kernel, dim3(nb(size_t(H) * W)), dim3(256), 0, s, in, logits,
+                     pred, target, H, W, inv_count, gz, err);
   return hipGetLastError();
 }
 hipError_t adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,

@@ -118,6 +118,7 @@ struct mdwt_renderer {
   int* d_win = nullptr;
   uint32_t* d_wave_cost = nullptr;  // per-wave ticks of the last recorded frame
   bool aov_on = false;
+  int aov_frames = 0;                // accumulate the buffers over the first frames only (0: all)
   Float4* d_aov = nullptr;          // 2 per pixel when aov_on
   bool cost_ready = false;
   bool need_resort = true;  // record costs and re-sort after the next frames
@@ -556,7 +557,8 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.n_items = 0;
   p.fr.work = nullptr;
   p.fr.item_ticks = nullptr;
-  p.fr.aov = h->d_aov;
+  const bool aov_now = h->aov_frames == 0 || h->accum_id < uint32_t(h->aov_frames);
+  p.fr.aov = aov_now ? h->d_aov : nullptr;
   return p;
 }
 
@@ -1744,6 +1746,14 @@ int mdwt_set_aov(mdwt_renderer* h, int on) {
   int rc = alloc_frame(h);  // (re)allocates the buffers; accumulation restarts
   reset_acc(h);
   return rc;
+}
+
+int mdwt_set_aov_frames(mdwt_renderer* h, int frames) {
+  if (!h || frames < 0) return MDWT_ERR_ARG;
+  if (frames == h->aov_frames) return MDWT_OK;
+  h->aov_frames = frames;
+  reset_acc(h);
+  return MDWT_OK;
 }
 
 int mdwt_read_aov(mdwt_renderer* h, float* out) {

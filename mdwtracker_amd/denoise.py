@@ -6,8 +6,8 @@ accumulates: the image, its second moment and the auxiliary buffers of
 mdwt_set_aov (`network_input`). Training pairs are rendered by the method
 itself: one accumulation run per view is read at several frame counts
 (inputs at every noise level) against one independent high-frame render of
-the same view (the target); the view is a zoomed crop camera whose pixel
-footprint matches the full-size image (`crop_camera`). Plain accumulation
+the same view (the target), at the workload's image size and framing, and
+cut into tiles for training (`make_training_set`). Plain accumulation
 stays the fallback: nothing here changes how frames are rendered.
 
 Errors are measured against a plain high-frame render whose own noise is
@@ -33,11 +33,6 @@ INPUTS = 11          # values per pixel, kInputs of the kernels
 DEPTH_SCALE = 0.25   # first-hit distances of the orbit cameras are ~1..4 world units
 STDERR_SCALE = 4.0   # standard error of the pixel mean, ~0.002..0.5
 FRAMES_SCALE = 1.0 / 16.0  # log2 of the frame count, 0..16
-
-# world units per pixel at the centre of the volume for the 1024^2 orbit
-# camera at distance 1.8 with a 45 degree field of view
-ORBIT_FOOTPRINT = 1.8 * 2.0 * math.tan(math.radians(22.5)) / 1024.0
-
 
 def _f(a: np.ndarray):
     return a.ctypes.data_as(_fp)
@@ -138,12 +133,17 @@ def read_input(r: MdwtRenderer, frames: int) -> np.ndarray:
     return network_input(a, m2, r.read_aov(), frames)
 
 
-def accumulate_inputs(r: MdwtRenderer, levels, seed: int):
+AOV_FRAMES = 64  # the auxiliary buffers converge long before the colour
+
+
+def accumulate_inputs(r: MdwtRenderer, levels, seed: int, aov_frames: int = AOV_FRAMES):
     """Yields the network inputs of one accumulation run (seed) read at each
     frame count of `levels` (ascending); the run continues between the
-    reads, so the renderer must not be used while the generator is live."""
+    reads, so the renderer must not be used while the generator is live.
+    The auxiliary buffers accumulate over the first aov_frames frames only."""
     r.set_seed(seed)
     r.set_aov(True)
+    r.set_aov_frames(aov_frames)
     r.reset_accumulation()
     done = 0
     for k in levels:
@@ -172,49 +172,57 @@ def noise_corrected_mse(img: np.ndarray, ref: np.ndarray, ref_var: np.ndarray):
     return float(e.mean()), float(e.std() / math.sqrt(e.size))
 
 
-def crop_camera(rng: np.random.Generator, size: int, footprint: float = ORBIT_FOOTPRINT,
-                jitter: float = 1.4):
-    """(from, at, up, fovy) of a size x size view whose pixel footprint at the
-    aim point is footprint times a log-uniform factor in [1/jitter, jitter]:
-    a crop of a larger image of the same pixel scale."""
+def orbit_view(rng: np.random.Generator, centre=(0.5, 0.5, 0.5)):
+    """(from, at, up, fovy) of a random orbit camera framed like the
+    workload's: a direction uniform on the sphere, distance 1.5..2.3, the aim
+    point within 0.15 of the centre, 45 degree field of view."""
     v = rng.normal(size=3)
     v /= np.linalg.norm(v)
-    at = 0.5 + rng.uniform(-0.3, 0.3, 3)
-    frm = np.array([0.5, 0.5, 0.5]) + rng.uniform(1.5, 2.6) * v
-    dist = float(np.linalg.norm(frm - at))
-    fp = footprint * math.exp(rng.uniform(-math.log(jitter), math.log(jitter)))
-    fovy = math.degrees(2.0 * math.atan(fp * size / (2.0 * dist)))
+    at = np.asarray(centre) + rng.uniform(-0.15, 0.15, 3)
+    frm = np.asarray(centre) + rng.uniform(1.5, 2.3) * v
     up = (0.0, 1.0, 0.0) if abs(v[1]) < 0.95 else (1.0, 0.0, 0.0)
-    return tuple(float(c) for c in frm), tuple(float(c) for c in at), up, fovy
+    return tuple(float(c) for c in frm), tuple(float(c) for c in at), up, 45.0
 
 
-TRAIN_LEVELS = (1, 2, 4, 16, 64, 256, 1024, 2048, 4096, 8192, 16384)
+def tiles(img: np.ndarray, size: int):
+    """Non-overlapping size x size tiles of img [h][w][...] (row-major)."""
+    h, w = img.shape[:2]
+    return [img[y:y + size, x:x + size] for y in range(0, h - size + 1, size)
+            for x in range(0, w - size + 1, size)]
 
 
-def make_training_set(n_scenes: int = 4, crops: int = 6, size: int = 256, levels=TRAIN_LEVELS,
-                      target_frames: int = 32768, seed: int = 0, scale: int = 1,
-                      workload: str = "sync_4ch256", log=None):
+TRAIN_LEVELS = (1, 2, 4, 16, 64, 256, 1024, 2048, 4096, 8192)
+
+
+def make_training_set(n_scenes: int = 3, views: int = 2, size: int = 256, levels=TRAIN_LEVELS,
+                      target_frames: int = 16384, seed: int = 0, scale: int = 1,
+                      workload: str = "sync_4ch256", min_signal: float = 0.01, log=None):
     """(inputs [N][size][size][INPUTS], targets [N][size][size][3], frame
     counts [N]) from n_scenes seeded volumes of the workload family (seeds
-    drawn from `seed`, so held out from the fixed-seed workload), `crops`
-    crop cameras each, every crop read at each level of `levels`."""
+    drawn from `seed`, so held out from the fixed-seed workload) seen from
+    `views` random orbit cameras each at the workload's image size: one
+    target render and one input accumulation read at each of `levels` per
+    view, cut into size x size tiles; tiles whose target mean is below
+    min_signal (background) are dropped."""
     rng = np.random.default_rng(seed)
     xs, ts, ks = [], [], []
     for s in range(n_scenes):
         sc = scenes.make_scene(workload, seed=int(rng.integers(1, 2**31)), scale=scale)
         r = MdwtRenderer(0)
         scenes.load_scene(r, sc, 2)
-        r.set_frame(size, size, 1, True, (0.0, 0.0, 0.0))
-        fp = ORBIT_FOOTPRINT * scale
-        for _ in range(crops):
-            r.set_camera(*crop_camera(rng, size, fp))
+        for _ in range(views):
+            r.set_camera(*orbit_view(rng))
             s_in, s_t = (int(v) for v in rng.integers(1, 2**31, 2))
             r.set_aov(False)
             t, _ = render_mean(r, target_frames, s_t)
+            t_tiles = tiles(t, size)
+            keep = [i for i, tt in enumerate(t_tiles) if float(tt.mean()) >= min_signal]
             for k, x in zip(levels, accumulate_inputs(r, levels, s_in)):
-                xs.append(x)
-                ts.append(t)
-                ks.append(k)
+                x_tiles = tiles(x, size)
+                for i in keep:
+                    xs.append(x_tiles[i].copy())
+                    ts.append(t_tiles[i].copy())
+                    ks.append(k)
         r.close()
         if log:
             log(f"[denoise] training scene {s + 1}/{n_scenes}: {len(xs)} pairs")

@@ -1,7 +1,7 @@
 """CPU checks of the denoiser's host logic: the network input built from the
 accumulated moments, the noise-corrected error estimate, the log-log crossing
-used for time-to-RMSE, the crop cameras' pixel footprint and the loss
-weights. No GPU calls (the library is loaded, nothing is launched)."""
+used for time-to-RMSE, the training views and tiles, and the loss weights.
+No GPU calls (the library is loaded, nothing is launched)."""
 import math
 
 import numpy as np
@@ -66,16 +66,23 @@ def test_frames_to_target(dn):
     assert dn.frames_to_target([1, 2, 4], [1.0, 0.1, 0.5], 0.2) < 2.0
 
 
-def test_crop_camera_footprint(dn):
+def test_orbit_view(dn):
     rng = np.random.default_rng(2)
-    size = 256
     for _ in range(50):
-        frm, at, up, fovy = dn.crop_camera(rng, size)
-        dist = math.dist(frm, at)
-        fp = dist * 2 * math.tan(math.radians(fovy) / 2) / size
-        assert dn.ORBIT_FOOTPRINT / 1.4 - 1e-12 <= fp <= dn.ORBIT_FOOTPRINT * 1.4 + 1e-12
-        d = np.subtract(at, frm) / dist
+        frm, at, up, fovy = dn.orbit_view(rng)
+        assert fovy == 45.0
+        assert np.all(np.abs(np.subtract(at, 0.5)) <= 0.15)
+        assert 1.5 - 1e-9 <= math.dist(frm, (0.5, 0.5, 0.5)) <= 2.3 + 1e-9
+        d = np.subtract(at, frm) / math.dist(frm, at)
         assert abs(float(np.dot(d, up))) < 0.999  # up is never along the view
+
+
+def test_tiles(dn):
+    img = np.arange(8 * 12 * 2).reshape(8, 12, 2)
+    t = dn.tiles(img, 4)
+    assert len(t) == 6 and all(x.shape == (4, 4, 2) for x in t)
+    np.testing.assert_array_equal(t[4], img[4:8, 4:8])
+    assert len(dn.tiles(img, 5)) == 2  # partial tiles are left out
 
 
 def test_level_weights(dn):

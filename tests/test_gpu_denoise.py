@@ -19,7 +19,7 @@ torch = pytest.importorskip("torch")
 
 LAYERS = [(11, 32, True), (32, 32, True), (32, 64, True), (64, 64, True), (64, 96, True),
           (96, 96, True), (160, 64, True), (64, 64, True), (96, 32, True), (32, 32, True),
-          (32, 3, False)]
+          (32, 49, False)]
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +45,8 @@ def _unpack(p):
 
 
 def _torch_forward(params, x9):
-    """float32 restatement of the U-Net with the kernels' bf16 rounding."""
+    """float32 restatement of the U-Net and its kernel-predicting output with
+    the kernels' bf16 rounding."""
     F = torch.nn.functional
     layers = []
     for (w, b), (cin, cout, relu) in zip(_unpack(params), LAYERS):
@@ -60,6 +61,7 @@ def _torch_forward(params, x9):
         return _bf(y) if store_bf16 else y
 
     x = torch.tensor(x9).permute(2, 0, 1)[None]
+    H, W = x9.shape[:2]
     x0 = _bf(x)
     a1 = conv(1, conv(0, x0))
     p1 = _bf(F.avg_pool2d(a1, 2))
@@ -70,8 +72,12 @@ def _torch_forward(params, x9):
     a7 = conv(7, conv(6, u2))
     u1 = torch.cat([F.interpolate(a7, scale_factor=2, mode="nearest"), a1], 1)
     a9 = conv(9, conv(8, u1))
-    out = conv(10, a9, store_bf16=False)
-    pred = x[:, :3] + out
+    logits = conv(10, a9, store_bf16=False)
+    # 7x7 kernel prediction over the clamped neighbourhood of the input colour
+    w = torch.softmax(logits, 1)  # [1][49][H][W]
+    rgb = F.pad(x[:, :3], (3, 3, 3, 3), mode="replicate")
+    patches = F.unfold(rgb, 7).reshape(1, 3, 49, H, W)
+    pred = (patches * w[:, None]).sum(2)
     return pred[0].permute(1, 2, 0), layers
 
 
@@ -101,7 +107,7 @@ def test_forward_matches_torch_reference(dnmod):
     want = want.numpy()
     err = np.abs(got - want)
     scale = np.abs(want - x[..., :3]).max()
-    print(f"[denoise] forward: max |err| {err.max():.3e}, correction scale {scale:.3e}, {ms:.3f} ms", flush=True)
+    print(f"[denoise] forward: max |err| {err.max():.3e}, change scale {scale:.3e}, {ms:.3f} ms", flush=True)
     assert scale > 1e-2
     # bf16 storage of activations: a stored value can round the other way
     # after a different fp32 summation order (1 bf16 ulp, 2^-8 relative)
@@ -157,8 +163,10 @@ def test_training_reduces_loss(dnmod):
     xs, ts = np.stack(xs), np.stack(ts)
     dn = dnmod.Denoiser(w, h, seed=9)
     losses = dnmod.train(dn, xs, ts, steps=120, batch=4, lr=2e-3)
-    print(f"[denoise] loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
-    assert np.mean(losses[-10:]) < 0.5 * losses[0]
+    print(f"[denoise] loss {np.mean(losses[:5]):.4e} -> {np.mean(losses[-10:]):.4e}", flush=True)
+    # the untrained network already filters with a Gaussian; training must
+    # still find a clearly better filter for these smooth images
+    assert np.mean(losses[-10:]) < 0.7 * np.mean(losses[:5])
     dn.close()
 
 
@@ -180,8 +188,22 @@ def test_aov_buffers():
     hit = 1.0 - aov[..., 1]
     assert hit.max() > 0.3 and np.all(aov[..., 0][hit == 0] == 0)
     assert np.all(aov[..., 4:6] == 0)  # two channels only
-    a.close()
-    b.close()
+    # a frame limit: the buffers hold the means over the first 2 frames (the
+    # same frame ids as a 2-frame run) and the image is unchanged
+    c, d = capi.MdwtRenderer(0), capi.MdwtRenderer(0)
+    for x in (c, d):
+        scenes.load_scene(x, sc, 2)
+        x.set_frame(64, 48, 2, True, (0.0, 0.0, 0.0))
+        x.set_aov(True)
+    c.set_aov_frames(2)
+    c.render(3)
+    d.render(2)
+    np.testing.assert_array_equal(c.read_accum(), a.read_accum())
+    np.testing.assert_array_equal(c.read_aov(), d.read_aov())
+    with pytest.raises(capi.MdwtError):
+        c.set_aov_frames(-1)
+    for x in (a, b, c, d):
+        x.close()
 
 
 def test_train_step_weights(dnmod):

@@ -51,6 +51,7 @@ struct mdwt_denoiser {
   uint16_t *x0, *a0, *a1, *p1, *a2, *a3, *p2, *a4, *a5, *u2, *a6, *a7, *u1, *a8, *a9, *gm;
   // fp32: network io and gradients
   float *in32, *tgt, *out32, *pred, *err_sum;
+  float* wpart;  // partial weight gradients of the row bands
   float *gout, *g9, *g8, *gu1, *g7, *g6, *gu2, *g5, *g4, *gp2, *g3, *g2, *gp1, *g1, *g0;
   int step = 0;
 };
@@ -87,6 +88,9 @@ const int kLayers[11][3] = {{mdwt::dn::kInputs, 32, 1}, {32, 32, 1}, {32, 64, 1}
 
 // width in pixels of the Gaussian filter the untrained network applies
 constexpr float kInitSigma = 1.5f;
+// floats of partial weight gradients (16 MiB): at least 9 * cout * cin of
+// every layer, and room for enough row bands to fill the GPU
+constexpr size_t kWgradPart = size_t(4) << 20;
 
 int setup(mdwt_denoiser* d, uint32_t seed) {
   std::mt19937 gen(seed);
@@ -138,7 +142,8 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
     return rc;
   if ((rc = alloc(d, &d->in32, n * mdwt::dn::kInputs)) || (rc = alloc(d, &d->tgt, n * 3)) ||
       (rc = alloc(d, &d->out32, n * mdwt::dn::kLogitStride)) || (rc = alloc(d, &d->pred, n * 3)) ||
-      (rc = alloc(d, &d->err_sum, 1)) || (rc = alloc(d, &d->gout, n * mdwt::dn::kLogitStride)) ||
+      (rc = alloc(d, &d->err_sum, 1)) || (rc = alloc(d, &d->wpart, kWgradPart)) ||
+      (rc = alloc(d, &d->gout, n * mdwt::dn::kLogitStride)) ||
       (rc = alloc(d, &d->g9, n * 32)) || (rc = alloc(d, &d->g8, n * 32)) ||
       (rc = alloc(d, &d->gu1, n * 96)) || (rc = alloc(d, &d->g7, n4 * 64)) ||
       (rc = alloc(d, &d->g6, n4 * 64)) || (rc = alloc(d, &d->gu2, n4 * 160)) ||
@@ -201,7 +206,8 @@ int bwd(mdwt_denoiser* d, int i, const float* gy, int gs, const uint16_t* y, int
   const Conv& L = d->c[i];
   hipStream_t s = d->stream;
   DNCHK(d, mdwt::dn::relu_grad(gy, gs, y, ys, 0, H * W, L.cout, L.cout_pad, L.relu, d->gm, L.gb, s));
-  DNCHK(d, mdwt::dn::wgrad(d->gm, L.cout_pad, x, xs, xo, H, W, L.cin, L.cout, L.gw, s));
+  DNCHK(d, mdwt::dn::wgrad(d->gm, L.cout_pad, x, xs, xo, H, W, L.cin, L.cout, L.gw, d->wpart,
+                           kWgradPart, s));
   if (gx)
     DNCHK(d, mdwt::dn::conv3x3(d->gm, H, W, L.cout_pad, 0, L.cout_pad, L.wd, nullptr, L.cin,
                                L.cin_pad32, nullptr, gx, gxs, 0, false, false, s));

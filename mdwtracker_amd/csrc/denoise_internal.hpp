@@ -29,9 +29,11 @@ static_assert(kTaps <= kLogitStride, "logit stride");
 hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, const uint16_t* w,
                    const float* bias, int cout, int cout_pad, uint16_t* y16, float* y32, int ys,
                    int yo, bool relu, bool accum, hipStream_t s);
-// dw [9][cout][cin] += sum over pixels of g[p][co] * x[p + tap][ci]
+// dw [9][cout][cin] += sum over pixels of g[p][co] * x[p + tap][ci]; part is
+// scratch of part_cap floats (at least 9 * cout * cin). Strides and xo are
+// multiples of 4 and the rows hold cin (cout) rounded up to 4 channels.
 hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
-                 int cin, int cout, float* dw, hipStream_t s);
+                 int cin, int cout, float* dw, float* part, size_t part_cap, hipStream_t s);
 // gm = bf16(gy * relu'(y)) [npix][cout_pad]; db += per-channel sums
 hipError_t relu_grad(const float* gy, int gs, const uint16_t* y, int ys, int yo, int npix, int cout,
                      int cout_pad, bool relu, uint16_t* gm, float* db, hipStream_t s);

@@ -83,47 +83,97 @@ __global__ __launch_bounds__(64) void conv3x3_kernel(const uint16_t* __restrict_
   }
 }
 
-// dW[tap][co][ci] += sum_p g[p][co] * x[p + tap][ci] over a chunk of pixels.
+// Weight gradient, GEMM M = output channels, N = input channels, K =
+// pixels:
+//   part[band][tap][co][ci] = sum over the band's pixels p of g[p][co] * x[p + tap][ci]
 // g: bf16 [H*W][gs] (the masked output gradient), x: bf16 [H*W][xs] channels
-// xo .. xo+cin. One wave per (tap, co tile, ci tile, pixel chunk); atomics
-// reduce the chunks. dw is fp32 [9][cout][cin].
-__global__ __launch_bounds__(64) void wgrad_kernel(const uint16_t* __restrict__ g, int gs,
-                                                   const uint16_t* __restrict__ x, int xs, int xo,
-                                                   int H, int W, int cin, int cout, int chunk,
-                                                   float* dw) {
+// xo .. xo+cin. One wave per (ci tile, co tile, band of image rows, tap row
+// dy): the 3 taps of one row share the staged input. Each 32-pixel row
+// segment is staged through LDS once: the gradient tile transposed ([co][px],
+// so a lane's 4 consecutive pixels are one 8-byte read) and the input row
+// y + dy with a one-pixel halo ([ci][px + 1]); the 3 taps read shifted
+// windows of it. Bands write partial sums, which wgrad_reduce_kernel adds
+// into dw in a fixed order (no atomics).
+constexpr int kSeg = 32;         // pixels per row segment: 4 MFMA K steps
+constexpr int kRow = kSeg + 4;   // LDS row (u16): kSeg + 2 pixels, 8-byte multiple
+
+__global__ __launch_bounds__(64) void wgrad_band_kernel(const uint16_t* __restrict__ g, int gs,
+                                                        const uint16_t* __restrict__ x, int xs,
+                                                        int xo, int H, int W, int cin, int cout,
+                                                        int rows, float* __restrict__ part) {
+  __shared__ __align__(16) uint16_t gT[32 * kRow];
+  __shared__ __align__(16) uint16_t xT[32 * kRow];
   const int l = threadIdx.x;
-  const int tap = blockIdx.z % 9, pc = blockIdx.z / 9;
-  const int co0 = blockIdx.y * 32, ci0 = blockIdx.x * 32;
-  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-  const int npix = H * W;
-  const int pbeg = pc * chunk, pend = min(npix, pbeg + chunk);
-  const int co = co0 + (l & 31), ci = ci0 + (l & 31);
-  const int kq = (l >> 5) * 4;
-  f16v acc = {};
-  for (int pb = pbeg; pb < pend; pb += 8) {
-    s4 a, b;
+  const int ci0 = blockIdx.x * 32, co0 = blockIdx.y * 32;
+  const int band = blockIdx.z / 3, dyi = blockIdx.z % 3;
+  const int y0 = band * rows, y1 = min(H, y0 + rows);
+  const int r32 = l & 31, kq = (l >> 5) * 4;
+  f16v acc[3] = {f16v{}, f16v{}, f16v{}};
+  for (int y = y0; y < y1; ++y) {
+    const int sy = y + dyi - 1;
+    for (int x0 = 0; x0 < W; x0 += kSeg) {
+      for (int i = l; i < kSeg * 8; i += 64) {  // gradient: 32 px x 8 groups of 4 channels
+        const int px = i >> 3, grp = i & 7, c = co0 + grp * 4, xx = x0 + px;
+        const s4 v = load4(g + (size_t(y) * W + xx) * gs + c, xx < W && c < cout);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int p = pb + kq + j;
-      uint16_t av = 0, bv = 0;
-      if (p < pend) {
-        if (co < cout) av = g[size_t(p) * gs + co];
-        const int py = p / W, px = p - py * W;
-        const int sy = py + dy, sx = px + dx;
-        if (ci < cin && sy >= 0 && sy < H && sx >= 0 && sx < W)
-          bv = x[size_t(sy * W + sx) * xs + xo + ci];
+        for (int j = 0; j < 4; ++j)
+          gT[(grp * 4 + j) * kRow + px] = c + j < cout ? uint16_t(v[j]) : uint16_t(0);
       }
-      a[j] = short(av);
-      b[j] = short(bv);
-    }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-  }
-  const int cc = ci0 + (l & 31);
-  if (cc >= cin) return;
+      for (int i = l; i < (kSeg + 2) * 8; i += 64) {  // input row sy: 34 px x 8 groups
+        const int grp = i & 7, px = i >> 3;
+        const int sx = x0 + px - 1, c = ci0 + grp * 4;
+        const bool in = sy >= 0 && sy < H && sx >= 0 && sx < W && c < cin;
+        const s4 v = load4(x + (size_t(in ? sy * W + sx : 0) * xs + xo + (in ? c : 0)), in);
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int r = co0 + (v >> 2) * 8 + (l >> 5) * 4 + (v & 3);
-    if (r < cout) atomicAdd(&dw[(size_t(tap) * cout + r) * cin + cc], acc[v]);
+        for (int j = 0; j < 4; ++j)
+          xT[(grp * 4 + j) * kRow + px] = c + j < cin ? uint16_t(v[j]) : uint16_t(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < kSeg / 8; ++ks) {
+        const int kb = ks * 8 + kq;  // this lane's 4 pixels of the K step
+        const s4 a = *reinterpret_cast<const s4*>(&gT[r32 * kRow + kb]);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const uint16_t* xr = &xT[r32 * kRow + kb + dx];
+          const s4 b = s4{short(xr[0]), short(xr[1]), short(xr[2]), short(xr[3])};
+          acc[dx] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[dx], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int ci = ci0 + r32;
+  if (ci >= cin) return;
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) {
+    const int tap = dyi * 3 + dx;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int co = co0 + (v >> 2) * 8 + kq + (v & 3);
+      if (co < cout) part[((size_t(band) * 9 + tap) * cout + co) * cin + ci] = acc[dx][v];
+    }
+  }
+}
+
+// dw[i] += sum over bands of part[band][i]. Block: 16 consecutive outputs x
+// 16 band groups (group g sums bands g, g + 16, ...); the groups are then
+// added in order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* __restrict__ part,
+                                                           int bands, int n,
+                                                           float* __restrict__ dw) {
+  __shared__ float sums[16][17];
+  const int t = threadIdx.x, o = t & 15, grp = t >> 4;
+  const int i = blockIdx.x * 16 + o;
+  float sum = 0.f;
+  if (i < n)
+    for (int b = grp; b < bands; b += 16) sum += part[size_t(b) * n + i];
+  sums[grp][o] = sum;
+  __syncthreads();
+  if (t < 16 && i < n) {
+    float acc = 0.f;
+    for (int g2 = 0; g2 < 16; ++g2) acc += sums[g2][t];
+    dw[i] += acc;
   }
 }
 
@@ -337,13 +387,23 @@ hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, con
 }
 
 hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
-                 int cin, int cout, float* dw, hipStream_t s) {
-  const int chunk = 2048;
-  const int npix = H * W;
-  const dim3 grid(unsigned((cin + 31) / 32), unsigned((cout + 31) / 32),
-                  unsigned(9 * ((npix + chunk - 1) / chunk)));
-  hipLaunchKernelGGL(wgrad_kernel, grid, dim3(64), 0, s, g, gs, x, xs, xo, H, W, cin, cout, chunk,
-                     dw);
+                 int cin, int cout, float* dw, float* part, size_t part_cap, hipStream_t s) {
+  // 8-byte channel groups: strides and offsets in whole groups, and the
+  // groups that hold the last channels inside the row
+  if (gs % 4 || xs % 4 || xo % 4 || (cout + 3) / 4 * 4 > gs || xo + (cin + 3) / 4 * 4 > xs)
+    return hipErrorInvalidValue;
+  const int tiles = ((cin + 31) / 32) * ((cout + 31) / 32);
+  const size_t per_band = size_t(9) * cout * cin;
+  if (per_band > part_cap) return hipErrorInvalidValue;
+  // about 2048 waves (3 tap rows per band), within the partial-sum buffer
+  int bands = std::max(1, std::min(H, std::min((683 + tiles - 1) / tiles, int(part_cap / per_band))));
+  const int rows = (H + bands - 1) / bands;
+  bands = (H + rows - 1) / rows;
+  const dim3 grid(unsigned((cin + 31) / 32), unsigned((cout + 31) / 32), unsigned(bands * 3));
+  hipLaunchKernelGGL(wgrad_band_kernel, grid, dim3(64), 0, s, g, gs, x, xs, xo, H, W, cin, cout,
+                     rows, part);
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(nb(per_band, 16)), dim3(256), 0, s, part, bands,
+                     int(per_band), dw);
   return hipGetLastError();
 }
 

@@ -239,22 +239,39 @@ def level_weights(ks: np.ndarray) -> np.ndarray:
 def train(dn: Denoiser, xs: np.ndarray, ts: np.ndarray, steps: int, batch: int = 8,
           lr: float = 1e-3, seed: int = 0, weights=None, log=None) -> list:
     """Adam over random batches with random flips; the learning rate decays
-    exponentially to lr / 10 over the run."""
+    exponentially to lr / 10 over the run. The next batch is gathered on a
+    helper thread while the GPU trains on the current one (the C calls
+    release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
-    losses = []
-    for i in range(steps):
+    plan = []
+    for _ in range(steps):
         idx = rng.choice(len(xs), size=min(batch, len(xs)), replace=False)
+        plan.append((idx, rng.random() < 0.5, rng.random() < 0.5))
+
+    def gather(i):
+        idx, fx, fy = plan[i]
         # flips keep the pairs valid and widen the data
         x, t = xs[idx], ts[idx]
-        if rng.random() < 0.5:
+        if fx:
             x, t = x[:, :, ::-1], t[:, :, ::-1]
-        if rng.random() < 0.5:
+        if fy:
             x, t = x[:, ::-1], t[:, ::-1]
-        step_lr = lr * (0.1 ** (i / max(1, steps)))
         w = None if weights is None else np.asarray(weights)[idx]
-        losses.append(dn.train_step(x, t, step_lr, w))
-        if log and (i % 200 == 0 or i == steps - 1):
-            log(f"[denoise] step {i}: loss {losses[-1]:.3e} lr {step_lr:.2e}")
+        return np.ascontiguousarray(x), np.ascontiguousarray(t), w
+
+    losses = []
+    with ThreadPoolExecutor(1) as pool:
+        nxt = pool.submit(gather, 0) if steps else None
+        for i in range(steps):
+            x, t, w = nxt.result()
+            if i + 1 < steps:
+                nxt = pool.submit(gather, i + 1)
+            step_lr = lr * (0.1 ** (i / max(1, steps)))
+            losses.append(dn.train_step(x, t, step_lr, w))
+            if log and (i % 500 == 0 or i == steps - 1):
+                log(f"[denoise] step {i}: loss {losses[-1]:.3e} lr {step_lr:.2e}")
     return losses
 
 
@@ -276,3 +293,38 @@ def frames_to_target(levels, mses, target_mse: float):
 
 def rmse(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.sqrt(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)))
+
+
+def _main() -> int:
+    """Time the denoiser's kernels on synthetic data: training steps at the
+    training tile size and inference at the workload's image size."""
+    import argparse
+    import time
+
+    ap = argparse.ArgumentParser(description="time denoiser training and inference (synthetic data)")
+    ap.add_argument("--size", type=int, default=256, help="training tile side")
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--infer", type=int, default=1024, help="inference image side")
+    ap.add_argument("--infer-runs", type=int, default=5)
+    args = ap.parse_args()
+    rng = np.random.default_rng(0)
+    xs = rng.uniform(0, 1, (2 * args.batch, args.size, args.size, INPUTS)).astype(np.float32)
+    ts = rng.uniform(0, 1, (2 * args.batch, args.size, args.size, 3)).astype(np.float32)
+    dn = Denoiser(args.size, args.size, seed=1)
+    train(dn, xs, ts, 3, batch=args.batch)
+    t = time.perf_counter()
+    train(dn, xs, ts, args.steps, batch=args.batch)
+    step_ms = (time.perf_counter() - t) * 1000.0 / max(1, args.steps)
+    print(f"train step {step_ms:.1f} ms (batch {args.batch}, {args.size}^2)", flush=True)
+    dn.close()
+    big = Denoiser(args.infer, args.infer, seed=1)
+    x = rng.uniform(0, 1, (args.infer, args.infer, INPUTS)).astype(np.float32)
+    ms = [big.infer(x)[1] for _ in range(args.infer_runs)]
+    print(f"inference {args.infer}^2: {float(np.median(ms)):.3f} ms (median of {len(ms)})", flush=True)
+    big.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())

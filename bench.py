@@ -112,7 +112,7 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
                                            size=args.dn_size, seed=args.dn_seed,
                                            workload=scene.name, log=log)
     t1 = time.perf_counter()
-    small = denoise.Denoiser(args.dn_size, args.dn_size, seed=1)
+    small = denoise.Denoiser(args.dn_size, args.dn_size, channels=scene.channels, seed=1)
     losses = denoise.train(small, xs, ts, args.dn_steps, batch=8, lr=args.dn_lr,
                            weights=denoise.level_weights(ks), log=log)
     params = small.get_params()
@@ -122,7 +122,7 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     del xs, ts
     log(f"[bench] denoiser: {n_pairs} pairs in {t1 - t0:.1f} s, {args.dn_steps} steps in {t2 - t1:.1f} s")
 
-    dn = denoise.Denoiser(scene.width, scene.height, seed=1)
+    dn = denoise.Denoiser(scene.width, scene.height, channels=scene.channels, seed=1)
     dn.set_params(params)
     r.set_aov(True)
     r.set_aov_frames(0)

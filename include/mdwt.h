@@ -191,14 +191,14 @@ int mdwt_set_macrocells(mdwt_renderer* h, const int mc[3]);
 /* Auxiliary buffers for the denoiser (no reference counterpart): when on,
  * every frame also accumulates per pixel the running means of the first-hit
  * distance (0 on a miss), the transmittance (1 - hit fraction) and the hit
- * fractions of channels 0..3. Switching restarts the accumulation. */
+ * fraction of every channel. Switching restarts the accumulation. */
 int mdwt_set_aov(mdwt_renderer* h, int on);
 /* Accumulate the auxiliary buffers during the first `frames` frames of each
  * accumulation only (0, the default: every frame); later frames render as
  * with the buffers off and the buffers keep the means over those frames.
  * Changing it restarts the accumulation. */
 int mdwt_set_aov_frames(mdwt_renderer* h, int frames);
-/* width * height * 6 floats in that order */
+/* width * height * (2 + channels) floats per pixel in that order */
 int mdwt_read_aov(mdwt_renderer* h, float* out);
 
 /* ---- batched views (no reference counterpart: the reference renders one

@@ -8,13 +8,13 @@
  * kernel-predicting (Bako et al. 2017): per pixel, the softmax of 49 logits
  * weighs the input colour of the 7x7 neighbourhood (coordinates clamped to
  * the image), so every output is a convex combination of input colours; the
- * untrained network applies a Gaussian of 1.5 pixels. Input per pixel, 11 floats:
- * r, g, b (the accumulated image), first-hit depth, transmittance (1 - hit
- * fraction), the hit fractions of channels 0..3 (mdwt_set_aov /
- * mdwt_read_aov), the standard error of the pixel mean (from the second
- * moment of mdwt_read_accum) and log2 of the accumulated frame count, the
- * last two in the scaling the caller trained with. Images are row-major,
- * pixel-interleaved float arrays.
+ * untrained network applies a Gaussian of 1.5 pixels. Input per pixel,
+ * 7 + channels floats: r, g, b (the accumulated image), first-hit depth,
+ * transmittance (1 - hit fraction), the hit fraction of each channel
+ * (mdwt_set_aov / mdwt_read_aov), the standard error of the pixel mean (from
+ * the second moment of mdwt_read_accum) and log2 of the accumulated frame
+ * count, the last two in the scaling the caller trained with. Images are
+ * row-major, pixel-interleaved float arrays.
  * Width and height must be multiples of 4 (two pooling levels). One HIP
  * stream per handle; not thread-safe. Status codes are those of mdwt.h. */
 #ifndef MDWT_DENOISE_H
@@ -30,17 +30,19 @@ extern "C" {
 
 typedef struct mdwt_denoiser mdwt_denoiser;
 
-/* A denoiser for width x height images on HIP device hip_device; weights
- * initialised from seed (He initialisation; output layer near zero with the
- * Gaussian's log-weights as bias). */
-int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, uint32_t seed);
+/* A denoiser for width x height images of a volume with `channels` channels
+ * (1..12) on HIP device hip_device; weights initialised from seed (He
+ * initialisation; output layer near zero with the Gaussian's log-weights as
+ * bias). */
+int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, int channels,
+                   uint32_t seed);
 int mdwt_dn_destroy(mdwt_denoiser* d);
 const char* mdwt_dn_last_error(const mdwt_denoiser* d);
 /* number of trainable parameters (weights [9][cout][cin] then bias, per layer) */
 int mdwt_dn_num_params(const mdwt_denoiser* d, int* n);
 int mdwt_dn_get_params(mdwt_denoiser* d, float* out);
 int mdwt_dn_set_params(mdwt_denoiser* d, const float* in);
-/* Denoise one image: input [h][w][11] -> output [h][w][3]; ms = GPU time of
+/* Denoise one image: input [h][w][7 + channels] -> output [h][w][3]; ms = GPU time of
  * the network (uploads and downloads excluded). */
 int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms);
 /* Gradient of the mean squared error against target [h][w][3] for one

@@ -81,7 +81,7 @@ SIGNATURES = {
     "mdwt_set_aov": [_h, C.c_int],
     "mdwt_set_aov_frames": [_h, C.c_int],
     "mdwt_read_aov": [_h, _fp],
-    "mdwt_dn_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_uint32],
+    "mdwt_dn_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32],
     "mdwt_dn_destroy": [_h],
     "mdwt_dn_last_error": [_h],
     "mdwt_dn_num_params": [_h, _ip],
@@ -324,8 +324,8 @@ class MdwtRenderer:
         self._chk(lib.mdwt_set_aov_frames(self._h, frames))
 
     def read_aov(self) -> np.ndarray:
-        """[h][w][6]: first-hit distance, transmittance, hit fractions of channels 0..3."""
-        a = np.empty((self.height, self.width, 6), np.float32)
+        """[h][w][2 + C]: first-hit distance, transmittance, hit fraction of each channel."""
+        a = np.empty((self.height, self.width, 2 + self.info()["channels"]), np.float32)
         self._chk(lib.mdwt_read_aov(self._h, _f(a)))
         return a
 

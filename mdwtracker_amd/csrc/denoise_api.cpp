@@ -4,10 +4,9 @@
 // FP32 accumulation on the matrix cores, trained in place with Adam on pairs
 // of low- and high-sample renders. Its output is kernel-predicting (Bako et
 // al. 2017): per pixel, softmax weights of a 7x7 filter applied to the input
-// colour. Input per pixel (kInputs = 11 values): rgb, first-hit
-// depth, transmittance, the four channel fractions of the hits
-// (mdwt_set_aov), the standard error of the pixel mean and log2 of the frame
-// count.
+// colour. Input per pixel (7 + channels values): rgb, first-hit depth,
+// transmittance, the hit fraction of each channel (mdwt_set_aov), the
+// standard error of the pixel mean and log2 of the frame count.
 // No reference counterpart; C ABI in include/mdwt_denoise.h.
 #include "mdwt_denoise.h"
 
@@ -43,6 +42,7 @@ struct mdwt_denoiser {
   std::string err;
   int device = 0;
   int H = 0, W = 0;
+  int channels = 4, inputs = 11, in_pad = 16;  // input values per pixel, padded to 8
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Conv c[11];
@@ -80,9 +80,9 @@ int alloc(mdwt_denoiser* d, T** p, size_t n) {
   return MDWT_OK;
 }
 
-// layer table: (cin, cout, relu); c0 reads the input padded to 16 channels,
+// layer table: (cin, cout, relu); c0 reads the input (cin set per handle),
 // c10 writes the kTaps filter logits
-const int kLayers[11][3] = {{mdwt::dn::kInputs, 32, 1}, {32, 32, 1}, {32, 64, 1},
+const int kLayers[11][3] = {{0, 32, 1}, {32, 32, 1}, {32, 64, 1},
                             {64, 64, 1},  {64, 96, 1}, {96, 96, 1}, {160, 64, 1},
                             {64, 64, 1},  {96, 32, 1}, {32, 32, 1}, {32, mdwt::dn::kTaps, 0}};
 
@@ -96,10 +96,10 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
   std::mt19937 gen(seed);
   for (int i = 0; i < 11; ++i) {
     Conv& L = d->c[i];
-    L.cin = kLayers[i][0];
+    L.cin = i == 0 ? d->inputs : kLayers[i][0];
     L.cout = kLayers[i][1];
     L.relu = kLayers[i][2] != 0;
-    L.cin_pad = i == 0 ? 16 : round_up(L.cin, 8);
+    L.cin_pad = i == 0 ? d->in_pad : round_up(L.cin, 8);
     L.cout_pad = round_up(L.cout, 32);
     L.cin_pad32 = round_up(L.cin, 32);
     L.nw = size_t(9) * L.cout * L.cin;
@@ -131,7 +131,7 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
   }
   const size_t n = size_t(d->H) * d->W, n4 = n / 4, n16 = n / 16;
   int rc;
-  if ((rc = alloc(d, &d->x0, n * 16)) || (rc = alloc(d, &d->a0, n * 32)) ||
+  if ((rc = alloc(d, &d->x0, n * d->in_pad)) || (rc = alloc(d, &d->a0, n * 32)) ||
       (rc = alloc(d, &d->a1, n * 32)) || (rc = alloc(d, &d->p1, n4 * 32)) ||
       (rc = alloc(d, &d->a2, n4 * 64)) || (rc = alloc(d, &d->a3, n4 * 64)) ||
       (rc = alloc(d, &d->p2, n16 * 64)) || (rc = alloc(d, &d->a4, n16 * 96)) ||
@@ -140,7 +140,7 @@ int setup(mdwt_denoiser* d, uint32_t seed) {
       (rc = alloc(d, &d->u1, n * 96)) || (rc = alloc(d, &d->a8, n * 32)) ||
       (rc = alloc(d, &d->a9, n * 32)) || (rc = alloc(d, &d->gm, n * mdwt::dn::kLogitStride)))
     return rc;
-  if ((rc = alloc(d, &d->in32, n * mdwt::dn::kInputs)) || (rc = alloc(d, &d->tgt, n * 3)) ||
+  if ((rc = alloc(d, &d->in32, n * d->inputs)) || (rc = alloc(d, &d->tgt, n * 3)) ||
       (rc = alloc(d, &d->out32, n * mdwt::dn::kLogitStride)) || (rc = alloc(d, &d->pred, n * 3)) ||
       (rc = alloc(d, &d->err_sum, 1)) || (rc = alloc(d, &d->wpart, kWgradPart)) ||
       (rc = alloc(d, &d->gout, n * mdwt::dn::kLogitStride)) ||
@@ -176,8 +176,8 @@ int forward(mdwt_denoiser* d) {
   const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
   hipStream_t s = d->stream;
   int rc;
-  DNCHK(d, mdwt::dn::pack_input(d->in32, n, d->x0, s));
-  if ((rc = fwd(d, 0, d->x0, H, W, 16, 0, d->a0, nullptr, 32))) return rc;
+  DNCHK(d, mdwt::dn::pack_input(d->in32, n, d->inputs, d->in_pad, d->x0, s));
+  if ((rc = fwd(d, 0, d->x0, H, W, d->in_pad, 0, d->a0, nullptr, 32))) return rc;
   if ((rc = fwd(d, 1, d->a0, H, W, 32, 0, d->a1, nullptr, 32))) return rc;
   DNCHK(d, mdwt::dn::pool(d->a1, H, W, 32, d->p1, s));
   if ((rc = fwd(d, 2, d->p1, H2, W2, 32, 0, d->a2, nullptr, 64))) return rc;
@@ -194,7 +194,7 @@ int forward(mdwt_denoiser* d) {
   if ((rc = fwd(d, 8, d->u1, H, W, 96, 0, d->a8, nullptr, 32))) return rc;
   if ((rc = fwd(d, 9, d->a8, H, W, 32, 0, d->a9, nullptr, 32))) return rc;
   if ((rc = fwd(d, 10, d->a9, H, W, 32, 0, nullptr, d->out32, mdwt::dn::kLogitStride))) return rc;
-  DNCHK(d, mdwt::dn::kpn_apply(d->in32, d->out32, H, W, d->pred, s));
+  DNCHK(d, mdwt::dn::kpn_apply(d->in32, d->inputs, d->out32, H, W, d->pred, s));
   return MDWT_OK;
 }
 
@@ -239,7 +239,7 @@ int backward(mdwt_denoiser* d) {
   if ((rc = bwd(d, 2, d->g2, 64, d->a2, 64, d->p1, 32, 0, H2, W2, d->gp1, 32))) return rc;
   DNCHK(d, mdwt::dn::pool_grad(d->gp1, H, W, 32, d->g1, s));
   if ((rc = bwd(d, 1, d->g1, 32, d->a1, 32, d->a0, 32, 0, H, W, d->g0, 32))) return rc;
-  if ((rc = bwd(d, 0, d->g0, 32, d->a0, 32, d->x0, 16, 0, H, W, nullptr, 0))) return rc;
+  if ((rc = bwd(d, 0, d->g0, 32, d->a0, 32, d->x0, d->in_pad, 0, H, W, nullptr, 0))) return rc;
   return MDWT_OK;
 }
 
@@ -254,13 +254,13 @@ int zero_grads(mdwt_denoiser* d) {
 // upload one (input, target) pair and accumulate its loss gradient
 int one_pair(mdwt_denoiser* d, const float* in, const float* tgt, float inv_count) {
   const size_t n = size_t(d->H) * d->W;
-  DNCHK(d, hipMemcpyAsync(d->in32, in, n * mdwt::dn::kInputs * sizeof(float),
+  DNCHK(d, hipMemcpyAsync(d->in32, in, n * d->inputs * sizeof(float),
                           hipMemcpyHostToDevice, d->stream));
   DNCHK(d, hipMemcpyAsync(d->tgt, tgt, n * 3 * sizeof(float), hipMemcpyHostToDevice, d->stream));
   int rc = forward(d);
   if (rc) return rc;
-  DNCHK(d, mdwt::dn::kpn_loss_grad(d->in32, d->out32, d->pred, d->tgt, d->H, d->W, inv_count,
-                                   d->gout, d->err_sum, d->stream));
+  DNCHK(d, mdwt::dn::kpn_loss_grad(d->in32, d->inputs, d->out32, d->pred, d->tgt, d->H, d->W,
+                                   inv_count, d->gout, d->err_sum, d->stream));
   return backward(d);
 }
 
@@ -268,16 +268,21 @@ int one_pair(mdwt_denoiser* d, const float* in, const float* tgt, float inv_coun
 
 extern "C" {
 
-int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, uint32_t seed) {
+int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, int channels,
+                   uint32_t seed) {
   if (!out) return MDWT_ERR_ARG;
   *out = nullptr;
-  if (width < 4 || height < 4 || width % 4 || height % 4 || width > 8192 || height > 8192)
+  if (width < 4 || height < 4 || width % 4 || height % 4 || width > 8192 || height > 8192 ||
+      channels < 1 || channels > 12)
     return MDWT_ERR_ARG;
   auto* d = new (std::nothrow) mdwt_denoiser();
   if (!d) return MDWT_ERR_OOM;
   d->device = hip_device;
   d->W = width;
   d->H = height;
+  d->channels = channels;
+  d->inputs = mdwt::dn::input_width(channels);
+  d->in_pad = round_up(d->inputs, 8);
   int rc = MDWT_OK;
   if (hipSetDevice(hip_device) != hipSuccess ||
       hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -349,7 +354,7 @@ int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms
   if (!d || !input || !output) return MDWT_ERR_ARG;
   DNCHK(d, hipSetDevice(d->device));
   const size_t n = size_t(d->H) * d->W;
-  DNCHK(d, hipMemcpyAsync(d->in32, input, n * mdwt::dn::kInputs * sizeof(float),
+  DNCHK(d, hipMemcpyAsync(d->in32, input, n * d->inputs * sizeof(float),
                           hipMemcpyHostToDevice, d->stream));
   DNCHK(d, hipEventRecord(d->ev0, d->stream));
   int rc = forward(d);
@@ -399,7 +404,7 @@ int mdwt_dn_train_step(mdwt_denoiser* d, const float* inputs, const float* targe
   DNCHK(d, hipMemsetAsync(d->err_sum, 0, sizeof(float), d->stream));
   const float inv = 1.f / float(n * 3 * size_t(count));
   for (int i = 0; i < count; ++i) {
-    const float* xi = inputs + size_t(i) * n * mdwt::dn::kInputs;
+    const float* xi = inputs + size_t(i) * n * d->inputs;
     const float wi = weights ? inv * weights[i] : inv;
     if ((rc = one_pair(d, xi, targets + size_t(i) * n * 3, wi))) return rc;
     DNCHK(d, hipStreamSynchronize(d->stream));  // host inputs are reused by the caller

@@ -11,10 +11,10 @@
 namespace mdwt {
 namespace dn {
 
-// values per pixel of the network input: rgb, first-hit depth, transmittance,
-// the four channel hit fractions, the standard error of the pixel mean and
-// log2 of the frame count (both scaled)
-constexpr int kInputs = 11;
+// values per pixel of the network input for a volume of `channels` channels:
+// rgb, first-hit depth, transmittance, the hit fraction of each channel, the
+// standard error of the pixel mean and log2 of the frame count (both scaled)
+constexpr int input_width(int channels) { return 7 + channels; }
 // kernel-predicting output: kKernel x kKernel filter weights per pixel
 constexpr int kKernelRadius = 3;
 constexpr int kKernel = 2 * kKernelRadius + 1;
@@ -44,13 +44,14 @@ hipError_t up_grad(const float* gy, int H, int W, int ys, int yo, int C, float* 
 hipError_t copy_slice(const uint16_t* x, int xs, int xo, int n, int C, uint16_t* y, int ys, int yo,
                       hipStream_t s);
 hipError_t add_slice(const float* gy, int ys, int yo, int n, int C, float* gx, hipStream_t s);
-hipError_t pack_input(const float* in, int n, uint16_t* x, hipStream_t s);
+// in: fp32 [n][inputs] -> x: bf16 [n][in_pad], channels from inputs up zero
+hipError_t pack_input(const float* in, int n, int inputs, int in_pad, uint16_t* x, hipStream_t s);
 // pred [H*W][3] = softmax(logits) over the kTaps neighbourhood of the input
-// colour (in: fp32 [H*W][kInputs]; logits: fp32 [H*W][kLogitStride])
-hipError_t kpn_apply(const float* in, const float* logits, int H, int W, float* pred,
+// colour (in: fp32 [H*W][inputs], rgb first; logits: fp32 [H*W][kLogitStride])
+hipError_t kpn_apply(const float* in, int inputs, const float* logits, int H, int W, float* pred,
                      hipStream_t s);
 // err += sum (pred - target)^2; gz = d(inv_count * that sum) / d logits
-hipError_t kpn_loss_grad(const float* in, const float* logits, const float* pred,
+hipError_t kpn_loss_grad(const float* in, int inputs, const float* logits, const float* pred,
                          const float* target, int H, int W, float inv_count, float* gz, float* err,
                          hipStream_t s);
 hipError_t adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,

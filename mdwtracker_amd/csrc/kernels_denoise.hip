@@ -259,13 +259,13 @@ __global__ void add_slice_kernel(const float* gy, int ys, int yo, int n, int C, 
   gx[i] += gy[size_t(p) * ys + yo + c];
 }
 
-// network input: in (fp32 [n][kInputs]) -> bf16 [n][16] (channels from
-// kInputs up zero)
-__global__ void pack_input_kernel(const float* in, int n, uint16_t* x) {
+// network input: in (fp32 [n][inputs]) -> bf16 [n][in_pad] (channels from
+// inputs up zero)
+__global__ void pack_input_kernel(const float* in, int n, int inputs, int in_pad, uint16_t* x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * 16) return;
-  const int p = i / 16, c = i - p * 16;
-  x[i] = c < kInputs ? f2bf(in[size_t(p) * kInputs + c]) : uint16_t(0);
+  if (i >= n * in_pad) return;
+  const int p = i / in_pad, c = i - p * in_pad;
+  x[i] = c < inputs ? f2bf(in[size_t(p) * inputs + c]) : uint16_t(0);
 }
 // Kernel-predicting output (Bako et al. 2017): per pixel, the softmax of
 // the kTaps logits (fp32 [n][kLogitStride]) weighs the input colour of the
@@ -290,8 +290,9 @@ __device__ __forceinline__ float kpn_weights(const float* logits, int p, float* 
   return sum;
 }
 // pred [n][3] = sum_j softmax(z)_j * rgb(neighbour j)
-__global__ void kpn_apply_kernel(const float* __restrict__ in, const float* __restrict__ logits,
-                                 int H, int W, float* __restrict__ pred) {
+__global__ void kpn_apply_kernel(const float* __restrict__ in, int inputs,
+                                 const float* __restrict__ logits, int H, int W,
+                                 float* __restrict__ pred) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= H * W) return;
   const int py = p / W, px = p - py * W;
@@ -300,7 +301,7 @@ __global__ void kpn_apply_kernel(const float* __restrict__ in, const float* __re
   float r = 0.f, g = 0.f, b = 0.f;
 #pragma unroll
   for (int j = 0; j < kTaps; ++j) {
-    const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * kInputs;
+    const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * inputs;
     r += w[j] * c[0];
     g += w[j] * c[1];
     b += w[j] * c[2];
@@ -313,7 +314,8 @@ __global__ void kpn_apply_kernel(const float* __restrict__ in, const float* __re
 // the gradient of inv_count * that sum with respect to the logits,
 // dz_j = w_j * (c_j - pred) . gpred with gpred = 2 inv_count (pred - target),
 // into gz [n][kLogitStride] (channels < kTaps).
-__global__ void kpn_loss_grad_kernel(const float* __restrict__ in, const float* __restrict__ logits,
+__global__ void kpn_loss_grad_kernel(const float* __restrict__ in, int inputs,
+                                     const float* __restrict__ logits,
                                      const float* __restrict__ pred, const float* __restrict__ target,
                                      int H, int W, float inv_count, float* __restrict__ gz,
                                      float* err) {
@@ -332,7 +334,7 @@ __global__ void kpn_loss_grad_kernel(const float* __restrict__ in, const float* 
     const float base = pr * gr + pg * gg + pb * gb;
 #pragma unroll
     for (int j = 0; j < kTaps; ++j) {
-      const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * kInputs;
+      const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * inputs;
       gz[size_t(p) * kLogitStride + j] = w[j] * inv * (c[0] * gr + c[1] * gg + c[2] * gb - base);
     }
   }
@@ -450,21 +452,22 @@ hipError_t add_slice(const float* gy, int ys, int yo, int n, int C, float* gx, h
                      gx);
   return hipGetLastError();
 }
-hipError_t pack_input(const float* in, int n, uint16_t* x, hipStream_t s) {
-  hipLaunchKernelGGL(pack_input_kernel, dim3(nb(size_t(n) * 16)), dim3(256), 0, s, in, n, x);
+hipError_t pack_input(const float* in, int n, int inputs, int in_pad, uint16_t* x, hipStream_t s) {
+  hipLaunchKernelGGL(pack_input_kernel, dim3(nb(size_t(n) * in_pad)), dim3(256), 0, s, in, n,
+                     inputs, in_pad, x);
   return hipGetLastError();
 }
-hipError_t kpn_apply(const float* in, const float* logits, int H, int W, float* pred,
+hipError_t kpn_apply(const float* in, int inputs, const float* logits, int H, int W, float* pred,
                      hipStream_t s) {
-  hipLaunchKernelGGL(kpn_apply_kernel, dim3(nb(size_t(H) * W)), dim3(256), 0, s, in, logits, H, W,
-                     pred);
+  hipLaunchKernelGGL(kpn_apply_kernel, dim3(nb(size_t(H) * W)), dim3(256), 0, s, in, inputs,
+                     logits, H, W, pred);
   return hipGetLastError();
 }
-hipError_t kpn_loss_grad(const float* in, const float* logits, const float* pred,
+hipError_t kpn_loss_grad(const float* in, int inputs, const float* logits, const float* pred,
                          const float* target, int H, int W, float inv_count, float* gz, float* err,
                          hipStream_t s) {
-  hipLaunchKernelGGL(kpn_loss_grad_kernel, dim3(nb(size_t(H) * W)), dim3(256), 0, s, in, logits,
-                     pred, target, H, W, inv_count, gz, err);
+  hipLaunchKernelGGL(kpn_loss_grad_kernel, dim3(nb(size_t(H) * W)), dim3(256), 0, s, in, inputs,
+                     logits, pred, target, H, W, inv_count, gz, err);
   return hipGetLastError();
 }
 hipError_t adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,

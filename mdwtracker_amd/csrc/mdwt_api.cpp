@@ -417,8 +417,9 @@ int alloc_frame(mdwt_renderer* h) {
   HIPCHK(h, hipMemsetAsync(h->d_m2, 0, n * sizeof(Float4), h->stream));
   h->fb_pixels = n;
   if (h->aov_on) {
-    HIPCHK(h, hipMalloc(&h->d_aov, 2 * n * sizeof(Float4)));
-    HIPCHK(h, hipMemsetAsync(h->d_aov, 0, 2 * n * sizeof(Float4), h->stream));
+    const size_t na = size_t(mdwt::aov_groups(h->C)) * n;
+    HIPCHK(h, hipMalloc(&h->d_aov, na * sizeof(Float4)));
+    HIPCHK(h, hipMemsetAsync(h->d_aov, 0, na * sizeof(Float4), h->stream));
   }
   // (the flat / persistent schedules only run with 16x16 blocks: mdwt_set_block
   // and mdwt_set_schedule keep bw = bh = 16 whenever schedule != 0)
@@ -1372,6 +1373,7 @@ mdwt::RenderParams batch_params(const mdwt_renderer* h) {
   p.fr.n_items = int(h->items.size());
   p.fr.work = h->d_work;
   p.fr.item_ticks = h->d_item_ticks;
+  p.fr.aov = nullptr;  // batched views have no auxiliary buffers
   return p;
 }
 
@@ -1761,18 +1763,11 @@ int mdwt_read_aov(mdwt_renderer* h, float* out) {
   if (!h->aov_on || !h->d_aov) return fail(h, MDWT_ERR_STATE, "auxiliary buffers are off");
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  std::vector<Float4> tmp(2 * h->fb_pixels);
-  HIPCHK(h, hipMemcpy(tmp.data(), h->d_aov, tmp.size() * sizeof(Float4), hipMemcpyDeviceToHost));
-  for (size_t p = 0; p < h->fb_pixels; ++p) {
-    const Float4 a = tmp[2 * p], b = tmp[2 * p + 1];
-    float* o = out + 6 * p;
-    o[0] = a.x;
-    o[1] = a.y;
-    o[2] = a.z;
-    o[3] = a.w;
-    o[4] = b.x;
-    o[5] = b.y;
-  }
+  const int ng = mdwt::aov_groups(h->C), nv = 2 + h->C;
+  std::vector<float> tmp(size_t(4 * ng) * h->fb_pixels);
+  HIPCHK(h, hipMemcpy(tmp.data(), h->d_aov, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t p = 0; p < h->fb_pixels; ++p)
+    std::memcpy(out + size_t(nv) * p, tmp.data() + size_t(4 * ng) * p, size_t(nv) * sizeof(float));
   return MDWT_OK;
 }
 

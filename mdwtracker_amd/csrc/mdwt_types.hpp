@@ -8,6 +8,9 @@ namespace mdwt {
 
 enum { MDWT_DT_U8 = 0, MDWT_DT_U16 = 1, MDWT_DT_F16 = 2, MDWT_DT_F32 = 3 };
 constexpr int kMaxChannels = 12;   // deviceCode.h:13
+// auxiliary buffers: per pixel first-hit distance, transmittance and the hit
+// fraction of every channel, packed into float4 groups
+constexpr int aov_groups(int channels) { return (2 + channels + 3) / 4; }
 constexpr int kBlockPixels = 16;   // a frame workgroup covers 16x16 pixels
 constexpr int kThreads = 256;      // 4 wave64, each an 8x8 pixel quad
 constexpr int kMaxTexelsLds = 10240;  // C * stride float4 texels (160 KiB)

@@ -550,21 +550,27 @@ struct ViewCtx {
 
 // Denoiser inputs of one sample folded into the pixel's running means:
 // first-hit distance (0 on a miss), transmittance (1 on a miss, 0 on a hit)
-// and a one-hot of the hit channel (channels 0..3).
+// and a one-hot of the hit channel, aov_groups(C) float4 per pixel.
 __device__ __forceinline__ void aov_sample(const RenderParams& P, const ViewCtx& V, uint32_t pixel,
                                            int s, const HitRec& h) {
-  float4* a = reinterpret_cast<float4*>(P.fr.aov) + 2 * size_t(pixel);
+  const int ng = aov_groups(P.vol.channels);
+  float4* a = reinterpret_cast<float4*>(P.fr.aov) + size_t(ng) * pixel;
   const uint32_t n = (P.fr.accumulate ? V.accum_id * uint32_t(V.spp) : 0u) + uint32_t(s);
   const float k = float(n), k1 = float(n + 1u);
   const bool hit = !h.missed;
-  const float d = hit ? h.tHit : 0.f, T = hit ? 0.f : 1.f;
-  const float c0 = hit && h.channel == 0 ? 1.f : 0.f, c1 = hit && h.channel == 1 ? 1.f : 0.f;
-  const float c2 = hit && h.channel == 2 ? 1.f : 0.f, c3 = hit && h.channel == 3 ? 1.f : 0.f;
   const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 a0 = n ? a[0] : z, a1 = n ? a[1] : z;
-  a[0] = make_float4((d + k * a0.x) / k1, (T + k * a0.y) / k1, (c0 + k * a0.z) / k1,
-                     (c1 + k * a0.w) / k1);
-  a[1] = make_float4((c2 + k * a1.x) / k1, (c3 + k * a1.y) / k1, 0.f, 0.f);
+  for (int g = 0; g < ng; ++g) {
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int slot = 4 * g + j;
+      v[j] = slot == 0 ? (hit ? h.tHit : 0.f)
+                       : slot == 1 ? (hit ? 0.f : 1.f) : (hit && h.channel == slot - 2 ? 1.f : 0.f);
+    }
+    const float4 o = n ? a[g] : z;
+    a[g] = make_float4((v[0] + k * o.x) / k1, (v[1] + k * o.y) / k1, (v[2] + k * o.z) / k1,
+                       (v[3] + k * o.w) / k1);
+  }
 }
 
 // One workgroup's pixel block (bw x bh pixels, one 8x8 quad per wave): every

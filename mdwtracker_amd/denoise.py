@@ -29,7 +29,12 @@ from .capi import MdwtError, MdwtRenderer, lib
 
 _fp = C.POINTER(C.c_float)
 
-INPUTS = 11          # values per pixel, kInputs of the kernels
+def input_width(channels: int) -> int:
+    """Values per pixel of the network input for a volume of `channels` channels."""
+    return 7 + channels
+
+
+INPUTS = input_width(4)  # the 4-channel workloads
 DEPTH_SCALE = 0.25   # first-hit distances of the orbit cameras are ~1..4 world units
 STDERR_SCALE = 4.0   # standard error of the pixel mean, ~0.002..0.5
 FRAMES_SCALE = 1.0 / 16.0  # log2 of the frame count, 0..16
@@ -39,15 +44,17 @@ def _f(a: np.ndarray):
 
 
 class Denoiser:
-    """One denoiser instance for width x height images (multiples of 4)."""
+    """One denoiser instance for width x height images (multiples of 4) of a
+    volume with `channels` channels."""
 
-    def __init__(self, width: int, height: int, seed: int = 1, device: int = 0):
+    def __init__(self, width: int, height: int, channels: int = 4, seed: int = 1, device: int = 0):
         h = C.c_void_p()
-        rc = lib.mdwt_dn_create(C.byref(h), device, width, height, seed & 0xFFFFFFFF)
+        rc = lib.mdwt_dn_create(C.byref(h), device, width, height, channels, seed & 0xFFFFFFFF)
         if rc:
-            raise MdwtError(rc, f"mdwt_dn_create({width}x{height})")
+            raise MdwtError(rc, f"mdwt_dn_create({width}x{height}, {channels} channels)")
         self._h = h
-        self.width, self.height = width, height
+        self.width, self.height, self.channels = width, height, channels
+        self.inputs = input_width(channels)
 
     def close(self) -> None:
         if self._h:
@@ -80,9 +87,9 @@ class Denoiser:
         self._chk(lib.mdwt_dn_set_params(self._h, _f(p)))
 
     def infer(self, x: np.ndarray):
-        """x [h][w][INPUTS] -> (denoised [h][w][3], GPU ms of the network)."""
+        """x [h][w][inputs] -> (denoised [h][w][3], GPU ms of the network)."""
         x = np.ascontiguousarray(x, np.float32)
-        assert x.shape == (self.height, self.width, INPUTS)
+        assert x.shape == (self.height, self.width, self.inputs)
         out = np.empty((self.height, self.width, 3), np.float32)
         ms = C.c_float()
         self._chk(lib.mdwt_dn_infer(self._h, _f(x), _f(out), C.byref(ms)))
@@ -99,7 +106,7 @@ class Denoiser:
     def train_step(self, xs: np.ndarray, ts: np.ndarray, lr: float, weights=None) -> float:
         xs = np.ascontiguousarray(xs, np.float32)
         ts = np.ascontiguousarray(ts, np.float32)
-        assert xs.shape[1:] == (self.height, self.width, INPUTS)
+        assert xs.shape[1:] == (self.height, self.width, self.inputs)
         assert ts.shape[1:] == (self.height, self.width, 3) and len(ts) == len(xs)
         w = None
         if weights is not None:
@@ -112,19 +119,20 @@ class Denoiser:
 
 
 def network_input(accum: np.ndarray, m2: np.ndarray, aov: np.ndarray, frames: int) -> np.ndarray:
-    """[h][w][INPUTS] input from an accumulation of `frames` frames: its mean
-    (accum), second moment (m2) and auxiliary buffers (aov)."""
-    x = np.empty(accum.shape[:2] + (INPUTS,), np.float32)
+    """[h][w][7 + C] input from an accumulation of `frames` frames: its mean
+    (accum), second moment (m2) and auxiliary buffers (aov, [h][w][2 + C])."""
+    nc = aov.shape[-1] - 2
+    x = np.empty(accum.shape[:2] + (input_width(nc),), np.float32)
     x[..., :3] = accum[..., :3]
     x[..., 3] = aov[..., 0] * DEPTH_SCALE
-    x[..., 4:9] = aov[..., 1:6]
+    x[..., 4:5 + nc] = aov[..., 1:]
     if frames > 1:
         mean = accum[..., :3].astype(np.float64)
         var = np.maximum(m2[..., :3].astype(np.float64) - mean * mean, 0.0) * frames / (frames - 1)
-        x[..., 9] = np.sqrt(var.mean(-1) / frames) * STDERR_SCALE
+        x[..., 5 + nc] = np.sqrt(var.mean(-1) / frames) * STDERR_SCALE
     else:
-        x[..., 9] = 0.0
-    x[..., 10] = math.log2(frames) * FRAMES_SCALE
+        x[..., 5 + nc] = 0.0
+    x[..., 6 + nc] = math.log2(frames) * FRAMES_SCALE
     return x
 
 
@@ -197,7 +205,7 @@ TRAIN_LEVELS = (1, 2, 4, 16, 64, 256, 1024, 2048, 4096, 8192)
 def make_training_set(n_scenes: int = 3, views: int = 2, size: int = 256, levels=TRAIN_LEVELS,
                       target_frames: int = 16384, seed: int = 0, scale: int = 1,
                       workload: str = "sync_4ch256", min_signal: float = 0.01, log=None):
-    """(inputs [N][size][size][INPUTS], targets [N][size][size][3], frame
+    """(inputs [N][size][size][7 + C], targets [N][size][size][3], frame
     counts [N]) from n_scenes seeded volumes of the workload family (seeds
     drawn from `seed`, so held out from the fixed-seed workload) seen from
     `views` random orbit cameras each at the workload's image size: one

@@ -34,6 +34,13 @@ def test_network_input_channels(dn):
     np.testing.assert_allclose(x[..., 10], math.log2(k) * dn.FRAMES_SCALE)
     x1 = dn.network_input(accum, m2, aov, 1)
     assert np.all(x1[..., 9] == 0) and np.all(x1[..., 10] == 0)
+    # ten channels: every channel's hit fraction, then the noise level
+    aov10 = rng.uniform(0, 1, (h, w, 12)).astype(np.float32)
+    x10 = dn.network_input(accum, m2, aov10, k)
+    assert x10.shape == (h, w, dn.input_width(10)) == (h, w, 17)
+    np.testing.assert_array_equal(x10[..., 4:15], aov10[..., 1:])
+    np.testing.assert_allclose(x10[..., 15], se, rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(x10[..., 16], math.log2(k) * dn.FRAMES_SCALE)
 
 
 def test_noise_corrected_mse_is_unbiased(dn):

@@ -17,7 +17,7 @@ from mdwtracker_amd import scenes
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-LAYERS = [(11, 32, True), (32, 32, True), (32, 64, True), (64, 64, True), (64, 96, True),
+LAYERS = [(None, 32, True), (32, 32, True), (32, 64, True), (64, 64, True), (64, 96, True),
           (96, 96, True), (160, 64, True), (64, 64, True), (96, 32, True), (32, 32, True),
           (32, 49, False)]
 
@@ -32,9 +32,13 @@ def _bf(t):
     return t.to(torch.bfloat16).to(torch.float32)
 
 
-def _unpack(p):
+def _layers(channels):
+    return [(7 + channels if cin is None else cin, cout, relu) for cin, cout, relu in LAYERS]
+
+
+def _unpack(p, channels):
     out, o = [], 0
-    for cin, cout, _ in LAYERS:
+    for cin, cout, _ in _layers(channels):
         w = p[o:o + 9 * cout * cin].reshape(9, cout, cin)
         o += 9 * cout * cin
         b = p[o:o + cout]
@@ -44,12 +48,12 @@ def _unpack(p):
     return out
 
 
-def _torch_forward(params, x9):
+def _torch_forward(params, x9, channels=4):
     """float32 restatement of the U-Net and its kernel-predicting output with
     the kernels' bf16 rounding."""
     F = torch.nn.functional
     layers = []
-    for (w, b), (cin, cout, relu) in zip(_unpack(params), LAYERS):
+    for (w, b), (cin, cout, relu) in zip(_unpack(params, channels), _layers(channels)):
         wt = torch.tensor(w).reshape(3, 3, cout, cin).permute(2, 3, 0, 1).contiguous()
         layers.append((_bf(wt).requires_grad_(True), torch.tensor(b).requires_grad_(True), relu))
 
@@ -81,29 +85,30 @@ def _torch_forward(params, x9):
     return pred[0].permute(1, 2, 0), layers
 
 
-def _rand_input(rng, h, w):
-    x = np.zeros((h, w, 11), np.float32)
+def _rand_input(rng, h, w, channels=4):
+    x = np.zeros((h, w, 7 + channels), np.float32)
     x[..., :3] = rng.uniform(0, 1.2, (h, w, 3))
     x[..., 3] = rng.uniform(0, 1, (h, w))
-    frac = rng.dirichlet(np.ones(5), size=(h, w)).astype(np.float32)
-    x[..., 4:9] = frac
-    x[..., 9] = rng.uniform(0, 1, (h, w))
-    x[..., 10] = rng.uniform(0, 1)
+    frac = rng.dirichlet(np.ones(channels + 1), size=(h, w)).astype(np.float32)
+    x[..., 4:5 + channels] = frac
+    x[..., 5 + channels] = rng.uniform(0, 1, (h, w))
+    x[..., 6 + channels] = rng.uniform(0, 1)
     return x
 
 
-def test_forward_matches_torch_reference(dnmod):
+@pytest.mark.parametrize("channels", [4, 10])
+def test_forward_matches_torch_reference(dnmod, channels):
     rng = np.random.default_rng(0)
     h, w = 32, 48
-    dn = dnmod.Denoiser(w, h, seed=3)
+    dn = dnmod.Denoiser(w, h, channels=channels, seed=3)
     p = dn.get_params()
     # a non-trivial output layer (the default starts near zero)
     p = p + rng.normal(0, 0.02, p.size).astype(np.float32)
     dn.set_params(p)
-    x = _rand_input(rng, h, w)
+    x = _rand_input(rng, h, w, channels)
     got, ms = dn.infer(x)
     with torch.no_grad():
-        want, _ = _torch_forward(p, x)
+        want, _ = _torch_forward(p, x, channels)
     want = want.numpy()
     err = np.abs(got - want)
     scale = np.abs(want - x[..., :3]).max()
@@ -130,7 +135,7 @@ def test_gradient_matches_torch_reference(dnmod):
     tl.backward()
     assert abs(loss - tl.item()) < 1e-2 * tl.item()
     o = 0
-    for i, ((cin, cout, _), (wt, bt, _)) in enumerate(zip(LAYERS, layers)):
+    for i, ((cin, cout, _), (wt, bt, _)) in enumerate(zip(_layers(4), layers)):
         gw = g[o:o + 9 * cout * cin].reshape(3, 3, cout, cin).transpose(2, 3, 0, 1)
         o += 9 * cout * cin
         gb = g[o:o + cout]
@@ -183,11 +188,11 @@ def test_aov_buffers():
     b.render(3)
     np.testing.assert_array_equal(a.read_accum(), b.read_accum())
     aov = b.read_aov()
-    s = aov[..., 1] + aov[..., 2:6].sum(-1)
+    assert aov.shape == (48, 64, 4)  # depth, transmittance, two channels
+    s = aov[..., 1] + aov[..., 2:].sum(-1)
     np.testing.assert_allclose(s, 1.0, atol=1e-5)
     hit = 1.0 - aov[..., 1]
     assert hit.max() > 0.3 and np.all(aov[..., 0][hit == 0] == 0)
-    assert np.all(aov[..., 4:6] == 0)  # two channels only
     # a frame limit: the buffers hold the means over the first 2 frames (the
     # same frame ids as a 2-frame run) and the image is unchanged
     c, d = capi.MdwtRenderer(0), capi.MdwtRenderer(0)
@@ -204,6 +209,23 @@ def test_aov_buffers():
         c.set_aov_frames(-1)
     for x in (a, b, c, d):
         x.close()
+
+
+def test_aov_buffers_ten_channels():
+    from mdwtracker_amd import capi
+    sc = scenes.make_scene("large_10ch_f32", seed=6, scale=8)
+    r = capi.MdwtRenderer(0)
+    scenes.load_scene(r, sc, 2)
+    r.set_frame(64, 64, 1, True, (0.0, 0.0, 0.0))
+    r.set_aov(True)
+    r.render(8)
+    aov = r.read_aov()
+    assert aov.shape == (64, 64, 12)
+    np.testing.assert_allclose(aov[..., 1] + aov[..., 2:].sum(-1), 1.0, atol=1e-5)
+    frac = aov[..., 2:].sum((0, 1))
+    print(f"[denoise] ten-channel hit fractions {np.round(frac / frac.sum(), 3)}", flush=True)
+    assert np.count_nonzero(frac[4:]) >= 3  # channels past the fourth are recorded
+    r.close()
 
 
 def test_train_step_weights(dnmod):

@@ -31,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "ms/frame @1spp 1024²; time-to-RMSE 1e-3, denoised vs plain, 1 GPU"
 HBM_PEAK_GBPS = 8000.0
+VOXEL_NAMES = {"uint8": "u8", "uint16": "u16", "float16": "f16", "float32": "f32"}
 
 
 def log(*a):
@@ -238,7 +239,7 @@ def main() -> int:
                 "traffic": traffic, "algorithmic_bytes": round(B),
                 "measured_copy_gbps": round(bw, 1),
                 "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
-                "kernel": "render_kernel<2,u8>" if args.mode == 2 else f"render_kernel<{args.mode},u8>"}
+                "kernel": f"render_kernel<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>"}
     if traffic_src:
         roofline["traffic_source"] = traffic_src
 

@@ -218,7 +218,9 @@ def make_training_set(n_scenes: int = 3, views: int = 2, size: int = 256, levels
         sc = scenes.make_scene(workload, seed=int(rng.integers(1, 2**31)), scale=scale)
         r = MdwtRenderer(0)
         scenes.load_scene(r, sc, 2)
-        for _ in range(views):
+        for vi in range(views):
+            if log:
+                log(f"[denoise] scene {s + 1}/{n_scenes} view {vi + 1}/{views}")
             r.set_camera(*orbit_view(rng))
             s_in, s_t = (int(v) for v in rng.integers(1, 2**31, 2))
             r.set_aov(False)

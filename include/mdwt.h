@@ -202,11 +202,12 @@ int mdwt_set_aov_frames(mdwt_renderer* h, int frames);
 int mdwt_read_aov(mdwt_renderer* h, float* out);
 
 /* ---- batched views (no reference counterpart: the reference renders one
- * camera per launch). Many views of uneven size and spp rendered by one
- * persistent launch whose workgroups pull (view, pixel block) items from a
- * device counter; the launch sequence is captured once in a HIP graph and
- * replayed every frame. A view's image equals a single-view render with the
- * same camera, size, spp and seed (the RNG is keyed by pixel), so views never
+ * camera per launch). Many views of uneven size and spp, each with its own
+ * accumulators, rendered frame by frame with one frame-kernel launch per view
+ * on the handle's stream (a persistent one-launch variant exists in the
+ * library but is switched off until a hang seen on its first GPU run is
+ * explained). A view's image equals a single-view render with the same
+ * camera, size, spp and seed (the RNG is keyed by pixel), so views never
  * depend on the batch they are in. */
 typedef struct mdwt_view {
   float from[3], at[3], up[3]; /* camera.h setOrientation arguments */
@@ -214,13 +215,9 @@ typedef struct mdwt_view {
   int width, height, spp;
   uint32_t seed;
 } mdwt_view;
-/* Replaces the batch (1..4096 views); each view gets its own accumulators.
- * Currently returns MDWT_ERR_STATE: the batched launch is switched off until
- * a hang seen on its first GPU run is explained. */
+/* Replaces the batch (1..4096 views); each view gets its own accumulators. */
 int mdwt_batch_set_views(mdwt_renderer* h, const mdwt_view* views, int n);
-/* Renders n_frames frames of every view (ms = total GPU time). The first
- * frame after mdwt_batch_set_views runs eagerly and times every item; later
- * frames issue items heaviest first. */
+/* Renders n_frames frames of every view (ms = total GPU time). */
 int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms);
 /* Counting pass over the batch: {samples, cell visits, rejections, rays}. */
 int mdwt_batch_count(mdwt_renderer* h, uint64_t out[4]);
@@ -228,8 +225,8 @@ int mdwt_batch_count(mdwt_renderer* h, uint64_t out[4]);
 int mdwt_batch_reset(mdwt_renderer* h);
 /* width * height float4 running means (and second moments) of one view. */
 int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2);
-/* Per view: completion time in microseconds of its last item in the last
- * batched frame, from the first item's start (100 MHz device clock). */
+/* Per view: GPU time in microseconds of its launch in the last batched
+ * frame (HIP events); MDWT_ERR_STATE before the first batched frame. */
 int mdwt_batch_view_times(mdwt_renderer* h, float* us);
 /* Autotuner: times `frames` frames (after one warm-up frame) for every
  * combination of macrocell size s (voxels per macrocell side; grid = dims /

@@ -150,6 +150,12 @@ struct mdwt_renderer {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
   mdwt::RenderParams graph_params;  // what the captured graph launches
+  // per-view launches (the batch path in use): each view's block order and
+  // block count, events around each view's launch, the last frame's times
+  std::vector<uint32_t*> view_order;
+  std::vector<int> view_blocks;
+  std::vector<hipEvent_t> view_ev;  // one per view plus one after the last
+  std::vector<float> view_us;
 };
 
 namespace {
@@ -1310,7 +1316,10 @@ int mdwt_hilbert_d2xy(int bits, uint32_t d, uint32_t* x, uint32_t* y) {
 // ===================================================== batched views ======
 namespace {
 
-constexpr int kBatchEnabled = 0;
+// The persistent one-launch batch (kernels_batch.hip, graph capture and
+// replay) is switched off: its first GPU run hung and the cause is not yet
+// shown. Batches render with one single-view launch per view instead.
+constexpr int kPersistentBatch = 0;
 
 void free_batch(mdwt_renderer* h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
@@ -1331,6 +1340,33 @@ void free_batch(mdwt_renderer* h) {
   h->d_work = nullptr;
   h->d_item_ticks = nullptr;
   h->items.clear();
+  for (uint32_t* q : h->view_order)
+    if (q) (void)hipFree(q);
+  for (hipEvent_t e : h->view_ev)
+    if (e) (void)hipEventDestroy(e);
+  h->view_order.clear();
+  h->view_blocks.clear();
+  h->view_ev.clear();
+  h->view_us.clear();
+}
+
+// Per-view block orders (Hilbert tiles of the current block shape) for the
+// per-view launches.
+int build_view_orders(mdwt_renderer* h) {
+  for (uint32_t* q : h->view_order)
+    if (q) (void)hipFree(q);
+  h->view_order.assign(h->views.size(), nullptr);
+  h->view_blocks.assign(h->views.size(), 0);
+  for (size_t i = 0; i < h->views.size(); ++i) {
+    const auto& vw = h->views[i];
+    const int nbx = (vw.width + h->bw - 1) / h->bw, nby = (vw.height + h->bh - 1) / h->bh;
+    const std::vector<uint32_t> order = block_order(nbx, nby, h->bw, h->bh, h->tile, true);
+    HIPCHK(h, hipMalloc(&h->view_order[i], order.size() * sizeof(uint32_t)));
+    HIPCHK(h, hipMemcpy(h->view_order[i], order.data(), order.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice));
+    h->view_blocks[i] = int(order.size());
+  }
+  return MDWT_OK;
 }
 
 // Work items of all views: views by decreasing work (pixels x spp), each
@@ -1361,6 +1397,64 @@ int build_items(mdwt_renderer* h) {
   HIPCHK(h, hipMemset(h->d_item_ticks, 0, 2 * items.size() * sizeof(unsigned long long)));
   h->items = std::move(items);
   h->batch_sorted = false;
+  return build_view_orders(h);
+}
+
+// The single-view launch of view i at the batch's frame and accumulation ids.
+mdwt::RenderParams view_params(const mdwt_renderer* h, size_t i) {
+  mdwt::RenderParams p = render_params(h);
+  const mdwt::ViewParams& v = h->views[i];
+  p.cam = v.cam;
+  p.fr.width = v.width;
+  p.fr.height = v.height;
+  p.fr.spp = v.spp;
+  p.fr.seed = v.seed;
+  p.fr.frame_id = h->batch_frame;
+  p.fr.accum_id = h->batch_accum;
+  p.fr.accum = v.accum;
+  p.fr.m2 = v.m2;
+  p.fr.block_order = h->view_order[i];
+  p.fr.n_blocks = h->view_blocks[i];
+  p.fr.schedule = 0;
+  p.fr.wave_cost = nullptr;
+  p.fr.aov = nullptr;  // batched views have no auxiliary buffers
+  return p;
+}
+
+// n_frames frames of every view, one launch per view per frame; the events
+// around the last frame's launches give the per-view times.
+int batch_render_views(mdwt_renderer* h, int n_frames, float* ms_out) {
+  const size_t n = h->views.size();
+  if (h->view_ev.size() != n + 1) {
+    for (hipEvent_t e : h->view_ev)
+      if (e) (void)hipEventDestroy(e);
+    h->view_ev.assign(n + 1, nullptr);
+    for (auto& e : h->view_ev) HIPCHK(h, hipEventCreate(&e));
+  }
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int f = 0; f < n_frames; ++f) {
+    for (size_t i = 0; i < n; ++i) {
+      HIPCHK(h, hipEventRecord(h->view_ev[i], h->stream));
+      HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, false, h->stream));
+    }
+    HIPCHK(h, hipEventRecord(h->view_ev[n], h->stream));
+    h->batch_frame++;
+    h->batch_accum = h->accumulate ? h->batch_accum + 1u : 0u;
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (n_frames > 0) {
+    h->view_us.assign(n, 0.f);
+    for (size_t i = 0; i < n; ++i) {
+      float t = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&t, h->view_ev[i], h->view_ev[i + 1]));
+      h->view_us[i] = t * 1000.f;
+    }
+  }
+  h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
+  if (ms_out) *ms_out = ms;
   return MDWT_OK;
 }
 
@@ -1497,10 +1591,6 @@ int mdwt_set_macrocells(mdwt_renderer* h, const int mc[3]) {
 
 int mdwt_batch_set_views(mdwt_renderer* h, const mdwt_view* views, int n) {
   if (!h || !views) return MDWT_ERR_ARG;
-  // Switched off: the first GPU run of a batched launch hung and the cause is
-  // not yet shown; the path stays unreachable until it is.
-  if (kBatchEnabled == 0)
-    return fail(h, MDWT_ERR_STATE, "batched views are disabled in this build (unresolved GPU hang)");
   if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
   if (n < 1 || n > 4096) return fail(h, MDWT_ERR_ARG, "1 to 4096 views");
   for (int i = 0; i < n; ++i) {
@@ -1547,6 +1637,7 @@ int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms_out) {
   int rc = batch_ready(h);
   if (rc != MDWT_OK) return rc;
   if (n_frames < 0) return fail(h, MDWT_ERR_ARG, "negative frame count");
+  if (!kPersistentBatch) return batch_render_views(h, n_frames, ms_out);
   const mdwt::RenderParams p = batch_params(h);
   int grid = 0;
   if ((rc = batch_grid(h, p, false, &grid)) != MDWT_OK) return rc;
@@ -1598,6 +1689,15 @@ int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
   if (!h || !counts) return MDWT_ERR_ARG;
   int rc = batch_ready(h);
   if (rc != MDWT_OK) return rc;
+  if (!kPersistentBatch) {
+    HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
+    for (size_t i = 0; i < h->views.size(); ++i)
+      HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, true, h->stream));
+    HIPCHK(h, hipMemcpyAsync(counts, h->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MDWT_OK;
+  }
   mdwt::RenderParams p = batch_params(h);
   p.fr.item_ticks = nullptr;
   int grid = 0;
@@ -1613,6 +1713,10 @@ int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
 int mdwt_batch_reset(mdwt_renderer* h) {
   if (!h) return MDWT_ERR_ARG;
   if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views");
+  if (!kPersistentBatch) {
+    h->batch_accum = 0;
+    return MDWT_OK;
+  }
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const unsigned w[3] = {0u, h->batch_frame, 0u};
@@ -1636,6 +1740,12 @@ int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2) {
 int mdwt_batch_view_times(mdwt_renderer* h, float* us) {
   if (!h || !us) return MDWT_ERR_ARG;
   if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views");
+  if (!kPersistentBatch) {
+    if (h->view_us.size() != h->views.size())
+      return fail(h, MDWT_ERR_STATE, "no batched frame rendered yet");
+    for (size_t k = 0; k < h->view_us.size(); ++k) us[k] = h->view_us[k];
+    return MDWT_OK;
+  }
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const size_t n = h->items.size();

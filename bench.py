@@ -183,6 +183,61 @@ def cpu_baseline(scene, mode: int, frames: int) -> dict:
                       f"(of {scene.frames}), OpenMP over pixel rows, {dt:.1f} s"}
 
 
+def batch_views(n: int, seed: int) -> list:
+    """n views on a seeded sphere (radius 2.2 around the volume centre):
+    width and height uniform in 256..1024, spp uniform in 1..16."""
+    import numpy as np
+    from mdwtracker_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    cams = scenes.orbit_cameras(n, rng, radius=2.2)
+    views = []
+    for i in range(n):
+        views.append({"from": tuple(float(c) for c in cams[i]), "at": (0.5, 0.5, 0.5),
+                      "up": (0.0, 1.0, 0.0) if abs(float(cams[i][1]) - 0.5) < 0.95 * 2.2 else (1.0, 0.0, 0.0),
+                      "fovy": 45.0, "width": int(rng.integers(256, 1025)),
+                      "height": int(rng.integers(256, 1025)), "spp": int(rng.integers(1, 17)),
+                      "seed": int(rng.integers(0, 2**31))})
+    return views
+
+
+def batch_bench(args, scene, r) -> dict:
+    """Batched views: the autotuner's sweep over macrocell size, workgroup
+    and Hilbert tile on the batch, then `--steps` batched frames at the best
+    setting: ms per batched frame, aggregate samples/s, per-view latency."""
+    import numpy as np
+
+    n_views = int(scene.extra.get("views", 64))
+    views = batch_views(n_views, scene.seed)
+    r.batch_set_views(views)
+    px = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    log(f"[bench] batch: {n_views} views, {px / 1e6:.1f} M pixel samples per batched frame")
+    t = time.perf_counter()
+    table, best = r.autotune((4, 8, 16, 32), (64, 128, 256), (8, 16, 32), frames=2)
+    log(f"[bench] autotune ({time.perf_counter() - t:.1f} s): best s={best[0]} threads={best[1]} "
+        f"tile={best[2]}")
+    counts = r.batch_count()
+    r.batch_render(args.warmup)
+    r.batch_reset()
+    t0 = time.perf_counter()
+    gpu_ms = r.batch_render(args.steps)
+    wall_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    us = r.batch_view_times()
+    kernel_ms = gpu_ms / args.steps
+    rows = [[round(float(v), 4) for v in row] for row in table]
+    return {"value": round(wall_ms, 3), "kernel_ms": round(kernel_ms, 3),
+            "samples_per_s": round(counts["samples"] / (kernel_ms * 1e-3)),
+            "pixel_samples_per_s": round(px / (kernel_ms * 1e-3)),
+            "view_latency_us": {"p50": round(float(np.percentile(us, 50)), 1),
+                                "p99": round(float(np.percentile(us, 99)), 1),
+                                "max": round(float(us.max()), 1)},
+            "work_per_frame": counts, "views": n_views,
+            "autotune": {"best": {"macrocell": best[0], "threads": best[1], "tile": best[2]},
+                         "columns": ["s", "threads", "tile", "ms/frame", "samples/s", "visits/s"],
+                         "table": rows},
+            "launch": "one frame-kernel launch per view per frame"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +274,24 @@ def main() -> int:
     scenes.load_scene(r, scene, args.mode)
     bw = r.stream_copy(2 << 30, 5)
     log(f"[bench] measured copy bandwidth {bw:.0f} GB/s (2 GiB read + 2 GiB write)")
+    if "views" in scene.extra:
+        b = batch_bench(args, scene, r)
+        line = {"metric": "ms per batched frame (views of 256..1024 px, spp 1..16); "
+                          "samples/s; per-view latency, 1 GPU",
+                "value": b.pop("value"), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": None, "higher_is_better": False,
+                "scaling": "weak", "vs_baseline": None,
+                "dtype": f"f32 ({VOXEL_NAMES[str(scene.volumes[0].dtype)]} voxels)",
+                "data": "synthetic (seeded fBm + Gaussian blobs, seeded views)",
+                "config": {"workload": scene.name, "channels": scene.channels,
+                           "volume": list(scene.volumes[0].shape),
+                           "voxel": str(scene.volumes[0].dtype), "mode": args.mode,
+                           "parallelism": "single GPU"}}
+        line["ms_per_step"] = line["value"]
+        line.update(b)
+        r.close()
+        print(json.dumps(line), flush=True)
+        return 0
 
     r.render(args.warmup)
     counts = frame_counts(r, 4)

@@ -167,7 +167,20 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     return out
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, mode: int, frames: int) -> dict:
+    """The oracle on the host's cores (OpenMP over pixel rows, up to 16
+    threads) for `frames` full frames of the workload, and on one thread for
+    one full frame."""
     from oracle import pyoracle
     from mdwtracker_amd import scenes
 
@@ -177,10 +190,44 @@ def cpu_baseline(scene, mode: int, frames: int) -> dict:
     t0 = time.perf_counter()
     o.render(frames, threads=threads)
     dt = time.perf_counter() - t0
+    o1 = pyoracle.OracleRenderer(threads=1)
+    scenes.load_scene(o1, scene, mode)
+    t1 = time.perf_counter()
+    o1.render(1, threads=1)
+    dt1 = time.perf_counter() - t1
     return {"value": round(dt * 1000.0 / frames, 3), "unit": "ms/frame", "cores": threads,
             "kind": "port",
             "sample": f"{scene.name} -m {mode}, {scene.width}x{scene.height}, {frames} full frames "
-                      f"(of {scene.frames}), OpenMP over pixel rows, {dt:.1f} s"}
+                      f"(of {scene.frames}), OpenMP over pixel rows, {dt:.1f} s",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "single_thread_ms_per_frame": round(dt1 * 1000.0, 1),
+            "samples_per_s": None}
+
+
+def small_case(capi, scenes) -> dict:
+    """The 2-channel 64^3 u8 CPU case (256^2, -m 0, 64 frames): the oracle on
+    all host cores and on one thread, and the GPU, in ms/frame."""
+    from oracle import pyoracle
+
+    sc = scenes.make_scene("ndda_cpu_2ch64")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = {"workload": sc.name, "mode": 0, "image": [sc.width, sc.height], "frames": sc.frames}
+    for n in (threads, 1):
+        o = pyoracle.OracleRenderer(threads=n)
+        scenes.load_scene(o, sc, 0)
+        t = time.perf_counter()
+        o.render(sc.frames, threads=n)
+        out[f"cpu_{n}_threads_ms_per_frame"] = round((time.perf_counter() - t) * 1000.0 / sc.frames, 3)
+    g = capi.MdwtRenderer(0)
+    scenes.load_scene(g, sc, 0)
+    g.render(4)
+    g.reset_accumulation()
+    t = time.perf_counter()
+    g.render(sc.frames)
+    out["gpu_ms_per_frame"] = round((time.perf_counter() - t) * 1000.0 / sc.frames, 4)
+    out["gpu_kernel_ms_per_frame"] = round(g.last_kernel_ms(), 4)
+    g.close()
+    return out
 
 
 def batch_views(n: int, seed: int) -> list:
@@ -366,7 +413,11 @@ def main() -> int:
     line.update(extra_modes)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, args.mode, args.cpu_frames)
-        log(f"[bench] cpu baseline {line['cpu_baseline']}")
+        cb = line["cpu_baseline"]
+        cb["samples_per_s"] = round(counts["samples"] / (cb["value"] * 1e-3))
+        log(f"[bench] cpu baseline {cb}")
+        line["small_case"] = small_case(capi, scenes)
+        log(f"[bench] small case {line['small_case']}")
     r.close()
     print(json.dumps(line), flush=True)
     return 0

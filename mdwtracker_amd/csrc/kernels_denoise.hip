@@ -40,46 +40,87 @@ __device__ __forceinline__ s4 load4(const uint16_t* p, bool ok) {
 // w [9][cout_pad][cin] (tap-major, K contiguous), output channel n of pixel p
 // to y16[p * ys + yo + n] (bf16) and/or y32[p * ys + yo + n] (fp32, added to
 // the existing value when accum). cin is a multiple of 8, cout_pad of 32;
-// channels n >= cout are not written. Grid: (ceil(H*W / 32), cout_pad / 32),
-// one wave per block.
-__global__ __launch_bounds__(64) void conv3x3_kernel(const uint16_t* __restrict__ x, int H, int W,
-                                                     int xs, int xo, int cin,
-                                                     const uint16_t* __restrict__ w,
-                                                     const float* __restrict__ bias, int cout,
-                                                     int cout_pad, uint16_t* y16, float* y32,
-                                                     int ys, int yo, int relu, int accum) {
-  const int l = threadIdx.x;
-  const int npix = H * W;
-  const int p0 = blockIdx.x * 32;
-  const int n0 = blockIdx.y * 32;
-  const int pa = p0 + (l & 31);  // pixel of this lane's A row
-  const int pay = pa / W, pax = pa - pay * W;
-  const int kq = (l >> 5) * 4;  // this lane's 4 K values inside a chunk of 8
-  f16v acc = {};
-  for (int tap = 0; tap < 9; ++tap) {
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int sy = pay + dy, sx = pax + dx;
-    const bool in = pa < npix && sy >= 0 && sy < H && sx >= 0 && sx < W;
-    const uint16_t* xa = x + (size_t(in ? sy * W + sx : 0) * xs + xo + kq);
-    const uint16_t* wb = w + (size_t(tap) * cout_pad + n0 + (l & 31)) * cin + kq;
-    for (int kc = 0; kc < cin; kc += 8) {
-      const s4 a = load4(xa + kc, in);
-      const s4 b = load4(wb + kc, true);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+// channels n >= cout are not written.
+// A workgroup (4 waves) computes R image rows x TP consecutive pixels x TN
+// output channels, TP x TN = 64 x 64 (NWN = 2 waves along the channels) or
+// 128 x 32 (NWN = 1). Per chunk of 32 input channels it stages the input's
+// R + 2 halo rows (TP + 2 pixels) and the 9 taps' weights of its TN channels
+// in LDS once; each wave then runs, per tap and K step, one weight fragment
+// against its R rows (R accumulators of 32 x 32). LDS rows are padded to 36
+// bf16 (72 bytes): 8-byte reads stay aligned and neighbouring lanes fall in
+// different banks.
+constexpr int kChunk = 32;        // input channels per staged chunk
+constexpr int kLdsRow = kChunk + 4;
+constexpr int kConvRows = 2;      // image rows per workgroup
+
+template <int NWN, int R>
+__global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t* __restrict__ x, int H, int W,
+                                                      int xs, int xo, int cin,
+                                                      const uint16_t* __restrict__ w,
+                                                      const float* __restrict__ bias, int cout,
+                                                      int cout_pad, uint16_t* y16, float* y32,
+                                                      int ys, int yo, int relu, int accum) {
+  constexpr int TN = 32 * NWN, TP = 32 * (4 / NWN), XP = TP + 2;
+  __shared__ __align__(16) uint16_t sA[(R + 2) * XP * kLdsRow];
+  __shared__ __align__(16) uint16_t sB[9 * TN * kLdsRow];
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int tiles_x = (W + TP - 1) / TP;
+  const int y0 = (blockIdx.x / tiles_x) * R, x0 = (blockIdx.x % tiles_x) * TP;
+  const int n0 = blockIdx.y * TN;
+  const int wp = (wv % (4 / NWN)) * 32, wn = (wv / (4 / NWN)) * 32;  // wave's pixel / channel offset
+  const int r32 = l & 31, kq = (l >> 5) * 4;
+  f16v acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = f16v{};
+  for (int kc = 0; kc < cin; kc += kChunk) {
+    const int ch = min(kChunk, cin - kc);  // multiple of 8
+    const int ng = ch / 4;                 // 8-byte groups per pixel / weight row
+    for (int i = t; i < (R + 2) * XP * ng; i += 256) {  // input halo rows y0-1 .. y0+R
+      const int g = i % ng, q = i / ng, row = q / XP, px = q - row * XP;
+      const int sy = y0 + row - 1, sx = x0 + px - 1;
+      const bool in = sy >= 0 && sy < H && sx >= 0 && sx < W;
+      const s4 v = load4(x + (size_t(in ? sy * W + sx : 0) * xs + xo + kc + 4 * g), in);
+      *reinterpret_cast<s4*>(&sA[(row * XP + px) * kLdsRow + 4 * g]) = v;
     }
+    for (int i = t; i < 9 * TN * ng; i += 256) {  // weights: 9 taps x TN channels
+      const int g = i % ng, q = i / ng, tap = q / TN, n = q - tap * TN;
+      const s4 v = load4(w + (size_t(tap) * cout_pad + n0 + n) * cin + kc + 4 * g, true);
+      *reinterpret_cast<s4*>(&sB[(tap * TN + n) * kLdsRow + 4 * g]) = v;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dyi = tap / 3, dx = tap % 3;
+      const uint16_t* ar = &sA[(dyi * XP + wp + r32 + dx) * kLdsRow + kq];
+      const uint16_t* br = &sB[(tap * TN + wn + r32) * kLdsRow + kq];
+      for (int k = 0; k < ch; k += 8) {
+        const s4 bv = *reinterpret_cast<const s4*>(br + k);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const s4 av = *reinterpret_cast<const s4*>(ar + r * XP * kLdsRow + k);
+          acc[r] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc[r], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
   }
-  const int n = n0 + (l & 31);
+  const int n = n0 + wn + r32;
   if (n >= cout) return;
   const float bn = bias ? bias[n] : 0.f;
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int p = p0 + (v >> 2) * 8 + (l >> 5) * 4 + (v & 3);
-    if (p >= npix) continue;
-    float r = acc[v] + bn;
-    if (relu) r = fmaxf(r, 0.f);
-    const size_t o = size_t(p) * ys + yo + n;
-    if (y16) y16[o] = f2bf(r);
-    if (y32) y32[o] = accum ? y32[o] + r : r;
+  for (int r = 0; r < R; ++r) {
+    const int y = y0 + r;
+    if (y >= H) break;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int xx = x0 + wp + (v >> 2) * 8 + kq + (v & 3);
+      if (xx >= W) continue;
+      float o32 = acc[r][v] + bn;
+      if (relu) o32 = fmaxf(o32, 0.f);
+      const size_t o = (size_t(y) * W + xx) * ys + yo + n;
+      if (y16) y16[o] = f2bf(o32);
+      if (y32) y32[o] = accum ? y32[o] + o32 : o32;
+    }
   }
 }
 
@@ -382,9 +423,19 @@ inline unsigned nb(size_t n, unsigned t = 256) { return unsigned((n + t - 1) / t
 hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, const uint16_t* w,
                    const float* bias, int cout, int cout_pad, uint16_t* y16, float* y32, int ys,
                    int yo, bool relu, bool accum, hipStream_t s) {
-  const dim3 grid(nb(size_t(H) * W, 32), unsigned(cout_pad / 32));
-  hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(64), 0, s, x, H, W, xs, xo, cin, w, bias, cout,
-                     cout_pad, y16, y32, ys, yo, relu ? 1 : 0, accum ? 1 : 0);
+  // 8-byte input and weight groups
+  if (cin % 8 || cout_pad % 32 || xs % 4 || xo % 4 || xo + cin > xs) return hipErrorInvalidValue;
+  constexpr int R = kConvRows;
+  const unsigned rows = unsigned((H + R - 1) / R);
+  if (cout_pad % 64 == 0) {
+    const dim3 grid(unsigned((W + 63) / 64) * rows, unsigned(cout_pad / 64));
+    hipLaunchKernelGGL((conv3x3_kernel<2, R>), grid, dim3(256), 0, s, x, H, W, xs, xo, cin, w,
+                       bias, cout, cout_pad, y16, y32, ys, yo, relu ? 1 : 0, accum ? 1 : 0);
+  } else {
+    const dim3 grid(unsigned((W + 127) / 128) * rows, unsigned(cout_pad / 32));
+    hipLaunchKernelGGL((conv3x3_kernel<1, R>), grid, dim3(256), 0, s, x, H, W, xs, xo, cin, w,
+                       bias, cout, cout_pad, y16, y32, ys, yo, relu ? 1 : 0, accum ? 1 : 0);
+  }
   return hipGetLastError();
 }
 

@@ -205,7 +205,9 @@ int bwd(mdwt_denoiser* d, int i, const float* gy, int gs, const uint16_t* y, int
         const uint16_t* x, int xs, int xo, int H, int W, float* gx, int gxs) {
   const Conv& L = d->c[i];
   hipStream_t s = d->stream;
-  DNCHK(d, mdwt::dn::relu_grad(gy, gs, y, ys, 0, H * W, L.cout, L.cout_pad, L.relu, d->gm, L.gb, s));
+  // the bias partial sums use the weight-gradient scratch, reduced before wgrad runs
+  DNCHK(d, mdwt::dn::relu_grad(gy, gs, y, ys, 0, H * W, L.cout, L.cout_pad, L.relu, d->gm, L.gb,
+                               d->wpart, kWgradPart, s));
   DNCHK(d, mdwt::dn::wgrad(d->gm, L.cout_pad, x, xs, xo, H, W, L.cin, L.cout, L.gw, d->wpart,
                            kWgradPart, s));
   if (gx)

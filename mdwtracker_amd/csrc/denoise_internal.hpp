@@ -34,9 +34,12 @@ hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, con
 // multiples of 4 and the rows hold cin (cout) rounded up to 4 channels.
 hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
                  int cin, int cout, float* dw, float* part, size_t part_cap, hipStream_t s);
-// gm = bf16(gy * relu'(y)) [npix][cout_pad]; db += per-channel sums
+// gm = bf16(gy * relu'(y)) [npix][cout_pad]; db += per-channel sums (partial
+// sums of 4096-pixel chunks in part, at least ceil(npix / 4096) * cout of
+// part_cap floats, added in chunk order)
 hipError_t relu_grad(const float* gy, int gs, const uint16_t* y, int ys, int yo, int npix, int cout,
-                     int cout_pad, bool relu, uint16_t* gm, float* db, hipStream_t s);
+                     int cout_pad, bool relu, uint16_t* gm, float* db, float* part,
+                     size_t part_cap, hipStream_t s);
 hipError_t pool(const uint16_t* x, int H, int W, int C, uint16_t* y, hipStream_t s);
 hipError_t pool_grad(const float* gy, int H, int W, int C, float* gx, hipStream_t s);
 hipError_t up(const uint16_t* x, int H, int W, int C, uint16_t* y, int ys, int yo, hipStream_t s);

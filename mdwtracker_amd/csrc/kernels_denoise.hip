@@ -235,20 +235,29 @@ __global__ void relu_grad_kernel(const float* __restrict__ gy, int gs,
   gm[size_t(p) * cout_pad + c] = f2bf(v);
 }
 
-// db[c] += sum over a chunk of pixels of gm[p][c]: block (c, chunk), 256
-// threads, one atomic per block
+// part[chunk][c] = sum over a chunk of pixels of gm[p][c]: block (c, chunk),
+// 256 threads, a fixed-order reduction inside the block
 __global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ gm,
                                                         int cout_pad, int npix, int chunk,
-                                                        float* db) {
-  __shared__ float part[4];
+                                                        float* __restrict__ part) {
+  __shared__ float sums[4];
   const int c = blockIdx.x;
   const int p0 = blockIdx.y * chunk, p1 = min(npix, p0 + chunk);
   float sum = 0.f;
   for (int p = p0 + int(threadIdx.x); p < p1; p += blockDim.x) sum += bf2f(gm[size_t(p) * cout_pad + c]);
   for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+  if ((threadIdx.x & 63) == 0) sums[threadIdx.x >> 6] = sum;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(&db[c], part[0] + part[1] + part[2] + part[3]);
+  if (threadIdx.x == 0) part[size_t(blockIdx.y) * gridDim.x + c] = sums[0] + sums[1] + sums[2] + sums[3];
+}
+// db[c] += sum over chunks of part[chunk][c], chunks in order (no atomics)
+__global__ void bias_reduce_kernel(const float* __restrict__ part, int chunks, int cout,
+                                   float* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cout) return;
+  float sum = 0.f;
+  for (int k = 0; k < chunks; ++k) sum += part[size_t(k) * cout + c];
+  db[c] += sum;
 }
 
 // 2x2 average pool: y[(py, px)] = mean of x over the 2x2 block (bf16, C channels)
@@ -461,14 +470,20 @@ hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, i
 }
 
 hipError_t relu_grad(const float* gy, int gs, const uint16_t* y, int ys, int yo, int npix, int cout,
-                     int cout_pad, bool relu, uint16_t* gm, float* db, hipStream_t s) {
+                     int cout_pad, bool relu, uint16_t* gm, float* db, float* part,
+                     size_t part_cap, hipStream_t s) {
+  const int chunk = 4096;
+  const int chunks = (npix + chunk - 1) / chunk;
+  if (size_t(chunks) * cout > part_cap) return hipErrorInvalidValue;
   hipLaunchKernelGGL(relu_grad_kernel, dim3(nb(size_t(npix) * cout_pad)), dim3(256), 0, s, gy, gs,
                      y, ys, yo, npix, cout, cout_pad, relu ? 1 : 0, gm);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int chunk = 4096;
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(unsigned(cout), unsigned((npix + chunk - 1) / chunk)),
-                     dim3(256), 0, s, gm, cout_pad, npix, chunk, db);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(unsigned(cout), unsigned(chunks)), dim3(256), 0, s, gm,
+                     cout_pad, npix, chunk, part);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(bias_reduce_kernel, dim3(nb(size_t(cout))), dim3(256), 0, s, part, chunks,
+                     cout, db);
   return hipGetLastError();
 }
 

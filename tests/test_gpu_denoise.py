@@ -242,14 +242,11 @@ def test_train_step_weights(dnmod):
     # pairs give exactly the gradient of the first pair alone (powers of two)
     la = a.train_step(xs, ts, 1e-3, np.array([2.0, 0.0], np.float32))
     lb = b.train_step(xs[:1], ts[:1], 1e-3)
-    da, db = a.get_params() - p0, b.get_params() - p0
-    assert np.abs(da).max() > 1e-4
-    # fp32 atomic sums may round differently; a gradient that is nearly zero
-    # can then take an Adam step of another size or sign
-    cos = float(np.dot(da, db) / (np.linalg.norm(da) * np.linalg.norm(db)))
-    close = float(np.mean(np.abs(da - db) <= 1e-6 + 1e-3 * np.abs(db)))
-    print(f"[denoise] weighted step: cos {cos:.6f}, matching {close:.4f}", flush=True)
-    assert cos > 0.999 and close > 0.98
+    pa, pb = a.get_params(), b.get_params()
+    assert np.abs(pa - p0).max() > 1e-4
+    # gradients are reduced in a fixed order (no atomics), the zero-weight
+    # pair adds exact zeros: the steps are identical bit for bit
+    np.testing.assert_array_equal(pa, pb)
     assert lb > 0 and la > 0  # the reported loss is the unweighted mean
     # all weights zero: no step at all
     c.train_step(xs, ts, 1e-3, np.zeros(2, np.float32))
@@ -258,6 +255,21 @@ def test_train_step_weights(dnmod):
         c.train_step(xs, ts, 1e-3, np.array([1.0, -1.0], np.float32))
     for d in (a, b, c):
         d.close()
+
+
+def test_training_is_deterministic(dnmod):
+    rng = np.random.default_rng(7)
+    h = w = 32
+    xs = np.stack([_rand_input(rng, h, w) for _ in range(4)])
+    ts = np.clip(xs[..., :3] + rng.normal(0, 0.1, (4, h, w, 3)), 0, None).astype(np.float32)
+    out = []
+    for _ in range(2):
+        d = dnmod.Denoiser(w, h, seed=13)
+        for i in range(3):
+            d.train_step(xs[i:i + 2], ts[i:i + 2], 1e-3, np.array([1.0, 0.5], np.float32))
+        out.append(d.get_params())
+        d.close()
+    np.testing.assert_array_equal(out[0], out[1])
 
 
 def test_accumulated_inputs_and_error_estimate(dnmod):

@@ -298,10 +298,10 @@ def main() -> int:
     ap.add_argument("--rmse-frames", type=int, default=4096,
                     help="frames of the reference render for time-to-RMSE (0: skip)")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoised time-to-RMSE")
-    ap.add_argument("--dn-scenes", type=int, default=3)
-    ap.add_argument("--dn-views", type=int, default=2)
+    ap.add_argument("--dn-scenes", type=int, default=2)
+    ap.add_argument("--dn-views", type=int, default=3)
     ap.add_argument("--dn-size", type=int, default=256)
-    ap.add_argument("--dn-steps", type=int, default=3000)
+    ap.add_argument("--dn-steps", type=int, default=6000)
     ap.add_argument("--dn-lr", type=float, default=1e-3)
     ap.add_argument("--dn-seed", type=int, default=77)
     args = ap.parse_args()

@@ -203,10 +203,12 @@ int mdwt_read_aov(mdwt_renderer* h, float* out);
 
 /* ---- batched views (no reference counterpart: the reference renders one
  * camera per launch). Many views of uneven size and spp, each with its own
- * accumulators, rendered frame by frame with one frame-kernel launch per view
- * on the handle's stream (a persistent one-launch variant exists in the
- * library but is switched off until a hang seen on its first GPU run is
- * explained). A view's image equals a single-view render with the same
+ * accumulators, rendered frame by frame with one frame-kernel launch per view,
+ * the views spread round-robin over four HIP streams so that small views run
+ * side by side (a persistent one-launch variant exists in the library but is
+ * switched off until a hang seen on its first GPU run is explained). Calls
+ * are ordered with the handle's other work and return when the frames are
+ * done. A view's image equals a single-view render with the same
  * camera, size, spp and seed (the RNG is keyed by pixel), so views never
  * depend on the batch they are in. */
 typedef struct mdwt_view {
@@ -226,7 +228,8 @@ int mdwt_batch_reset(mdwt_renderer* h);
 /* width * height float4 running means (and second moments) of one view. */
 int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2);
 /* Per view: GPU time in microseconds of its launch in the last batched
- * frame (HIP events); MDWT_ERR_STATE before the first batched frame. */
+ * frame (HIP events on its stream, so it includes time shared with views on
+ * the other streams); MDWT_ERR_STATE before the first batched frame. */
 int mdwt_batch_view_times(mdwt_renderer* h, float* us);
 /* Autotuner: times `frames` frames (after one warm-up frame) for every
  * combination of macrocell size s (voxels per macrocell side; grid = dims /

@@ -154,7 +154,9 @@ struct mdwt_renderer {
   // block count, events around each view's launch, the last frame's times
   std::vector<uint32_t*> view_order;
   std::vector<int> view_blocks;
-  std::vector<hipEvent_t> view_ev;  // one per view plus one after the last
+  std::vector<hipEvent_t> view_ev, view_ev_end;  // around each view's launch
+  std::vector<hipEvent_t> join_ev;               // end of each batch stream's work
+  std::vector<hipStream_t> batch_streams;        // views run round-robin on these
   std::vector<float> view_us;
 };
 
@@ -1340,13 +1342,20 @@ void free_batch(mdwt_renderer* h) {
   h->d_work = nullptr;
   h->d_item_ticks = nullptr;
   h->items.clear();
+  for (hipStream_t q : h->batch_streams) {
+    (void)hipStreamSynchronize(q);
+    (void)hipStreamDestroy(q);
+  }
+  h->batch_streams.clear();
   for (uint32_t* q : h->view_order)
     if (q) (void)hipFree(q);
-  for (hipEvent_t e : h->view_ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto* v : {&h->view_ev, &h->view_ev_end, &h->join_ev}) {
+    for (hipEvent_t e : *v)
+      if (e) (void)hipEventDestroy(e);
+    v->clear();
+  }
   h->view_order.clear();
   h->view_blocks.clear();
-  h->view_ev.clear();
   h->view_us.clear();
 }
 
@@ -1421,25 +1430,52 @@ mdwt::RenderParams view_params(const mdwt_renderer* h, size_t i) {
   return p;
 }
 
-// n_frames frames of every view, one launch per view per frame; the events
-// around the last frame's launches give the per-view times.
+// Streams the views of a batch are spread over (round-robin, default
+// priority): small views do not fill the GPU one launch at a time.
+constexpr int kBatchStreams = 4;
+
+int make_events(mdwt_renderer* h, std::vector<hipEvent_t>& v, size_t n) {
+  if (v.size() == n) return MDWT_OK;
+  for (hipEvent_t e : v)
+    if (e) (void)hipEventDestroy(e);
+  v.assign(n, nullptr);
+  for (auto& e : v) HIPCHK(h, hipEventCreate(&e));
+  return MDWT_OK;
+}
+
+// n_frames frames of every view, one launch per view per frame, view i on
+// batch stream i % kBatchStreams (a view's frames stay in order on its
+// stream). The batch streams start after the handle's stream has reached
+// this call and the handle's stream waits for all of them at the end, so
+// the call is ordered like any other on the handle. Events around the last
+// frame's launches give the per-view times.
 int batch_render_views(mdwt_renderer* h, int n_frames, float* ms_out) {
   const size_t n = h->views.size();
-  if (h->view_ev.size() != n + 1) {
-    for (hipEvent_t e : h->view_ev)
-      if (e) (void)hipEventDestroy(e);
-    h->view_ev.assign(n + 1, nullptr);
-    for (auto& e : h->view_ev) HIPCHK(h, hipEventCreate(&e));
+  int rc;
+  if ((rc = make_events(h, h->view_ev, n)) != MDWT_OK ||
+      (rc = make_events(h, h->view_ev_end, n)) != MDWT_OK ||
+      (rc = make_events(h, h->join_ev, kBatchStreams)) != MDWT_OK)
+    return rc;
+  while (h->batch_streams.size() < size_t(kBatchStreams)) {
+    hipStream_t q = nullptr;
+    HIPCHK(h, hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    h->batch_streams.push_back(q);
   }
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (hipStream_t q : h->batch_streams) HIPCHK(h, hipStreamWaitEvent(q, h->ev0, 0));
   for (int f = 0; f < n_frames; ++f) {
     for (size_t i = 0; i < n; ++i) {
-      HIPCHK(h, hipEventRecord(h->view_ev[i], h->stream));
-      HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, false, h->stream));
+      hipStream_t q = h->batch_streams[i % kBatchStreams];
+      HIPCHK(h, hipEventRecord(h->view_ev[i], q));
+      HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, false, q));
+      HIPCHK(h, hipEventRecord(h->view_ev_end[i], q));
     }
-    HIPCHK(h, hipEventRecord(h->view_ev[n], h->stream));
     h->batch_frame++;
     h->batch_accum = h->accumulate ? h->batch_accum + 1u : 0u;
+  }
+  for (int k = 0; k < kBatchStreams; ++k) {
+    HIPCHK(h, hipEventRecord(h->join_ev[size_t(k)], h->batch_streams[size_t(k)]));
+    HIPCHK(h, hipStreamWaitEvent(h->stream, h->join_ev[size_t(k)], 0));
   }
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipEventSynchronize(h->ev1));
@@ -1449,7 +1485,7 @@ int batch_render_views(mdwt_renderer* h, int n_frames, float* ms_out) {
     h->view_us.assign(n, 0.f);
     for (size_t i = 0; i < n; ++i) {
       float t = 0.f;
-      HIPCHK(h, hipEventElapsedTime(&t, h->view_ev[i], h->view_ev[i + 1]));
+      HIPCHK(h, hipEventElapsedTime(&t, h->view_ev[i], h->view_ev_end[i]));
       h->view_us[i] = t * 1000.f;
     }
   }

@@ -54,8 +54,9 @@ def test_host_validation_without_gpu(capi):
         r2.push_channel(np.zeros((4, 4, 4), np.float32))  # dtype differs
     with pytest.raises(capi.MdwtError):
         r2.set_bounds([1.0, float("nan"), 1.0])
-    with pytest.raises(capi.MdwtError):
-        r2.init(1)  # -m 1 is not part of this library
+    with pytest.raises(capi.MdwtError) as e:
+        r2.init(10)  # modes are 0-9
+    assert e.value.code == 1
     with pytest.raises(capi.MdwtError):
         r2.init(2, (0, -1, 0))
     with pytest.raises(capi.MdwtError) as e:
@@ -75,6 +76,38 @@ def test_host_validation_without_gpu(capi):
     with pytest.raises(capi.MdwtError) as e:
         capi.MdwtRenderer(-1)
     assert e.value.code == 1
+
+
+def test_denoiser_and_aov_entry_points_validate_without_gpu(capi):
+    import ctypes as C
+    lib = capi.lib
+    h = C.c_void_p()
+    # sizes must be multiples of 4 up to 8192, channels 1..12
+    for w, hh, ch in ((30, 32, 4), (32, 30, 4), (0, 32, 4), (8196, 32, 4), (32, 32, 0),
+                      (32, 32, 13)):
+        assert lib.mdwt_dn_create(C.byref(h), 0, w, hh, ch, 1) == 1, (w, hh, ch)
+        assert not h.value
+    assert lib.mdwt_dn_create(None, 0, 32, 32, 4, 1) == 1
+    n = C.c_int()
+    buf = np.zeros(16, np.float32)
+    fp = buf.ctypes.data_as(C.POINTER(C.c_float))
+    assert lib.mdwt_dn_destroy(None) == 1
+    assert lib.mdwt_dn_num_params(None, C.byref(n)) == 1
+    assert lib.mdwt_dn_get_params(None, fp) == 1
+    assert lib.mdwt_dn_set_params(None, fp) == 1
+    assert lib.mdwt_dn_infer(None, fp, fp, None) == 1
+    assert lib.mdwt_dn_gradient(None, fp, fp, fp, None) == 1
+    assert lib.mdwt_dn_train_step(None, fp, fp, None, 1, 1e-3, None) == 1
+    assert lib.mdwt_dn_last_error(None) == b""
+    # auxiliary buffers: a negative frame limit, a missing handle
+    assert lib.mdwt_set_aov_frames(None, 4) == 1
+    r = capi.MdwtRenderer(0)
+    with pytest.raises(capi.MdwtError) as e:
+        r.set_aov_frames(-1)
+    assert e.value.code == 1
+    with pytest.raises(capi.MdwtError) as e:
+        r.read_aov()  # before init
+    r.close()
 
 
 def test_hilbert_order_bijective_and_adjacent(capi):

@@ -66,6 +66,22 @@ def read_traffic(workload: str, mode: int):
 REF_SEED = 0x13579BDF  # seed of the plain reference render; other renders use other seeds
 
 
+def read_valu(workload: str, mode: int, kernel_ms: float):
+    """VALU issue of the frame kernel: the profiled VALU instructions per
+    launch over the live launch time, against one wave64 VALU instruction
+    per CU per cycle at the profiled clock."""
+    p = ROOT / "profiles" / f"valu_{workload}_m{mode}.json"
+    if not p.exists():
+        return None
+    d = json.loads(p.read_text())
+    achieved = d["valu_instructions_per_launch"] / (kernel_ms * 1e-3)
+    peak = d["compute_units"] * d["clock_ghz"] * 1e9
+    return {"achieved": round(achieved / 1e9, 1), "peak": round(peak / 1e9, 1),
+            "unit": "G wave64 VALU instructions/s", "frac": round(achieved / peak, 4),
+            "instructions_per_launch": d["valu_instructions_per_launch"],
+            "source": str(p.relative_to(ROOT))}
+
+
 def time_to_rmse(r, frames: int, target: float, ms_per_frame: float):
     """Frames (and seconds at the measured frame time) that plain accumulation
     needs for an expected RMSE of `target` against the converged image.
@@ -362,6 +378,9 @@ def main() -> int:
                 "kernel": f"render_kernel<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>"}
     if traffic_src:
         roofline["traffic_source"] = traffic_src
+    valu = read_valu(scene.name, args.mode, kernel_ms)
+    if valu:
+        roofline["valu_issue"] = valu
 
     ttr = None
     if args.rmse_frames > 1:

@@ -49,3 +49,32 @@ def test_offline_cli_argument_errors():
     r = subprocess.run([str(exe), "-fr", "a_4x4x4_uint8.raw", "-l", "1", "0", "0"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "-l takes 5 to 7 values" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parent.parent
+    p = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_bench_batch_views_are_seeded_and_bounded():
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import bench
+
+    a, b = bench.batch_views(64, 7), bench.batch_views(64, 7)
+    assert a == b and len(a) == 64
+    for v in a:
+        assert 256 <= v["width"] <= 1024 and 256 <= v["height"] <= 1024 and 1 <= v["spp"] <= 16
+        d = [t - f for t, f in zip(v["at"], v["from"])]
+        norm = sum(x * x for x in d) ** 0.5
+        # the up vector is never along the view direction
+        assert abs(sum(x * u for x, u in zip(d, v["up"])) / norm) < 0.99

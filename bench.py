@@ -344,7 +344,7 @@ def main() -> int:
                 "value": b.pop("value"), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps,
                 "warmup": args.warmup, "ms_per_step": None, "higher_is_better": False,
                 "scaling": "weak", "vs_baseline": None,
-                "dtype": f"f32 ({VOXEL_NAMES[str(scene.volumes[0].dtype)]} voxels)",
+                "dtype": "f32",
                 "data": "synthetic (seeded fBm + Gaussian blobs, seeded views)",
                 "config": {"workload": scene.name, "channels": scene.channels,
                            "volume": list(scene.volumes[0].shape),
@@ -415,7 +415,7 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (u8 voxels)",
+        "dtype": "f32",
         "data": "synthetic (seeded fBm + Gaussian blobs, seeded transfer functions)",
         "config": {"workload": scene.name, "channels": scene.channels,
                    "volume": list(scene.volumes[0].shape), "voxel": str(scene.volumes[0].dtype),

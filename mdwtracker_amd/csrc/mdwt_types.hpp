@@ -92,9 +92,9 @@ struct FrameParams {
   int n_items;
   unsigned* work;
   unsigned long long* item_ticks;  // start / end wall clock per item, nullptr = off
-  // auxiliary per-pixel buffers for the denoiser: 2 float4 per pixel, running
-  // means over samples of (first-hit distance, transmittance, hit fraction of
-  // channels 0..3); nullptr = off
+  // auxiliary per-pixel buffers for the denoiser: aov_groups(C) float4 per
+  // pixel, running means over samples of (first-hit distance, transmittance,
+  // hit fraction of each channel); nullptr = off
   void* aov;
 };
 

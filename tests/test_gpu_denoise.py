@@ -96,10 +96,11 @@ def _rand_input(rng, h, w, channels=4):
     return x
 
 
-@pytest.mark.parametrize("channels", [4, 10])
-def test_forward_matches_torch_reference(dnmod, channels):
+@pytest.mark.parametrize("channels,h,w", [(4, 32, 48), (10, 32, 48), (4, 12, 20), (4, 68, 132)])
+def test_forward_matches_torch_reference(dnmod, channels, h, w):
+    # partial convolution tiles at every level: 12 x 20 pools to 3 x 5,
+    # 68 x 132 leaves a partial 64-pixel tile and an odd row pair
     rng = np.random.default_rng(0)
-    h, w = 32, 48
     dn = dnmod.Denoiser(w, h, channels=channels, seed=3)
     p = dn.get_params()
     # a non-trivial output layer (the default starts near zero)
@@ -121,9 +122,9 @@ def test_forward_matches_torch_reference(dnmod, channels):
     dn.close()
 
 
-def test_gradient_matches_torch_reference(dnmod):
+@pytest.mark.parametrize("h,w", [(16, 24), (20, 28)])
+def test_gradient_matches_torch_reference(dnmod, h, w):
     rng = np.random.default_rng(1)
-    h, w = 16, 24
     dn = dnmod.Denoiser(w, h, seed=5)
     p = dn.get_params() + rng.normal(0, 0.02, dn.num_params()).astype(np.float32)
     dn.set_params(p)

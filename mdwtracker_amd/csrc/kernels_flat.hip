@@ -160,7 +160,9 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
         L.b3 = o[3];
       }
       const float u1 = to_unit((cur & 1u) ? L.b2 : L.b0);
-      L.t = L.t - (plogf(1.0f - u1) / L.maj) * X.dt * L.w2g;
+      // -m 0 / -m 2 steps use the division-free log, -m 3 the series log
+      const float lg = MODE == 3 ? plogf(1.0f - u1) : plogf_poly(1.0f - u1);
+      L.t = L.t - (lg / L.maj) * X.dt * L.w2g;
       if (MODE == 2) {  // altMultiMajDTCH (deviceCode.cu:800-848)
         bool next_ch = false;
         if (L.t >= L.minT) {

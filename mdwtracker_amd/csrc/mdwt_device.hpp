@@ -121,6 +121,33 @@ __device__ __forceinline__ float plogf(float x) {
   return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
 }
 
+// The same reduction with log m = f P(f), f = m - 1, P of degree 9 in Horner
+// form with fused multiply-adds (no division): the -m 0 and -m 2 free-flight
+// steps use it. Same operations as the host side (fmaf rounds once on both).
+__device__ __forceinline__ float plogf_poly(float x) {
+  const uint32_t bits = __float_as_uint(x);
+  int e = int((bits >> 23) & 0xffu) - 127;
+  float m = __uint_as_float((bits & 0x007fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e = e + 1;
+  }
+  const float f = m - 1.0f;
+  float p = -7.637512684e-02f;
+  p = fmaf(p, f, 1.291354448e-01f);
+  p = fmaf(p, f, -1.324101686e-01f);
+  p = fmaf(p, f, 1.418020874e-01f);
+  p = fmaf(p, f, -1.660911888e-01f);
+  p = fmaf(p, f, 2.000206262e-01f);
+  p = fmaf(p, f, -2.500158548e-01f);
+  p = fmaf(p, f, 3.333332539e-01f);
+  p = fmaf(p, f, -4.999998808e-01f);
+  p = fmaf(p, f, 1.000000000e+00f);
+  const float r = f * p;
+  const float fe = float(e);
+  return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
+}
+
 // fp32 exp: x = n ln2 + r (Cody-Waite, the ln2 split of plogf), e^r by its
 // Taylor polynomial to r^6 (|r| <= ln2/2), times 2^n from exponent bits.
 // 0 below -87.3 (no denormals), +inf above 88. Same operations as the host.

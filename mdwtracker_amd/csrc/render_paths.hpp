@@ -50,7 +50,7 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
       float t = t0;
       if (majorant == 0.0f) continue;
       while (true) {
-        t = t - (plogf(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
         if (t >= minT) break;
         const float tWorld = t * g.gridToWorldT;
         const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -110,7 +110,7 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
       if (majorant != 0.00f) {
         float t = t0;
         while (true) {
-          t = t - (plogf(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+          t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
           if (t >= t1) break;
           if (tMaxHit < t) {
             stop = true;

@@ -144,6 +144,36 @@ inline float plogf(float x) {
   return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
 }
 
+// The same reduction with log m = f P(f), f = m - 1, P a degree-9 polynomial
+// (least-squares fit of log1p(f)/f on Chebyshev nodes of [sqrt(1/2)-1,
+// sqrt(2)-1], P(0) = 1) in Horner form with fused multiply-adds: no division,
+// so fewer instructions per free-flight step. fmaf rounds once on the host and
+// on the GPU, so both sides agree bit for bit. Used by the -m 0 and -m 2 steps
+// (deviceCode.cu:913, 797); the other modes keep plogf.
+inline float plogf_poly(float x) {
+  uint32_t bits;
+  std::memcpy(&bits, &x, 4);
+  int e = int((bits >> 23) & 0xffu) - 127;
+  uint32_t mb = (bits & 0x007fffffu) | 0x3f800000u;
+  float m;
+  std::memcpy(&m, &mb, 4);
+  if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+  const float f = m - 1.0f;
+  float p = -7.637512684e-02f;
+  p = std::fmaf(p, f, 1.291354448e-01f);
+  p = std::fmaf(p, f, -1.324101686e-01f);
+  p = std::fmaf(p, f, 1.418020874e-01f);
+  p = std::fmaf(p, f, -1.660911888e-01f);
+  p = std::fmaf(p, f, 2.000206262e-01f);
+  p = std::fmaf(p, f, -2.500158548e-01f);
+  p = std::fmaf(p, f, 3.333332539e-01f);
+  p = std::fmaf(p, f, -4.999998808e-01f);
+  p = std::fmaf(p, f, 1.000000000e+00f);
+  const float r = f * p;
+  const float fe = float(e);
+  return fe * 0.693145751953125f + (r + fe * 1.42860677e-06f);
+}
+
 // Portable fp32 exp: x = n ln2 + r (Cody-Waite, same ln2 split as plogf),
 // e^r by its Taylor polynomial to r^6 (|r| <= ln2/2, truncation < 2^-24
 // relative), scaled by 2^n built from the exponent bits. Flushes to 0 below
@@ -668,7 +698,7 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       if (tMax < t) return false;
       const float tWorld = t * g.gridToWorldT;
@@ -772,7 +802,7 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
       float t = t0;
       if (majorant == 0.0f) continue;
       while (true) {
-        t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
         if (t >= minT) break;
         const float tWorld = t * g.gridToWorldT;
         const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -1553,6 +1583,7 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
 }
 
 float oracle_logf(float x) { return plogf(x); }
+float oracle_logf_poly(float x) { return plogf_poly(x); }
 float oracle_expf(float x) { return pexpf(x); }
 float oracle_powf(float x, float y) { return ppowf(x, y); }
 

@@ -89,6 +89,7 @@ int oracle_clip_rays(oracle_renderer* o, const float* org, const float* dir, int
 /* unit hooks */
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float oracle_logf(float x);
+float oracle_logf_poly(float x);
 float oracle_expf(float x);
 float oracle_powf(float x, float y);
 float oracle_sample(oracle_renderer* o, int ch, const float p[3]);

@@ -56,6 +56,7 @@ _SIG = {
     "oracle_clip_rays": ([_h, _fp, _fp, C.c_int, _fp, _ip], C.c_int),
     "oracle_philox4x32_10": ([_up, _up, _up], None),
     "oracle_logf": ([C.c_float], C.c_float),
+    "oracle_logf_poly": ([C.c_float], C.c_float),
     "oracle_expf": ([C.c_float], C.c_float),
     "oracle_powf": ([C.c_float, C.c_float], C.c_float),
     "oracle_sample": ([_h, C.c_int, _fp], C.c_float),
@@ -108,6 +109,11 @@ def philox4x32_10(ctr, key) -> np.ndarray:
 
 def logf(x: float) -> float:
     return lib.oracle_logf(float(x))
+
+
+def logf_poly(x: float) -> float:
+    """The division-free portable log of the -m 0 / -m 2 free-flight steps."""
+    return lib.oracle_logf_poly(float(x))
 
 
 def expf(x: float) -> float:

@@ -49,6 +49,26 @@ def test_log_accuracy_within_2ulp():
     assert np.all(np.diff(sweep) >= 0)
 
 
+@pytest.mark.parametrize("fn", ["logf", "logf_poly"])
+def test_both_logs_within_2ulp_and_monotone(fn):
+    log = getattr(po, fn)
+    xs = np.concatenate([np.arange(1, 1 << 24, 97, dtype=np.float64) * 2.0 ** -24,
+                         np.float32(np.sqrt(0.5)) + np.arange(-64, 64) * np.float64(2.0 ** -24),
+                         np.float32(np.sqrt(2.0)) + np.arange(-64, 64) * np.float64(2.0 ** -23),
+                         [1.0, 2.0 ** -24, 0.5, 3.0, 1e-30, 1e30, 7.5e5]]).astype(np.float32)
+    got = np.array([log(float(x)) for x in xs], np.float64)
+    ref = np.log(xs.astype(np.float64))
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    err = np.abs(got - ref) / ulp
+    assert err.max() <= 2.0, (fn, err.max(), xs[err.argmax()])
+    assert log(1.0) == 0.0
+    # monotone across the reduction's switch at sqrt(2) and on a sweep of (0, 1]
+    for grid in (np.sort(xs[(xs > 0.6) & (xs < 1.5)]),
+                 np.linspace(1e-6, 1.0, 20001, dtype=np.float32)):
+        v = np.array([log(float(x)) for x in grid])
+        assert np.all(np.diff(v) >= 0), fn
+
+
 def _renderer(vols, tfs, mode=2, mc=(2, 2, 2), ranges=None):
     r = po.OracleRenderer()
     for v in vols:

@@ -232,7 +232,7 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
       for (int i = 0; i < kMaxChannels; ++i) {
         if (i < X.C) {
           if (maj[i] > 0.0f)
-            ts[i] = ts[i] - (plogf(1.0f - rng.step()) / maj[i]) * step * g.worldToGridT;
+            ts[i] = ts[i] - (plogf_poly(1.0f - rng.step()) / maj[i]) * step * g.worldToGridT;
           else
             ts[i] = 1e20f;
         }
@@ -271,7 +271,7 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
           }
         }
         cnt.rejections++;
-        const float nt = minT - (plogf(1.0f - rand) / mj) * step * g.worldToGridT;
+        const float nt = minT - (plogf_poly(1.0f - rand) / mj) * step * g.worldToGridT;
 #pragma unroll
         for (int i = 0; i < kMaxChannels; ++i)
           if (i == sel) ts[i] = nt;
@@ -342,7 +342,7 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
     if (majorant != 0.00f) {
       float t = t0;
       while (true) {
-        t = t - (plogf(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
         if (t >= t1) break;
         const float tWorld = t * g.gridToWorldT;
         const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;

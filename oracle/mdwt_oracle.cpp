@@ -148,8 +148,9 @@ inline float plogf(float x) {
 // (least-squares fit of log1p(f)/f on Chebyshev nodes of [sqrt(1/2)-1,
 // sqrt(2)-1], P(0) = 1) in Horner form with fused multiply-adds: no division,
 // so fewer instructions per free-flight step. fmaf rounds once on the host and
-// on the GPU, so both sides agree bit for bit. Used by the -m 0 and -m 2 steps
-// (deviceCode.cu:913, 797); the other modes keep plogf.
+// on the GPU, so both sides agree bit for bit. Used by every free-flight step
+// (deviceCode.cu:913, 805, 676, 739, 1023) except the -m 3 one
+// (deviceCode.cu:564), which keeps plogf.
 inline float plogf_poly(float x) {
   uint32_t bits;
   std::memcpy(&bits, &x, 4);
@@ -750,7 +751,7 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
     if (majorantSum <= 0.0000001f) return true;
     for (int i = 0; i < C; ++i) {
       if (majorants[i] > 0.0f)
-        ts[i] = ts[i] - (plogf(1.0f - rng.step()) / majorants[i]) * unit * g.worldToGridT;
+        ts[i] = ts[i] - (plogf_poly(1.0f - rng.step()) / majorants[i]) * unit * g.worldToGridT;
       else
         ts[i] = 1e20f;
     }
@@ -782,7 +783,7 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
         }
       }
       prd.rejections++;
-      ts[sel] = ts[sel] - (plogf(1.0f - rand) / majorants[sel]) * unit * g.worldToGridT;
+      ts[sel] = ts[sel] - (plogf_poly(1.0f - rand) / majorants[sel]) * unit * g.worldToGridT;
     }
     return true;
   };
@@ -917,7 +918,7 @@ void trace_blend(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;

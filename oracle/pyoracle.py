@@ -112,7 +112,7 @@ def logf(x: float) -> float:
 
 
 def logf_poly(x: float) -> float:
-    """The division-free portable log of the -m 0 / -m 2 free-flight steps."""
+    """The division-free portable log of the free-flight steps of every mode but -m 3."""
     return lib.oracle_logf_poly(float(x))
 
 

@@ -203,10 +203,12 @@ int mdwt_read_aov(mdwt_renderer* h, float* out);
 
 /* ---- batched views (no reference counterpart: the reference renders one
  * camera per launch). Many views of uneven size and spp, each with its own
- * accumulators, rendered frame by frame with one frame-kernel launch per view,
- * the views spread round-robin over four HIP streams so that small views run
- * side by side (a persistent one-launch variant exists in the library but is
- * switched off until a hang seen on its first GPU run is explained). Calls
+ * accumulators, rendered frame by frame, either with one frame-kernel launch
+ * per view, the views spread round-robin over four HIP streams so that small
+ * views run side by side (launch form 0, the default), or with one launch per
+ * frame whose grid holds every view's pixel blocks (launch form 1). A
+ * persistent one-launch variant exists in the library but is switched off
+ * until a hang seen on its first GPU run is explained. Calls
  * are ordered with the handle's other work and return when the frames are
  * done. A view's image equals a single-view render with the same
  * camera, size, spp and seed (the RNG is keyed by pixel), so views never
@@ -225,11 +227,18 @@ int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms);
 int mdwt_batch_count(mdwt_renderer* h, uint64_t out[4]);
 /* Restarts the views' accumulation (frame ids keep counting). */
 int mdwt_batch_reset(mdwt_renderer* h);
+/* Launch form of the batch: 0 = one launch per view over four streams (each
+ * view finishes as soon as its own blocks do: low per-view latency), 1 = one
+ * launch for all views per frame (every view finishes with the frame: highest
+ * throughput when views are small). Images and counters do not depend on it. */
+int mdwt_batch_set_launch(mdwt_renderer* h, int launch);
 /* width * height float4 running means (and second moments) of one view. */
 int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2);
-/* Per view: GPU time in microseconds of its launch in the last batched
- * frame (HIP events on its stream, so it includes time shared with views on
- * the other streams); MDWT_ERR_STATE before the first batched frame. */
+/* Per view: GPU time in microseconds of the view in the last batched frame.
+ * Launch form 0: HIP events around its launch on its stream (so it includes
+ * time shared with views on the other streams); form 1: from the launch's
+ * first workgroup start to the view's last workgroup end (device wall
+ * clock). MDWT_ERR_STATE before the first batched frame. */
 int mdwt_batch_view_times(mdwt_renderer* h, float* us);
 /* Autotuner: times `frames` frames (after one warm-up frame) for every
  * combination of macrocell size s (voxels per macrocell side; grid = dims /

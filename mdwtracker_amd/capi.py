@@ -75,6 +75,7 @@ SIGNATURES = {
     "mdwt_batch_render": [_h, C.c_int, _fp],
     "mdwt_batch_count": [_h, _u64p],
     "mdwt_batch_reset": [_h],
+    "mdwt_batch_set_launch": [_h, C.c_int],
     "mdwt_batch_read_view": [_h, C.c_int, _fp, _fp],
     "mdwt_batch_view_times": [_h, _fp],
     "mdwt_autotune": [_h, _ip, C.c_int, _ip, C.c_int, _ip, C.c_int, C.c_int, _fp, _ip],
@@ -290,6 +291,10 @@ class MdwtRenderer:
 
     def batch_reset(self) -> None:
         self._chk(lib.mdwt_batch_reset(self._h))
+
+    def batch_set_launch(self, launch: int) -> None:
+        """0: one launch per view over four streams; 1: one launch for all views."""
+        self._chk(lib.mdwt_batch_set_launch(self._h, int(launch)))
 
     def batch_read_view(self, i: int, with_m2: bool = False):
         w, h = self._views[i]

@@ -1,11 +1,14 @@
-// Batched views: one persistent launch renders many views of uneven size and
-// spp. Its workgroups pull work items (view, pixel block) from a device
-// counter until the list is exhausted, so small and large views share the
-// machine without a launch per view, and the heaviest items (measured on the
-// first frame) go first. The counter reset and the frame / accumulation id
-// advance are kernels, not memsets, so the three-launch sequence is captured
-// once in a HIP graph and replayed every frame (mdwt_api.cpp). No reference
-// counterpart: the reference renders one camera per owlLaunch2D
+// Batched views of uneven size and spp in one launch per frame.
+// render_views_kernel: one workgroup per (view, pixel block) item, every view
+// of the frame in a single grid, so small views share the machine with large
+// ones without a launch each.
+// render_batch_kernel (not reachable from the C ABI): a persistent launch
+// whose workgroups pull work items from a device counter until the list is
+// exhausted, the heaviest items (measured on the first frame) first. Its
+// counter reset and frame / accumulation id advance are kernels, not memsets,
+// so the three-launch sequence can be captured once in a HIP graph and
+// replayed every frame (mdwt_api.cpp).
+// No reference counterpart: the reference renders one camera per owlLaunch2D
 // (renderer.cpp:949).
 #include "mdwt_internal.hpp"
 #include "render_paths.hpp"
@@ -57,6 +60,47 @@ __global__ __launch_bounds__(256) void render_batch_kernel(RenderParams P) {
   if (COUNT) flush_counts(P, cnt);
 }
 
+// One launch for all views, one workgroup per work item: workgroup i renders
+// item i (view << 20 | by << 10 | bx). Items run heaviest view first, each
+// view's blocks in Hilbert order, so consecutive workgroups (dealt
+// round-robin over the XCDs) cover neighbouring blocks of one view. Frame and
+// accumulation ids are launch arguments shared by every view. With
+// item_ticks set, each workgroup records its start and end wall clock.
+template <int MODE, int DT, bool COUNT, bool LIT>
+__global__ __launch_bounds__(256) void render_views_kernel(RenderParams P) {
+  extern __shared__ float4 s_tf[];
+  __shared__ ChanTf s_ch[kMaxChannels];
+  const uint64_t t_begin = P.fr.item_ticks ? wall_clock64() : 0ull;
+  load_tf_lds(P, s_tf, s_ch);
+  __syncthreads();
+
+  Ctx<DT> X;
+  X.init(P, s_tf, s_ch);
+  Counts cnt = {0, 0, 0, 0};
+  const uint32_t code = P.fr.items[blockIdx.x];
+  const ViewParams* vp = P.fr.views + (code >> 20);
+  ViewCtx V;
+  V.cam = &vp->cam;
+  V.W = vp->width;
+  V.H = vp->height;
+  V.spp = vp->spp;
+  V.seed = vp->seed;
+  V.frame_id = P.fr.frame_id;
+  V.accum_id = P.fr.accum_id;
+  V.accum = vp->accum;
+  V.m2 = vp->m2;
+  render_block<MODE, DT, COUNT, LIT>(P, X, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu),
+                                     cnt);
+  if (COUNT) flush_counts(P, cnt);
+  if (!COUNT && P.fr.item_ticks) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      P.fr.item_ticks[2 * blockIdx.x] = t_begin;
+      P.fr.item_ticks[2 * blockIdx.x + 1] = wall_clock64();
+    }
+  }
+}
+
 __global__ void batch_begin_kernel(unsigned* work) {
   if (threadIdx.x == 0) work[0] = 0u;
 }
@@ -76,6 +120,35 @@ RenderFn batch_fn_dt(int dtype) {
     default: return render_batch_kernel<MODE, MDWT_DT_F32, COUNT, LIT>;
   }
 }
+
+template <int MODE, bool COUNT, bool LIT>
+RenderFn views_fn_dt(int dtype) {
+  switch (dtype) {
+    case MDWT_DT_U8: return render_views_kernel<MODE, MDWT_DT_U8, COUNT, LIT>;
+    case MDWT_DT_U16: return render_views_kernel<MODE, MDWT_DT_U16, COUNT, LIT>;
+    case MDWT_DT_F16: return render_views_kernel<MODE, MDWT_DT_F16, COUNT, LIT>;
+    default: return render_views_kernel<MODE, MDWT_DT_F32, COUNT, LIT>;
+  }
+}
+
+#define MDWT_VIEWS(M)                                                                 \
+  (count ? (lit ? views_fn_dt<M, true, true>(dtype) : views_fn_dt<M, true, false>(dtype)) \
+         : (lit ? views_fn_dt<M, false, true>(dtype) : views_fn_dt<M, false, false>(dtype)))
+RenderFn select_views(int mode, int dtype, bool count, bool lit) {
+  switch (mode) {
+    case 0: return MDWT_VIEWS(0);
+    case 1: return MDWT_VIEWS(1);
+    case 2: return MDWT_VIEWS(2);
+    case 3: return MDWT_VIEWS(3);
+    case 4: return MDWT_VIEWS(4);
+    case 5: return MDWT_VIEWS(5);
+    case 6: return MDWT_VIEWS(6);
+    case 7: return MDWT_VIEWS(7);
+    case 8: return MDWT_VIEWS(8);
+    default: return MDWT_VIEWS(9);
+  }
+}
+#undef MDWT_VIEWS
 
 #define MDWT_BATCH(M)                                                                 \
   (count ? (lit ? batch_fn_dt<M, true, true>(dtype) : batch_fn_dt<M, true, false>(dtype)) \
@@ -132,6 +205,16 @@ hipError_t launch_batch(const RenderParams& p, int mode, int dtype, bool count, 
     e = hipGetLastError();
   }
   return e;
+}
+
+hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s) {
+  if (p.fr.n_items <= 0) return hipSuccess;
+  const size_t lds = render_lds_bytes(p);
+  const RenderFn f = select_views(mode, dtype, count, is_lit(p));
+  const hipError_t e = prepare(f, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(f, dim3(unsigned(p.fr.n_items)), dim3(unsigned(p.fr.bw * p.fr.bh)), lds, s, p);
+  return hipGetLastError();
 }
 
 }  // namespace mdwt

@@ -158,6 +158,7 @@ struct mdwt_renderer {
   std::vector<hipEvent_t> join_ev;               // end of each batch stream's work
   std::vector<hipStream_t> batch_streams;        // views run round-robin on these
   std::vector<float> view_us;
+  int batch_launch = 0;  // 0: a launch per view over the batch streams, 1: one launch for all
 };
 
 namespace {
@@ -1320,7 +1321,8 @@ namespace {
 
 // The persistent one-launch batch (kernels_batch.hip, graph capture and
 // replay) is switched off: its first GPU run hung and the cause is not yet
-// shown. Batches render with one single-view launch per view instead.
+// shown. Batches render with one single-view launch per view, or with one
+// ordinary (non-persistent) launch whose grid holds every view's blocks.
 constexpr int kPersistentBatch = 0;
 
 void free_batch(mdwt_renderer* h) {
@@ -1488,6 +1490,68 @@ int batch_render_views(mdwt_renderer* h, int n_frames, float* ms_out) {
       HIPCHK(h, hipEventElapsedTime(&t, h->view_ev[i], h->view_ev_end[i]));
       h->view_us[i] = t * 1000.f;
     }
+  }
+  h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
+  if (ms_out) *ms_out = ms;
+  return MDWT_OK;
+}
+
+// The one-launch form: every view's pixel blocks in one grid per frame, at
+// the batch's frame and accumulation ids.
+mdwt::RenderParams views_params(const mdwt_renderer* h) {
+  mdwt::RenderParams p = render_params(h);
+  p.fr.schedule = 0;
+  p.fr.wave_cost = nullptr;
+  p.fr.views = h->d_views;
+  p.fr.items = h->d_items;
+  p.fr.n_items = int(h->items.size());
+  p.fr.work = nullptr;
+  p.fr.item_ticks = nullptr;
+  p.fr.frame_id = h->batch_frame;
+  p.fr.accum_id = h->batch_accum;
+  p.fr.aov = nullptr;  // batched views have no auxiliary buffers
+  return p;
+}
+
+// Per-view time of the last one-launch frame from its workgroups' wall-clock
+// stamps (100 MHz): from the first workgroup's start to the view's last
+// workgroup's end.
+int views_times_from_ticks(mdwt_renderer* h) {
+  const size_t n = h->items.size();
+  std::vector<unsigned long long> t(2 * n);
+  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+  unsigned long long t0 = ~0ull;
+  for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[2 * i]);
+  std::vector<unsigned long long> end(h->views.size(), 0ull);
+  for (size_t i = 0; i < n; ++i) {
+    const size_t v = h->items[i] >> 20;
+    end[v] = std::max(end[v], t[2 * i + 1]);
+  }
+  h->view_us.assign(h->views.size(), 0.f);
+  for (size_t v = 0; v < end.size(); ++v)
+    h->view_us[v] = end[v] >= t0 ? float(double(end[v] - t0) / 100.0) : 0.f;
+  return MDWT_OK;
+}
+
+// n_frames frames of every view, one launch per frame on the handle's stream;
+// the last frame records its workgroups' wall clocks for the per-view times.
+int batch_render_one_launch(mdwt_renderer* h, int n_frames, float* ms_out) {
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int f = 0; f < n_frames; ++f) {
+    mdwt::RenderParams p = views_params(h);
+    if (f == n_frames - 1) p.fr.item_ticks = h->d_item_ticks;
+    HIPCHK(h, mdwt::launch_views(p, h->mode, h->dtype, false, h->stream));
+    h->batch_frame++;
+    h->batch_accum = h->accumulate ? h->batch_accum + 1u : 0u;
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (n_frames > 0) {
+    const int rc = views_times_from_ticks(h);
+    if (rc != MDWT_OK) return rc;
   }
   h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
   if (ms_out) *ms_out = ms;
@@ -1673,7 +1737,9 @@ int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms_out) {
   int rc = batch_ready(h);
   if (rc != MDWT_OK) return rc;
   if (n_frames < 0) return fail(h, MDWT_ERR_ARG, "negative frame count");
-  if (!kPersistentBatch) return batch_render_views(h, n_frames, ms_out);
+  if (!kPersistentBatch)
+    return h->batch_launch == 1 ? batch_render_one_launch(h, n_frames, ms_out)
+                                : batch_render_views(h, n_frames, ms_out);
   const mdwt::RenderParams p = batch_params(h);
   int grid = 0;
   if ((rc = batch_grid(h, p, false, &grid)) != MDWT_OK) return rc;
@@ -1727,8 +1793,11 @@ int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
   if (rc != MDWT_OK) return rc;
   if (!kPersistentBatch) {
     HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
-    for (size_t i = 0; i < h->views.size(); ++i)
-      HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, true, h->stream));
+    if (h->batch_launch == 1)
+      HIPCHK(h, mdwt::launch_views(views_params(h), h->mode, h->dtype, true, h->stream));
+    else
+      for (size_t i = 0; i < h->views.size(); ++i)
+        HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, true, h->stream));
     HIPCHK(h, hipMemcpyAsync(counts, h->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                              h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1743,6 +1812,15 @@ int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
   HIPCHK(h, hipMemcpyAsync(counts, h->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MDWT_OK;
+}
+
+int mdwt_batch_set_launch(mdwt_renderer* h, int launch) {
+  if (!h) return MDWT_ERR_ARG;
+  if (launch != 0 && launch != 1)
+    return fail(h, MDWT_ERR_ARG,
+                "batch launch must be 0 (a launch per view) or 1 (one launch for all views)");
+  h->batch_launch = launch;
   return MDWT_OK;
 }
 

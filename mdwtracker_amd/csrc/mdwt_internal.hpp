@@ -19,6 +19,9 @@ hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count,
 // (grid workgroups), frame / accumulation id advance (skipped when counting)
 hipError_t launch_batch(const RenderParams& p, int mode, int dtype, bool count, int grid,
                         hipStream_t s);
+// batched views in one launch: one workgroup per item of p.fr.items (view,
+// pixel block), frame and accumulation ids from p.fr.frame_id / accum_id
+hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
 // resident workgroups per CU of the batched frame kernel
 hipError_t batch_blocks_per_cu(const RenderParams& p, int mode, int dtype, bool count, int* out);
 // dynamic LDS of a frame launch (transfer-function texels)

@@ -5,11 +5,15 @@
 
 namespace mdwt {
 
+// Per-channel transfer-function parameters in LDS: value domain [dlo, dhi],
+// TF range start xlo, the correctly rounded reciprocals of the domain width
+// (idw) and of the TF range width (ixw), opacity scale, texel count and
+// texel offset.
 struct ChanTf {
-  float dlo, dhi, xlo, xhi, os;
+  float dlo, dhi, xlo, ixw, os;
   int n;
   int off;  // texel offset in LDS
-  int pad;
+  float idw;
 };
 
 struct Counts {
@@ -88,15 +92,16 @@ struct Ctx {
   }
 
   // transferFunction (deviceCode.cu:27-42), opacity only; colour is looked
-  // up only for the accepted collision (same lerp, same texels).
+  // up only for the accepted collision (same lerp, same texels). The two
+  // divisions by per-channel constants are products with their reciprocals.
   __device__ __forceinline__ float tf_alpha(float f, int c, float& r2out) const {
     const ChanTf t = chtf[c];
     if (f < t.dlo || f > t.dhi) {
       r2out = qnan();
       return 0.0f;
     }
-    const float r1 = (f - t.dlo) / (t.dhi - t.dlo);
-    const float r2 = (r1 - t.xlo) / (t.xhi - t.xlo);
+    const float r1 = (f - t.dlo) * t.idw;
+    const float r2 = (r1 - t.xlo) * t.ixw;
     r2out = r2;
     return tf_lerp_alpha(tf + t.off, t.n, r2) * t.os;
   }
@@ -105,8 +110,8 @@ struct Ctx {
   __device__ __forceinline__ float4 tf_rgba(float f, int c) const {
     const ChanTf t = chtf[c];
     if (f < t.dlo || f > t.dhi) return make_float4(1.f, 0.f, 1.f, 0.f);
-    const float r1 = (f - t.dlo) / (t.dhi - t.dlo);
-    const float r2 = (r1 - t.xlo) / (t.xhi - t.xlo);
+    const float r1 = (f - t.dlo) * t.idw;
+    const float r2 = (r1 - t.xlo) * t.ixw;
     float4 v = tf_lerp(tf + t.off, t.n, r2);
     v.w = v.w * t.os;
     return v;
@@ -168,11 +173,11 @@ __device__ __forceinline__ void load_tf_lds(const RenderParams& P, float4* s_tf,
     t.dlo = P.tf.dom_lo[c];
     t.dhi = P.tf.dom_hi[c];
     t.xlo = P.tf.xf_lo[c];
-    t.xhi = P.tf.xf_hi[c];
+    t.ixw = 1.0f / (P.tf.xf_hi[c] - P.tf.xf_lo[c]);
     t.os = P.tf.opacity[c];
     t.n = P.tf.n[c];
     t.off = c * P.tf.stride;
-    t.pad = 0;
+    t.idw = 1.0f / (P.tf.dom_hi[c] - P.tf.dom_lo[c]);
     s_ch[c] = t;
   }
 }

@@ -536,13 +536,17 @@ bool tfs_ready(const oracle_renderer* o) {
 }
 
 // ------------------------------------------------------ device programs ----
-// deviceCode.cu:27-42 transferFunction
+// deviceCode.cu:27-42 transferFunction. Its two divisions by per-channel
+// constants ((f - lo) / (hi - lo), then by the TF range width) are products
+// with the correctly rounded reciprocals of those constants here, as on the
+// GPU: the remapped value moves by about an ulp, far below one texel.
 inline vec4f transfer_function(const oracle_renderer* o, float f, int c) {
   const float vlo = o->dom_lo[c], vhi = o->dom_hi[c];
   if (f < vlo || f > vhi) return {1.f, 0.f, 1.f, 0.f};
   const TF& tf = o->tf[c];
-  const float r1 = (f - vlo) / (vhi - vlo);
-  const float r2 = (r1 - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  const float idw = 1.0f / (vhi - vlo), ixw = 1.0f / (tf.xf_hi - tf.xf_lo);
+  const float r1 = (f - vlo) * idw;
+  const float r2 = (r1 - tf.xf_lo) * ixw;
   vec4f xf = tf_lookup_u(tf, r2);
   xf.w = xf.w * tf.opacity_scale;
   return xf;
@@ -1132,6 +1136,7 @@ bool trace_woodcock(const oracle_renderer* o, vec3f org, vec3f dir, Rng& rng, Hi
 // enabled.
 void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
   const size_t fbOfs = size_t(px) + size_t(o->W) * py;
+  // deviceCode.cu:292 starts the sum at alpha 1: the stored alpha is 1 + 1/spp
   vec4f finalColor = {0.f, 0.f, 0.f, 1.f};
   const uint32_t pixel = uint32_t(fbOfs);
   for (int sampleID = 0; sampleID < o->spp; ++sampleID) {

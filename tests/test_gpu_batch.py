@@ -1,7 +1,8 @@
 """GPU checks of batched multi-view rendering, the workgroup shapes, the
 Hilbert tiles, the macrocell rebuild and the autotuner. Batches render with
-one single-view launch per view (the persistent one-launch kernel is switched
-off after an unexplained hang on its first run).
+one single-view launch per view (launch form 0) or with one launch whose grid
+holds every view's blocks (form 1); the persistent kernel is switched off
+after an unexplained hang on its first run.
 
 The RNG is keyed by (seed, pixel, frame, sample), so a batched view must give
 the same bits as a single-view render of the same camera, size, spp and seed,
@@ -57,15 +58,18 @@ def _single(capi, scene, mode, v, frames):
     return a, cnt
 
 
+@pytest.mark.parametrize("launch", [0, 1])
 @pytest.mark.parametrize("mode", [2, 0, 7])
-def test_batched_views_equal_single_view_renders(capi, scene, mode):
+def test_batched_views_equal_single_view_renders(capi, scene, mode, launch):
     views = _views(6, seed=mode)
-    print(f"[batch] mode={mode} views={[(v['width'], v['height'], v['spp']) for v in views]}", flush=True)
+    print(f"[batch] launch={launch} mode={mode} "
+          f"views={[(v['width'], v['height'], v['spp']) for v in views]}", flush=True)
     g = capi.MdwtRenderer(0)
     scenes.load_scene(g, scene, mode)
+    g.batch_set_launch(launch)
     g.batch_set_views(views)
     cb = g.batch_count()
-    # 1 eager frame (records item costs), then graph replays across calls
+    # frames split over several calls: ids and accumulation carry across them
     g.batch_render(1)
     g.batch_render(3)
     g.batch_render(2)
@@ -88,11 +92,14 @@ def test_batched_views_equal_single_view_renders(capi, scene, mode):
     g.close()
 
 
-def test_batch_with_lighting_equals_single(capi, scene):
+@pytest.mark.parametrize("launch", [0, 1])
+def test_batch_with_lighting_equals_single(capi, scene, launch):
     views = _views(3, seed=11)
+    print(f"[batch] lighting launch={launch}", flush=True)
     g = capi.MdwtRenderer(0)
     scenes.load_scene(g, scene, 2)
     g.set_light((0.3, -1.0, 0.2), True, True)
+    g.batch_set_launch(launch)
     g.batch_set_views(views)
     g.batch_render(2)
     for i, v in enumerate(views):
@@ -125,6 +132,28 @@ def test_block_shapes_and_tiles_do_not_change_images(capi, scene, threads, tile)
     g.render(3)
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(want))
     g.close()
+
+
+@pytest.mark.parametrize("threads", [64, 128])
+def test_one_launch_batch_block_shapes(capi, scene, threads):
+    """The one-launch batch with 8x8 and 16x8 blocks (items rebuilt for the
+    block shape) equals the default 16x16 per-view launches."""
+    views = _views(5, seed=21)
+    print(f"[batch] one launch, {threads} threads", flush=True)
+    out = []
+    for launch, th in ((0, 256), (1, threads)):
+        g = capi.MdwtRenderer(0)
+        g.set_block(th)
+        scenes.load_scene(g, scene, 2)
+        g.batch_set_launch(launch)
+        g.batch_set_views(views)
+        g.batch_render(2)
+        out.append([g.batch_read_view(i) for i in range(len(views))])
+        t = g.batch_view_times()
+        assert t.shape == (len(views),) and np.all(t > 0)
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(_bits(a), _bits(b))
 
 
 def test_set_macrocells_matches_fresh_init(capi, scene):
@@ -182,6 +211,8 @@ def test_batch_api_errors(capi, scene):
     g.batch_render(1)
     assert g.batch_view_times().shape == (2,)
     assert capi.lib.mdwt_batch_read_view(g._h, 2, None, None) != 0  # no such view
+    with pytest.raises(capi.MdwtError, match="batch launch"):
+        g.batch_set_launch(2)
     g.close()
 
 

@@ -98,6 +98,84 @@ def test_ranges_other_dtypes_and_bounds(capi, po, dtype):
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
 
 
+def _ragged_scene(n_ch, seed):
+    """n_ch u8 blob channels on a 23 x 17 x 30 (x, y, z) grid: no side a power
+    of two, macrocells (5, 4, 7) that do not divide it."""
+    rng = np.random.default_rng(seed)
+    shape = (30, 17, 23)
+    vols = [scenes.quantize_u8(scenes.gaussian_blobs(shape, 3, rng)) for _ in range(n_ch)]
+    tfs = [scenes.piecewise_tf(5, rng, c / n_ch, n_texels=64) for c in range(n_ch)]
+    return vols, tfs
+
+
+@pytest.mark.parametrize("n_ch", [1, 12])
+@pytest.mark.parametrize("mode", list(range(10)))
+def test_channel_count_extremes_ragged_bit_exact(capi, po, n_ch, mode):
+    """One channel and the maximum of twelve, on a volume and a frame whose
+    sizes divide nothing: images, second moments and counters stay bit-exact."""
+    vols, tfs = _ragged_scene(n_ch, 40 + n_ch)
+    print(f"[ragged] channels={n_ch} mode={mode}", flush=True)
+    g, o = capi.MdwtRenderer(0), po.OracleRenderer()
+    for r in (g, o):
+        for v in vols:
+            r.push_channel(v)
+        r.init(mode, (5, 4, 7), True)
+        for c, t in enumerate(tfs):
+            r.set_tf(c, t, 4.0, (0.0, 1.0))
+        r.set_frame(77, 45, 2, True, (0.1, 0.0, 0.2))
+        r.set_camera([1.9, 1.2, -0.8], [0.5, 0.45, 0.5], [0, 1, 0], 40.0)
+        r.set_seed(5)
+    np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o.read_majorants()))
+    cg = g.count_frame()
+    g.render(2)
+    o.render(2, threads=THREADS)
+    a_g, m_g = g.read_accum(with_m2=True)
+    a_o, m_o = o.read_accum(with_m2=True)
+    np.testing.assert_array_equal(_bits(a_g), _bits(a_o))
+    np.testing.assert_array_equal(_bits(m_g), _bits(m_o))
+    o1 = po.OracleRenderer()
+    for v in vols:
+        o1.push_channel(v)
+    o1.init(mode, (5, 4, 7), True)
+    for c, t in enumerate(tfs):
+        o1.set_tf(c, t, 4.0, (0.0, 1.0))
+    o1.set_frame(77, 45, 2, True, (0.1, 0.0, 0.2))
+    o1.set_camera([1.9, 1.2, -0.8], [0.5, 0.45, 0.5], [0, 1, 0], 40.0)
+    o1.set_seed(5)
+    o1.render(1, threads=THREADS)
+    assert cg == o1.read_counters() and cg["samples"] > 0
+    # a one-pixel frame of the same scene
+    for r in (g, o):
+        r.set_frame(1, 1, 3, True, (0.0, 0.0, 0.0))
+    g.render(2)
+    o.render(2, threads=THREADS)
+    np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+    g.close()
+
+
+def test_transparent_volume_renders_background_exactly(capi):
+    """Zero voxels under a transfer function that is transparent at 0: every
+    majorant is 0, no sample is taken, and every pixel is the background."""
+    g = capi.MdwtRenderer(0)
+    for _ in range(3):
+        g.push_channel(np.zeros((9, 10, 11), np.uint8))
+    g.init(2, (3, 3, 3), True)
+    tf = np.zeros((16, 4), np.float32)
+    tf[:, :3] = 1.0
+    tf[8:, 3] = 1.0  # opaque only above the middle of the range
+    for c in range(3):
+        g.set_tf(c, tf)
+    bg = (0.25, 0.5, 0.75)
+    g.set_frame(33, 21, 1, True, bg)
+    assert np.all(g.read_majorants() == 0)
+    cnt = g.count_frame()
+    g.render(3)
+    a = g.read_accum()
+    assert cnt["samples"] == 0
+    np.testing.assert_array_equal(a[..., :3], np.broadcast_to(np.float32(bg), a[..., :3].shape))
+    g.close()
+
+
 def test_camera_rays_and_clip_bit_exact(capi, po):
     sc = _scene("ndda_cpu_2ch64")
     g, o = _pair(capi, po, sc, 2)

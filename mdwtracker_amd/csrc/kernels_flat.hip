@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
       L.k = cur + 1u;
       if ((cur & 1u) == 0u) {
         uint32_t o[4];
-        philox10(frame, uint32_t(L.s), 1u + (cur >> 1), 0u, seed, L.pixel, o);
+        philox(frame, uint32_t(L.s), 1u + (cur >> 1), 0u, seed, L.pixel, o);
         L.b0 = o[0];
         L.b1 = o[1];
         L.b2 = o[2];
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
         L.hc = 0;
         cnt.rays++;
         uint32_t o[4];
-        philox10(frame, uint32_t(L.s), 0u, 0u, seed, L.pixel, o);
+        philox(frame, uint32_t(L.s), 0u, 0u, seed, L.pixel, o);
         V3 org, dir;
         camera_ray(P.cam, L.px, L.py, W, H, to_unit(o[0]), org, dir);
         float t0, t1;

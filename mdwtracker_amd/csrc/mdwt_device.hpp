@@ -32,14 +32,17 @@ __device__ __forceinline__ V3 load3(const float* p) { return {p[0], p[1], p[2]};
 __device__ __forceinline__ float qnan() { return __int_as_float(0x7fc00000); }
 
 // ------------------------------------------------------------- Philox ------
-// Philox4x32-10 (Salmon et al., SC'11). Stream key = (seed, pixel); counter =
-// (frame, sample, block, stream). Each round's two 32x32->64 products are
+// Philox4x32-7 (Salmon et al., SC'11: seven rounds is the fewest for which
+// Philox4x32 passes the BigCrush battery; the round function and key schedule
+// are those of the usual ten-round generator). Stream key = (seed, pixel);
+// counter = (frame, sample, block, stream). Each round's two 32x32->64 products are
 // written as 64-bit products so that each becomes one v_mad_u64_u32 (32-bit
 // integer multiplies are quarter rate on CDNA; the hi/lo pair would be two).
-__device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                         uint32_t k0, uint32_t k1, uint32_t out[4]) {
+constexpr int kPhiloxRounds = 7;
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
-  for (int r = 0; r < 10; ++r) {
+  for (int r = 0; r < kPhiloxRounds; ++r) {
     if (r > 0) {
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
@@ -81,7 +84,7 @@ struct Rng {
   }
   __device__ __forceinline__ float jitter() const {
     uint32_t o[4];
-    philox10(frame, sample, 0u, stream, k0, k1, o);
+    philox(frame, sample, 0u, stream, k0, k1, o);
     return to_unit(o[0]);
   }
   __device__ __forceinline__ float step() {
@@ -89,7 +92,7 @@ struct Rng {
     k += 1;
     if ((cur & 1u) == 0u) {
       uint32_t o[4];
-      philox10(frame, sample, 1u + (cur >> 1), stream, k0, k1, o);
+      philox(frame, sample, 1u + (cur >> 1), stream, k0, k1, o);
       b0 = o[0];
       b1 = o[1];
       b2 = o[2];

@@ -57,10 +57,14 @@ inline vec4f operator*(float s, vec4f a) { return {s * a.x, s * a.y, s * a.z, s 
 inline vec4f operator/(vec4f a, float s) { return {a.x / s, a.y / s, a.z / s, a.w / s}; }
 
 // ------------------------------------------------------------------- RNG ----
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
-// as 1, 2, 3", SC'11). Replaces OWL's LCG<4> (deviceCode.h:145,
-// deviceCode.cu:296-297), whose streams are keyed by threadIdx; here a stream
-// is keyed by (seed, pixel) and counted by (frame, sample, block, stream).
+// Philox4x32 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11) with 7 rounds, the fewest for which that paper reports
+// Philox4x32 passing BigCrush (the round function and key schedule are those
+// of the ten-round generator, whose published known answers pin them).
+// Replaces OWL's LCG<4> (deviceCode.h:145, deviceCode.cu:296-297), whose
+// streams are keyed by threadIdx; here a stream is keyed by (seed, pixel) and
+// counted by (frame, sample, block, stream).
+constexpr int kPhiloxRounds = 7;
 inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
   const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
   const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
@@ -70,10 +74,11 @@ inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
   const uint32_t n2 = hi0 ^ c[3] ^ k[1];
   c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
 }
-inline void philox10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4],
+                   int rounds = kPhiloxRounds) {
   uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
   uint32_t k[2] = {key[0], key[1]};
-  for (int r = 0; r < 10; ++r) {
+  for (int r = 0; r < rounds; ++r) {
     if (r > 0) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
     philox_round(c, k);
   }
@@ -106,7 +111,7 @@ struct Rng {
   float word(uint32_t n) {
     if (n / 4 != cached) {
       ctr[2] = n / 4;
-      philox10(ctr, key, buf);
+      philox(ctr, key, buf);
       cached = n / 4;
     }
     return float(buf[n % 4] >> 8) * 0x1p-24f;  // exact: 24-bit integer times 2^-24
@@ -1584,9 +1589,10 @@ int oracle_clip_rays(oracle_renderer* o, const float* org, const float* dir, int
   return 0;
 }
 
-void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
-  philox10(ctr, key, out);
+void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]) {
+  philox(ctr, key, out, rounds);
 }
+int oracle_philox_rounds(void) { return kPhiloxRounds; }
 
 float oracle_logf(float x) { return plogf(x); }
 float oracle_logf_poly(float x) { return plogf_poly(x); }

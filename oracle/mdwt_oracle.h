@@ -16,7 +16,7 @@
  *
  * Definitions that the reference leaves to hardware or to its un-vendored OWL
  * dependency are fixed here once and followed by the HIP kernels:
- *   - RNG: Philox4x32-10 keyed by (seed, pixel), counter (frame, sample, block,
+ *   - RNG: Philox4x32-7 keyed by (seed, pixel), counter (frame, sample, block,
  *     stream) instead of OWL's LCG<4> keyed by threadIdx;
  *   - log(1-u): a portable fp32 log (Cody-Waite reduction + atanh series);
  *   - texture filtering: fp32-weight trilinear / linear lerp a + f*(b-a)
@@ -87,7 +87,10 @@ int oracle_clip_rays(oracle_renderer* o, const float* org, const float* dir, int
                      float* t01, int32_t* hit);
 
 /* unit hooks */
-void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Philox4x32 with `rounds` rounds (the renderer's streams use
+ * oracle_philox_rounds(), 7; 10 checks the published known answers) */
+void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]);
+int oracle_philox_rounds(void);
 float oracle_logf(float x);
 float oracle_logf_poly(float x);
 float oracle_expf(float x);

@@ -54,7 +54,8 @@ _SIG = {
     "oracle_trace_cells": ([_h, _fp, _fp, C.c_int, C.c_int, _ip, _fp, _ip], C.c_int),
     "oracle_gen_rays": ([_h, C.c_uint32, C.c_uint32, _fp, _fp], C.c_int),
     "oracle_clip_rays": ([_h, _fp, _fp, C.c_int, _fp, _ip], C.c_int),
-    "oracle_philox4x32_10": ([_up, _up, _up], None),
+    "oracle_philox4x32": ([_up, _up, C.c_int, _up], None),
+    "oracle_philox_rounds": ([], C.c_int),
     "oracle_logf": ([C.c_float], C.c_float),
     "oracle_logf_poly": ([C.c_float], C.c_float),
     "oracle_expf": ([C.c_float], C.c_float),
@@ -81,6 +82,7 @@ def _load() -> C.CDLL:
 
 
 lib = _load()
+PHILOX_ROUNDS = int(lib.oracle_philox_rounds())
 
 
 def _f(a):
@@ -99,11 +101,13 @@ class OracleError(RuntimeError):
     pass
 
 
-def philox4x32_10(ctr, key) -> np.ndarray:
+def philox4x32(ctr, key, rounds=None) -> np.ndarray:
+    """One Philox4x32 block; rounds defaults to the renderer's (7)."""
     c = np.ascontiguousarray(ctr, np.uint32)
     k = np.ascontiguousarray(key, np.uint32)
     out = np.zeros(4, np.uint32)
-    lib.oracle_philox4x32_10(c.ctypes.data_as(_up), k.ctypes.data_as(_up), out.ctypes.data_as(_up))
+    n = PHILOX_ROUNDS if rounds is None else int(rounds)
+    lib.oracle_philox4x32(c.ctypes.data_as(_up), k.ctypes.data_as(_up), n, out.ctypes.data_as(_up))
     return out
 
 

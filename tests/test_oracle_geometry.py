@@ -163,7 +163,7 @@ def test_camera_rays_follow_philox_jitter():
     cam = r.get_camera().astype(np.float64)
     for i in [0, 5, 17, 64, 127]:
         px, py = i % 16, i // 16
-        w = po.philox4x32_10([3, 1, 0, 0], [99, i])[0]
+        w = po.philox4x32([3, 1, 0, 0], [99, i])[0]
         jit = (int(w) >> 8) * 2.0 ** -24
         p = cam[1] + (px + jit) / 16 * cam[2] + (py + jit) / 8 * cam[3]
         e = (p - cam[0]) / np.linalg.norm(p - cam[0])

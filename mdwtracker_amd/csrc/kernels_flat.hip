@@ -32,7 +32,9 @@ struct Lane {
   int px, py, s;
   uint32_t pixel;
   float ar, ag, ab;  // pixel sums over samples
-  uint32_t k;        // next free-flight step of the current sample
+  uint32_t k;        // next free-flight step since the last stream seek
+  uint32_t base;     // stream block offset (-m 0 / -m 2: visit * 16 + channel)
+  uint32_t nv;       // macrocells entered so far (-m 0: in the channel's pass)
   uint32_t b0, b1, b2, b3;
   V3 wdir;
   float w2g, g2w, t1w;
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
       L.k = cur + 1u;
       if ((cur & 1u) == 0u) {
         uint32_t o[4];
-        philox(frame, uint32_t(L.s), 1u + (cur >> 1), 0u, seed, L.pixel, o);
+        philox(frame, uint32_t(L.s), 1u + L.base, (cur >> 1) << 1, seed, L.pixel, o);
         L.b0 = o[0];
         L.b1 = o[1];
         L.b2 = o[2];
@@ -203,6 +205,8 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
             L.c = __builtin_ctz(L.mask);
             L.maj = cell_maj(X, L, L.c);
             L.t = L.ct0;
+            L.base = ((L.nv - 1u) << 4) | uint32_t(L.c);
+            L.k = 0;
           } else if (L.hit) {
             st = S_FINISH;
           } else {
@@ -286,6 +290,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
     // ------------------------------- -m 0: the next channel's DDA pass --
     if (MODE == 0 && st == S_PASS) {
       L.c += 1;
+      L.nv = 0;
       if (L.c >= C) {
         st = S_FINISH;
       } else {
@@ -313,6 +318,8 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
         }
       } else {
         L.k = 0;
+        L.base = 0;
+        L.nv = 0;
         L.hit = false;
         L.hc = 0;
         cnt.rays++;
@@ -364,20 +371,26 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
           const float* mp = X.maj + size_t(L.cid) * C;
           for (int c = 0; c < C; ++c) m |= (mp[c] == 0.0f ? 0u : 1u) << c;
         }
+        L.nv += 1;  // this macrocell is visit nv - 1
         if (m) {
           L.mask = m;
           L.c = __builtin_ctz(m);
           L.maj = cell_maj(X, L, L.c);
           L.t = L.ct0;
           L.minT = L.ct1;
+          L.base = ((L.nv - 1u) << 4) | uint32_t(L.c);
+          L.k = 0;
           st = S_STEP;
         } else if (!L.d.advance()) {
           st = S_FINISH;
         }
       } else if (MODE == 0) {
         L.maj = X.maj[size_t(L.cid) * C + L.c];
+        L.nv += 1;  // this macrocell is visit nv - 1 of the channel's pass
         if (L.maj != 0.00f) {
           L.t = L.ct0;
+          L.base = ((L.nv - 1u) << 4) | uint32_t(L.c);
+          L.k = 0;
           st = S_STEP;
         } else if (!L.d.advance()) {
           st = S_PASS;

@@ -65,13 +65,25 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
 // uniform in [0,1) with 24 bits: (w >> 8) * 2^-24, exact
 __device__ __forceinline__ float to_unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }
 
-// Word layout of a (pixel, frame, sample) stream: the jitter is word 0 (block
-// 0); free-flight step k owns words 4 + 2k (distance) and 5 + 2k (threshold),
-// i.e. block 1 + k/2, so one Philox block serves two consecutive steps.
+// Word layout of a (pixel, frame, sample, stream) stream: the jitter is word 0
+// of block (frame, sample, 0, stream). Free-flight steps are counted from the
+// last seek(b), or from the ray's start in the modes that never seek: step j
+// owns words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold) of block
+// (frame, sample, 1 + b, stream | 2 floor(j / 2)), so one Philox block serves
+// two consecutive steps. -m 2 seeks b = visit * 16 + channel at every
+// (macrocell, channel), -m 0 at every macrocell of a channel's pass (visit
+// counted within the pass): the lanes of a wave then all need a new block at their
+// first step in the cell, instead of at whichever step their own running
+// count happens to make even, so the block code runs for the wave about once
+// per cell and channel rather than at nearly every step.
 struct Rng {
   uint32_t k0, k1, frame, sample, stream;
-  uint32_t k, cur;
+  uint32_t k, cur, base;
   uint32_t b0, b1, b2, b3;
+  __device__ __forceinline__ void seek(uint32_t blk) {
+    base = blk;
+    k = 0;
+  }
   __device__ __forceinline__ void init(uint32_t seed, uint32_t pixel, uint32_t frame_, uint32_t sample_,
                                        uint32_t stream_) {
     k0 = seed;
@@ -81,6 +93,7 @@ struct Rng {
     stream = stream_;
     k = 0;
     cur = 0;
+    base = 0;
   }
   __device__ __forceinline__ float jitter() const {
     uint32_t o[4];
@@ -92,7 +105,7 @@ struct Rng {
     k += 1;
     if ((cur & 1u) == 0u) {
       uint32_t o[4];
-      philox(frame, sample, 1u + (cur >> 1), stream, k0, k1, o);
+      philox(frame, sample, 1u + base, stream | ((cur >> 1) << 1), k0, k1, o);
       b0 = o[0];
       b1 = o[1];
       b2 = o[2];

@@ -35,6 +35,7 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
   const bool regs = X.C <= 4;
   int cid = d.cx + d.cy * X.mx + d.cz * X.mx * X.my;
   float4 mcur = regs ? load_cell_maj(X.maj, cid, X.C) : make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t visit = 0;  // macrocells entered so far: the stream's block offset
   while (true) {
     cnt.visits++;
     float t0, t1;
@@ -49,6 +50,7 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
       const float majorant = regs ? pick(mcur, c) : X.maj[size_t(cid) * X.C + c];
       float t = t0;
       if (majorant == 0.0f) continue;
+      rng.seek((visit << 4) | uint32_t(c));
       while (true) {
         t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
         if (t >= minT) break;
@@ -82,6 +84,7 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
     d = dn;
     cid = ncid;
     mcur = mnext;
+    ++visit;
   }
 }
 
@@ -97,6 +100,7 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
   for (int c = 0; c < X.C; ++c) {
     Dda d = d0;
     float majorant = X.maj[size_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my) * X.C + c];
+    uint32_t visit = 0;
     while (true) {
       cnt.visits++;
       float t0, t1;
@@ -108,6 +112,7 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
           more ? X.maj[size_t(dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my) * X.C + c] : 0.f;
       bool stop = false;
       if (majorant != 0.00f) {
+        rng.seek((visit << 4) | uint32_t(c));
         float t = t0;
         while (true) {
           t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
@@ -145,6 +150,7 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
       if (!more) break;
       d = dn;
       majorant = mnext;
+      ++visit;
     }
   }
 }

@@ -85,45 +85,50 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
   for (int i = 0; i < 4; ++i) out[i] = c[i];
 }
 
-// One stream of uniforms in [0,1) per (pixel, frame, sample): word n is word
-// n%4 of Philox block n/4 (counter (frame, sample, n/4, stream), key (seed,
-// pixel)), converted as (w >> 8) * 2^-24. Fixed word layout: the pixel jitter
-// is word 0; free-flight step k of the ray (counted over the whole ray,
-// whichever cell or channel it is in) owns words 4 + 2k (distance) and
-// 5 + 2k (null-collision threshold, read only if the step lands inside the
-// cell). The reference draws the same quantities from one sequential LCG
-// stream (deviceCode.cu:296-298, 563, 583, 797, 830, 913, 931); fixed
-// positions let the GPU compute one Philox block per two steps.
+// One stream of uniforms in [0,1) per (pixel, frame, sample, stream): Philox
+// blocks under key (seed, pixel), words converted as (w >> 8) * 2^-24. Fixed
+// word layout: the pixel jitter is word 0 of block (frame, sample, 0, stream).
+// Free-flight steps are counted from the last seek(b), or from the ray's start
+// in the modes that never seek: step j owns words 2 (j mod 2) (distance) and
+// 2 (j mod 2) + 1 (null-collision threshold, read only if the step lands
+// inside the cell) of block (frame, sample, 1 + b, stream | 2 floor(j / 2)).
+// -m 2 seeks b = visit * 16 + channel at every (macrocell, channel) it tracks,
+// -m 0 at every macrocell of a channel's pass (visit counted within the pass).
+// The reference draws the same quantities from one sequential LCG stream
+// (deviceCode.cu:296-298, 563, 583, 797, 830, 913, 931); fixed positions let
+// the GPU compute one Philox block per two steps, and the per-cell blocks of
+// -m 2 let all lanes of a wave compute theirs at the same step.
 struct Rng {
   uint32_t key[2];
-  uint32_t ctr[4];
+  uint32_t frame, sample, stream;
   uint32_t buf[4];
-  uint32_t cached;  // block held in buf (~0u: none)
-  uint32_t k;       // next free-flight step
-  uint32_t cur;     // current step
-  Rng(uint32_t seed, uint32_t pixel, uint32_t frame, uint32_t sample, uint32_t stream) {
+  uint32_t c2 = 0, c3 = 0;  // counter words of the block held in buf
+  bool held = false;
+  uint32_t k = 0;     // next free-flight step since the last seek
+  uint32_t cur = 0;   // current step
+  uint32_t base = 0;  // block offset set by seek
+  Rng(uint32_t seed, uint32_t pixel, uint32_t frame_, uint32_t sample_, uint32_t stream_)
+      : frame(frame_), sample(sample_), stream(stream_) {
     key[0] = seed; key[1] = pixel;
-    ctr[0] = frame; ctr[1] = sample; ctr[2] = 0; ctr[3] = stream;
-    cached = ~0u;
-    k = 0;
-    cur = 0;
   }
-  float word(uint32_t n) {
-    if (n / 4 != cached) {
-      ctr[2] = n / 4;
+  const uint32_t* block(uint32_t w2, uint32_t w3) {
+    if (!held || w2 != c2 || w3 != c3) {
+      const uint32_t ctr[4] = {frame, sample, w2, w3};
       philox(ctr, key, buf);
-      cached = n / 4;
+      c2 = w2; c3 = w3; held = true;
     }
-    return float(buf[n % 4] >> 8) * 0x1p-24f;  // exact: 24-bit integer times 2^-24
+    return buf;
   }
-  float jitter() { return word(0); }
+  static float unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }  // exact
+  float jitter() { return unit(block(0u, stream)[0]); }
+  void seek(uint32_t b) { base = b; k = 0; }
   // distance uniform of the next free-flight step
   float step() {
     cur = k++;
-    return word(4 + 2 * cur);
+    return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u)]);
   }
   // threshold uniform of the current step
-  float thr() { return word(5 + 2 * cur); }
+  float thr() { return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u) + 1]); }
 };
 
 // --------------------------------------------------------------- logf -------
@@ -701,11 +706,14 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
   const float unit = o->dt;
   int curMesh = 0;
   float tMax = 1e20f;
+  uint32_t visit = 0;  // macrocells entered in this channel's pass: block offset
   auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
     prd.visits++;
     const int cid = cell_id(o, cellIdx);
+    const uint32_t vis = visit++;
     const float majorant = o->maj[size_t(cid) * C + curMesh];
     if (majorant == 0.00f) return true;
+    rng.seek((vis << 4) | uint32_t(curMesh));
     float t = t0;
     while (true) {
       t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
@@ -733,6 +741,7 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
   };
   for (int i = 0; i < C; ++i) {
     curMesh = i;
+    visit = 0;
     dda3(g.org, g.dir, 1e20f, o->mc, lambda);
   }
 }
@@ -803,14 +812,17 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
 void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
   const int C = int(o->ch.size());
   const float unit = o->dt;
+  uint32_t visit = 0;  // macrocells entered so far: the stream's block offset
   auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
     prd.visits++;
     const int cid = cell_id(o, cellIdx);
+    const uint32_t vis = visit++;
     float minT = t1;
     for (int c = 0; c < C; ++c) {
       const float majorant = o->maj[size_t(cid) * C + c];
       float t = t0;
       if (majorant == 0.0f) continue;
+      rng.seek((vis << 4) | uint32_t(c));
       while (true) {
         t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
         if (t >= minT) break;

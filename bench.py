@@ -402,7 +402,8 @@ def main() -> int:
                 "traffic": traffic, "algorithmic_bytes": round(B),
                 "measured_copy_gbps": round(bw, 1),
                 "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
-                "kernel": f"render_kernel<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>"}
+                "kernel": (f"render_kernel{'_w6' if args.mode in (0, 2) else ''}"
+                           f"<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")}
     if traffic_src:
         roofline["traffic_source"] = traffic_src
     valu = read_valu(scene.name, args.mode, kernel_ms)

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void render_batch_kernel(RenderParams P) {
 // accumulation ids are launch arguments shared by every view. With
 // item_ticks set, each workgroup records its start and end wall clock.
 template <int MODE, int DT, bool COUNT, bool LIT>
-__global__ __launch_bounds__(256) void render_views_kernel(RenderParams P) {
+__device__ __forceinline__ void render_views_body(const RenderParams& P) {
   extern __shared__ float4 s_tf[];
   __shared__ ChanTf s_ch[kMaxChannels];
   const uint64_t t_begin = P.fr.item_ticks ? wall_clock64() : 0ull;
@@ -101,6 +101,33 @@ __global__ __launch_bounds__(256) void render_views_kernel(RenderParams P) {
   }
 }
 
+// Occupancy targets as for the single-view frame kernel (kernels_render.hip):
+// six waves per SIMD for -m 0 / -m 2 without lighting, a stated three for
+// blend tracking, the natural budget otherwise.
+template <int MODE, int DT, bool COUNT, bool LIT>
+__global__ __launch_bounds__(256) void render_views_kernel(RenderParams P) {
+  render_views_body<MODE, DT, COUNT, LIT>(P);
+}
+template <int MODE, int DT, bool COUNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void render_views_kernel_w6(
+    RenderParams P) {
+  render_views_body<MODE, DT, COUNT, false>(P);
+}
+template <int MODE, int DT, bool COUNT, bool LIT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void render_views_kernel_w3(
+    RenderParams P) {
+  render_views_body<MODE, DT, COUNT, LIT>(P);
+}
+template <int MODE, int DT, bool COUNT, bool LIT>
+RenderFn views_fn() {
+  if constexpr ((MODE == 0 || MODE == 2) && !LIT)
+    return render_views_kernel_w6<MODE, DT, COUNT>;
+  else if constexpr (MODE == 4 || MODE == 5)
+    return render_views_kernel_w3<MODE, DT, COUNT, LIT>;
+  else
+    return render_views_kernel<MODE, DT, COUNT, LIT>;
+}
+
 __global__ void batch_begin_kernel(unsigned* work) {
   if (threadIdx.x == 0) work[0] = 0u;
 }
@@ -124,10 +151,10 @@ RenderFn batch_fn_dt(int dtype) {
 template <int MODE, bool COUNT, bool LIT>
 RenderFn views_fn_dt(int dtype) {
   switch (dtype) {
-    case MDWT_DT_U8: return render_views_kernel<MODE, MDWT_DT_U8, COUNT, LIT>;
-    case MDWT_DT_U16: return render_views_kernel<MODE, MDWT_DT_U16, COUNT, LIT>;
-    case MDWT_DT_F16: return render_views_kernel<MODE, MDWT_DT_F16, COUNT, LIT>;
-    default: return render_views_kernel<MODE, MDWT_DT_F32, COUNT, LIT>;
+    case MDWT_DT_U8: return views_fn<MODE, MDWT_DT_U8, COUNT, LIT>();
+    case MDWT_DT_U16: return views_fn<MODE, MDWT_DT_U16, COUNT, LIT>();
+    case MDWT_DT_F16: return views_fn<MODE, MDWT_DT_F16, COUNT, LIT>();
+    default: return views_fn<MODE, MDWT_DT_F32, COUNT, LIT>();
   }
 }
 

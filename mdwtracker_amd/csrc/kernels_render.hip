@@ -153,10 +153,24 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ sr
   for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// Blend tracking (-m 4 / 5) needs more than 128 VGPRs, so it runs at three
+// waves per SIMD whatever it is given. Without that stated, the scheduler
+// squeezed it towards four waves by issuing the blended sample's voxel loads
+// one or two at a time with a full wait in between (0.41 -> 0.9 ms per frame
+// from an unrelated four-register change); with the target stated it keeps
+// them in flight together.
+template <int MODE, int DT, bool COUNT, bool LIT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void render_kernel_w3(
+    RenderParams P) {
+  render_body<MODE, DT, COUNT, LIT>(P);
+}
+
 template <int MODE, int DT, bool COUNT, bool LIT>
 RenderFn static_fn() {
   if constexpr ((MODE == 0 || MODE == 2) && !LIT)
     return render_kernel_w6<MODE, DT, COUNT>;
+  else if constexpr (MODE == 4 || MODE == 5)
+    return render_kernel_w3<MODE, DT, COUNT, LIT>;
   else
     return render_kernel<MODE, DT, COUNT, LIT>;
 }

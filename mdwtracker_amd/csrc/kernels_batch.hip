@@ -102,8 +102,9 @@ __device__ __forceinline__ void render_views_body(const RenderParams& P) {
 }
 
 // Occupancy targets as for the single-view frame kernel (kernels_render.hip):
-// six waves per SIMD for -m 0 / -m 2 without lighting, a stated three for
-// blend tracking, the natural budget otherwise.
+// six waves per SIMD for -m 0 / -m 2 without lighting on at most four
+// channels of 1- or 2-byte voxels (wide), a stated three for blend tracking,
+// the natural budget otherwise.
 template <int MODE, int DT, bool COUNT, bool LIT>
 __global__ __launch_bounds__(256) void render_views_kernel(RenderParams P) {
   render_views_body<MODE, DT, COUNT, LIT>(P);
@@ -119,13 +120,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   render_views_body<MODE, DT, COUNT, LIT>(P);
 }
 template <int MODE, int DT, bool COUNT, bool LIT>
-RenderFn views_fn() {
-  if constexpr ((MODE == 0 || MODE == 2) && !LIT)
-    return render_views_kernel_w6<MODE, DT, COUNT>;
-  else if constexpr (MODE == 4 || MODE == 5)
-    return render_views_kernel_w3<MODE, DT, COUNT, LIT>;
-  else
+RenderFn views_fn(bool wide) {
+  if constexpr ((MODE == 0 || MODE == 2) && !LIT) {
+    if (wide) return render_views_kernel_w6<MODE, DT, COUNT>;
     return render_views_kernel<MODE, DT, COUNT, LIT>;
+  } else if constexpr (MODE == 4 || MODE == 5) {
+    return render_views_kernel_w3<MODE, DT, COUNT, LIT>;
+  } else {
+    return render_views_kernel<MODE, DT, COUNT, LIT>;
+  }
 }
 
 __global__ void batch_begin_kernel(unsigned* work) {
@@ -149,19 +152,19 @@ RenderFn batch_fn_dt(int dtype) {
 }
 
 template <int MODE, bool COUNT, bool LIT>
-RenderFn views_fn_dt(int dtype) {
+RenderFn views_fn_dt(int dtype, bool wide) {
   switch (dtype) {
-    case MDWT_DT_U8: return views_fn<MODE, MDWT_DT_U8, COUNT, LIT>();
-    case MDWT_DT_U16: return views_fn<MODE, MDWT_DT_U16, COUNT, LIT>();
-    case MDWT_DT_F16: return views_fn<MODE, MDWT_DT_F16, COUNT, LIT>();
-    default: return views_fn<MODE, MDWT_DT_F32, COUNT, LIT>();
+    case MDWT_DT_U8: return views_fn<MODE, MDWT_DT_U8, COUNT, LIT>(wide);
+    case MDWT_DT_U16: return views_fn<MODE, MDWT_DT_U16, COUNT, LIT>(wide);
+    case MDWT_DT_F16: return views_fn<MODE, MDWT_DT_F16, COUNT, LIT>(wide);
+    default: return views_fn<MODE, MDWT_DT_F32, COUNT, LIT>(wide);
   }
 }
 
 #define MDWT_VIEWS(M)                                                                 \
-  (count ? (lit ? views_fn_dt<M, true, true>(dtype) : views_fn_dt<M, true, false>(dtype)) \
-         : (lit ? views_fn_dt<M, false, true>(dtype) : views_fn_dt<M, false, false>(dtype)))
-RenderFn select_views(int mode, int dtype, bool count, bool lit) {
+  (count ? (lit ? views_fn_dt<M, true, true>(dtype, wide) : views_fn_dt<M, true, false>(dtype, wide)) \
+         : (lit ? views_fn_dt<M, false, true>(dtype, wide) : views_fn_dt<M, false, false>(dtype, wide)))
+RenderFn select_views(int mode, int dtype, bool count, bool lit, bool wide) {
   switch (mode) {
     case 0: return MDWT_VIEWS(0);
     case 1: return MDWT_VIEWS(1);
@@ -237,7 +240,8 @@ hipError_t launch_batch(const RenderParams& p, int mode, int dtype, bool count, 
 hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s) {
   if (p.fr.n_items <= 0) return hipSuccess;
   const size_t lds = render_lds_bytes(p);
-  const RenderFn f = select_views(mode, dtype, count, is_lit(p));
+  const bool wide = p.vol.channels <= 4 && (dtype == MDWT_DT_U8 || dtype == MDWT_DT_U16);
+  const RenderFn f = select_views(mode, dtype, count, is_lit(p), wide);
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(f, dim3(unsigned(p.fr.n_items)), dim3(unsigned(p.fr.bw * p.fr.bh)), lds, s, p);

@@ -162,8 +162,7 @@ __global__ __launch_bounds__(256) void render_flat(RenderParams P) {
         L.b3 = o[3];
       }
       const float u1 = to_unit((cur & 1u) ? L.b2 : L.b0);
-      // -m 3 keeps the series log, the other modes use the division-free one
-      const float lg = MODE == 3 ? plogf(1.0f - u1) : plogf_poly(1.0f - u1);
+      const float lg = plogf_poly(1.0f - u1);
       L.t = L.t - (lg / L.maj) * X.dt * L.w2g;
       if (MODE == 2) {  // altMultiMajDTCH (deviceCode.cu:800-848)
         bool next_ch = false;

@@ -138,10 +138,8 @@ __device__ __forceinline__ float plogf(float x) {
 }
 
 // The same reduction with log m = f P(f), f = m - 1, P of degree 9 in Horner
-// form with fused multiply-adds (no division): every free-flight step except
-// -m 3's uses it (that kernel sits at the register budget of 4 waves per SIMD
-// and the polynomial costs it one register). Same operations as the host side
-// (fmaf rounds once on both).
+// form with fused multiply-adds (no division): every free-flight step uses it.
+// Same operations as the host side (fmaf rounds once on both).
 __device__ __forceinline__ float plogf_poly(float x) {
   const uint32_t bits = __float_as_uint(x);
   int e = int((bits >> 23) & 0xffu) - 127;

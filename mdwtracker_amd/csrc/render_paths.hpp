@@ -173,7 +173,7 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
     if (majorant != 0.00f) {
       float t = t0;
       while (true) {
-        t = t - (plogf(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
         if (t >= t1) break;
         const float tWorld = t * g.gridToWorldT;
         const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;

@@ -159,8 +159,7 @@ inline float plogf(float x) {
 // sqrt(2)-1], P(0) = 1) in Horner form with fused multiply-adds: no division,
 // so fewer instructions per free-flight step. fmaf rounds once on the host and
 // on the GPU, so both sides agree bit for bit. Used by every free-flight step
-// (deviceCode.cu:913, 805, 676, 739, 1023) except the -m 3 one
-// (deviceCode.cu:564), which keeps plogf.
+// (deviceCode.cu:913, 805, 676, 739, 564, 1023); plogf remains for pow().
 inline float plogf_poly(float x) {
   uint32_t bits;
   std::memcpy(&bits, &x, 4);
@@ -863,7 +862,7 @@ void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, 
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;

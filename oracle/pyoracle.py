@@ -116,7 +116,7 @@ def logf(x: float) -> float:
 
 
 def logf_poly(x: float) -> float:
-    """The division-free portable log of the free-flight steps of every mode but -m 3."""
+    """The division-free portable log of every free-flight step."""
     return lib.oracle_logf_poly(float(x))
 
 

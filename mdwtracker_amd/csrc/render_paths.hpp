@@ -302,15 +302,29 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
 // blendChannels (deviceCode.cu:222-241): maxOpacityBlend (165-179) for -m 4
 // and 7, mixBlend (182-201) for -m 5 and 8, densityBlend (204-220) for -m 6;
 // (0,0,0,0) outside the unit cube.
+// The ray marchers (-m 6-8) sample every channel at every step: with at most
+// four channels all channels are sampled first, so their voxel loads are in
+// flight together instead of one channel's after another's; the blend then
+// runs in channel order on the same values. Blend tracking (-m 4/5) blends
+// only at tentative collisions and keeps the plain loop (fewer registers).
 template <int MODE, int DT>
 __device__ float4 blend_channels(const Ctx<DT>& X, V3 p) {
   if (p.x < 0.0f || p.x > 1.0f || p.y < 0.0f || p.y > 1.0f || p.z < 0.0f || p.z > 1.0f)
     return make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool pre = MODE >= 6 && X.C <= 4;
+  float4 v4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (pre) {
+    v4.x = X.sample(p, 0);
+    if (X.C > 1) v4.y = X.sample(p, 1);
+    if (X.C > 2) v4.z = X.sample(p, 2);
+    if (X.C > 3) v4.w = X.sample(p, 3);
+  }
+  auto value = [&](int c) { return pre ? pick(v4, c) : X.sample(p, c); };
   if (MODE == 6) {
     float sum = 0.0f;
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int c = 0; c < X.C; ++c) {
-      const float4 s = X.tf_rgba(X.sample(p, c), c);
+      const float4 s = X.tf_rgba(value(c), c);
       col.x = col.x + s.w * s.x;
       col.y = col.y + s.w * s.y;
       col.z = col.z + s.w * s.z;
@@ -323,14 +337,14 @@ __device__ float4 blend_channels(const Ctx<DT>& X, V3 p) {
   if (MODE == 4 || MODE == 7) {
     float4 best = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int c = 0; c < X.C; ++c) {
-      const float4 s = X.tf_rgba(X.sample(p, c), c);
+      const float4 s = X.tf_rgba(value(c), c);
       if (s.w > best.w) best = s;
     }
     return best;
   }
   float cr = 0.f, cg = 0.f, cb = 0.f, mo = 0.0f;  // -m 5 and 8
   for (int c = 0; c < X.C; ++c) {
-    const float4 s = X.tf_rgba(X.sample(p, c), c);
+    const float4 s = X.tf_rgba(value(c), c);
     cr = cr + s.x * s.w;
     cg = cg + s.y * s.w;
     cb = cb + s.z * s.w;

@@ -31,6 +31,8 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "ms/frame @1spp 1024²; time-to-RMSE 1e-3, denoised vs plain, 1 GPU"
 HBM_PEAK_GBPS = 8000.0
+# read-only bandwidth probe: 8 GiB, far beyond the 256 MB last-level cache
+READ_STREAM_BYTES = 8 << 30
 VOXEL_NAMES = {"uint8": "u8", "uint16": "u16", "float16": "f16", "float32": "f32"}
 
 
@@ -141,6 +143,16 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
 
     dn = denoise.Denoiser(scene.width, scene.height, channels=scene.channels, seed=1)
     dn.set_params(params)
+    # the denoiser reads the standard error of the pixel mean, so its
+    # accumulation keeps second moments on every frame: price its plain
+    # frames with them (plain accumulation alone runs without)
+    r.set_moments(True)
+    r.set_aov(False)
+    r.render(16)
+    r.reset_accumulation()
+    ta = time.perf_counter()
+    r.render(256)
+    ms_m2 = (time.perf_counter() - ta) * 1000.0 / 256
     r.set_aov(True)
     r.set_aov_frames(0)
     r.render(16)
@@ -165,7 +177,8 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     dn.close()
     ms_net = float(np.median(infer_ms[1:] if len(infer_ms) > 1 else infer_ms))
     kd = denoise.frames_to_target(SWEEP_LEVELS, [s["denoised_mse"] for s in sweep], target * target)
-    out = {"frame_ms_with_aov": round(ms_aov, 4), "aov_frames": n_aov,
+    out = {"frame_ms_with_aov": round(ms_aov, 4), "frame_ms_with_moments": round(ms_m2, 4),
+           "aov_frames": n_aov,
            "network_ms": round(ms_net, 3),
            "train": {"pairs": n_pairs, "size": args.dn_size, "steps": args.dn_steps,
                      "scenes": args.dn_scenes, "views": args.dn_views, "levels": list(denoise.TRAIN_LEVELS),
@@ -177,9 +190,16 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
         out.update({"frames": None, "seconds": None, "speedup": None})
     else:
         # the first n_aov frames carry the auxiliary buffers, the rest are plain
-        sec = (min(kd, n_aov) * ms_aov + max(0.0, kd - n_aov) * ms_per_frame + ms_net) * 1e-3
+        sec = (min(kd, n_aov) * ms_aov + max(0.0, kd - n_aov) * ms_m2 + ms_net) * 1e-3
+        # one-off cost: rendering the training pairs and training; it pays
+        # back after break_even_renders renders of RMSE 1e-3
+        one_off = (t1 - t0) + (t2 - t1)
+        saving = ttr["plain"]["seconds"] - sec
         out.update({"frames": int(math.ceil(kd)), "seconds": round(sec, 3),
-                    "speedup": round(ttr["plain"]["seconds"] / sec, 2)})
+                    "speedup": round(ttr["plain"]["seconds"] / sec, 2),
+                    "one_off_seconds": round(one_off, 1),
+                    "speedup_with_training_once": round(ttr["plain"]["seconds"] / (sec + one_off), 3),
+                    "break_even_renders": (int(math.ceil(one_off / saving)) if saving > 0 else None)})
     return out
 
 
@@ -362,8 +382,11 @@ def main() -> int:
         f"({time.perf_counter() - t:.1f} s)")
     r = capi.MdwtRenderer(0)
     scenes.load_scene(r, scene, args.mode)
-    bw = r.stream_copy(2 << 30, 5)
-    log(f"[bench] measured copy bandwidth {bw:.0f} GB/s (2 GiB read + 2 GiB write)")
+    copy_bw = r.stream_copy(2 << 30, 5)
+    bw = r.stream_read(READ_STREAM_BYTES, 5)
+    log(f"[bench] measured bandwidth: read-only stream {bw:.0f} GB/s over "
+        f"{READ_STREAM_BYTES >> 30} GiB (roofline denominator); copy {copy_bw:.0f} GB/s "
+        f"(2 GiB read + 2 GiB write)")
     if "views" in scene.extra:
         b = batch_bench(args, scene, r)
         line = {"metric": "ms per batched frame (views of 256..1024 px, spp 1..16); "
@@ -400,7 +423,9 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
                 "traffic": traffic, "algorithmic_bytes": round(B),
-                "measured_copy_gbps": round(bw, 1),
+                "measured_read_gbps": round(bw, 1),
+                "measured_read_bytes": READ_STREAM_BYTES,
+                "measured_copy_gbps": round(copy_bw, 1),
                 "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
                 "kernel": (f"render_kernel{'_w6' if args.mode in (0, 2) else ''}"
                            f"<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")}
@@ -419,6 +444,7 @@ def main() -> int:
             log(f"[bench] denoised time-to-RMSE 1e-3: "
                 f"{ {k: v for k, v in ttr['denoised'].items() if k != 'sweep'} }")
         del ref, ref_var
+        r.set_moments(False)
 
     extra_modes = {}
     if not args.no_extra_modes:

@@ -102,7 +102,8 @@ int mdwt_set_camera(mdwt_renderer* h, const float from[3], const float at[3], co
 /* Setdt (renderer.cpp:1460-1466) / Resetdt (renderer.cpp:1444-1453) */
 int mdwt_set_dt(mdwt_renderer* h, float dt);
 int mdwt_reset_dt(mdwt_renderer* h);
-/* fbSize/Resize (renderer.cpp:1046-1056), spp, enableAccumulation, bgColor */
+/* fbSize/Resize (renderer.cpp:1046-1056), spp, enableAccumulation, bgColor;
+ * at most 65536 pixels per side and 2^30 pixels per frame (MDWT_ERR_LIMIT) */
 int mdwt_set_frame(mdwt_renderer* h, int width, int height, int spp, int accumulate,
                    const float bg[3]);
 /* SetLightIntensity / SetAmbient (renderer.cpp:1476-1486) */
@@ -132,9 +133,17 @@ int mdwt_set_seed(mdwt_renderer* h, uint32_t seed);
 int mdwt_render(mdwt_renderer* h, int n_frames, float* ms_out);
 /* ResetAccumulation (renderer.cpp:1488-1495) */
 int mdwt_reset_accumulation(mdwt_renderer* h);
+/* Second-moment buffers on (1) or off (0, the default): when on, every frame
+ * also keeps the per-pixel running mean of the squared colour beside the
+ * running mean (an addition to the reference, which keeps only the mean,
+ * deviceCode.cu:463-469), for confidence intervals and the error estimate of
+ * time-to-RMSE; it costs 32 more bytes of HBM traffic per pixel and frame.
+ * Applies to the single view and to batched views; restarts the
+ * accumulation. No reference counterpart. */
+int mdwt_set_moments(mdwt_renderer* h, int on);
 /* accumBuffer (float4 per pixel, device-resident in the reference) and the
- * per-pixel running mean of squares (an addition, for confidence intervals);
- * either pointer may be NULL. */
+ * per-pixel running mean of squares (only with mdwt_set_moments on, else
+ * MDWT_ERR_STATE when m2 is not NULL); either pointer may be NULL. */
 int mdwt_read_accum(mdwt_renderer* h, float* rgba, float* m2);
 /* frameBuffer (uint32 RGBA8, renderer.cpp:908; make_rgba of the accumulated
  * colour, deviceCode.cu:463-469) */
@@ -257,9 +266,15 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
  * other than 0, 2 and 3 always run schedule 0. No reference counterpart (OptiX schedules the
  * reference's launch). */
 int mdwt_set_schedule(mdwt_renderer* h, int schedule);
-/* Read-dominated device copy of `bytes` bytes, `iters` times; GB/s of
- * (read + write) bytes. Measures the achievable HBM bandwidth. */
+/* Device copy of `bytes` bytes, `iters` times (one read and one write per
+ * byte); GB/s of (read + write) bytes. No reference counterpart. */
 int mdwt_stream_copy(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps);
+/* Read-only stream of `bytes` bytes (a reduction whose only writes are one
+ * float per workgroup), `iters` times; GB/s of bytes read. With at least
+ * 4 GiB the 256 MB last-level cache cannot serve it, so it measures the HBM
+ * bandwidth a read-dominated kernel reaches: the denominator of the frame
+ * kernel's roofline. No reference counterpart. */
+int mdwt_stream_read(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps);
 /* 2-D Hilbert curve of side 2^bits: index d -> (x, y) (host function, no GPU) */
 int mdwt_hilbert_d2xy(int bits, uint32_t d, uint32_t* x, uint32_t* y);
 

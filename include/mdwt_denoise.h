@@ -33,7 +33,8 @@ typedef struct mdwt_denoiser mdwt_denoiser;
 /* A denoiser for width x height images of a volume with `channels` channels
  * (1..12) on HIP device hip_device; weights initialised from seed (He
  * initialisation; output layer near zero with the Gaussian's log-weights as
- * bias). */
+ * bias). width and height: multiples of 4 up to 8192 (MDWT_ERR_ARG), at most
+ * 2^24 pixels in all (MDWT_ERR_LIMIT; e.g. 4096 x 4096). */
 int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, int channels,
                    uint32_t seed);
 int mdwt_dn_destroy(mdwt_denoiser* d);

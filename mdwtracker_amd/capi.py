@@ -68,6 +68,8 @@ SIGNATURES = {
     "mdwt_set_tile_order": [_h, C.c_int, C.c_int],
     "mdwt_set_schedule": [_h, C.c_int],
     "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
+    "mdwt_stream_read": [_h, C.c_uint64, C.c_int, _fp],
+    "mdwt_set_moments": [_h, C.c_int],
     "mdwt_hilbert_d2xy": [C.c_int, C.c_uint32, _up, _up],
     "mdwt_set_block": [_h, C.c_int],
     "mdwt_set_macrocells": [_h, _ip],
@@ -335,7 +337,9 @@ class MdwtRenderer:
         return a
 
     def set_schedule(self, schedule: int) -> None:
-        """0: one thread per pixel; 1: persistent waves (default)."""
+        """0: one thread per pixel, nested loops (default); 1: flat step loop in
+        persistent waves with lane refill; 2: flat step loop, a workgroup per
+        16x16 block; 3: wave-coherent tracking (-m 0 / -m 2 only)."""
         self._chk(lib.mdwt_set_schedule(self._h, schedule))
 
     # -- frames -------------------------------------------------------------
@@ -436,3 +440,13 @@ class MdwtRenderer:
         v = C.c_float()
         self._chk(lib.mdwt_stream_copy(self._h, nbytes, iters, C.byref(v)))
         return v.value
+
+    def stream_read(self, nbytes: int, iters: int = 10) -> float:
+        """GB/s of a read-only stream over nbytes (mdwt_stream_read)."""
+        v = C.c_float()
+        self._chk(lib.mdwt_stream_read(self._h, nbytes, iters, C.byref(v)))
+        return v.value
+
+    def set_moments(self, on: bool = True) -> None:
+        """Second-moment buffers (needed by read_accum(with_m2=True))."""
+        self._chk(lib.mdwt_set_moments(self._h, 1 if on else 0))

@@ -82,6 +82,9 @@ int alloc(mdwt_denoiser* d, T** p, size_t n) {
 
 // layer table: (cin, cout, relu); c0 reads the input (cin set per handle),
 // c10 writes the kTaps filter logits
+// largest image (pixels) a denoiser handle accepts
+constexpr size_t kMaxPixels = size_t(1) << 24;
+
 const int kLayers[11][3] = {{0, 32, 1}, {32, 32, 1}, {32, 64, 1},
                             {64, 64, 1},  {64, 96, 1}, {96, 96, 1}, {160, 64, 1},
                             {64, 64, 1},  {96, 32, 1}, {32, 32, 1}, {32, mdwt::dn::kTaps, 0}};
@@ -277,6 +280,10 @@ int mdwt_dn_create(mdwt_denoiser** out, int hip_device, int width, int height, i
   if (width < 4 || height < 4 || width % 4 || height % 4 || width > 8192 || height > 8192 ||
       channels < 1 || channels > 12)
     return MDWT_ERR_ARG;
+  // The element-wise kernels index pixel x channel in 32-bit ints; the widest
+  // full-resolution tensor has 96 channels (the level-0 concatenation), so
+  // 2^24 pixels keep every index below 2^31.
+  if (size_t(width) * size_t(height) > kMaxPixels) return MDWT_ERR_LIMIT;
   auto* d = new (std::nothrow) mdwt_denoiser();
   if (!d) return MDWT_ERR_OOM;
   d->device = hip_device;

@@ -153,6 +153,36 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ sr
   for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// Read-only stream over n float4 (the HBM bandwidth a read-dominated kernel
+// can reach): each thread sums four independent float4 loads per iteration of
+// a grid-stride loop, and each workgroup writes one partial sum, so the only
+// writes are kReadGrid floats per launch.
+__global__ __launch_bounds__(256) void read_kernel(const float4* __restrict__ src, size_t n,
+                                                   float* __restrict__ sums) {
+  __shared__ float s_part[4];
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, d = a;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const float4 x0 = src[i], x1 = src[i + stride], x2 = src[i + 2 * stride],
+                 x3 = src[i + 3 * stride];
+    a.x += x0.x; a.y += x0.y; a.z += x0.z; a.w += x0.w;
+    b.x += x1.x; b.y += x1.y; b.z += x1.z; b.w += x1.w;
+    c.x += x2.x; c.y += x2.y; c.z += x2.z; c.w += x2.w;
+    d.x += x3.x; d.y += x3.y; d.z += x3.z; d.w += x3.w;
+  }
+  for (; i < n; i += stride) {
+    const float4 x0 = src[i];
+    a.x += x0.x; a.y += x0.y; a.z += x0.z; a.w += x0.w;
+  }
+  float v = (a.x + a.y + a.z + a.w) + (b.x + b.y + b.z + b.w) + (c.x + c.y + c.z + c.w) +
+            (d.x + d.y + d.z + d.w);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+}
+
 // Blend tracking (-m 4 / 5) needs more than 128 VGPRs, so it runs at three
 // waves per SIMD whatever it is given. Without that stated, the scheduler
 // squeezed it towards four waves by issuing the blended sample's voxel loads
@@ -285,6 +315,12 @@ hipError_t launch_trace_cells(const VolumeParams& vol, const int mc[3], const fl
 hipError_t launch_rgba8(const void* accum, uint32_t* out, int n, hipStream_t s) {
   hipLaunchKernelGGL(rgba8_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
                      reinterpret_cast<const float4*>(accum), out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_read(const void* src, size_t n16, float* sums, hipStream_t s) {
+  hipLaunchKernelGGL(read_kernel, dim3(unsigned(kReadGrid)), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(src), n16, sums);
   return hipGetLastError();
 }
 

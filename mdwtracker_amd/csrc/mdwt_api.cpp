@@ -109,7 +109,8 @@ struct mdwt_renderer {
   Float4* d_tf = nullptr;
   size_t d_tf_texels = 0;
   Float4* d_accum = nullptr;
-  Float4* d_m2 = nullptr;
+  Float4* d_m2 = nullptr;             // only while moments is on
+  bool moments = false;                // second-moment buffers (mdwt_set_moments)
   size_t fb_pixels = 0;
   unsigned long long* d_counters = nullptr;
   uint32_t* d_order = nullptr;
@@ -421,9 +422,11 @@ int alloc_frame(mdwt_renderer* h) {
   free_frame(h);
   const size_t n = size_t(h->W) * h->H;
   HIPCHK(h, hipMalloc(&h->d_accum, n * sizeof(Float4)));
-  HIPCHK(h, hipMalloc(&h->d_m2, n * sizeof(Float4)));
   HIPCHK(h, hipMemsetAsync(h->d_accum, 0, n * sizeof(Float4), h->stream));
-  HIPCHK(h, hipMemsetAsync(h->d_m2, 0, n * sizeof(Float4), h->stream));
+  if (h->moments) {
+    HIPCHK(h, hipMalloc(&h->d_m2, n * sizeof(Float4)));
+    HIPCHK(h, hipMemsetAsync(h->d_m2, 0, n * sizeof(Float4), h->stream));
+  }
   h->fb_pixels = n;
   if (h->aov_on) {
     const size_t na = size_t(mdwt::aov_groups(h->C)) * n;
@@ -945,7 +948,9 @@ int mdwt_set_frame(mdwt_renderer* h, int width, int height, int spp, int accumul
                    const float bg[3]) {
   if (!h) return MDWT_ERR_ARG;
   if (width <= 0 || height <= 0 || spp <= 0) return fail(h, MDWT_ERR_ARG, "frame sizes must be positive");
-  if (width > 16384 * 4 || height > 16384 * 4) return fail(h, MDWT_ERR_LIMIT, "frame too large");
+  // pixel indices are 32-bit in the kernels: at most 2^30 pixels per frame
+  if (width > 16384 * 4 || height > 16384 * 4 || uint64_t(width) * uint64_t(height) > (1ull << 30))
+    return fail(h, MDWT_ERR_LIMIT, "frame too large (at most 2^30 pixels, 65536 per side)");
   if (bg && !finite3(bg)) return fail(h, MDWT_ERR_ARG, "non-finite background");
   const bool resize = width != h->W || height != h->H;
   h->W = width;
@@ -1071,6 +1076,7 @@ int mdwt_read_accum(mdwt_renderer* h, float* rgba, float* m2) {
   if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
   HIPCHK(h, hipSetDevice(h->device));
   const size_t bytes = h->fb_pixels * sizeof(Float4);
+  if (m2 && !h->d_m2) return fail(h, MDWT_ERR_STATE, "second moments are off (mdwt_set_moments)");
   if (rgba) HIPCHK(h, hipMemcpyAsync(rgba, h->d_accum, bytes, hipMemcpyDeviceToHost, h->stream));
   if (m2) HIPCHK(h, hipMemcpyAsync(m2, h->d_m2, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1271,6 +1277,75 @@ int mdwt_set_schedule(mdwt_renderer* h, int schedule) {
   if (schedule != 0 && (h->bw != 16 || h->bh != 16))
     return fail(h, MDWT_ERR_ARG, "the flat schedules need 256-thread (16x16) blocks");
   h->schedule = schedule;
+  return MDWT_OK;
+}
+
+int mdwt_set_moments(mdwt_renderer* h, int on) {
+  if (!h) return MDWT_ERR_ARG;
+  const bool want = on != 0;
+  if (want == h->moments) return MDWT_OK;
+  h->moments = want;
+  reset_acc(h);
+  if (!h->inited) return MDWT_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->fb_pixels) {
+    const int rc = alloc_frame(h);
+    if (rc != MDWT_OK) return rc;
+  }
+  if (!h->views.empty()) {
+    for (auto& v : h->views) {
+      const size_t bytes = size_t(v.width) * v.height * sizeof(Float4);
+      if (want && !v.m2) {
+        if (hipMalloc(&v.m2, bytes) != hipSuccess) {
+          v.m2 = nullptr;
+          return fail(h, MDWT_ERR_OOM, "view second moments");
+        }
+        HIPCHK(h, hipMemset(v.m2, 0, bytes));
+      } else if (!want && v.m2) {
+        (void)hipFree(v.m2);
+        v.m2 = nullptr;
+      }
+    }
+    HIPCHK(h, hipMemcpy(h->d_views, h->views.data(), h->views.size() * sizeof(mdwt::ViewParams),
+                        hipMemcpyHostToDevice));
+    h->batch_accum = 0;
+  }
+  return MDWT_OK;
+}
+
+int mdwt_stream_read(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps) {
+  if (!h || !gbps) return MDWT_ERR_ARG;
+  if (bytes < 16 || iters < 1) return fail(h, MDWT_ERR_ARG, "bad read size");
+  if (!h->hip_ready) {
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreate(&h->ev0));
+    HIPCHK(h, hipEventCreate(&h->ev1));
+    h->hip_ready = true;
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  const size_t n16 = size_t(bytes / 16);
+  const unsigned grid = mdwt::kReadGrid;
+  void *a = nullptr, *sums = nullptr;
+  if (hipMalloc(&a, n16 * 16) != hipSuccess) return fail(h, MDWT_ERR_OOM, "read buffer");
+  if (hipMalloc(&sums, size_t(grid) * sizeof(float)) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(h, MDWT_ERR_OOM, "read partial sums");
+  }
+  hipError_t e = hipMemsetAsync(a, 1, n16 * 16, h->stream);
+  if (e == hipSuccess) e = mdwt::launch_read(a, n16, static_cast<float*>(sums), h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+  for (int i = 0; i < iters && e == hipSuccess; ++i)
+    e = mdwt::launch_read(a, n16, static_cast<float*>(sums), h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+  (void)hipFree(a);
+  (void)hipFree(sums);
+  if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("stream_read: ") + hipGetErrorString(e));
+  *gbps = float(double(n16 * 16) * iters / (double(ms) * 1e-3) / 1e9);
   return MDWT_OK;
 }
 
@@ -1637,7 +1712,8 @@ int reorder(mdwt_renderer* h) {
     int rc = alloc_frame(h);
     if (rc != MDWT_OK) return rc;
     HIPCHK(h, hipMemcpy(h->d_accum, acc, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
-    HIPCHK(h, hipMemcpy(h->d_m2, m2, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
+    if (m2 && h->d_m2)
+      HIPCHK(h, hipMemcpy(h->d_m2, m2, h->fb_pixels * sizeof(Float4), hipMemcpyDeviceToDevice));
     (void)hipFree(acc);
     (void)hipFree(m2);
     h->accum_id = keep;
@@ -1713,14 +1789,15 @@ int mdwt_batch_set_views(mdwt_renderer* h, const mdwt_view* views, int n) {
     vp.spp = v.spp;
     vp.seed = v.seed;
     const size_t bytes = size_t(v.width) * v.height * sizeof(Float4);
-    if (hipMalloc(&vp.accum, bytes) != hipSuccess || hipMalloc(&vp.m2, bytes) != hipSuccess) {
+    if (hipMalloc(&vp.accum, bytes) != hipSuccess ||
+        (h->moments && hipMalloc(&vp.m2, bytes) != hipSuccess)) {
       h->views.push_back(vp);
       free_batch(h);
       return fail(h, MDWT_ERR_OOM, "view accumulators");
     }
     h->views.push_back(vp);
     HIPCHK(h, hipMemset(vp.accum, 0, bytes));
-    HIPCHK(h, hipMemset(vp.m2, 0, bytes));
+    if (vp.m2) HIPCHK(h, hipMemset(vp.m2, 0, bytes));
   }
   HIPCHK(h, hipMalloc(&h->d_views, h->views.size() * sizeof(mdwt::ViewParams)));
   HIPCHK(h, hipMemcpy(h->d_views, h->views.data(), h->views.size() * sizeof(mdwt::ViewParams),
@@ -1846,6 +1923,7 @@ int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const auto& v = h->views[size_t(view)];
   const size_t bytes = size_t(v.width) * v.height * sizeof(Float4);
+  if (m2 && !v.m2) return fail(h, MDWT_ERR_STATE, "second moments are off (mdwt_set_moments)");
   if (rgba) HIPCHK(h, hipMemcpy(rgba, v.accum, bytes, hipMemcpyDeviceToHost));
   if (m2) HIPCHK(h, hipMemcpy(m2, v.m2, bytes, hipMemcpyDeviceToHost));
   return MDWT_OK;

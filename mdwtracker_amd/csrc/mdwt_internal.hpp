@@ -37,5 +37,9 @@ hipError_t launch_trace_cells(const VolumeParams& vol, const int mc[3], const fl
                               int* counts, hipStream_t s);
 hipError_t launch_rgba8(const void* accum, uint32_t* out, int n, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
+// read-only reduction over n16 float4 (bandwidth probe): kReadGrid workgroups,
+// one partial sum each into sums
+constexpr unsigned kReadGrid = 4096;
+hipError_t launch_read(const void* src, size_t n16, float* sums, hipStream_t s);
 
 }  // namespace mdwt

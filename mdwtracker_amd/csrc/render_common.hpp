@@ -208,26 +208,35 @@ __device__ __forceinline__ void flush_counts(const RenderParams& P, const Counts
 // and each XCD still works along the curve, every 8th block.
 __device__ __forceinline__ uint32_t block_slot(uint32_t b, uint32_t /*n*/) { return b; }
 
-// running mean of the colour and of its square (deviceCode.cu:463-469; the
-// second moment is an addition)
+// running mean of the colour (deviceCode.cu:463-469) and, when m2 is set, of
+// its square (an addition for confidence intervals and the error estimate;
+// off by default, so a plain frame moves only the accumulator's 32 bytes per
+// pixel)
 __device__ __forceinline__ void accumulate_pixel(const RenderParams& P, void* accum, void* m2,
                                                  uint32_t accum_id, uint32_t pixel, float fr,
                                                  float fg, float fb, float fa) {
   float4* acc = reinterpret_cast<float4*>(accum) + pixel;
-  float4* sq = reinterpret_cast<float4*>(m2) + pixel;
-  const float4 q = make_float4(fr * fr, fg * fg, fb * fb, fa * fa);
   if (P.fr.accumulate) {
     const float k = float(accum_id);
     const float k1 = float(accum_id + 1);
     const float4 a = *acc;
-    const float4 m = *sq;
     *acc = make_float4((fr + k * a.x) / k1, (fg + k * a.y) / k1, (fb + k * a.z) / k1,
                        (fa + k * a.w) / k1);
-    *sq = make_float4((q.x + k * m.x) / k1, (q.y + k * m.y) / k1, (q.z + k * m.z) / k1,
-                      (q.w + k * m.w) / k1);
   } else {
     *acc = make_float4(fr, fg, fb, fa);
-    *sq = q;
+  }
+  if (m2) {
+    float4* sq = reinterpret_cast<float4*>(m2) + pixel;
+    const float4 q = make_float4(fr * fr, fg * fg, fb * fb, fa * fa);
+    if (P.fr.accumulate) {
+      const float k = float(accum_id);
+      const float k1 = float(accum_id + 1);
+      const float4 m = *sq;
+      *sq = make_float4((q.x + k * m.x) / k1, (q.y + k * m.y) / k1, (q.z + k * m.z) / k1,
+                        (q.w + k * m.w) / k1);
+    } else {
+      *sq = q;
+    }
   }
 }
 

@@ -149,6 +149,7 @@ def accumulate_inputs(r: MdwtRenderer, levels, seed: int, aov_frames: int = AOV_
     frame count of `levels` (ascending); the run continues between the
     reads, so the renderer must not be used while the generator is live.
     The auxiliary buffers accumulate over the first aov_frames frames only."""
+    r.set_moments(True)
     r.set_seed(seed)
     r.set_aov(True)
     r.set_aov_frames(aov_frames)
@@ -164,6 +165,7 @@ def accumulate_inputs(r: MdwtRenderer, levels, seed: int, aov_frames: int = AOV_
 def render_mean(r: MdwtRenderer, frames: int, seed: int):
     """(mean image [h][w][3], variance of that mean per pixel and channel) of
     a plain `frames`-frame accumulation with seed."""
+    r.set_moments(True)
     r.set_seed(seed)
     r.reset_accumulation()
     r.render(frames)

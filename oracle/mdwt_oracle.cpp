@@ -98,12 +98,20 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 // (deviceCode.cu:296-298, 563, 583, 797, 830, 913, 931); fixed positions let
 // the GPU compute one Philox block per two steps, and the per-cell blocks of
 // -m 2 let all lanes of a wave compute theirs at the same step.
+//
+// Sequential mode (the reference-faithful path, oracle_set_faithful): seek()
+// does nothing and every draw after the jitter takes the next word of one
+// running stream (word i & 3 of block (frame, sample, 1 + i / 4, stream)),
+// in the order the reference calls prd.rng(): each step's distance draw, and
+// its threshold draw only when the step lands inside the cell.
 struct Rng {
   uint32_t key[2];
   uint32_t frame, sample, stream;
   uint32_t buf[4];
   uint32_t c2 = 0, c3 = 0;  // counter words of the block held in buf
   bool held = false;
+  bool seq = false;   // one running stream (reference draw order)
+  uint32_t n = 0;     // next word of the running stream
   uint32_t k = 0;     // next free-flight step since the last seek
   uint32_t cur = 0;   // current step
   uint32_t base = 0;  // block offset set by seek
@@ -121,14 +129,26 @@ struct Rng {
   }
   static float unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }  // exact
   float jitter() { return unit(block(0u, stream)[0]); }
-  void seek(uint32_t b) { base = b; k = 0; }
+  void seek(uint32_t b) {
+    if (seq) return;
+    base = b;
+    k = 0;
+  }
+  float next() {
+    const uint32_t i = n++;
+    return unit(block(1u + (i >> 2), stream)[i & 3u]);
+  }
   // distance uniform of the next free-flight step
   float step() {
+    if (seq) return next();
     cur = k++;
     return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u)]);
   }
   // threshold uniform of the current step
-  float thr() { return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u) + 1]); }
+  float thr() {
+    if (seq) return next();
+    return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u) + 1]);
+  }
 };
 
 // --------------------------------------------------------------- logf -------
@@ -331,6 +351,9 @@ struct oracle_renderer {
   std::vector<vec4f> accum, m2;
   uint64_t counters[4] = {0, 0, 0, 0};
   int build_threads = 1;
+  // reference-faithful tracking (oracle_set_faithful): libm log, the two
+  // divisions of the TF remap, one sequential RNG stream per ray
+  bool faithful = false;
 };
 
 namespace {
@@ -549,16 +572,29 @@ bool tfs_ready(const oracle_renderer* o) {
 // constants ((f - lo) / (hi - lo), then by the TF range width) are products
 // with the correctly rounded reciprocals of those constants here, as on the
 // GPU: the remapped value moves by about an ulp, far below one texel.
+// The reference-faithful path keeps the two divisions as written.
 inline vec4f transfer_function(const oracle_renderer* o, float f, int c) {
   const float vlo = o->dom_lo[c], vhi = o->dom_hi[c];
   if (f < vlo || f > vhi) return {1.f, 0.f, 1.f, 0.f};
   const TF& tf = o->tf[c];
-  const float idw = 1.0f / (vhi - vlo), ixw = 1.0f / (tf.xf_hi - tf.xf_lo);
-  const float r1 = (f - vlo) * idw;
-  const float r2 = (r1 - tf.xf_lo) * ixw;
+  float r2;
+  if (o->faithful) {
+    r2 = ((f - vlo) / (vhi - vlo) - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  } else {
+    const float idw = 1.0f / (vhi - vlo), ixw = 1.0f / (tf.xf_hi - tf.xf_lo);
+    const float r1 = (f - vlo) * idw;
+    r2 = (r1 - tf.xf_lo) * ixw;
+  }
   vec4f xf = tf_lookup_u(tf, r2);
   xf.w = xf.w * tf.opacity_scale;
   return xf;
+}
+
+// log(1 - u) of a free-flight step (deviceCode.cu:563, 805, 913): the
+// division-free polynomial shared with the GPU, or libm's logf on the
+// reference-faithful path.
+inline float flight_log(const oracle_renderer* o, float x) {
+  return o->faithful ? std::log(x) : plogf_poly(x);
 }
 
 // deviceCode.cu:122-132 sampleVolumeTexture -> sample 91-120 (one texture):
@@ -715,7 +751,7 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
     rng.seek((vis << 4) | uint32_t(curMesh));
     float t = t0;
     while (true) {
-      t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       if (tMax < t) return false;
       const float tWorld = t * g.gridToWorldT;
@@ -768,7 +804,7 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
     if (majorantSum <= 0.0000001f) return true;
     for (int i = 0; i < C; ++i) {
       if (majorants[i] > 0.0f)
-        ts[i] = ts[i] - (plogf_poly(1.0f - rng.step()) / majorants[i]) * unit * g.worldToGridT;
+        ts[i] = ts[i] - (flight_log(o, 1.0f - rng.step()) / majorants[i]) * unit * g.worldToGridT;
       else
         ts[i] = 1e20f;
     }
@@ -800,7 +836,7 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
         }
       }
       prd.rejections++;
-      ts[sel] = ts[sel] - (plogf_poly(1.0f - rand) / majorants[sel]) * unit * g.worldToGridT;
+      ts[sel] = ts[sel] - (flight_log(o, 1.0f - rand) / majorants[sel]) * unit * g.worldToGridT;
     }
     return true;
   };
@@ -823,7 +859,7 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
       if (majorant == 0.0f) continue;
       rng.seek((vis << 4) | uint32_t(c));
       while (true) {
-        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+        t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
         if (t >= minT) break;
         const float tWorld = t * g.gridToWorldT;
         const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -862,7 +898,7 @@ void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, 
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -938,7 +974,7 @@ void trace_blend(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
     if (majorant == 0.00f) return true;
     float t = t0;
     while (true) {
-      t = t - (plogf_poly(1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -1157,6 +1193,7 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
   const uint32_t pixel = uint32_t(fbOfs);
   for (int sampleID = 0; sampleID < o->spp; ++sampleID) {
     Rng rng(o->seed, pixel, o->frame_id, uint32_t(sampleID), 0u);
+    rng.seq = o->faithful;
     const float jit = rng.jitter();
     const float su = (float(px) + jit) / float(o->W);
     const float sv = (float(py) + jit) / float(o->H);
@@ -1209,6 +1246,7 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
       if (calculateShadows) {
         // shadow ray from the hit towards the light, own RNG stream
         Rng srng(o->seed, pixel, o->frame_id, uint32_t(sampleID), 1u);
+        srng.seq = o->faithful;
         Hit sp;
         trace_woodcock(o, org + prd.tHit * dir, -o->light_dir, srng, sp);
         const float sw = sp.missed ? 0.0f : sp.rgba.w;
@@ -1415,6 +1453,11 @@ int oracle_set_frame(oracle_renderer* o, int w, int h, int spp, int accumulate, 
 int oracle_set_seed(oracle_renderer* o, uint32_t seed) {
   o->seed = seed;
   o->accum_id = 0;
+  return 0;
+}
+
+int oracle_set_faithful(oracle_renderer* o, int on) {
+  o->faithful = on != 0;
   return 0;
 }
 

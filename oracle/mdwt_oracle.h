@@ -55,6 +55,14 @@ int oracle_set_lighting(oracle_renderer* o, float light_intensity, float ambient
 int oracle_set_light(oracle_renderer* o, const float dir[3], int shadows, int gradient);
 int oracle_set_material(oracle_renderer* o, int ch, float diffuse, float specular,
                         float shininess, float gradient_step, const float spec[3]);
+/* Reference-faithful tracking on (1) / off (0, default): libm logf for the
+ * free-flight step (deviceCode.cu:563,805,913), the two divisions of the TF
+ * remap (deviceCode.cu:36-37) and one sequential RNG stream per ray in the
+ * reference's draw order (prd.rng(), deviceCode.cu:296-298,797,830) instead
+ * of the GPU-shared fixed word layout. Statistically the same estimator: it
+ * checks the default path against the reference's semantics (the default
+ * path's images are what the GPU must match bit for bit). */
+int oracle_set_faithful(oracle_renderer* o, int on);
 /* OpenMP threads for the range-grid build (result does not depend on it) */
 int oracle_set_threads(oracle_renderer* o, int n);
 /* Render n_frames frames; rows [y0, y1) only (y1 <= 0 means all rows).

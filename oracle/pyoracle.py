@@ -40,6 +40,7 @@ _SIG = {
     "oracle_set_light": ([_h, _fp, C.c_int, C.c_int], C.c_int),
     "oracle_set_material": ([_h, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _fp], C.c_int),
     "oracle_set_threads": ([_h, C.c_int], C.c_int),
+    "oracle_set_faithful": ([_h, C.c_int], C.c_int),
     "oracle_render": ([_h, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "oracle_reset_accumulation": ([_h], C.c_int),
     "oracle_read_accum": ([_h, _fp, _fp], C.c_int),
@@ -183,6 +184,11 @@ class OracleRenderer:
 
     def set_seed(self, seed):
         self._chk(lib.oracle_set_seed(self._h, seed & 0xFFFFFFFF))
+
+    def set_faithful(self, on=True):
+        """Reference-faithful tracking: libm log, the TF remap's divisions,
+        one sequential RNG stream per ray (see oracle_set_faithful)."""
+        self._chk(lib.oracle_set_faithful(self._h, int(bool(on))))
 
     def set_lighting(self, light_intensity, ambient):
         self._chk(lib.oracle_set_lighting(self._h, light_intensity, ambient))

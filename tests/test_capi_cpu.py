@@ -88,6 +88,10 @@ def test_denoiser_and_aov_entry_points_validate_without_gpu(capi):
         assert lib.mdwt_dn_create(C.byref(h), 0, w, hh, ch, 1) == 1, (w, hh, ch)
         assert not h.value
     assert lib.mdwt_dn_create(None, 0, 32, 32, 4, 1) == 1
+    # more than 2^24 pixels: 32-bit pixel x channel indices would wrap
+    for w, hh in ((8192, 4096), (8192, 8192), (4100, 4096)):
+        assert lib.mdwt_dn_create(C.byref(h), 0, w, hh, 4, 1) == 4, (w, hh)
+        assert not h.value
     n = C.c_int()
     buf = np.zeros(16, np.float32)
     fp = buf.ctypes.data_as(C.POINTER(C.c_float))
@@ -130,3 +134,26 @@ def test_hilbert_order_matches_reference_hilbert_h(capi):
         for d in range(1 << (2 * bits)):
             ref.ref_hilbert_i2c(2, bits, d, c)
             assert capi.hilbert_d2xy(bits, d) == (c[0], c[1]), (bits, d)
+
+
+def test_frame_limit_moments_and_stream_entry_points_without_gpu(capi):
+    """Frames beyond 2^30 pixels (32-bit pixel indices in the kernels) are
+    refused before any allocation; second moments are opt-in and can be
+    switched before init; the bandwidth probes validate their arguments."""
+    import ctypes as C
+    lib = capi.lib
+    r = capi.MdwtRenderer(0)
+    for w, hh in ((65536, 16385), (40000, 40000)):
+        with pytest.raises(capi.MdwtError) as e:
+            r.set_frame(w, hh)
+        assert e.value.code == 4, (w, hh)
+    r.set_frame(32768, 32768)  # exactly 2^30 pixels: accepted (nothing allocated before init)
+    r.set_moments(True)
+    r.set_moments(False)
+    assert lib.mdwt_set_moments(None, 1) == 1
+    g = C.c_float()
+    assert lib.mdwt_stream_read(None, 1 << 20, 1, C.byref(g)) == 1
+    assert lib.mdwt_stream_read(r._h, 8, 1, C.byref(g)) == 1   # fewer than 16 bytes
+    assert lib.mdwt_stream_read(r._h, 1 << 20, 0, C.byref(g)) == 1
+    assert lib.mdwt_stream_read(r._h, 1 << 20, 1, None) == 1
+    r.close()

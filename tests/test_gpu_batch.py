@@ -47,6 +47,7 @@ def _views(n, seed=3):
 
 def _single(capi, scene, mode, v, frames):
     g = capi.MdwtRenderer(0)
+    g.set_moments(True)
     scenes.load_scene(g, scene, mode)
     g.set_frame(v["width"], v["height"], v["spp"], True, (0.0, 0.0, 0.0))
     g.set_camera(v["from"], v["at"], v["up"], v["fovy"])
@@ -65,6 +66,7 @@ def test_batched_views_equal_single_view_renders(capi, scene, mode, launch):
     print(f"[batch] launch={launch} mode={mode} "
           f"views={[(v['width'], v['height'], v['spp']) for v in views]}", flush=True)
     g = capi.MdwtRenderer(0)
+    g.set_moments(True)
     scenes.load_scene(g, scene, mode)
     g.batch_set_launch(launch)
     g.batch_set_views(views)

@@ -28,6 +28,7 @@ def po():
 
 def _pair(capi, po, scene, mode):
     g = capi.MdwtRenderer(0)
+    g.set_moments(True)  # second moments compared too
     o = po.OracleRenderer()
     scenes.load_scene(g, scene, mode)
     scenes.load_scene(o, scene, mode)
@@ -116,6 +117,7 @@ def test_channel_count_extremes_ragged_bit_exact(capi, po, n_ch, mode):
     vols, tfs = _ragged_scene(n_ch, 40 + n_ch)
     print(f"[ragged] channels={n_ch} mode={mode}", flush=True)
     g, o = capi.MdwtRenderer(0), po.OracleRenderer()
+    g.set_moments(True)
     for r in (g, o):
         for v in vols:
             r.push_channel(v)
@@ -295,6 +297,7 @@ def test_schedules_agree_bit_exact(capi, mode):
     out = []
     for schedule in (0, 1, 2):
         g = capi.MdwtRenderer(0)
+        g.set_moments(True)
         g.set_schedule(schedule)
         scenes.load_scene(g, sc, mode)
         g.set_frame(sc.width + 13, sc.height - 7, 2, True, (0.05, 0.1, 0.15))

@@ -427,8 +427,9 @@ def main() -> int:
                 "measured_read_bytes": READ_STREAM_BYTES,
                 "measured_copy_gbps": round(copy_bw, 1),
                 "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
-                "kernel": (f"render_kernel{'_w6' if args.mode in (0, 2) else ''}"
-                           f"<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")}
+                "kernel": ((f"render_wave<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")
+                           if args.mode in (0, 2) and scene.channels <= 4 else
+                           f"render_kernel<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")}
     if traffic_src:
         roofline["traffic_source"] = traffic_src
     valu = read_valu(scene.name, args.mode, kernel_ms)
@@ -449,13 +450,20 @@ def main() -> int:
     extra_modes = {}
     if not args.no_extra_modes:
         for m in (0, 1, 2, 3):
-            if m == args.mode:
-                continue
             r.set_mode(m)
-            r.render(4)
-            r.reset_accumulation()
-            r.render(32)
-            extra_modes[f"m{m}_ms_per_frame"] = round(r.last_kernel_ms(), 4)
+            if m != args.mode:
+                r.render(4)
+                r.reset_accumulation()
+                r.render(32)
+                extra_modes[f"m{m}_ms_per_frame"] = round(r.last_kernel_ms(), 4)
+            if m in (0, 2):
+                # the thread-per-pixel schedule beside the default wave-coherent one
+                r.set_schedule(0)
+                r.render(4)
+                r.reset_accumulation()
+                r.render(32)
+                extra_modes[f"m{m}_thread_per_pixel_ms_per_frame"] = round(r.last_kernel_ms(), 4)
+                r.set_schedule(1)
         r.set_mode(args.mode)
 
     line = {

@@ -35,7 +35,7 @@ COMMON = [
     "-Wall",
 ]
 
-SOURCES = ["kernels_build.hip", "kernels_render.hip", "kernels_batch.hip", "kernels_flat.hip",
+SOURCES = ["kernels_build.hip", "kernels_render.hip", "kernels_batch.hip", "kernels_wave.hip",
            "kernels_denoise.hip", "mdwt_api.cpp", "denoise_api.cpp"]
 
 

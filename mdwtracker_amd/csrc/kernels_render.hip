@@ -2,7 +2,7 @@
 // ray generation, root clip, 3D-DDA over the majorant grid and the
 // per-channel Woodcock tracking of every -m mode (render_paths.hpp), plus
 // accumulation; also the test-hook kernels and the frame launcher (which also
-// dispatches the flat schedules of kernels_flat.hip).
+// dispatches the wave-coherent schedule of kernels_wave.hip).
 //
 // Reference: mainRG (deviceCode.cu:283-479) launched as one OptiX thread per
 // pixel and volumeIntersection (deviceCode.cu:1676-1726). Here OptiX is gone:
@@ -223,7 +223,7 @@ RenderFn static_fn_dt(int dtype, bool wide) {
          : (lit ? static_fn_dt<M, false, true>(dtype, wide) : static_fn_dt<M, false, false>(dtype, wide)))
 // (-m 9 has no lighting: its shadow code is commented out in the reference;
 // its LIT instance only adds the auxiliary outputs.)
-RenderFn select_render(int mode, int dtype, bool count, int schedule, bool lit, bool wide) {
+RenderFn select_render(int mode, int dtype, bool count, bool lit, bool wide) {
   switch (mode) {  // modes with the thread-per-pixel kernel only
     case 1: return MDWT_STATIC(1);
     case 4: return MDWT_STATIC(4);
@@ -234,8 +234,6 @@ RenderFn select_render(int mode, int dtype, bool count, int schedule, bool lit, 
     case 9: return MDWT_STATIC(9);
     default: break;
   }
-  if (!lit && (schedule == 1 || schedule == 2))
-    return select_flat_kernel(mode, dtype, count, schedule == 1);
   if (mode == 0) return MDWT_STATIC(0);
   if (mode == 2) return MDWT_STATIC(2);
   return MDWT_STATIC(3);
@@ -246,16 +244,6 @@ RenderFn select_render(int mode, int dtype, bool count, int schedule, bool lit, 
 
 size_t render_lds_bytes(const RenderParams& p) {
   return size_t(p.vol.channels) * size_t(p.tf.stride) * 16;
-}
-
-hipError_t persistent_blocks_per_cu(int mode, int dtype, bool count, size_t lds, int* out) {
-  const RenderFn f = select_render(mode, dtype, count, 1, false, false);
-  hipError_t e = hipSuccess;
-  if (lds > 65536 &&
-      (e = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds))) != hipSuccess)
-    return e;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, f, kThreads, lds);
 }
 
 namespace {
@@ -270,8 +258,12 @@ bool is_lit(const RenderParams& p) {
 bool six_waves(const RenderParams& p, int dtype) {
   return p.vol.channels <= 4 && (dtype == MDWT_DT_U8 || dtype == MDWT_DT_U16);
 }
+// Kernels whose static plus dynamic LDS passes 64 KiB must say so first.
 hipError_t prepare(RenderFn f, size_t lds) {
-  if (lds <= 65536) return hipSuccess;
+  hipFuncAttributes a{};
+  hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(f));
+  if (e != hipSuccess) return e;
+  if (a.sharedSizeBytes + lds <= 65536) return hipSuccess;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
 }
@@ -279,13 +271,14 @@ hipError_t prepare(RenderFn f, size_t lds) {
 
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s) {
   const size_t lds = render_lds_bytes(p);
-  const RenderFn f = select_render(mode, dtype, count, p.fr.schedule, is_lit(p), six_waves(p, dtype));
+  const bool lit = is_lit(p);
+  const RenderFn f = (p.fr.schedule == 1 && wave_schedule_applies(p, mode, count, lit))
+                         ? select_wave_kernel(mode, dtype)
+                         : select_render(mode, dtype, count, lit, six_waves(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
-  // persistent: a resident-sized grid; otherwise one workgroup per pixel block
-  const unsigned grid = unsigned(p.fr.schedule == 1 ? p.fr.grid : p.fr.n_blocks);
-  const unsigned threads = p.fr.schedule == 0 ? unsigned(p.fr.bw * p.fr.bh) : unsigned(kThreads);
-  hipLaunchKernelGGL(f, dim3(grid), dim3(threads), lds, s, p);
+  // one workgroup per pixel block
+  hipLaunchKernelGGL(f, dim3(unsigned(p.fr.n_blocks)), dim3(unsigned(p.fr.bw * p.fr.bh)), lds, s, p);
   return hipGetLastError();
 }
 

@@ -58,6 +58,11 @@ inline mdwt::ShadeParams default_shade() {
   return s;
 }
 
+// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 0 /
+// -m 2 on at most four channels (kernels_wave.hip), the thread-per-pixel
+// kernel everywhere else.
+constexpr int kDefaultSchedule = 1;
+
 struct mdwt_renderer {
   int device = 0;
   std::string err;
@@ -124,13 +129,9 @@ struct mdwt_renderer {
   bool cost_ready = false;
   bool need_resort = true;  // record costs and re-sort after the next frames
 
-  // frame-kernel schedule (0: thread per pixel, 1: persistent, 2: flat)
-  int schedule = 0;
-  unsigned* d_queues = nullptr;  // 2 x 8 pixel-queue counters
-  uint32_t launch_seq = 0;
+  // frame-kernel schedule (0: thread per pixel, 1: wave-coherent task queues)
+  int schedule = kDefaultSchedule;
   int cu_count = 0;
-  int grid_key = -1;  // (mode, count, lds) the cached grid was sized for
-  int grid = 0;
 
   // workgroup pixel block (64: 8x8, 128: 16x8, 256: 16x16 threads/pixels) and
   // the Hilbert tile (pixels) of the block order
@@ -433,8 +434,6 @@ int alloc_frame(mdwt_renderer* h) {
     HIPCHK(h, hipMalloc(&h->d_aov, na * sizeof(Float4)));
     HIPCHK(h, hipMemsetAsync(h->d_aov, 0, na * sizeof(Float4), h->stream));
   }
-  // (the flat / persistent schedules only run with 16x16 blocks: mdwt_set_block
-  // and mdwt_set_schedule keep bw = bh = 16 whenever schedule != 0)
   const int bw = h->bw, bh = h->bh;
   const int nbx = (h->W + bw - 1) / bw, nby = (h->H + bh - 1) / bh;
   std::vector<uint32_t> order = block_order(nbx, nby, bw, bh, h->tile, h->order_mode >= 1);
@@ -524,12 +523,12 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
   return v;
 }
 
-// The flat schedules cover -m 0/2/3; every other mode runs the thread-per-pixel kernel.
-// Shadow rays, gradient shading and auxiliary outputs run in the
-// thread-per-pixel kernel only.
+// The wave-coherent schedule covers -m 0 / -m 2 on at most four channels;
+// every other mode, shadow rays, gradient shading and auxiliary outputs run
+// the thread-per-pixel kernel (the launcher checks the same conditions).
 int eff_schedule(const mdwt_renderer* h) {
   if (h->sh.shadows || h->sh.gradient || h->aov_on) return 0;
-  return (h->mode == 0 || h->mode == 2 || h->mode == 3) ? h->schedule : 0;
+  return ((h->mode == 0 || h->mode == 2) && h->C <= 4) ? h->schedule : 0;
 }
 
 mdwt::RenderParams render_params(const mdwt_renderer* h) {
@@ -557,12 +556,8 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.counters = h->d_counters;
   p.fr.block_order = h->d_order;
   p.fr.n_blocks = h->n_blocks;
-  p.fr.queues = h->d_queues;
-  p.fr.launch_seq = h->launch_seq;
   p.fr.schedule = eff_schedule(h);
-  p.fr.grid = h->grid;
-  p.fr.wave_cost =
-      (h->order_mode == 2 && eff_schedule(h) == 0 && h->need_resort) ? h->d_wave_cost : nullptr;
+  p.fr.wave_cost = (h->order_mode == 2 && h->need_resort) ? h->d_wave_cost : nullptr;
   p.fr.bw = h->bw;
   p.fr.bh = h->bh;
   p.fr.views = nullptr;
@@ -575,25 +570,12 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   return p;
 }
 
-// One frame launch: sizes the persistent grid to the resident capacity
-// (cached per mode / counting / LDS size), launches, advances the queue set.
+// One frame launch (the counting pass always runs the thread-per-pixel kernel).
 int launch_frame(mdwt_renderer* h, bool count) {
   mdwt::RenderParams p = render_params(h);
-  if (eff_schedule(h) == 1) {
-    const size_t lds = mdwt::render_lds_bytes(p);
-    const int key = int(lds) * 8 + h->mode * 2 + (count ? 1 : 0);
-    if (key != h->grid_key) {
-      int per_cu = 0;
-      HIPCHK(h, mdwt::persistent_blocks_per_cu(h->mode, h->dtype, count, lds, &per_cu));
-      if (per_cu < 1) per_cu = 1;
-      h->grid = per_cu * h->cu_count;
-      h->grid_key = key;
-    }
-    p.fr.grid = h->grid;
-  }
+  if (count) p.fr.wave_cost = nullptr;
   HIPCHK(h, mdwt::launch_render(p, h->mode, h->dtype, count, h->stream));
-  h->launch_seq++;
-  if (p.fr.wave_cost && !count) h->cost_ready = true;
+  if (p.fr.wave_cost) h->cost_ready = true;
   return MDWT_OK;
 }
 
@@ -683,7 +665,7 @@ int mdwt_destroy(mdwt_renderer* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_frame(h);
     free_batch(h);
-    void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win, h->d_queues};
+    void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -817,10 +799,7 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   int rc0 = build_range_grid(h, wx, wy, wz);
   if (rc0 != MDWT_OK) return rc0;
   HIPCHK(h, hipMalloc(&h->d_counters, 4 * sizeof(unsigned long long)));
-  HIPCHK(h, hipMalloc(&h->d_queues, 16 * sizeof(unsigned)));
-  HIPCHK(h, hipMemsetAsync(h->d_queues, 0, 16 * sizeof(unsigned), h->stream));
   HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device));
-  h->launch_seq = 0;
   // the reference drops its scalar buffers after the build (renderer.cpp:738-739)
   std::vector<std::vector<uint8_t>>().swap(h->staging);
   h->cm.assign(h->C, std::vector<Float4>());
@@ -1272,10 +1251,8 @@ int mdwt_set_tile_order(mdwt_renderer* h, int order, int tile) {
 
 int mdwt_set_schedule(mdwt_renderer* h, int schedule) {
   if (!h) return MDWT_ERR_ARG;
-  if (schedule < 0 || schedule > 2)
-    return fail(h, MDWT_ERR_ARG, "schedule must be 0 (thread per pixel), 1 (persistent) or 2 (flat)");
-  if (schedule != 0 && (h->bw != 16 || h->bh != 16))
-    return fail(h, MDWT_ERR_ARG, "the flat schedules need 256-thread (16x16) blocks");
+  if (schedule < 0 || schedule > 1)
+    return fail(h, MDWT_ERR_ARG, "schedule must be 0 (thread per pixel) or 1 (wave-coherent)");
   h->schedule = schedule;
   return MDWT_OK;
 }
@@ -1735,8 +1712,6 @@ int mdwt_set_block(mdwt_renderer* h, int threads) {
     case 256: bw = 16; bh = 16; break;
     default: return fail(h, MDWT_ERR_ARG, "workgroup size must be 64, 128 or 256");
   }
-  if (h->schedule != 0 && threads != 256)
-    return fail(h, MDWT_ERR_ARG, "the flat schedules need 256-thread (16x16) blocks");
   if (bw == h->bw && bh == h->bh) return MDWT_OK;
   h->bw = bw;
   h->bh = bh;
@@ -1964,7 +1939,6 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
   const bool batch = !h->views.empty();
   int rc = batch ? batch_ready(h) : ready_to_render(h);
   if (rc != MDWT_OK) return rc;
-  if (h->schedule != 0) return fail(h, MDWT_ERR_STATE, "autotuning runs the thread-per-pixel schedule");
   const int keep_order = h->order_mode;
   const bool keep_resort = h->batch_resort;
   h->order_mode = 1;  // the tile order is what runs while timing

@@ -66,16 +66,21 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
 __device__ __forceinline__ float to_unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }
 
 // Word layout of a (pixel, frame, sample, stream) stream: the jitter is word 0
-// of block (frame, sample, 0, stream). Free-flight steps are counted from the
-// last seek(b), or from the ray's start in the modes that never seek: step j
-// owns words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold) of block
-// (frame, sample, 1 + b, stream | 2 floor(j / 2)), so one Philox block serves
-// two consecutive steps. -m 2 seeks b = visit * 16 + channel at every
-// (macrocell, channel), -m 0 at every macrocell of a channel's pass (visit
-// counted within the pass): the lanes of a wave then all need a new block at their
-// first step in the cell, instead of at whichever step their own running
-// count happens to make even, so the block code runs for the wave about once
-// per cell and channel rather than at nearly every step.
+// of block (frame, sample, 0, stream).
+// -m 0 / -m 2 track every (macrocell visit v, channel c) from the cell's
+// entry (v counts the cells of the ray's DDA walk):
+//   * the first free-flight distance of channel c is word c mod 4 of the cell
+//     block (frame, sample, 1 + 16 v + 12 + c / 4, stream): one Philox block
+//     gives the first step of up to four channels, and most (v, c) steps
+//     leave the cell at that first step;
+//   * after step j lands inside the cell, pair j of the (v, c) stream holds
+//     its threshold and the next step's distance: words 2 (j mod 2) and
+//     2 (j mod 2) + 1 of block (frame, sample, 1 + 16 v + c, stream |
+//     2 floor(j / 2)).
+// Every (v, c) step is thus a function of (v, c, j) alone, whichever lane
+// or loop computes it. The other modes count steps from the ray's start:
+// step j owns words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold)
+// of block (frame, sample, 1, stream | 2 floor(j / 2)).
 struct Rng {
   uint32_t k0, k1, frame, sample, stream;
   uint32_t k, cur, base;
@@ -114,7 +119,38 @@ struct Rng {
     return to_unit((cur & 1u) ? b2 : b0);
   }
   __device__ __forceinline__ float thr() const { return to_unit((cur & 1u) ? b3 : b1); }
+
+  // -m 0 / -m 2: the cell block of channel group g at visit v
+  __device__ __forceinline__ void cell_block(uint32_t v, uint32_t g, uint32_t o[4]) const {
+    philox(frame, sample, 1u + ((v << 4) | (12u + g)), stream, k0, k1, o);
+  }
+  // start the pairs of (visit v, channel c)
+  __device__ __forceinline__ void start_pairs(uint32_t v, uint32_t c) {
+    base = (v << 4) | c;
+    k = 0;
+  }
+  // threshold of the current step (pair k, word 0); computes the pair's block
+  __device__ __forceinline__ float pair_thr() {
+    if ((k & 1u) == 0u) {
+      uint32_t o[4];
+      philox(frame, sample, 1u + base, stream | ((k >> 1) << 1), k0, k1, o);
+      b0 = o[0];
+      b1 = o[1];
+      b2 = o[2];
+      b3 = o[3];
+    }
+    return to_unit((k & 1u) ? b2 : b0);
+  }
+  // distance of the next step (pair k, word 1); moves on to the next pair
+  __device__ __forceinline__ float pair_dist() {
+    const uint32_t p = k++;
+    return to_unit((p & 1u) ? b3 : b1);
+  }
 };
+
+__device__ __forceinline__ uint32_t pick_word(const uint32_t w[4], uint32_t i) {
+  return i == 0u ? w[0] : (i == 1u ? w[1] : (i == 2u ? w[2] : w[3]));
+}
 
 // ------------------------------------------------------------ log ----------
 // fp32 natural log for normal x > 0: Cody-Waite reduction x = 2^e m,

@@ -9,8 +9,10 @@ namespace mdwt {
 
 // host stub of a frame kernel
 using RenderFn = void (*)(RenderParams);
-// flat-loop frame kernels (kernels_flat.hip): refill = persistent schedule
-RenderFn select_flat_kernel(int mode, int dtype, bool count, bool refill);
+// wave-coherent frame kernels (kernels_wave.hip, schedule 1): -m 0 / -m 2,
+// at most four channels, no lighting, not the counting pass
+bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit);
+RenderFn select_wave_kernel(int mode, int dtype);
 
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
 hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);
@@ -26,8 +28,6 @@ hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, 
 hipError_t batch_blocks_per_cu(const RenderParams& p, int mode, int dtype, bool count, int* out);
 // dynamic LDS of a frame launch (transfer-function texels)
 size_t render_lds_bytes(const RenderParams& p);
-// resident workgroups per CU of the persistent frame kernel
-hipError_t persistent_blocks_per_cu(int mode, int dtype, bool count, size_t lds, int* out);
 hipError_t launch_gen_rays(const CameraParams& cam, int W, int H, uint32_t seed, uint32_t frame,
                            uint32_t sample, float* org, float* dir, hipStream_t s);
 hipError_t launch_clip(const VolumeParams& vol, const float* org, const float* dir, int n,

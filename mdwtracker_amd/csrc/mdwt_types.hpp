@@ -73,12 +73,7 @@ struct FrameParams {
   unsigned long long* counters;  // samples, visits, rejections, rays
   const uint32_t* block_order;   // (bx | by << 16) per 16x16 pixel block, Hilbert order
   int n_blocks;
-  // persistent schedule: two sets of 8 pixel-queue counters (one per XCD);
-  // launch k uses set k & 1 and clears set (k + 1) & 1 for the next launch
-  unsigned* queues;
-  uint32_t launch_seq;
-  int schedule;  // 0: one thread per pixel, 1: persistent waves, 2: flat loop
-  int grid;      // workgroups of the persistent launch
+  int schedule;  // 0: one thread per pixel, 1: wave-coherent (-m 0 / -m 2, <= 4 channels)
   // cost-sorted order: wall-clock ticks (100 MHz) of each wave, indexed by
   // waves_per_block * (position in block_order) + wave; nullptr = not recorded
   uint32_t* wave_cost;

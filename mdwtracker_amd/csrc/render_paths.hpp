@@ -46,13 +46,19 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
     const float4 mnext = (regs && more) ? load_cell_maj(X.maj, ncid, X.C) : mcur;
     float minT = t1;
     bool terminate = false;
+    // first distances of up to four channels from one cell block (all lanes
+    // of the wave compute it together, once per cell and channel group)
+    uint32_t cw[4];
+    if (regs) rng.cell_block(visit, 0u, cw);
     for (int c = 0; c < X.C; ++c) {
       const float majorant = regs ? pick(mcur, c) : X.maj[size_t(cid) * X.C + c];
       float t = t0;
+      if (!regs && (c & 3) == 0) rng.cell_block(visit, uint32_t(c >> 2), cw);
       if (majorant == 0.0f) continue;
-      rng.seek((visit << 4) | uint32_t(c));
+      float u = to_unit(pick_word(cw, uint32_t(c & 3)));
+      rng.start_pairs(visit, uint32_t(c));
       while (true) {
-        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - u) / majorant) * step * g.worldToGridT;
         if (t >= minT) break;
         const float tWorld = t * g.gridToWorldT;
         const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -61,7 +67,8 @@ __device__ void trace_sync(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
           if (c == X.C - 1) terminate = true;
           break;
         }
-        const float thr = rng.thr() * majorant;
+        const float thr = rng.pair_thr() * majorant;
+        u = rng.pair_dist();
         const float value = X.sample(xT, c);
         cnt.samples++;
         if (!(value != value)) {
@@ -112,10 +119,13 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
           more ? X.maj[size_t(dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my) * X.C + c] : 0.f;
       bool stop = false;
       if (majorant != 0.00f) {
-        rng.seek((visit << 4) | uint32_t(c));
+        uint32_t cw[4];
+        rng.cell_block(visit, uint32_t(c >> 2), cw);
+        float u = to_unit(pick_word(cw, uint32_t(c & 3)));
+        rng.start_pairs(visit, uint32_t(c));
         float t = t0;
         while (true) {
-          t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+          t = t - (plogf_poly(1.0f - u) / majorant) * step * g.worldToGridT;
           if (t >= t1) break;
           if (tMaxHit < t) {
             stop = true;
@@ -128,22 +138,24 @@ __device__ void trace_ndda(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng
             stop = true;
             break;
           }
-          const float thr = rng.thr() * majorant;
+          const float thr = rng.pair_thr() * majorant;
           const float value = X.sample(xT, c);
           cnt.samples++;
-          if (value != value) continue;
-          float r2;
-          const float a = X.tf_alpha(value, c, r2);
-          if (a > 0.0f && thr < a) {
-            h.missed = false;
-            h.tHit = tWorld;
-            h.channel = c;
-            X.tf_rgb(r2, c, h);
-            tMaxHit = fminf(tMaxHit, t);
-            stop = true;
-            break;
+          if (!(value != value)) {
+            float r2;
+            const float a = X.tf_alpha(value, c, r2);
+            if (a > 0.0f && thr < a) {
+              h.missed = false;
+              h.tHit = tWorld;
+              h.channel = c;
+              X.tf_rgb(r2, c, h);
+              tMaxHit = fminf(tMaxHit, t);
+              stop = true;
+              break;
+            }
+            cnt.rejections++;
           }
-          cnt.rejections++;
+          u = rng.pair_dist();
         }
       }
       if (stop) break;

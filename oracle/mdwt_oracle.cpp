@@ -88,16 +88,26 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 // One stream of uniforms in [0,1) per (pixel, frame, sample, stream): Philox
 // blocks under key (seed, pixel), words converted as (w >> 8) * 2^-24. Fixed
 // word layout: the pixel jitter is word 0 of block (frame, sample, 0, stream).
-// Free-flight steps are counted from the last seek(b), or from the ray's start
-// in the modes that never seek: step j owns words 2 (j mod 2) (distance) and
-// 2 (j mod 2) + 1 (null-collision threshold, read only if the step lands
-// inside the cell) of block (frame, sample, 1 + b, stream | 2 floor(j / 2)).
-// -m 2 seeks b = visit * 16 + channel at every (macrocell, channel) it tracks,
-// -m 0 at every macrocell of a channel's pass (visit counted within the pass).
+//
+// -m 0 and -m 2 track each (macrocell visit v, channel c) from the cell's
+// entry (v counts the cells of the ray's DDA walk, the same cells for every
+// channel pass of -m 0):
+//   * the first free-flight distance of every channel of group g = c / 4 at
+//     visit v is word c mod 4 of the cell block (frame, sample,
+//     1 + 16 v + 12 + g, stream): one block serves up to four channels, and
+//     most (v, c) steps leave the cell at that first step;
+//   * after a step j lands inside the cell, pair j of the (v, c) stream holds
+//     its null-collision threshold and the next step's distance: words
+//     2 (j mod 2) and 2 (j mod 2) + 1 of block (frame, sample, 1 + 16 v + c,
+//     stream | 2 floor(j / 2)).
+// The other modes count free-flight steps from the ray's start: step j owns
+// words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold, read only if
+// the step lands inside the cell) of block (frame, sample, 1, stream | 2
+// floor(j / 2)).
 // The reference draws the same quantities from one sequential LCG stream
 // (deviceCode.cu:296-298, 563, 583, 797, 830, 913, 931); fixed positions let
-// the GPU compute one Philox block per two steps, and the per-cell blocks of
-// -m 2 let all lanes of a wave compute theirs at the same step.
+// the GPU compute one Philox block for all first steps of a cell, and draw
+// any (v, c) step independently of the others.
 //
 // Sequential mode (the reference-faithful path, oracle_set_faithful): seek()
 // does nothing and every draw after the jitter takes the next word of one
@@ -137,6 +147,25 @@ struct Rng {
   float next() {
     const uint32_t i = n++;
     return unit(block(1u + (i >> 2), stream)[i & 3u]);
+  }
+  // -m 0 / -m 2: first distance of channel c at visit v; starts the (v, c)
+  // stream's pairs
+  float first_dist(uint32_t v, uint32_t c) {
+    if (seq) return next();
+    base = (v << 4) | c;
+    cur = 0;
+    return unit(block(1u + ((v << 4) | (12u + (c >> 2))), stream)[c & 3u]);
+  }
+  // threshold of the current step (pair cur, word 0)
+  float pair_thr() {
+    if (seq) return next();
+    return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u)]);
+  }
+  // distance of the next step (pair cur, word 1); advances to that step
+  float pair_dist() {
+    if (seq) return next();
+    const uint32_t p = cur++;
+    return unit(block(1u + base, stream | ((p >> 1) << 1))[2 * (p & 1u) + 1]);
   }
   // distance uniform of the next free-flight step
   float step() {
@@ -748,10 +777,10 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
     const uint32_t vis = visit++;
     const float majorant = o->maj[size_t(cid) * C + curMesh];
     if (majorant == 0.00f) return true;
-    rng.seek((vis << 4) | uint32_t(curMesh));
     float t = t0;
+    float u = rng.first_dist(vis, uint32_t(curMesh));
     while (true) {
-      t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (flight_log(o, 1.0f - u) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       if (tMax < t) return false;
       const float tWorld = t * g.gridToWorldT;
@@ -760,17 +789,19 @@ void trace_ndda(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
         prd.rejections++;
         return false;
       }
-      const float nullColThreshold = rng.thr() * majorant;
+      const float nullColThreshold = rng.pair_thr() * majorant;
       const float value = sample_volume(o, xTexture, curMesh);
       prd.samples++;
-      if (std::isnan(value)) continue;
-      const vec4f s = transfer_function(o, value, curMesh);
-      if (s.w > 0.0f && nullColThreshold < s.w) {
-        record_hit(prd, tWorld, s, curMesh);
-        tMax = fminf(tMax, t);
-        return false;
+      if (!std::isnan(value)) {
+        const vec4f s = transfer_function(o, value, curMesh);
+        if (s.w > 0.0f && nullColThreshold < s.w) {
+          record_hit(prd, tWorld, s, curMesh);
+          tMax = fminf(tMax, t);
+          return false;
+        }
+        prd.rejections++;
       }
-      prd.rejections++;
+      u = rng.pair_dist();
     }
     return true;
   };
@@ -857,9 +888,9 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
       const float majorant = o->maj[size_t(cid) * C + c];
       float t = t0;
       if (majorant == 0.0f) continue;
-      rng.seek((vis << 4) | uint32_t(c));
+      float u = rng.first_dist(vis, uint32_t(c));
       while (true) {
-        t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+        t = t - (flight_log(o, 1.0f - u) / majorant) * unit * g.worldToGridT;
         if (t >= minT) break;
         const float tWorld = t * g.gridToWorldT;
         const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -868,7 +899,8 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
           if (c == C - 1) return false;
           break;
         }
-        const float nullColThreshold = rng.thr() * majorant;
+        const float nullColThreshold = rng.pair_thr() * majorant;
+        u = rng.pair_dist();
         const float value = sample_volume(o, xTexture, c);
         prd.samples++;
         if (!std::isnan(value)) {

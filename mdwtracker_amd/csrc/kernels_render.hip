@@ -270,11 +270,11 @@ hipError_t prepare(RenderFn f, size_t lds) {
 }  // namespace
 
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s) {
-  const size_t lds = render_lds_bytes(p);
   const bool lit = is_lit(p);
-  const RenderFn f = (p.fr.schedule == 1 && wave_schedule_applies(p, mode, count, lit))
-                         ? select_wave_kernel(mode, dtype)
-                         : select_render(mode, dtype, count, lit, six_waves(p, dtype));
+  const bool wave = p.fr.schedule == 1 && wave_schedule_applies(p, mode, count, lit);
+  const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
+  const RenderFn f = wave ? select_wave_kernel(mode, dtype)
+                          : select_render(mode, dtype, count, lit, six_waves(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
   // one workgroup per pixel block

@@ -51,25 +51,26 @@
 namespace mdwt {
 namespace {
 
-constexpr int kRecords = 256;  // records per wave and round
+constexpr int kRecords = 192;  // records per wave and round
 constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most
-constexpr int kMinAhead = 2;
-constexpr int kStack = 320;    // sample stack: < 64 left + 64 records x 4 channels pushed
+constexpr int kStack = 128;    // sample stack: < 64 left + one channel's 64 pushes
 
 enum : uint32_t { kLeave = 0u, kBoxEnd = 1u, kHit = 2u, kPending = 3u };
 
+// Per-wave LDS. Record slot = k * n_live + rank for the k-th cell of the
+// rank-th live ray, so the lanes of a walk or a resolve step touch
+// consecutive words (no bank conflicts).
 struct WaveLds {
-  // records of the round (slot = rank * K + k for the rank-th live ray's k-th cell)
   uint32_t cid[kRecords];
   float t0[kRecords], t1[kRecords];
   uint32_t visit[kRecords];
-  uint16_t order[kRecords];  // packed list -> slot
+  uint16_t order[kRecords];  // packed list of the round's slots, ascending
   // first event of each (record, channel): 2-bit code per channel (bits 2c,
   // 2c + 1; pending entries are settled with LDS atomic AND, so lanes that
   // settle different channels of one record never overwrite each other) and
   // the event's t (grid units)
   uint32_t code[kRecords];
-  float tev[kRecords][4];
+  float4 tev[kRecords];
   // sample stack: slot | channel << 8 | pair << 10, and the step's t
   uint32_t item[kStack];
   float item_t[kStack];
@@ -83,18 +84,6 @@ struct WaveLds {
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
-// exclusive prefix sum over the wave of v (0 <= v < 32)
-__device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
-  uint32_t pre = 0;
-  total = 0;
-#pragma unroll
-  for (int b = 0; b < 5; ++b) {
-    const uint64_t m = __ballot((v >> b) & 1u);
-    pre += lane_rank(m) << b;
-    total += uint32_t(__popcll(m)) << b;
-  }
-  return pre;
-}
 // Orders the wave's LDS accesses across a phase boundary (lanes read what
 // other lanes of the same wave wrote; the waves of a workgroup never share
 // LDS, so no workgroup barrier is needed or allowed: they loop independently).
@@ -106,6 +95,15 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ void settle(uint32_t* word, int c, uint32_t cd) {
   atomicAnd(word, ~((kPending ^ cd) << (2 * c)));
 }
+__device__ __forceinline__ float comp(const float4& v, int c) {
+  return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ void set_comp(float4& v, int c, float x) {
+  if (c == 0) v.x = x;
+  else if (c == 1) v.y = x;
+  else if (c == 2) v.z = x;
+  else v.w = x;
+}
 
 template <int DT>
 __device__ __forceinline__ float4 cell_majorants(const Ctx<DT>& X, uint32_t cid) {
@@ -113,15 +111,27 @@ __device__ __forceinline__ float4 cell_majorants(const Ctx<DT>& X, uint32_t cid)
   const float* p = X.maj + size_t(cid) * X.C;
   return make_float4(p[0], X.C > 1 ? p[1] : 0.f, X.C > 2 ? p[2] : 0.f, 0.f);
 }
-__device__ __forceinline__ float pick4(const float4& m, int c) {
-  return c == 0 ? m.x : (c == 1 ? m.y : (c == 2 ? m.z : m.w));
+
+// transferFunction's opacity (deviceCode.cu:27-42) from the LDS table of
+// opacities: the same operations as Ctx::tf_alpha on the float4 table.
+__device__ __forceinline__ float alpha_lookup(const float* s_a, const ChanTf& t, float f) {
+  if (f < t.dlo || f > t.dhi) return 0.0f;
+  const float r1 = (f - t.dlo) * t.idw;
+  const float r2 = (r1 - t.xlo) * t.ixw;
+  const float x = r2 * float(t.n) - 0.5f;
+  const float fl = floorf(x);
+  const float fr = x - fl;
+  const float a = s_a[t.off + clamp_index(fl, t.n)];
+  const float b = s_a[t.off + clamp_index_plus1(fl, t.n)];
+  return (a + fr * (b - a)) * t.os;
 }
 
 // Pops up to 64 items of the sample stack (n_items) and tracks each one step
 // further: null-collision test at the item's t, then the next free-flight step.
 template <int DT>
 __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                            int K, uint32_t& n_items, V3 w2u, uint32_t sample) {
+                                            const float* s_a, uint32_t n_live, uint32_t& n_items,
+                                            V3 w2u, uint32_t sample) {
   const int lane = threadIdx.x & 63;
   const uint32_t n = n_items < 64u ? n_items : 64u;
   const uint32_t base = n_items - n;
@@ -135,11 +145,11 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   wave_sync();
   bool push = false;
   float tn = 0.f;
-  uint32_t slot = it & 0xffu;
+  const uint32_t slot = it & 0xffu;
   const int c = int((it >> 8) & 3u);
   const uint32_t j = it >> 10;
   if (act) {
-    const int rl = L.rank_lane[slot / uint32_t(K)];
+    const int rl = L.rank_lane[slot % n_live];
     const uint32_t cid = L.cid[slot];
     const float t1 = L.t1[slot];
     const uint32_t v = L.visit[slot];
@@ -155,19 +165,18 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
     const float value = X.sample(xT, c);
     bool hit = false;
     if (!(value != value)) {
-      float r2;
-      const float a = X.tf_alpha(value, c, r2);
+      const float a = alpha_lookup(s_a, X.chtf[c], value);
       hit = a > 0.0f && thr < a;
     }
     if (hit) {
-      L.tev[slot][c] = t;
+      reinterpret_cast<float*>(&L.tev[slot])[c] = t;
       settle(&L.code[slot], c, kHit);
     } else {
       tn = t - (plogf_poly(1.0f - un) / m) * X.dt * w2g;
       if (tn >= t1) {
         settle(&L.code[slot], c, kLeave);
       } else if (tn * g2w >= L.tend[rl]) {
-        L.tev[slot][c] = tn;
+        reinterpret_cast<float*>(&L.tev[slot])[c] = tn;
         settle(&L.code[slot], c, kBoxEnd);
       } else {
         push = true;
@@ -185,17 +194,41 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   n_items = base + uint32_t(__popcll(pm));
 }
 
+// Transfer-function opacities and per-channel parameters -> LDS (the caller
+// synchronises the workgroup afterwards). Colours stay in HBM: only a ray's
+// hit reads them.
+__device__ __forceinline__ void load_alpha_lds(const RenderParams& P, float* s_a, ChanTf* s_ch) {
+  const int C = P.vol.channels;
+  const float4* g = reinterpret_cast<const float4*>(P.tf.tab);
+  const int total = C * P.tf.stride;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) s_a[i] = g[i].w;
+  if (int(threadIdx.x) < C) {
+    const int c = threadIdx.x;
+    ChanTf t;
+    t.dlo = P.tf.dom_lo[c];
+    t.dhi = P.tf.dom_hi[c];
+    t.xlo = P.tf.xf_lo[c];
+    t.ixw = 1.0f / (P.tf.xf_hi[c] - P.tf.xf_lo[c]);
+    t.os = P.tf.opacity[c];
+    t.n = P.tf.n[c];
+    t.off = c * P.tf.stride;
+    t.idw = 1.0f / (P.tf.dom_hi[c] - P.tf.dom_lo[c]);
+    s_ch[c] = t;
+  }
+}
+
 template <int MODE, int DT>
 __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
-  extern __shared__ float4 s_tf[];
+  extern __shared__ float s_alpha[];
   __shared__ ChanTf s_ch[kMaxChannels];
   __shared__ WaveLds s_wave[kThreads / 64];
   const uint64_t t_begin = P.fr.wave_cost ? wall_clock64() : 0ull;
-  load_tf_lds(P, s_tf, s_ch);
+  load_alpha_lds(P, s_alpha, s_ch);
   __syncthreads();  // the only workgroup barrier: the waves run independently from here
 
   Ctx<DT> X;
-  X.init(P, s_tf, s_ch);
+  // colours of a hit are looked up in the HBM copy of the tables
+  X.init(P, reinterpret_cast<const float4*>(P.tf.tab), s_ch);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   WaveLds& L = s_wave[wave];
   const int C = X.C;
@@ -250,7 +283,7 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
       if (live == 0ull) break;
       const uint32_t n_live = uint32_t(__popcll(live));
       int K = int(uint32_t(kRecords) / n_live);
-      K = K > kMaxAhead ? kMaxAhead : (K < kMinAhead ? kMinAhead : K);
+      K = K > kMaxAhead ? kMaxAhead : K;
       const uint32_t rank = lane_rank(live);
       // ---- 1. walk: up to K cells per live ray
       uint32_t cnt = 0u;
@@ -261,7 +294,7 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           for (int k = 0; k < K; ++k) {
             float t0, t1;
             d.range(t0, t1);
-            const uint32_t sl = rank * uint32_t(K) + uint32_t(k);
+            const uint32_t sl = uint32_t(k) * n_live + rank;
             L.cid[sl] = uint32_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my);
             L.t0[sl] = t0;
             L.t1[sl] = t1;
@@ -275,9 +308,13 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           }
         }
       }
-      uint32_t total;
-      const uint32_t off = wave_prefix(cnt, total);
-      for (uint32_t k = 0; k < cnt; ++k) L.order[off + k] = uint16_t(rank * uint32_t(K) + k);
+      // the round's slots, ascending (k-major)
+      uint32_t total = 0u;
+      for (int k = 0; k < K; ++k) {
+        const uint64_t m = __ballot(uint32_t(k) < cnt);
+        if (uint32_t(k) < cnt) L.order[total + lane_rank(m)] = uint16_t(uint32_t(k) * n_live + rank);
+        total += uint32_t(__popcll(m));
+      }
       wave_sync();
       // ---- 2. enter: first step of every channel of every record
       uint32_t n_items = 0u;
@@ -285,11 +322,11 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
         const uint32_t i = b + uint32_t(lane);
         const bool act = i < total;
         uint32_t sl = 0u;
-        bool need[4] = {false, false, false, false};
-        float tq[4] = {0.f, 0.f, 0.f, 0.f};
+        float4 tq = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t need = 0u;
         if (act) {
           sl = L.order[i];
-          const int rl = L.rank_lane[sl / uint32_t(K)];
+          const int rl = L.rank_lane[sl % n_live];
           const uint32_t cid = L.cid[sl];
           const float t0 = L.t0[sl], t1 = L.t1[sl];
           const uint32_t v = L.visit[sl];
@@ -302,55 +339,57 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             uint32_t cd = kLeave;
-            float te = 0.f;
-            const float m = pick4(m4, c);
+            const float m = comp(m4, c);
             if (c < C && m != 0.0f && ((msk >> c) & 1u)) {
               const float t = t0 - (plogf_poly(1.0f - to_unit(w[c])) / m) * X.dt * w2g;
               if (t < t1) {
+                set_comp(tq, c, t);
                 if (t * g2w >= tend) {
                   cd = kBoxEnd;
-                  te = t;
                 } else {
                   cd = kPending;
-                  need[c] = true;
-                  tq[c] = t;
+                  need |= 1u << c;
                 }
               }
             }
             codes |= cd << (2 * c);
-            L.tev[sl][c] = te;
           }
           L.code[sl] = codes;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const uint64_t pm = __ballot(need[c]);
-          if (need[c]) {
-            const uint32_t pos = n_items + lane_rank(pm);
-            L.item[pos] = sl | (uint32_t(c) << 8);
-            L.item_t[pos] = tq[c];
-          }
-          n_items += uint32_t(__popcll(pm));
+          L.tev[sl] = tq;
         }
         wave_sync();
-        // ---- 3. sample: full batches as soon as there are any
-        while (n_items >= 64u) sample_pass<DT>(P, X, L, K, n_items, w2u, uint32_t(s));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const bool nc = (need >> c) & 1u;
+          const uint64_t pm = __ballot(nc);
+          if (nc) {
+            const uint32_t pos = n_items + lane_rank(pm);
+            L.item[pos] = sl | (uint32_t(c) << 8);
+            L.item_t[pos] = comp(tq, c);
+          }
+          n_items += uint32_t(__popcll(pm));
+          wave_sync();
+          // ---- 3. sample: full batches as soon as there are any
+          while (n_items >= 64u)
+            sample_pass<DT>(P, X, L, s_alpha, n_live, n_items, w2u, uint32_t(s));
+        }
       }
-      while (n_items > 0u) sample_pass<DT>(P, X, L, K, n_items, w2u, uint32_t(s));
+      while (n_items > 0u) sample_pass<DT>(P, X, L, s_alpha, n_live, n_items, w2u, uint32_t(s));
       wave_sync();
       // ---- 4. resolve: the ray's records in cell order
       if (alive) {
         bool done = false;
         for (uint32_t k = 0; k < cnt && !done; ++k) {
-          const uint32_t sl = rank * uint32_t(K) + k;
+          const uint32_t sl = k * n_live + rank;
+          const uint32_t codes = L.code[sl];
+          const float4 te = L.tev[sl];
           if (MODE == 2) {
             float minT = L.t1[sl];
             int hc = -1;
             bool term = false;
-            const uint32_t codes = L.code[sl];
             for (int c = 0; c < C; ++c) {
               const uint32_t cd = (codes >> (2 * c)) & 3u;
-              const float tc = L.tev[sl][c];
+              const float tc = comp(te, c);
               if (cd == kHit && tc < minT) {
                 hc = c;
                 minT = tc;
@@ -367,11 +406,10 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           } else {
             float best = 1e20f;
             int hc = -1;
-            const uint32_t codes = L.code[sl];
             for (int c = 0; c < C; ++c) {
               if (!((mask >> c) & 1u)) continue;
               const uint32_t cd = (codes >> (2 * c)) & 3u;
-              const float tc = L.tev[sl][c];
+              const float tc = comp(te, c);
               if (cd == kHit && tc <= best) {
                 hc = c;
                 best = tc;
@@ -438,6 +476,10 @@ bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit
 
 RenderFn select_wave_kernel(int mode, int dtype) {
   return mode == 0 ? wave_fn<0>(dtype) : wave_fn<2>(dtype);
+}
+
+size_t wave_lds_bytes(const RenderParams& p) {
+  return size_t(p.vol.channels) * size_t(p.tf.stride) * sizeof(float);
 }
 
 }  // namespace mdwt

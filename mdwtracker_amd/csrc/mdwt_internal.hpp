@@ -13,6 +13,8 @@ using RenderFn = void (*)(RenderParams);
 // at most four channels, no lighting, not the counting pass
 bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit);
 RenderFn select_wave_kernel(int mode, int dtype);
+// dynamic LDS of a wave-coherent launch (transfer-function opacities)
+size_t wave_lds_bytes(const RenderParams& p);
 
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
 hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);

@@ -358,6 +358,8 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
+    ap.add_argument("--schedule", type=int, default=1, choices=(0, 1),
+                    help="frame-kernel schedule: 1 wave-coherent (default), 0 thread per pixel")
     ap.add_argument("--rmse-frames", type=int, default=4096,
                     help="frames of the reference render for time-to-RMSE (0: skip)")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoised time-to-RMSE")
@@ -381,6 +383,7 @@ def main() -> int:
         f"{scene.volumes[0].dtype}, {scene.width}x{scene.height}, mc {scene.mc} "
         f"({time.perf_counter() - t:.1f} s)")
     r = capi.MdwtRenderer(0)
+    r.set_schedule(args.schedule)
     scenes.load_scene(r, scene, args.mode)
     copy_bw = r.stream_copy(2 << 30, 5)
     bw = r.stream_read(READ_STREAM_BYTES, 5)
@@ -428,7 +431,7 @@ def main() -> int:
                 "measured_copy_gbps": round(copy_bw, 1),
                 "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
                 "kernel": ((f"render_wave<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")
-                           if args.mode in (0, 2) and scene.channels <= 4 else
+                           if args.mode in (0, 2) and scene.channels <= 4 and args.schedule == 1 else
                            f"render_kernel<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")}
     if traffic_src:
         roofline["traffic_source"] = traffic_src
@@ -463,7 +466,7 @@ def main() -> int:
                 r.reset_accumulation()
                 r.render(32)
                 extra_modes[f"m{m}_thread_per_pixel_ms_per_frame"] = round(r.last_kernel_ms(), 4)
-                r.set_schedule(1)
+                r.set_schedule(args.schedule)
         r.set_mode(args.mode)
 
     line = {

@@ -51,34 +51,53 @@
 namespace mdwt {
 namespace {
 
-constexpr int kRecords = 192;  // records per wave and round
+constexpr int kRecords = 160;  // records per wave and round
 constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most
-constexpr int kStack = 128;    // sample stack: < 64 left + one channel's 64 pushes
+constexpr int kStack = 128;    // each stack: < 64 left + one channel's 64 pushes
+// Safety factor of the certain-exit test (see the enter phase): far above the
+// few fp32 roundings (relative 2^-21) that it has to absorb.
+constexpr float kExitSafety = 1.0001f;
 
 enum : uint32_t { kLeave = 0u, kBoxEnd = 1u, kHit = 2u, kPending = 3u };
 
+// A record: the macrocell of one (ray, visit), its entry and exit t (grid
+// units) and the visit number, read and written as one 16-byte word.
+struct Record {
+  uint32_t cid;
+  float t0, t1;
+  uint32_t visit;
+};
+// A ray of the wave (one per lane): world origin and direction, grid <-> world
+// t scales, end of the root box (world t), 1 / (dt * worldToGridT), pixel and
+// the channels still tracked (-m 0).
+struct WaveRay {
+  float4 org_w2g;   // origin, worldToGridT
+  float4 dir_g2w;   // direction, gridToWorldT
+  float tend, idw;
+  uint32_t pixel, mask;
+};
+// A stack entry: slot | channel << 8 | ray lane << 10 | pair << 16, a t or a
+// uniform, and the channel's majorant in the record's cell.
+struct Item {
+  uint32_t id;
+  float x, m;
+};
+
 // Per-wave LDS. Record slot = k * n_live + rank for the k-th cell of the
 // rank-th live ray, so the lanes of a walk or a resolve step touch
-// consecutive words (no bank conflicts).
+// consecutive records.
 struct WaveLds {
-  uint32_t cid[kRecords];
-  float t0[kRecords], t1[kRecords];
-  uint32_t visit[kRecords];
-  uint16_t order[kRecords];  // packed list of the round's slots, ascending
+  Record rec[kRecords];
+  uint16_t order[kRecords];  // the round's slots, ascending: slot | ray lane << 8
   // first event of each (record, channel): 2-bit code per channel (bits 2c,
   // 2c + 1; pending entries are settled with LDS atomic AND, so lanes that
   // settle different channels of one record never overwrite each other) and
   // the event's t (grid units)
   uint32_t code[kRecords];
   float4 tev[kRecords];
-  // sample stack: slot | channel << 8 | pair << 10, and the step's t
-  uint32_t item[kStack];
-  float item_t[kStack];
-  // live rays of the round by rank, and every lane's ray
-  uint8_t rank_lane[64];
-  uint32_t pixel[64], mask[64];
-  float w2g[64], g2w[64], tend[64];
-  float ox[64], oy[64], oz[64], dx[64], dy[64], dz[64];
+  Item sample[kStack];  // steps that need a null-collision test (x = t)
+  Item first[kStack];   // first steps to compute exactly (x = uniform)
+  WaveRay ray[64];
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
@@ -130,38 +149,33 @@ __device__ __forceinline__ float alpha_lookup(const float* s_a, const ChanTf& t,
 // further: null-collision test at the item's t, then the next free-flight step.
 template <int DT>
 __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                            const float* s_a, uint32_t n_live, uint32_t& n_items,
-                                            V3 w2u, uint32_t sample) {
+                                            const float* s_a, uint32_t& n_items, V3 w2u,
+                                            uint32_t sample) {
   const int lane = threadIdx.x & 63;
   const uint32_t n = n_items < 64u ? n_items : 64u;
   const uint32_t base = n_items - n;
   const bool act = uint32_t(lane) < n;
-  uint32_t it = 0u;
-  float t = 0.f;
-  if (act) {
-    it = L.item[base + lane];
-    t = L.item_t[base + lane];
-  }
+  Item it = {0u, 0.f, 0.f};
+  if (act) it = L.sample[base + lane];
   wave_sync();
   bool push = false;
   float tn = 0.f;
-  const uint32_t slot = it & 0xffu;
-  const int c = int((it >> 8) & 3u);
-  const uint32_t j = it >> 10;
+  const uint32_t slot = it.id & 0xffu;
+  const int c = int((it.id >> 8) & 3u);
+  const uint32_t j = it.id >> 16;
   if (act) {
-    const int rl = L.rank_lane[slot % n_live];
-    const uint32_t cid = L.cid[slot];
-    const float t1 = L.t1[slot];
-    const uint32_t v = L.visit[slot];
-    const float m = X.maj[size_t(cid) * X.C + c];
-    const float w2g = L.w2g[rl], g2w = L.g2w[rl];
+    const int rl = int((it.id >> 10) & 63u);
+    const Record R = L.rec[slot];
+    const WaveRay& Y = L.ray[rl];
+    const float4 ow = Y.org_w2g, dg = Y.dir_g2w;
+    const float t = it.x, m = it.m;
     uint32_t w[4];
-    philox(P.fr.frame_id, sample, 1u + ((v << 4) | uint32_t(c)), (j >> 1) << 1,
-           P.fr.seed, L.pixel[rl], w);
+    philox(P.fr.frame_id, sample, 1u + ((R.visit << 4) | uint32_t(c)), (j >> 1) << 1,
+           P.fr.seed, Y.pixel, w);
     const float thr = to_unit((j & 1u) ? w[2] : w[0]) * m;
     const float un = to_unit((j & 1u) ? w[3] : w[1]);
-    const float tWorld = t * g2w;
-    const V3 xT = (v3(L.ox[rl], L.oy[rl], L.oz[rl]) + tWorld * v3(L.dx[rl], L.dy[rl], L.dz[rl])) * w2u;
+    const float tWorld = t * dg.w;
+    const V3 xT = (v3(ow.x, ow.y, ow.z) + tWorld * v3(dg.x, dg.y, dg.z)) * w2u;
     const float value = X.sample(xT, c);
     bool hit = false;
     if (!(value != value)) {
@@ -172,10 +186,10 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
       reinterpret_cast<float*>(&L.tev[slot])[c] = t;
       settle(&L.code[slot], c, kHit);
     } else {
-      tn = t - (plogf_poly(1.0f - un) / m) * X.dt * w2g;
-      if (tn >= t1) {
+      tn = t - (plogf_poly(1.0f - un) / m) * X.dt * ow.w;
+      if (tn >= R.t1) {
         settle(&L.code[slot], c, kLeave);
-      } else if (tn * g2w >= L.tend[rl]) {
+      } else if (tn * dg.w >= Y.tend) {
         reinterpret_cast<float*>(&L.tev[slot])[c] = tn;
         settle(&L.code[slot], c, kBoxEnd);
       } else {
@@ -186,12 +200,60 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   wave_sync();
   const uint64_t pm = __ballot(push);
   if (push) {
-    const uint32_t pos = base + lane_rank(pm);
-    L.item[pos] = slot | (uint32_t(c) << 8) | ((j + 1u) << 10);
-    L.item_t[pos] = tn;
+    Item o;
+    o.id = (it.id & 0xffffu) | ((j + 1u) << 16);
+    o.x = tn;
+    o.m = it.m;
+    L.sample[base + lane_rank(pm)] = o;
   }
   wave_sync();
   n_items = base + uint32_t(__popcll(pm));
+}
+
+// Pops up to 64 items of the first-step stack (n_first): the first free-flight
+// step of (record, channel) computed exactly, as the sequential loop does;
+// steps that land inside the cell and before the box's end go on the sample
+// stack.
+__device__ __forceinline__ void first_pass(float dt, WaveLds& L, uint32_t& n_first,
+                                           uint32_t& n_items) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t n = n_first < 64u ? n_first : 64u;
+  const uint32_t base = n_first - n;
+  const bool act = uint32_t(lane) < n;
+  Item it = {0u, 0.f, 0.f};
+  if (act) it = L.first[base + lane];
+  wave_sync();
+  bool push = false;
+  float t = 0.f;
+  const uint32_t slot = it.id & 0xffu;
+  const int c = int((it.id >> 8) & 3u);
+  if (act) {
+    const int rl = int((it.id >> 10) & 63u);
+    const Record R = L.rec[slot];
+    const WaveRay& Y = L.ray[rl];
+    const float w2g = Y.org_w2g.w, g2w = Y.dir_g2w.w;
+    t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * dt * w2g;
+    if (t >= R.t1) {
+      settle(&L.code[slot], c, kLeave);
+    } else if (t * g2w >= Y.tend) {
+      reinterpret_cast<float*>(&L.tev[slot])[c] = t;
+      settle(&L.code[slot], c, kBoxEnd);
+    } else {
+      push = true;
+    }
+  }
+  wave_sync();
+  const uint64_t pm = __ballot(push);
+  if (push) {
+    Item o;
+    o.id = it.id & 0xffffu;  // pair 0
+    o.x = t;
+    o.m = it.m;
+    L.sample[n_items + lane_rank(pm)] = o;
+  }
+  wave_sync();
+  n_first = base;
+  n_items += uint32_t(__popcll(pm));
 }
 
 // Transfer-function opacities and per-channel parameters -> LDS (the caller
@@ -250,6 +312,7 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
     // ---- ray set-up (mainRG, deviceCode.cu:293-321; the program's grid ray)
     Dda d;
     bool alive = false;
+    float idw = 0.f;
     {
       Rng rng;
       rng.init(P.fr.seed, pixel, P.fr.frame_id, uint32_t(s), 0u);
@@ -257,21 +320,22 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
       V3 org, dir;
       camera_ray(P.cam, px, py, P.fr.width, P.fr.height, jit, org, dir);
       float c0, c1;
+      WaveRay Y;
+      Y.org_w2g = make_float4(org.x, org.y, org.z, 0.f);
+      Y.dir_g2w = make_float4(dir.x, dir.y, dir.z, 0.f);
+      Y.tend = 0.f;
       if (valid && clip_root(org, dir, blo, bhi, c0, c1)) {
         const GridRay g = make_grid_ray(org, dir, blo, bhi, X.mx, X.my, X.mz);
         alive = d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz);
-        L.w2g[lane] = g.worldToGridT;
-        L.g2w[lane] = g.gridToWorldT;
-        L.tend[lane] = c1;
-        L.ox[lane] = org.x;
-        L.oy[lane] = org.y;
-        L.oz[lane] = org.z;
-        L.dx[lane] = dir.x;
-        L.dy[lane] = dir.y;
-        L.dz[lane] = dir.z;
+        Y.org_w2g.w = g.worldToGridT;
+        Y.dir_g2w.w = g.gridToWorldT;
+        Y.tend = c1;
+        idw = 1.0f / (X.dt * g.worldToGridT);
       }
-      L.pixel[lane] = pixel;
-      L.mask[lane] = all_ch;
+      Y.idw = idw;
+      Y.pixel = pixel;
+      Y.mask = all_ch;
+      L.ray[lane] = Y;
     }
     bool exhausted = false;
     uint32_t visit = 0u, mask = all_ch;
@@ -288,17 +352,17 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
       // ---- 1. walk: up to K cells per live ray
       uint32_t cnt = 0u;
       if (alive) {
-        L.rank_lane[rank] = uint8_t(lane);
-        L.mask[lane] = mask;
+        L.ray[lane].mask = mask;
         if (!exhausted) {
           for (int k = 0; k < K; ++k) {
             float t0, t1;
             d.range(t0, t1);
-            const uint32_t sl = uint32_t(k) * n_live + rank;
-            L.cid[sl] = uint32_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my);
-            L.t0[sl] = t0;
-            L.t1[sl] = t1;
-            L.visit[sl] = visit;
+            Record R;
+            R.cid = uint32_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my);
+            R.t0 = t0;
+            R.t1 = t1;
+            R.visit = visit;
+            L.rec[uint32_t(k) * n_live + rank] = R;
             ++visit;
             ++cnt;
             if (!d.advance()) {
@@ -308,54 +372,57 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           }
         }
       }
-      // the round's slots, ascending (k-major)
+      // the round's slots, ascending (k-major), each with its ray's lane
       uint32_t total = 0u;
       for (int k = 0; k < K; ++k) {
         const uint64_t m = __ballot(uint32_t(k) < cnt);
-        if (uint32_t(k) < cnt) L.order[total + lane_rank(m)] = uint16_t(uint32_t(k) * n_live + rank);
+        if (uint32_t(k) < cnt)
+          L.order[total + lane_rank(m)] = uint16_t((uint32_t(k) * n_live + rank) | (uint32_t(lane) << 8));
         total += uint32_t(__popcll(m));
       }
       wave_sync();
-      // ---- 2. enter: first step of every channel of every record
-      uint32_t n_items = 0u;
+      // ---- 2. enter: first step of every channel of every record. The
+      // sequential loop's first step leaves the cell when
+      //   t0 - (log(1 - u) / m) dt w2g >= t1   (fp32, each operation rounded).
+      // Since -log(1 - u) >= u, the portable log is within 2^-22 of log, and
+      // each rounding moves a value by at most 2^-24 relative, the step
+      // certainly leaves when
+      //   u >= ((t1 - t0) + t1 2^-23) m / (dt w2g) (1 + 2^-21),
+      // which the test below over-estimates by the safety factor. Most steps
+      // pass it with a few instructions; the rest are computed exactly, in
+      // full batches, from the first-step stack.
+      uint32_t n_items = 0u, n_first = 0u;
       for (uint32_t b = 0; b < total; b += 64u) {
         const uint32_t i = b + uint32_t(lane);
         const bool act = i < total;
-        uint32_t sl = 0u;
-        float4 tq = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t head = 0u;
+        float4 uq = make_float4(0.f, 0.f, 0.f, 0.f), m4 = uq;
         uint32_t need = 0u;
         if (act) {
-          sl = L.order[i];
-          const int rl = L.rank_lane[sl % n_live];
-          const uint32_t cid = L.cid[sl];
-          const float t0 = L.t0[sl], t1 = L.t1[sl];
-          const uint32_t v = L.visit[sl];
-          const float w2g = L.w2g[rl], g2w = L.g2w[rl], tend = L.tend[rl];
-          const uint32_t msk = L.mask[rl];
-          const float4 m4 = cell_majorants(X, cid);
+          const uint32_t o = L.order[i];
+          const uint32_t sl = o & 0xffu, rl = o >> 8;
+          head = sl | (rl << 10);
+          const Record R = L.rec[sl];
+          const WaveRay& Y = L.ray[rl];
+          const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
+          m4 = cell_majorants(X, R.cid);
           uint32_t w[4];
-          philox(P.fr.frame_id, uint32_t(s), 1u + ((v << 4) | 12u), 0u, P.fr.seed, L.pixel[rl], w);
+          philox(P.fr.frame_id, uint32_t(s), 1u + ((R.visit << 4) | 12u), 0u, P.fr.seed, Y.pixel, w);
+          const uint32_t msk = Y.mask;
           uint32_t codes = 0u;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            uint32_t cd = kLeave;
             const float m = comp(m4, c);
             if (c < C && m != 0.0f && ((msk >> c) & 1u)) {
-              const float t = t0 - (plogf_poly(1.0f - to_unit(w[c])) / m) * X.dt * w2g;
-              if (t < t1) {
-                set_comp(tq, c, t);
-                if (t * g2w >= tend) {
-                  cd = kBoxEnd;
-                } else {
-                  cd = kPending;
-                  need |= 1u << c;
-                }
+              const float u = to_unit(w[c]);
+              if (!(u >= q * m)) {
+                set_comp(uq, c, u);
+                need |= 1u << c;
+                codes |= kPending << (2 * c);
               }
             }
-            codes |= cd << (2 * c);
           }
           L.code[sl] = codes;
-          L.tev[sl] = tq;
         }
         wave_sync();
 #pragma unroll
@@ -363,18 +430,27 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           const bool nc = (need >> c) & 1u;
           const uint64_t pm = __ballot(nc);
           if (nc) {
-            const uint32_t pos = n_items + lane_rank(pm);
-            L.item[pos] = sl | (uint32_t(c) << 8);
-            L.item_t[pos] = comp(tq, c);
+            Item e;
+            e.id = head | (uint32_t(c) << 8);
+            e.x = comp(uq, c);
+            e.m = comp(m4, c);
+            L.first[n_first + lane_rank(pm)] = e;
           }
-          n_items += uint32_t(__popcll(pm));
+          n_first += uint32_t(__popcll(pm));
           wave_sync();
-          // ---- 3. sample: full batches as soon as there are any
-          while (n_items >= 64u)
-            sample_pass<DT>(P, X, L, s_alpha, n_live, n_items, w2u, uint32_t(s));
+          // full batches as soon as there are any
+          while (n_first >= 64u) {
+            first_pass(X.dt, L, n_first, n_items);
+            // ---- 3. sample
+            while (n_items >= 64u) sample_pass<DT>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+          }
         }
       }
-      while (n_items > 0u) sample_pass<DT>(P, X, L, s_alpha, n_live, n_items, w2u, uint32_t(s));
+      while (n_first > 0u) {
+        first_pass(X.dt, L, n_first, n_items);
+        while (n_items >= 64u) sample_pass<DT>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+      }
+      while (n_items > 0u) sample_pass<DT>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
       wave_sync();
       // ---- 4. resolve: the ray's records in cell order
       if (alive) {
@@ -382,9 +458,10 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
         for (uint32_t k = 0; k < cnt && !done; ++k) {
           const uint32_t sl = k * n_live + rank;
           const uint32_t codes = L.code[sl];
+          if (codes == 0u) continue;  // every channel left the cell
           const float4 te = L.tev[sl];
           if (MODE == 2) {
-            float minT = L.t1[sl];
+            float minT = L.rec[sl].t1;
             int hc = -1;
             bool term = false;
             for (int c = 0; c < C; ++c) {
@@ -436,10 +513,10 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
     h.channel = hit_c;
     h.r = h.g = h.b = 0.f;
     if (hit) {
-      const float g2w = L.g2w[lane];
-      const float tWorld = hit_t * g2w;
-      const V3 xT = (v3(L.ox[lane], L.oy[lane], L.oz[lane]) +
-                     tWorld * v3(L.dx[lane], L.dy[lane], L.dz[lane])) * w2u;
+      const WaveRay& Y = L.ray[lane];
+      const float tWorld = hit_t * Y.dir_g2w.w;
+      const V3 xT = (v3(Y.org_w2g.x, Y.org_w2g.y, Y.org_w2g.z) +
+                     tWorld * v3(Y.dir_g2w.x, Y.dir_g2w.y, Y.dir_g2w.z)) * w2u;
       float r2;
       (void)X.tf_alpha(X.sample(xT, hit_c), hit_c, r2);
       X.tf_rgb(r2, hit_c, h);

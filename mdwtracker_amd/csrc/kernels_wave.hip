@@ -145,9 +145,54 @@ __device__ __forceinline__ float alpha_lookup(const float* s_a, const ChanTf& t,
   return (a + fr * (b - a)) * t.os;
 }
 
-// Pops up to 64 items of the sample stack (n_items) and tracks each one step
-// further: null-collision test at the item's t, then the next free-flight step.
+// The null-collision test of a step that landed at t inside its cell (before
+// the root box's end), then the next free-flight step; pair j of the
+// (visit, channel) stream. Returns true when the step needs another test (tn
+// inside the cell and before the box's end); otherwise the (record, channel)
+// is settled here.
 template <int DT>
+__device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
+                                          const float* s_a, V3 w2u, uint32_t sample,
+                                          uint32_t slot, int c, const Record& R,
+                                          const WaveRay& Y, uint32_t j, float t, float m,
+                                          float& tn) {
+  const float4 ow = Y.org_w2g, dg = Y.dir_g2w;
+  uint32_t w[4];
+  philox(P.fr.frame_id, sample, 1u + ((R.visit << 4) | uint32_t(c)), (j >> 1) << 1, P.fr.seed,
+         Y.pixel, w);
+  const float thr = to_unit((j & 1u) ? w[2] : w[0]) * m;
+  const float un = to_unit((j & 1u) ? w[3] : w[1]);
+  const float tWorld = t * dg.w;
+  const V3 xT = (v3(ow.x, ow.y, ow.z) + tWorld * v3(dg.x, dg.y, dg.z)) * w2u;
+  const float value = X.sample(xT, c);
+  bool hit = false;
+  if (!(value != value)) {
+    const float a = alpha_lookup(s_a, X.chtf[c], value);
+    hit = a > 0.0f && thr < a;
+  }
+  if (hit) {
+    reinterpret_cast<float*>(&L.tev[slot])[c] = t;
+    settle(&L.code[slot], c, kHit);
+    return false;
+  }
+  tn = t - (plogf_poly(1.0f - un) / m) * X.dt * ow.w;
+  if (tn >= R.t1) {
+    settle(&L.code[slot], c, kLeave);
+    return false;
+  }
+  if (tn * dg.w >= Y.tend) {
+    reinterpret_cast<float*>(&L.tev[slot])[c] = tn;
+    settle(&L.code[slot], c, kBoxEnd);
+    return false;
+  }
+  return true;
+}
+
+// Pops up to 64 items of the sample stack (n_items) and tests each. Full
+// passes push the next step back; the last pass of a round (DRAIN, fewer
+// than 64 items) keeps every lane on its item until it is settled, so the
+// rare long runs of null collisions cost loop iterations instead of passes.
+template <int DT, bool DRAIN>
 __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                             const float* s_a, uint32_t& n_items, V3 w2u,
                                             uint32_t sample) {
@@ -160,48 +205,31 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   wave_sync();
   bool push = false;
   float tn = 0.f;
-  const uint32_t slot = it.id & 0xffu;
-  const int c = int((it.id >> 8) & 3u);
-  const uint32_t j = it.id >> 16;
+  uint32_t j = it.id >> 16;
   if (act) {
-    const int rl = int((it.id >> 10) & 63u);
+    const uint32_t slot = it.id & 0xffu;
+    const int c = int((it.id >> 8) & 3u);
     const Record R = L.rec[slot];
-    const WaveRay& Y = L.ray[rl];
-    const float4 ow = Y.org_w2g, dg = Y.dir_g2w;
-    const float t = it.x, m = it.m;
-    uint32_t w[4];
-    philox(P.fr.frame_id, sample, 1u + ((R.visit << 4) | uint32_t(c)), (j >> 1) << 1,
-           P.fr.seed, Y.pixel, w);
-    const float thr = to_unit((j & 1u) ? w[2] : w[0]) * m;
-    const float un = to_unit((j & 1u) ? w[3] : w[1]);
-    const float tWorld = t * dg.w;
-    const V3 xT = (v3(ow.x, ow.y, ow.z) + tWorld * v3(dg.x, dg.y, dg.z)) * w2u;
-    const float value = X.sample(xT, c);
-    bool hit = false;
-    if (!(value != value)) {
-      const float a = alpha_lookup(s_a, X.chtf[c], value);
-      hit = a > 0.0f && thr < a;
-    }
-    if (hit) {
-      reinterpret_cast<float*>(&L.tev[slot])[c] = t;
-      settle(&L.code[slot], c, kHit);
-    } else {
-      tn = t - (plogf_poly(1.0f - un) / m) * X.dt * ow.w;
-      if (tn >= R.t1) {
-        settle(&L.code[slot], c, kLeave);
-      } else if (tn * dg.w >= Y.tend) {
-        reinterpret_cast<float*>(&L.tev[slot])[c] = tn;
-        settle(&L.code[slot], c, kBoxEnd);
-      } else {
+    const WaveRay Y = L.ray[(it.id >> 10) & 63u];
+    float t = it.x;
+    while (null_test<DT>(P, X, L, s_a, w2u, sample, slot, c, R, Y, j, t, it.m, tn)) {
+      ++j;
+      if (!DRAIN) {
         push = true;
+        break;
       }
+      t = tn;
     }
   }
   wave_sync();
+  if (DRAIN) {
+    n_items = base;
+    return;
+  }
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = (it.id & 0xffffu) | ((j + 1u) << 16);
+    o.id = (it.id & 0xffffu) | (j << 16);
     o.x = tn;
     o.m = it.m;
     L.sample[base + lane_rank(pm)] = o;
@@ -211,11 +239,14 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
 }
 
 // Pops up to 64 items of the first-step stack (n_first): the first free-flight
-// step of (record, channel) computed exactly, as the sequential loop does;
-// steps that land inside the cell and before the box's end go on the sample
-// stack.
-__device__ __forceinline__ void first_pass(float dt, WaveLds& L, uint32_t& n_first,
-                                           uint32_t& n_items) {
+// step of (record, channel) computed exactly, as the sequential loop does,
+// and, when it lands inside the cell before the box's end (nearly always for
+// these items), its null-collision test; a rejected step's successor goes on
+// the sample stack.
+template <int DT>
+__device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
+                                           const float* s_a, uint32_t& n_first,
+                                           uint32_t& n_items, V3 w2u, uint32_t sample) {
   const int lane = threadIdx.x & 63;
   const uint32_t n = n_first < 64u ? n_first : 64u;
   const uint32_t base = n_first - n;
@@ -224,30 +255,28 @@ __device__ __forceinline__ void first_pass(float dt, WaveLds& L, uint32_t& n_fir
   if (act) it = L.first[base + lane];
   wave_sync();
   bool push = false;
-  float t = 0.f;
-  const uint32_t slot = it.id & 0xffu;
-  const int c = int((it.id >> 8) & 3u);
+  float tn = 0.f;
   if (act) {
-    const int rl = int((it.id >> 10) & 63u);
+    const uint32_t slot = it.id & 0xffu;
+    const int c = int((it.id >> 8) & 3u);
     const Record R = L.rec[slot];
-    const WaveRay& Y = L.ray[rl];
-    const float w2g = Y.org_w2g.w, g2w = Y.dir_g2w.w;
-    t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * dt * w2g;
+    const WaveRay Y = L.ray[(it.id >> 10) & 63u];
+    const float t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
     if (t >= R.t1) {
       settle(&L.code[slot], c, kLeave);
-    } else if (t * g2w >= Y.tend) {
+    } else if (t * Y.dir_g2w.w >= Y.tend) {
       reinterpret_cast<float*>(&L.tev[slot])[c] = t;
       settle(&L.code[slot], c, kBoxEnd);
     } else {
-      push = true;
+      push = null_test<DT>(P, X, L, s_a, w2u, sample, slot, c, R, Y, 0u, t, it.m, tn);
     }
   }
   wave_sync();
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = it.id & 0xffffu;  // pair 0
-    o.x = t;
+    o.id = (it.id & 0xffffu) | (1u << 16);  // pair 1
+    o.x = tn;
     o.m = it.m;
     L.sample[n_items + lane_rank(pm)] = o;
   }
@@ -440,17 +469,18 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           wave_sync();
           // full batches as soon as there are any
           while (n_first >= 64u) {
-            first_pass(X.dt, L, n_first, n_items);
+            first_pass<DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
             // ---- 3. sample
-            while (n_items >= 64u) sample_pass<DT>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+            while (n_items >= 64u)
+              sample_pass<DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
           }
         }
       }
-      while (n_first > 0u) {
-        first_pass(X.dt, L, n_first, n_items);
-        while (n_items >= 64u) sample_pass<DT>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+      if (n_first > 0u) {  // fewer than 64 left
+        first_pass<DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
+        while (n_items >= 64u) sample_pass<DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
       }
-      while (n_items > 0u) sample_pass<DT>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+      if (n_items > 0u) sample_pass<DT, true>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
       wave_sync();
       // ---- 4. resolve: the ray's records in cell order
       if (alive) {

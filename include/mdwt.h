@@ -148,6 +148,13 @@ int mdwt_read_accum(mdwt_renderer* h, float* rgba, float* m2);
 /* frameBuffer (uint32 RGBA8, renderer.cpp:908; make_rgba of the accumulated
  * colour, deviceCode.cu:463-469) */
 int mdwt_read_fb(mdwt_renderer* h, uint32_t* rgba8);
+/* mdwt_read_fb without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under graph capture,
+ * tight serving loops). The RGBA8 conversion is staged in a static device
+ * array of 2^20 pixels per GPU (4 MiB, held until the process ends): frames
+ * of more pixels return MDWT_ERR_LIMIT. Calls that overlap on one GPU share
+ * that array and race. Same output as mdwt_read_fb. */
+int mdwt_read_fb_noalloc(mdwt_renderer* h, uint32_t* rgba8);
 
 /* ---- inspection hooks (tests, bench) ---- */
 /* value-range grid: float2 (min, max) per (cell, channel), index cell*C + ch */

@@ -70,6 +70,7 @@ SIGNATURES = {
     "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_stream_read": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_set_moments": [_h, C.c_int],
+    "mdwt_read_fb_noalloc": [_h, _up],
     "mdwt_hilbert_d2xy": [C.c_int, C.c_uint32, _up, _up],
     "mdwt_set_block": [_h, C.c_int],
     "mdwt_set_macrocells": [_h, _ip],

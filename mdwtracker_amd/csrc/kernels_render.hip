@@ -11,11 +11,26 @@
 // same macrocells); blocks are issued in Hilbert or cost-sorted order, dealt
 // round-robin over the eight XCDs; transfer functions sit in LDS; voxels are
 // read in their native dtype from planar channels in HBM.
+#include <mutex>
+
 #include "mdwt_internal.hpp"
 #include "render_paths.hpp"
 
 namespace mdwt {
+
+// Staging area of the allocation-free frame-buffer read
+// (mdwt_read_fb_noalloc): RGBA8 of up to kFbScratchPixels pixels. Every GPU
+// the library runs on holds its own copy until the process ends; calls that
+// overlap on one GPU share it and race. Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address): it
+// uses the per-device address that fb_scratch_setup takes from
+// hipGetSymbolAddress.
+__device__ __align__(256) uint32_t scratch[kFbScratchPixels];
+
 namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {};
 
 // LIT: shadow rays, gradient shading and the denoiser's auxiliary outputs
 // compiled in. The other instances keep the register budget of the plain
@@ -303,6 +318,23 @@ hipError_t launch_trace_cells(const VolumeParams& vol, const int mc[3], const fl
   hipLaunchKernelGGL(trace_cells_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, vol,
                      mc[0], mc[1], mc[2], org, dir, n, max_cells, cells, t01, counts);
   return hipGetLastError();
+}
+
+hipError_t fb_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[device] = static_cast<uint32_t*>(p);
+  return hipSuccess;
+}
+
+uint32_t* fb_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch[device];
 }
 
 hipError_t launch_rgba8(const void* accum, uint32_t* out, int n, hipStream_t s) {

@@ -800,6 +800,8 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
   if (rc0 != MDWT_OK) return rc0;
   HIPCHK(h, hipMalloc(&h->d_counters, 4 * sizeof(unsigned long long)));
   HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device));
+  // this device's copy of the allocation-free read's staging array
+  HIPCHK(h, mdwt::fb_scratch_setup(h->device));
   // the reference drops its scalar buffers after the build (renderer.cpp:738-739)
   std::vector<std::vector<uint8_t>>().swap(h->staging);
   h->cm.assign(h->C, std::vector<Float4>());
@@ -1075,6 +1077,27 @@ int mdwt_read_fb(mdwt_renderer* h, uint32_t* rgba8) {
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(h, MDWT_ERR_HIP, std::string("read_fb: ") + hipGetErrorString(e));
+  return MDWT_OK;
+}
+
+int mdwt_read_fb_noalloc(mdwt_renderer* h, uint32_t* rgba8) {
+  if (!h) return MDWT_ERR_ARG;
+  if (!h->inited) return fail(h, MDWT_ERR_STATE, "renderer not initialized");
+  if (!rgba8) return fail(h, MDWT_ERR_ARG, "null output");
+  if (h->fb_pixels > mdwt::kFbScratchPixels)
+    return fail(h, MDWT_ERR_LIMIT,
+                "read_fb_noalloc: frame of " + std::to_string(h->fb_pixels) +
+                    " pixels exceeds the static staging area of " +
+                    std::to_string(mdwt::kFbScratchPixels) + " pixels");
+  HIPCHK(h, hipSetDevice(h->device));
+  uint32_t* d = mdwt::fb_scratch(h->device);  // set up by mdwt_init on this device
+  if (!d) return fail(h, MDWT_ERR_STATE, "read_fb_noalloc: staging area not set up on this device");
+  hipError_t e = mdwt::launch_rgba8(h->d_accum, d, int(h->fb_pixels), h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rgba8, d, h->fb_pixels * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess)
+    return fail(h, MDWT_ERR_HIP, std::string("read_fb_noalloc: ") + hipGetErrorString(e));
   return MDWT_OK;
 }
 

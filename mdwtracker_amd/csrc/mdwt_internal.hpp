@@ -38,6 +38,13 @@ hipError_t launch_trace_cells(const VolumeParams& vol, const int mc[3], const fl
                               const float* dir, int n, int max_cells, int* cells, float* t01,
                               int* counts, hipStream_t s);
 hipError_t launch_rgba8(const void* accum, uint32_t* out, int n, hipStream_t s);
+// Allocation-free frame-buffer read: a file-scope device array of
+// kFbScratchPixels RGBA8 pixels per GPU. fb_scratch_setup (once per device,
+// with that device current; called by mdwt_init) takes its address with
+// hipGetSymbolAddress; fb_scratch returns it (nullptr before the set-up).
+constexpr size_t kFbScratchPixels = size_t(1) << 20;
+hipError_t fb_scratch_setup(int device);
+uint32_t* fb_scratch(int device);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 // read-only reduction over n16 float4 (bandwidth probe): kReadGrid workgroups,
 // one partial sum each into sums

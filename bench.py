@@ -288,8 +288,9 @@ def batch_bench(args, scene, r) -> dict:
     """Batched views: the autotuner's sweep over macrocell size, workgroup
     and Hilbert tile on the batch, then `--steps` batched frames at the best
     setting in each launch form (a launch per view over four streams; one
-    launch for all views): ms per batched frame, aggregate samples/s,
-    per-view latency. `value` is the faster form."""
+    launch for all views; one persistent launch per frame replayed from a
+    captured HIP graph): ms per batched frame, aggregate samples/s, per-view
+    latency. `value` is the fastest form."""
     import numpy as np
 
     n_views = int(scene.extra.get("views", 64))
@@ -302,7 +303,7 @@ def batch_bench(args, scene, r) -> dict:
     log(f"[bench] autotune ({time.perf_counter() - t:.1f} s): best s={best[0]} threads={best[1]} "
         f"tile={best[2]}")
     forms = {}
-    for launch, name in ((0, "per_view_streams"), (1, "one_launch")):
+    for launch, name in ((0, "per_view_streams"), (1, "one_launch"), (2, "persistent_graph")):
         r.batch_set_launch(launch)
         counts = r.batch_count()
         r.batch_render(args.warmup)

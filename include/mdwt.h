@@ -219,16 +219,17 @@ int mdwt_read_aov(mdwt_renderer* h, float* out);
 
 /* ---- batched views (no reference counterpart: the reference renders one
  * camera per launch). Many views of uneven size and spp, each with its own
- * accumulators, rendered frame by frame, either with one frame-kernel launch
- * per view, the views spread round-robin over four HIP streams so that small
- * views run side by side (launch form 0, the default), or with one launch per
- * frame whose grid holds every view's pixel blocks (launch form 1). A
- * persistent one-launch variant exists in the library but is switched off
- * until a hang seen on its first GPU run is explained. Calls
- * are ordered with the handle's other work and return when the frames are
- * done. A view's image equals a single-view render with the same
- * camera, size, spp and seed (the RNG is keyed by pixel), so views never
- * depend on the batch they are in. */
+ * accumulators, rendered frame by frame: with one frame-kernel launch per
+ * view, the views spread round-robin over four HIP streams so that small
+ * views run side by side (launch form 0, the default); with one launch per
+ * frame whose grid holds every view's pixel blocks (launch form 1); or with
+ * one persistent launch per frame (a resident grid, each workgroup taking
+ * every G-th pixel block of the batch) that is captured once in a HIP graph
+ * together with the kernel advancing the frame ids, and replayed every frame
+ * (launch form 2). Calls are ordered with the handle's other work and return
+ * when the frames are done. A view's image equals a single-view render with
+ * the same camera, size, spp and seed (the RNG is keyed by pixel), so views
+ * never depend on the batch they are in. */
 typedef struct mdwt_view {
   float from[3], at[3], up[3]; /* camera.h setOrientation arguments */
   float fovy;                  /* degrees */
@@ -246,7 +247,9 @@ int mdwt_batch_reset(mdwt_renderer* h);
 /* Launch form of the batch: 0 = one launch per view over four streams (each
  * view finishes as soon as its own blocks do: low per-view latency), 1 = one
  * launch for all views per frame (every view finishes with the frame: highest
- * throughput when views are small). Images and counters do not depend on it. */
+ * throughput when views are small), 2 = one persistent launch per frame
+ * replayed from a captured HIP graph (captured again when the launch's
+ * parameters change). Images and counters do not depend on it. */
 int mdwt_batch_set_launch(mdwt_renderer* h, int launch);
 /* width * height float4 running means (and second moments) of one view. */
 int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2);

@@ -296,7 +296,8 @@ class MdwtRenderer:
         self._chk(lib.mdwt_batch_reset(self._h))
 
     def batch_set_launch(self, launch: int) -> None:
-        """0: one launch per view over four streams; 1: one launch for all views."""
+        """0: one launch per view over four streams; 1: one launch for all views;
+        2: one persistent launch for all views, replayed from a captured HIP graph."""
         self._chk(lib.mdwt_batch_set_launch(self._h, int(launch)))
 
     def batch_read_view(self, i: int, with_m2: bool = False):

@@ -142,16 +142,15 @@ struct mdwt_renderer {
   mdwt::ViewParams* d_views = nullptr;
   std::vector<uint32_t> items;           // (view << 20) | (by << 10) | bx, issue order
   uint32_t* d_items = nullptr;
-  unsigned* d_work = nullptr;            // item counter, frame id, accumulation id
+  unsigned* d_work = nullptr;            // [1] frame id, [2] accumulation id of launch form 2
   unsigned long long* d_item_ticks = nullptr;
-  bool batch_sorted = false;             // items re-sorted by measured cost
-  bool batch_resort = true;              // sort after the first recorded frame
   uint32_t batch_frame = 0, batch_accum = 0;
-  int batch_grid = 0, batch_grid_key = -1;
-  hipGraph_t graph = nullptr;
+  int batch_grid = 0, batch_grid_key = -1;  // resident grid of launch form 2
+  hipGraph_t graph = nullptr;               // launch form 2: one captured frame
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
   mdwt::RenderParams graph_params;  // what the captured graph launches
+  int graph_grid = 0;
   // per-view launches (the batch path in use): each view's block order and
   // block count, events around each view's launch, the last frame's times
   std::vector<uint32_t*> view_order;
@@ -160,7 +159,9 @@ struct mdwt_renderer {
   std::vector<hipEvent_t> join_ev;               // end of each batch stream's work
   std::vector<hipStream_t> batch_streams;        // views run round-robin on these
   std::vector<float> view_us;
-  int batch_launch = 0;  // 0: a launch per view over the batch streams, 1: one launch for all
+  // 0: a launch per view over the batch streams, 1: one launch for all, 2: one
+  // persistent launch for all, replayed from a captured HIP graph
+  int batch_launch = 0;
 };
 
 namespace {
@@ -1394,12 +1395,6 @@ int mdwt_hilbert_d2xy(int bits, uint32_t d, uint32_t* x, uint32_t* y) {
 // ===================================================== batched views ======
 namespace {
 
-// The persistent one-launch batch (kernels_batch.hip, graph capture and
-// replay) is switched off: its first GPU run hung and the cause is not yet
-// shown. Batches render with one single-view launch per view, or with one
-// ordinary (non-persistent) launch whose grid holds every view's blocks.
-constexpr int kPersistentBatch = 0;
-
 void free_batch(mdwt_renderer* h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -1482,7 +1477,6 @@ int build_items(mdwt_renderer* h) {
   HIPCHK(h, hipMemcpy(h->d_items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(h->d_item_ticks, 0, 2 * items.size() * sizeof(unsigned long long)));
   h->items = std::move(items);
-  h->batch_sorted = false;
   return build_view_orders(h);
 }
 
@@ -1633,51 +1627,81 @@ int batch_render_one_launch(mdwt_renderer* h, int n_frames, float* ms_out) {
   return MDWT_OK;
 }
 
-mdwt::RenderParams batch_params(const mdwt_renderer* h) {
-  mdwt::RenderParams p = render_params(h);
-  p.fr.schedule = 0;
-  p.fr.wave_cost = nullptr;
-  p.fr.views = h->d_views;
-  p.fr.items = h->d_items;
-  p.fr.n_items = int(h->items.size());
+// Launch form 2: the one-launch parameters with the device frame and
+// accumulation ids (the captured graph advances them itself) and the per-item
+// wall clocks.
+mdwt::RenderParams persistent_params(const mdwt_renderer* h) {
+  mdwt::RenderParams p = views_params(h);
+  p.fr.frame_id = 0;  // read from work[1] / work[2] instead
+  p.fr.accum_id = 0;
   p.fr.work = h->d_work;
   p.fr.item_ticks = h->d_item_ticks;
-  p.fr.aov = nullptr;  // batched views have no auxiliary buffers
   return p;
 }
 
-// resident workgroups of the persistent batched launch (cached per kernel
-// choice and LDS size)
-int batch_grid(mdwt_renderer* h, const mdwt::RenderParams& p, bool count, int* grid) {
+// resident workgroups of the persistent launch (cached per kernel choice),
+// at most one per item
+int persistent_grid(mdwt_renderer* h, const mdwt::RenderParams& p, int* grid) {
   const bool lit = p.fr.sh.shadows || p.fr.sh.gradient || p.fr.aov != nullptr;
   const int key = int(mdwt::render_lds_bytes(p)) * 64 + h->mode * 16 + h->bw * h->bh / 64 * 2 +
-                  (count ? 1 : 0) + (lit ? 1 << 30 : 0);
+                  (lit ? 1 << 30 : 0);
   if (key != h->batch_grid_key) {
     int per_cu = 0;
-    HIPCHK(h, mdwt::batch_blocks_per_cu(p, h->mode, h->dtype, count, &per_cu));
+    HIPCHK(h, mdwt::views_persistent_blocks_per_cu(p, h->mode, h->dtype, &per_cu));
     h->batch_grid = std::max(1, per_cu) * h->cu_count;
     h->batch_grid_key = key;
   }
-  *grid = h->batch_grid;
+  *grid = std::min(h->batch_grid, std::max(1, p.fr.n_items));
   return MDWT_OK;
 }
 
-// Items sorted by their measured duration (heaviest first), as the single
-// view's blocks are.
-int resort_items(mdwt_renderer* h) {
-  const size_t n = h->items.size();
-  std::vector<unsigned long long> t(2 * n);
-  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
-                      hipMemcpyDeviceToHost));
-  std::vector<std::pair<uint64_t, uint32_t>> cost(n);
-  for (size_t i = 0; i < n; ++i) cost[i] = {t[2 * i + 1] - t[2 * i], h->items[i]};
-  std::stable_sort(cost.begin(), cost.end(),
-                   [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
-                     return a.first > b.first;
-                   });
-  for (size_t i = 0; i < n; ++i) h->items[i] = cost[i].second;
-  HIPCHK(h, hipMemcpy(h->d_items, h->items.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  h->batch_sorted = true;
+// n_frames frames of every view by graph replay: the persistent launch and
+// the id advance are captured once (again whenever the parameters change)
+// and the graph is replayed once per frame. The device ids are set from the
+// host's before the replays, so frames split over calls and resets carry.
+int batch_render_persistent(mdwt_renderer* h, int n_frames, float* ms_out) {
+  const mdwt::RenderParams p = persistent_params(h);
+  int grid = 0;
+  int rc = persistent_grid(h, p, &grid);
+  if (rc != MDWT_OK) return rc;
+  if (!h->graph_valid || grid != h->graph_grid ||
+      std::memcmp(&p, &h->graph_params, sizeof(p)) != 0) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_valid = false;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t le = mdwt::launch_views_persistent(p, h->mode, h->dtype, grid, h->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(h, MDWT_ERR_HIP, std::string("graph capture: ") +
+                                       hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    h->graph = g;
+    HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+    h->graph_params = p;
+    h->graph_grid = grid;
+    h->graph_valid = true;
+  }
+  const unsigned ids[3] = {0u, h->batch_frame, h->batch_accum};
+  HIPCHK(h, hipMemcpyAsync(h->d_work, ids, sizeof(ids), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int f = 0; f < n_frames; ++f) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->batch_frame += uint32_t(n_frames);
+  h->batch_accum = h->accumulate ? h->batch_accum + uint32_t(n_frames) : 0u;
+  if (n_frames > 0) {
+    rc = views_times_from_ticks(h);
+    if (rc != MDWT_OK) return rc;
+  }
+  h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
+  if (ms_out) *ms_out = ms;
   return MDWT_OK;
 }
 
@@ -1812,78 +1836,24 @@ int mdwt_batch_render(mdwt_renderer* h, int n_frames, float* ms_out) {
   int rc = batch_ready(h);
   if (rc != MDWT_OK) return rc;
   if (n_frames < 0) return fail(h, MDWT_ERR_ARG, "negative frame count");
-  if (!kPersistentBatch)
-    return h->batch_launch == 1 ? batch_render_one_launch(h, n_frames, ms_out)
-                                : batch_render_views(h, n_frames, ms_out);
-  const mdwt::RenderParams p = batch_params(h);
-  int grid = 0;
-  if ((rc = batch_grid(h, p, false, &grid)) != MDWT_OK) return rc;
-  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  int done = 0;
-  if (n_frames > 0 && !h->batch_sorted && h->batch_resort) {
-    // first frame eagerly; its per-item durations order the next frames
-    HIPCHK(h, mdwt::launch_batch(p, h->mode, h->dtype, false, grid, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    rc = resort_items(h);
-    if (rc != MDWT_OK) return rc;
-    done = 1;
+  switch (h->batch_launch) {
+    case 1: return batch_render_one_launch(h, n_frames, ms_out);
+    case 2: return batch_render_persistent(h, n_frames, ms_out);
+    default: return batch_render_views(h, n_frames, ms_out);
   }
-  if (n_frames > done) {
-    if (!h->graph_valid || std::memcmp(&p, &h->graph_params, sizeof(p)) != 0) {
-      if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-      if (h->graph) (void)hipGraphDestroy(h->graph);
-      h->graph_exec = nullptr;
-      h->graph = nullptr;
-      h->graph_valid = false;
-      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      const hipError_t le = mdwt::launch_batch(p, h->mode, h->dtype, false, grid, h->stream);
-      hipGraph_t g = nullptr;
-      const hipError_t ce = hipStreamEndCapture(h->stream, &g);
-      if (le != hipSuccess || ce != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return fail(h, MDWT_ERR_HIP, std::string("graph capture: ") +
-                                         hipGetErrorString(le != hipSuccess ? le : ce));
-      }
-      h->graph = g;
-      HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-      h->graph_params = p;
-      h->graph_valid = true;
-    }
-    for (int f = done; f < n_frames; ++f) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
-  }
-  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
-  HIPCHK(h, hipEventSynchronize(h->ev1));
-  float ms = 0.f;
-  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  h->batch_frame += uint32_t(n_frames);
-  h->batch_accum = h->accumulate ? h->batch_accum + uint32_t(n_frames) : 0u;
-  h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
-  if (ms_out) *ms_out = ms;
-  return MDWT_OK;
 }
 
 int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
   if (!h || !counts) return MDWT_ERR_ARG;
   int rc = batch_ready(h);
   if (rc != MDWT_OK) return rc;
-  if (!kPersistentBatch) {
-    HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
-    if (h->batch_launch == 1)
-      HIPCHK(h, mdwt::launch_views(views_params(h), h->mode, h->dtype, true, h->stream));
-    else
-      for (size_t i = 0; i < h->views.size(); ++i)
-        HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, true, h->stream));
-    HIPCHK(h, hipMemcpyAsync(counts, h->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                             h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    return MDWT_OK;
-  }
-  mdwt::RenderParams p = batch_params(h);
-  p.fr.item_ticks = nullptr;
-  int grid = 0;
-  if ((rc = batch_grid(h, p, true, &grid)) != MDWT_OK) return rc;
+  // the counting pass of form 2 is form 1's (the same per-item work)
   HIPCHK(h, hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
-  HIPCHK(h, mdwt::launch_batch(p, h->mode, h->dtype, true, grid, h->stream));
+  if (h->batch_launch >= 1)
+    HIPCHK(h, mdwt::launch_views(views_params(h), h->mode, h->dtype, true, h->stream));
+  else
+    for (size_t i = 0; i < h->views.size(); ++i)
+      HIPCHK(h, mdwt::launch_render(view_params(h, i), h->mode, h->dtype, true, h->stream));
   HIPCHK(h, hipMemcpyAsync(counts, h->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1892,9 +1862,10 @@ int mdwt_batch_count(mdwt_renderer* h, uint64_t counts[4]) {
 
 int mdwt_batch_set_launch(mdwt_renderer* h, int launch) {
   if (!h) return MDWT_ERR_ARG;
-  if (launch != 0 && launch != 1)
+  if (launch < 0 || launch > 2)
     return fail(h, MDWT_ERR_ARG,
-                "batch launch must be 0 (a launch per view) or 1 (one launch for all views)");
+                "batch launch must be 0 (a launch per view), 1 (one launch for all views) or 2 "
+                "(one persistent launch, graph replay)");
   h->batch_launch = launch;
   return MDWT_OK;
 }
@@ -1902,15 +1873,7 @@ int mdwt_batch_set_launch(mdwt_renderer* h, int launch) {
 int mdwt_batch_reset(mdwt_renderer* h) {
   if (!h) return MDWT_ERR_ARG;
   if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views");
-  if (!kPersistentBatch) {
-    h->batch_accum = 0;
-    return MDWT_OK;
-  }
-  HIPCHK(h, hipSetDevice(h->device));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  const unsigned w[3] = {0u, h->batch_frame, 0u};
-  HIPCHK(h, hipMemcpy(h->d_work, w, sizeof(w), hipMemcpyHostToDevice));
-  h->batch_accum = 0;
+  h->batch_accum = 0;  // form 2 copies the ids to the device before its replays
   return MDWT_OK;
 }
 
@@ -1930,27 +1893,9 @@ int mdwt_batch_read_view(mdwt_renderer* h, int view, float* rgba, float* m2) {
 int mdwt_batch_view_times(mdwt_renderer* h, float* us) {
   if (!h || !us) return MDWT_ERR_ARG;
   if (h->views.empty()) return fail(h, MDWT_ERR_STATE, "no batched views");
-  if (!kPersistentBatch) {
-    if (h->view_us.size() != h->views.size())
-      return fail(h, MDWT_ERR_STATE, "no batched frame rendered yet");
-    for (size_t k = 0; k < h->view_us.size(); ++k) us[k] = h->view_us[k];
-    return MDWT_OK;
-  }
-  HIPCHK(h, hipSetDevice(h->device));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  const size_t n = h->items.size();
-  std::vector<unsigned long long> t(2 * n);
-  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
-                      hipMemcpyDeviceToHost));
-  unsigned long long t0 = ~0ull;
-  for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[2 * i]);
-  std::vector<unsigned long long> end(h->views.size(), 0ull);
-  for (size_t i = 0; i < n; ++i) {
-    const size_t v = h->items[i] >> 20;
-    end[v] = std::max(end[v], t[2 * i + 1]);
-  }
-  for (size_t v = 0; v < end.size(); ++v)
-    us[v] = end[v] >= t0 ? float(double(end[v] - t0) / 100.0) : 0.f;  // 100 MHz clock
+  if (h->view_us.size() != h->views.size())
+    return fail(h, MDWT_ERR_STATE, "no batched frame rendered yet");
+  for (size_t k = 0; k < h->view_us.size(); ++k) us[k] = h->view_us[k];
   return MDWT_OK;
 }
 
@@ -1963,9 +1908,7 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
   int rc = batch ? batch_ready(h) : ready_to_render(h);
   if (rc != MDWT_OK) return rc;
   const int keep_order = h->order_mode;
-  const bool keep_resort = h->batch_resort;
   h->order_mode = 1;  // the tile order is what runs while timing
-  h->batch_resort = false;
   float best_ms = std::numeric_limits<float>::max();
   int bi[3] = {mc_sizes[0], threads[0], tiles[0]};
   int row = 0;
@@ -2013,17 +1956,13 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
     }
   }
   h->order_mode = keep_order;
-  h->batch_resort = keep_resort;
   if (rc != MDWT_OK) return rc;
   const int mc[3] = {std::max(1, h->dims[0] / bi[0]), std::max(1, h->dims[1] / bi[0]),
                      std::max(1, h->dims[2] / bi[0])};
   if ((rc = mdwt_set_macrocells(h, mc)) != MDWT_OK) return rc;
   if ((rc = mdwt_set_block(h, bi[1])) != MDWT_OK) return rc;
   if ((rc = mdwt_set_tile_order(h, keep_order, bi[2])) != MDWT_OK) return rc;
-  if (batch) {
-    h->batch_sorted = false;
-    if ((rc = mdwt_batch_reset(h)) != MDWT_OK) return rc;
-  }
+  if (batch && (rc = mdwt_batch_reset(h)) != MDWT_OK) return rc;
   reset_acc(h);
   if (best) {
     best[0] = bi[0];

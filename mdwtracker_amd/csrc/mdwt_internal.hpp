@@ -19,15 +19,17 @@ size_t wave_lds_bytes(const RenderParams& p);
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
 hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
-// batched views: item-counter reset, persistent frame kernel over p.fr.items
-// (grid workgroups), frame / accumulation id advance (skipped when counting)
-hipError_t launch_batch(const RenderParams& p, int mode, int dtype, bool count, int grid,
-                        hipStream_t s);
+// batched views, launch form 2: the persistent frame kernel over p.fr.items
+// (grid workgroups, workgroup g takes items g, g + grid, ...) at the ids in
+// p.fr.work[1..2], then the kernel that advances those ids; the pair is what a
+// graph captures for one frame
+hipError_t launch_views_persistent(const RenderParams& p, int mode, int dtype, int grid,
+                                   hipStream_t s);
+// resident workgroups per CU of that persistent kernel
+hipError_t views_persistent_blocks_per_cu(const RenderParams& p, int mode, int dtype, int* out);
 // batched views in one launch: one workgroup per item of p.fr.items (view,
 // pixel block), frame and accumulation ids from p.fr.frame_id / accum_id
 hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
-// resident workgroups per CU of the batched frame kernel
-hipError_t batch_blocks_per_cu(const RenderParams& p, int mode, int dtype, bool count, int* out);
 // dynamic LDS of a frame launch (transfer-function texels)
 size_t render_lds_bytes(const RenderParams& p);
 hipError_t launch_gen_rays(const CameraParams& cam, int W, int H, uint32_t seed, uint32_t frame,

@@ -1,8 +1,8 @@
 """GPU checks of batched multi-view rendering, the workgroup shapes, the
 Hilbert tiles, the macrocell rebuild and the autotuner. Batches render with
-one single-view launch per view (launch form 0) or with one launch whose grid
-holds every view's blocks (form 1); the persistent kernel is switched off
-after an unexplained hang on its first run.
+one single-view launch per view (launch form 0), with one launch whose grid
+holds every view's blocks (form 1), or with one persistent launch per frame
+replayed from a captured HIP graph (form 2).
 
 The RNG is keyed by (seed, pixel, frame, sample), so a batched view must give
 the same bits as a single-view render of the same camera, size, spp and seed,
@@ -59,7 +59,7 @@ def _single(capi, scene, mode, v, frames):
     return a, cnt
 
 
-@pytest.mark.parametrize("launch", [0, 1])
+@pytest.mark.parametrize("launch", [0, 1, 2])
 @pytest.mark.parametrize("mode", [2, 0, 7])
 def test_batched_views_equal_single_view_renders(capi, scene, mode, launch):
     views = _views(6, seed=mode)
@@ -94,7 +94,7 @@ def test_batched_views_equal_single_view_renders(capi, scene, mode, launch):
     g.close()
 
 
-@pytest.mark.parametrize("launch", [0, 1])
+@pytest.mark.parametrize("launch", [0, 1, 2])
 def test_batch_with_lighting_equals_single(capi, scene, launch):
     views = _views(3, seed=11)
     print(f"[batch] lighting launch={launch}", flush=True)
@@ -214,7 +214,7 @@ def test_batch_api_errors(capi, scene):
     assert g.batch_view_times().shape == (2,)
     assert capi.lib.mdwt_batch_read_view(g._h, 2, None, None) != 0  # no such view
     with pytest.raises(capi.MdwtError, match="batch launch"):
-        g.batch_set_launch(2)
+        g.batch_set_launch(3)
     g.close()
 
 
@@ -233,3 +233,31 @@ def test_autotune_single_view(capi, scene):
     g.render(2)
     assert np.all(np.isfinite(g.read_accum()))
     g.close()
+
+
+def test_persistent_graph_recaptures_on_changes(capi, scene):
+    """Launch form 2 replays one captured frame; changing the frame's
+    parameters (lighting, accumulation off, block shape) makes the next call
+    capture again, and every frame still equals the one-launch form's."""
+    views = _views(4, seed=31)
+    print("[batch] persistent graph, parameter changes between calls", flush=True)
+    out = []
+    for launch in (1, 2):
+        g = capi.MdwtRenderer(0)
+        scenes.load_scene(g, scene, 2)
+        g.batch_set_launch(launch)
+        g.batch_set_views(views)
+        g.batch_render(2)
+        g.set_light((0.2, -1.0, 0.4), True, False)
+        g.batch_reset()
+        g.batch_render(3)
+        a = [g.batch_read_view(i) for i in range(len(views))]
+        g.set_block(64)
+        g.batch_reset()
+        g.batch_render(2)
+        b = [g.batch_read_view(i) for i in range(len(views))]
+        out.append((a, b))
+        g.close()
+    for (a1, b1), (a2, b2) in [(out[0], out[1])]:
+        for x, y in zip(a1 + b1, a2 + b2):
+            np.testing.assert_array_equal(_bits(x), _bits(y))

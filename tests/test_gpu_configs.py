@@ -10,7 +10,9 @@ workloads, and on the default macrocell grid.
 * Four f16 channels (the batched 4 x 512^3 workload at 1/8 scale): 16 views
   with sizes and spp drawn like the batched workload's (sizes scaled by 1/8,
   one view above 256 px), in both launch forms: every view bit-identical to
-  its single-view render, two views bit-identical to the oracle; the
+  its single-view render, two views bit-identical to the oracle (launch
+  forms 0, 1 and 2: per-view launches, one launch, one persistent launch
+  replayed from a HIP graph); the
   autotuner's full macrocell x workgroup x tile sweep on that batch.
 * mc = 0 (CalculateMCGridDims(4), renderer.cpp:641-647,1342-1355) on a
   ragged volume: grid dims, range grid, majorants and image equal the
@@ -170,7 +172,7 @@ def _single_view(capi, sc, mode, v, frames):
     return a, cnt
 
 
-@pytest.mark.parametrize("launch", [0, 1])
+@pytest.mark.parametrize("launch", [0, 1, 2])
 def test_f16_batch_views_equal_single_views_and_oracle(capi, po, batch8, launch):
     sc = batch8
     views = _batch_views(16, seed=9)

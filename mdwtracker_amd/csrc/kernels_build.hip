@@ -6,10 +6,13 @@
 // float atomics. Here the same grid is gathered instead: the host derives, per
 // axis, the window of voxels whose projected boxes cover each macrocell index
 // (an interval, because both projected box ends grow monotonically with the
-// voxel index) and widens it by the neighbourhood; one workgroup per
-// (macrocell, channel) reduces its window with wave64 shuffles. No atomics,
-// one launch for all channels, and every voxel byte is read from HBM in
-// coalesced rows along x.
+// voxel index) and widens it by the neighbourhood. Because the window is a
+// box, its min/max separates: range_pencil_kernel gives one workgroup to a
+// pencil of cells along x (fixed y and z windows), reads those rows in full
+// with 16-byte loads, keeps per-column min/max and reduces each cell's x
+// window from LDS. Volumes whose rows are not a multiple of 16 bytes use
+// range_kernel, one workgroup per (macrocell, channel) over its window in
+// dword loads. No atomics, one launch for all channels.
 #include "mdwt_device.hpp"
 #include "mdwt_internal.hpp"
 
@@ -25,6 +28,23 @@ __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+template <int DT>
+__host__ __device__ constexpr int voxel_bytes() {
+  return DT == MDWT_DT_U8 ? 1 : DT == MDWT_DT_F32 ? 4 : 2;
+}
+// Voxel e of a little-endian dword holding 4 / voxel_bytes voxels, as
+// fetch_raw would convert it.
+template <int DT>
+__device__ __forceinline__ float word_voxel(uint32_t w, int e) {
+  if constexpr (DT == MDWT_DT_U8) return float((w >> (8 * e)) & 0xffu);
+  if constexpr (DT == MDWT_DT_U16) return float((w >> (16 * e)) & 0xffffu);
+  if constexpr (DT == MDWT_DT_F16) {
+    const uint16_t h = uint16_t(w >> (16 * e));
+    return float(__builtin_bit_cast(_Float16, h));
+  }
+  return __uint_as_float(w);
 }
 
 template <int DT>
@@ -47,19 +67,66 @@ __global__ __launch_bounds__(256) void range_kernel(RangeParams P) {
     return;
   }
   const int lx = x1 - x0 + 1, ly = y1 - y0 + 1, lz = z1 - z0 + 1;
-  const int total = lx * ly * lz;
   const uint8_t* base = P.vol + size_t(ch) * P.chan_stride;
-  const uint32_t nx = uint32_t(P.nx), nxy = uint32_t(P.nx) * uint32_t(P.ny);
   float vmin = __int_as_float(0x7f800000), vmax = -__int_as_float(0x7f800000);
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int ix = i % lx;
-    const int r = i / lx;
-    const int iy = r % ly;
-    const int iz = r / ly;
-    const uint32_t idx = uint32_t(x0 + ix) + uint32_t(y0 + iy) * nx + uint32_t(z0 + iz) * nxy;
-    const float v = fetch_raw<DT>(base, idx);
-    vmin = fminf(vmin, v);
-    vmax = fmaxf(vmax, v);
+  // One row segment (lx voxels along x at fixed y, z) spans at most ndw
+  // aligned dwords; gl lanes (a power of two >= ndw, at most 64) share a row
+  // and rp = 256 / gl rows are read per pass, four passes' loads issued before
+  // any is reduced. Rows advance by additions (iy, iz carried), no per-voxel
+  // index division. A dword may hold voxels outside the window: they are
+  // masked by their byte position. The channel's allocation is padded to 256
+  // bytes, so the dword holding its last voxel is inside it.
+  constexpr int ES = voxel_bytes<DT>();
+  const int span = lx * ES;
+  const int ndw = (span + 6) >> 2;
+  int lg = 0;
+  while ((1 << lg) < ndw && lg < 6) ++lg;
+  const int gl = 1 << lg, rp = int(blockDim.x) >> lg;
+  const int q = int(threadIdx.x) & (gl - 1), g = int(threadIdx.x) >> lg;
+  const int nrows = ly * lz;
+  const int dy = rp % ly, dz = rp / ly;
+  const uint32_t nx = uint32_t(P.nx), ny = uint32_t(P.ny);
+  for (int k0 = 0; k0 < ndw; k0 += gl) {
+    const int k = k0 + q;
+    int r = g, iy = g % ly, iz = g / ly;
+    while (r < nrows) {
+      uint32_t w[4];
+      int s[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ok[u] = false;
+        s[u] = 0;
+        w[u] = 0u;
+        if (r < nrows) {
+          const size_t row = (size_t(uint32_t(z0 + iz)) * ny + uint32_t(y0 + iy)) * nx + uint32_t(x0);
+          const uintptr_t lo = reinterpret_cast<uintptr_t>(base) + row * ES;
+          s[u] = int(lo & 3u);
+          ok[u] = k < ndw && 4 * k <= s[u] + span - 1;
+          if (ok[u]) w[u] = *reinterpret_cast<const uint32_t*>((lo & ~uintptr_t(3)) + 4u * uint32_t(k));
+          r += rp;
+          iy += dy;
+          iz += dz;
+          if (iy >= ly) {
+            iy -= ly;
+            ++iz;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!ok[u]) continue;
+        const int end = s[u] + span - 1;
+#pragma unroll
+        for (int e = 0; e < 4 / ES; ++e) {
+          const int pos = 4 * k + e * ES;
+          if (pos < s[u] || pos > end) continue;
+          const float v = word_voxel<DT>(w[u], e);
+          vmin = fminf(vmin, v);
+          vmax = fmaxf(vmax, v);
+        }
+      }
+    }
   }
   vmin = wave_min(vmin);
   vmax = wave_max(vmax);
@@ -79,6 +146,145 @@ __global__ __launch_bounds__(256) void range_kernel(RangeParams P) {
     // value conversion is monotone and exact on the integer types, so the
     // converted min/max equal min/max of converted voxels
     out[size_t(cell) * P.channels + ch] = make_float2(a * P.scale + P.offset, b * P.scale + P.offset);
+  }
+}
+
+// Same grid, gathered per pencil: one workgroup per (cy, cz, channel) reads
+// every voxel row of the pencil's y and z windows in full, as aligned 16-byte
+// chunks (J chunks per lane, gl lanes per row), keeps a running min/max per x
+// column in registers, merges the row slots by shuffles and the four waves in
+// LDS, then reduces each cell's x window from the LDS columns. A voxel row is
+// read once per (cy, cz) instead of once per cell, and each lane issues U
+// chunk loads before reducing any. Requires rows of a multiple of 16 bytes
+// (and at most kPencilChunks chunks); other volumes use range_kernel.
+constexpr int kPencilChunks = 128;  // 16-byte chunks per row
+constexpr int kPencilColumns = 2048;
+
+template <int DT>
+__device__ __forceinline__ float chunk_voxel(const uint4& c, int v) {
+  const uint32_t w[4] = {c.x, c.y, c.z, c.w};
+  if constexpr (DT == MDWT_DT_U8) return word_voxel<DT>(w[v >> 2], v & 3);
+  if constexpr (DT == MDWT_DT_U16 || DT == MDWT_DT_F16) return word_voxel<DT>(w[v >> 1], v & 1);
+  return word_voxel<DT>(w[v], 0);
+}
+
+template <int DT, int J>
+__global__ __launch_bounds__(256) void range_pencil_kernel(RangeParams P) {
+  constexpr int ES = voxel_bytes<DT>(), VPC = 16 / ES, U = J == 1 ? 8 : 4;
+  __shared__ float cmin[kPencilColumns], cmax[kPencilColumns];
+  const int cy = blockIdx.x, cz = blockIdx.y, ch = blockIdx.z;
+  const int mx = P.mc[0], my = P.mc[1];
+  const int* wx = P.win;
+  const int* wy = P.win + 2 * P.mc[0];
+  const int* wz = P.win + 2 * (P.mc[0] + P.mc[1]);
+  const int y0 = wy[2 * cy], y1 = wy[2 * cy + 1];
+  const int z0 = wz[2 * cz], z1 = wz[2 * cz + 1];
+  float2* out = reinterpret_cast<float2*>(P.ranges) + (size_t(cz) * my + cy) * mx * P.channels + ch;
+  if (y0 > y1 || z0 > z1) {  // no voxel covers any cell of the pencil
+    for (int cx = threadIdx.x; cx < mx; cx += blockDim.x)
+      out[size_t(cx) * P.channels] = make_float2(1e20f, -1e20f);
+    return;
+  }
+  const int ly = y1 - y0 + 1, lz = z1 - z0 + 1, nrows = ly * lz;
+  const int chunks = P.nx * ES / 16;
+  int lg = 0;
+  while ((1 << lg) * J < chunks && lg < 6) ++lg;
+  const int gl = 1 << lg, rp = int(blockDim.x) >> lg;
+  const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+  const int q = int(threadIdx.x) & (gl - 1), g = int(threadIdx.x) >> lg;
+  const uint8_t* base = P.vol + size_t(ch) * P.chan_stride;
+  const size_t row_bytes = size_t(P.nx) * ES;
+  float vmin[J][VPC], vmax[J][VPC];
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int v = 0; v < VPC; ++v) {
+      vmin[j][v] = __int_as_float(0x7f800000);
+      vmax[j][v] = -__int_as_float(0x7f800000);
+    }
+  const int dy = rp % ly, dz = rp / ly;
+  int r = g, iy = g % ly, iz = g / ly;
+  while (r < nrows) {
+    uint4 w[U][J];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = r < nrows;
+      if (ok[u]) {
+        const uint8_t* row = base + (size_t(z0 + iz) * P.ny + size_t(y0 + iy)) * row_bytes;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int c = q + j * gl;
+          w[u][j] = c < chunks ? *reinterpret_cast<const uint4*>(row + size_t(c) * 16) : uint4{};
+        }
+        r += rp;
+        iy += dy;
+        iz += dz;
+        if (iy >= ly) {
+          iy -= ly;
+          ++iz;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int v = 0; v < VPC; ++v) {
+          const float x = chunk_voxel<DT>(w[u][j], v);
+          vmin[j][v] = fminf(vmin[j][v], x);
+          vmax[j][v] = fmaxf(vmax[j][v], x);
+        }
+    }
+  }
+  // lanes q, q + gl, ... of a wave hold the same columns (other rows)
+  for (int o = gl; o < 64; o <<= 1)
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int v = 0; v < VPC; ++v) {
+        vmin[j][v] = fminf(vmin[j][v], __shfl_xor(vmin[j][v], o, 64));
+        vmax[j][v] = fmaxf(vmax[j][v], __shfl_xor(vmax[j][v], o, 64));
+      }
+  const int waves = int(blockDim.x) >> 6;
+  for (int wv = 0; wv < waves; ++wv) {
+    if (wave == wv && lane < gl) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int c = q + j * gl;
+        if (c >= chunks) continue;
+#pragma unroll
+        for (int v = 0; v < VPC; ++v) {
+          const int x = c * VPC + v;
+          cmin[x] = wv == 0 ? vmin[j][v] : fminf(cmin[x], vmin[j][v]);
+          cmax[x] = wv == 0 ? vmax[j][v] : fmaxf(cmax[x], vmax[j][v]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // 16 lanes per cell reduce its x window from the columns
+  const int sub = int(threadIdx.x) & 15;
+  for (int cx0 = 0; cx0 < mx; cx0 += int(blockDim.x) >> 4) {
+    const int cx = cx0 + (int(threadIdx.x) >> 4);
+    const bool live = cx < mx;
+    const int x0 = live ? wx[2 * cx] : 0, x1 = live ? wx[2 * cx + 1] : -1;
+    float a = __int_as_float(0x7f800000), b = -__int_as_float(0x7f800000);
+    for (int x = x0 + sub; x <= x1; x += 16) {
+      a = fminf(a, cmin[x]);
+      b = fmaxf(b, cmax[x]);
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      a = fminf(a, __shfl_xor(a, o, 64));
+      b = fmaxf(b, __shfl_xor(b, o, 64));
+    }
+    if (live && sub == 0)
+      out[size_t(cx) * P.channels] = x0 > x1 ? make_float2(1e20f, -1e20f)
+                                             : make_float2(a * P.scale + P.offset,
+                                                           b * P.scale + P.offset);
   }
 }
 
@@ -138,9 +344,30 @@ __global__ __launch_bounds__(256) void majorant_kernel(MajorantParams P) {
   }
 }
 
+template <int DT>
+void launch_pencils(const RangeParams& p, hipStream_t s) {
+  const dim3 grid(unsigned(p.mc[1]), unsigned(p.mc[2]), unsigned(p.channels));
+  if (p.nx * voxel_bytes<DT>() <= 64 * 16)
+    hipLaunchKernelGGL((range_pencil_kernel<DT, 1>), grid, dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((range_pencil_kernel<DT, 2>), grid, dim3(256), 0, s, p);
+}
+
+int dtype_bytes(int dtype) { return dtype == MDWT_DT_U8 ? 1 : dtype == MDWT_DT_F32 ? 4 : 2; }
+
 }  // namespace
 
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s) {
+  const int row = p.nx * dtype_bytes(dtype);
+  if (row % 16 == 0 && row / 16 <= kPencilChunks) {
+    switch (dtype) {
+      case MDWT_DT_U8: launch_pencils<MDWT_DT_U8>(p, s); break;
+      case MDWT_DT_U16: launch_pencils<MDWT_DT_U16>(p, s); break;
+      case MDWT_DT_F16: launch_pencils<MDWT_DT_F16>(p, s); break;
+      default: launch_pencils<MDWT_DT_F32>(p, s); break;
+    }
+    return hipGetLastError();
+  }
   const int n_cells = p.mc[0] * p.mc[1] * p.mc[2];
   const dim3 grid(unsigned(n_cells), unsigned(p.channels));
   switch (dtype) {

@@ -66,10 +66,14 @@ def test_ranges_and_majorants_bit_exact(capi, po, name, scale):
         g.close()
 
 
+# (z, y, x) shapes: rows of 33 voxels go through the per-cell range kernel
+# (not a multiple of 16 bytes), rows of 40 through the pencil kernel with one
+# 16-byte chunk per lane, rows of 300 (f32: 75 chunks) with two.
+@pytest.mark.parametrize("shape", [(19, 26, 33), (21, 18, 40), (6, 5, 300)])
 @pytest.mark.parametrize("dtype", [np.uint16, np.float16, np.float32])
-def test_ranges_other_dtypes_and_bounds(capi, po, dtype):
+def test_ranges_other_dtypes_and_bounds(capi, po, dtype, shape):
     rng = np.random.default_rng(17)
-    shape = (19, 26, 33)
+    print(f"[ranges] dtype={np.dtype(dtype).name} shape={shape}", flush=True)
     if dtype == np.uint16:
         vols = [rng.integers(0, 65535, shape, dtype=np.uint16) for _ in range(2)]
     else:
@@ -97,6 +101,30 @@ def test_ranges_other_dtypes_and_bounds(capi, po, dtype):
     g.render(3)
     o.render(3, threads=THREADS)
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+
+
+@pytest.mark.parametrize("shape,mc", [((3, 3, 16), (1, 2, 3)), ((5, 7, 48), (6, 3, 2)),
+                                      ((4, 6, 1040), (37, 2, 4)), ((9, 8, 2048), (64, 3, 5)),
+                                      ((7, 5, 2064), (50, 2, 3))])
+def test_ranges_u8_row_widths(capi, po, shape, mc):
+    """u8 rows of 1, 3, 65 and 128 16-byte chunks (the pencil kernel's lane
+    layouts, one and two chunks per lane) and of 129 chunks (beyond the
+    pencil kernel: per-cell kernel), random voxels, uneven macrocells."""
+    rng = np.random.default_rng(sum(shape))
+    print(f"[ranges] u8 shape={shape} mc={mc}", flush=True)
+    vols = [rng.integers(0, 256, shape, dtype=np.uint8) for _ in range(3)]
+    tfs = [scenes.piecewise_tf(5, rng, c / 3.0, n_texels=50) for c in range(3)]
+    rs = (capi.MdwtRenderer(0), po.OracleRenderer())
+    for r in rs:
+        for v in vols:
+            r.push_channel(v)
+        r.init(2, mc, True)
+        for c in range(3):
+            r.set_tf(c, tfs[c], 2.0, (0.0, 1.0))
+    g, o = rs
+    np.testing.assert_array_equal(_bits(g.read_ranges()), _bits(o.read_ranges()))
+    np.testing.assert_array_equal(_bits(g.read_majorants()), _bits(o.read_majorants()))
+    g.close()
 
 
 def _ragged_scene(n_ch, seed):

@@ -46,6 +46,11 @@ int mdwt_dn_set_params(mdwt_denoiser* d, const float* in);
 /* Denoise one image: input [h][w][7 + channels] -> output [h][w][3]; ms = GPU time of
  * the network (uploads and downloads excluded). */
 int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms);
+/* Inference kernel sequence: on (default) fuses the upsample-and-concatenate
+ * inputs, the pooling and the output filter into the convolutions; off runs
+ * them as separate kernels (the training forward). Both give identical
+ * outputs; the switch exists for testing and profiling. */
+int mdwt_dn_set_fused(mdwt_denoiser* d, int on);
 /* Gradient of the mean squared error against target [h][w][3] for one
  * image, in the parameter order of mdwt_dn_get_params; loss = that error. */
 int mdwt_dn_gradient(mdwt_denoiser* d, const float* input, const float* target, float* grad,

@@ -92,6 +92,7 @@ SIGNATURES = {
     "mdwt_dn_get_params": [_h, _fp],
     "mdwt_dn_set_params": [_h, _fp],
     "mdwt_dn_infer": [_h, _fp, _fp, _fp],
+    "mdwt_dn_set_fused": [_h, C.c_int],
     "mdwt_dn_gradient": [_h, _fp, _fp, _fp, _fp],
     "mdwt_dn_train_step": [_h, _fp, _fp, _fp, C.c_int, C.c_float, _fp],
 }

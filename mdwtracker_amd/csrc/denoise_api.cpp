@@ -54,6 +54,7 @@ struct mdwt_denoiser {
   float* wpart;  // partial weight gradients of the row bands
   float *gout, *g9, *g8, *gu1, *g7, *g6, *gu2, *g5, *g4, *gp2, *g3, *g2, *gp1, *g1, *g0;
   int step = 0;
+  bool fused = true;  // inference: fused upsample/concat, pooling and filter
 };
 
 namespace {
@@ -199,6 +200,69 @@ int forward(mdwt_denoiser* d) {
   if ((rc = fwd(d, 10, d->a9, H, W, 32, 0, nullptr, d->out32, mdwt::dn::kLogitStride))) return rc;
   DNCHK(d, mdwt::dn::kpn_apply(d->in32, d->inputs, d->out32, H, W, d->pred, s));
   return MDWT_OK;
+}
+
+// conv layer i forward with a fused input source / epilogue (ConvArgs)
+int fwd_fused(mdwt_denoiser* d, int i, mdwt::dn::ConvArgs a, int src, int epi) {
+  const Conv& L = d->c[i];
+  a.cin = L.cin_pad;
+  a.w = L.wf;
+  a.bias = L.b;
+  a.cout = L.cout;
+  a.cout_pad = L.cout_pad;
+  a.relu = L.relu ? 1 : 0;
+  DNCHK(d, mdwt::dn::conv(a, src, epi, d->stream));
+  return MDWT_OK;
+}
+
+// Inference forward: the same values as forward() (every fused step
+// computes what the kernel it replaces would have stored), without
+// materialising the pooled, upsampled and concatenated tensors or the
+// logits: c1 and c3 pool in their epilogue, c6 and c8 read their two
+// inputs directly, c10 applies the filter in its epilogue.
+int forward_fused(mdwt_denoiser* d) {
+  using namespace mdwt::dn;
+  const int H = d->H, W = d->W, n = H * W;
+  const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+  int rc;
+  auto args = [](const uint16_t* x, int xs, int Hh, int Ww, uint16_t* y16, int ys) {
+    ConvArgs a{};
+    a.x = x;
+    a.xs = xs;
+    a.H = Hh;
+    a.W = Ww;
+    a.y16 = y16;
+    a.ys = ys;
+    return a;
+  };
+  DNCHK(d, pack_input(d->in32, n, d->inputs, d->in_pad, d->x0, d->stream));
+  if ((rc = fwd_fused(d, 0, args(d->x0, d->in_pad, H, W, d->a0, 32), kSrcPlain, kEpiPlain))) return rc;
+  ConvArgs a = args(d->a0, 32, H, W, d->a1, 32);
+  a.pool16 = d->p1;
+  a.ps = 32;
+  if ((rc = fwd_fused(d, 1, a, kSrcPlain, kEpiPool))) return rc;
+  if ((rc = fwd_fused(d, 2, args(d->p1, 32, H2, W2, d->a2, 64), kSrcPlain, kEpiPlain))) return rc;
+  a = args(d->a2, 64, H2, W2, d->a3, 64);
+  a.pool16 = d->p2;
+  a.ps = 64;
+  if ((rc = fwd_fused(d, 3, a, kSrcPlain, kEpiPool))) return rc;
+  if ((rc = fwd_fused(d, 4, args(d->p2, 64, H4, W4, d->a4, 96), kSrcPlain, kEpiPlain))) return rc;
+  if ((rc = fwd_fused(d, 5, args(d->a4, 96, H4, W4, d->a5, 96), kSrcPlain, kEpiPlain))) return rc;
+  a = args(d->a3, 64, H2, W2, d->a6, 64);  // channels 0..95: up(a5), 96..159: a3
+  a.xl = d->a5;
+  a.cu = 96;
+  if ((rc = fwd_fused(d, 6, a, kSrcUpCat, kEpiPlain))) return rc;
+  if ((rc = fwd_fused(d, 7, args(d->a6, 64, H2, W2, d->a7, 64), kSrcPlain, kEpiPlain))) return rc;
+  a = args(d->a1, 32, H, W, d->a8, 32);  // channels 0..63: up(a7), 64..95: a1
+  a.xl = d->a7;
+  a.cu = 64;
+  if ((rc = fwd_fused(d, 8, a, kSrcUpCat, kEpiPlain))) return rc;
+  if ((rc = fwd_fused(d, 9, args(d->a8, 32, H, W, d->a9, 32), kSrcPlain, kEpiPlain))) return rc;
+  a = args(d->a9, 32, H, W, nullptr, kLogitStride);
+  a.kin = d->in32;
+  a.inputs = d->inputs;
+  a.pred = d->pred;
+  return fwd_fused(d, 10, a, kSrcPlain, kEpiKpn);
 }
 
 // backward of conv layer i: gy (fp32, stride gs) of its output y (bf16,
@@ -366,7 +430,7 @@ int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms
   DNCHK(d, hipMemcpyAsync(d->in32, input, n * d->inputs * sizeof(float),
                           hipMemcpyHostToDevice, d->stream));
   DNCHK(d, hipEventRecord(d->ev0, d->stream));
-  int rc = forward(d);
+  int rc = d->fused ? forward_fused(d) : forward(d);
   if (rc) return rc;
   DNCHK(d, hipEventRecord(d->ev1, d->stream));
   DNCHK(d, hipMemcpyAsync(output, d->pred, n * 3 * sizeof(float), hipMemcpyDeviceToHost, d->stream));
@@ -374,6 +438,12 @@ int mdwt_dn_infer(mdwt_denoiser* d, const float* input, float* output, float* ms
   float t = 0.f;
   DNCHK(d, hipEventElapsedTime(&t, d->ev0, d->ev1));
   if (ms) *ms = t;
+  return MDWT_OK;
+}
+
+int mdwt_dn_set_fused(mdwt_denoiser* d, int on) {
+  if (!d) return MDWT_ERR_ARG;
+  d->fused = on != 0;
   return MDWT_OK;
 }
 

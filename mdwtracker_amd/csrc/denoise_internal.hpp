@@ -24,11 +24,40 @@ static_assert(kTaps <= kLogitStride, "logit stride");
 
 // 3x3 convolution (zero padding) as an implicit GEMM on the matrix cores.
 // w: bf16 [9][cout_pad][cin]; cin % 8 == 0, cout_pad % 32 == 0, xs and xo
-// multiples of 4. Writes channels < cout of y16 (bf16) and/or y32 (fp32,
+// multiples of 8. Writes channels < cout of y16 (bf16) and/or y32 (fp32,
 // added when accum).
 hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, const uint16_t* w,
                    const float* bias, int cout, int cout_pad, uint16_t* y16, float* y32, int ys,
                    int yo, bool relu, bool accum, hipStream_t s);
+// The same convolution with a choice of input source and epilogue:
+// kSrcPlain reads x; kSrcUpCat reads channel c < cu from xl (H/2 x W/2,
+// stride cu) at (y / 2, x / 2) and channel c >= cu from x at c - cu (the
+// U-Net's nearest upsample and concatenation). kEpiPlain stores y16 / y32;
+// kEpiPool also stores the 2 x 2 mean of y16 into pool16 (stride ps), as
+// pool() would; kEpiKpn (cout_pad == kLogitStride) applies the
+// kernel-predicting filter to kin and writes pred instead of the logits.
+// H and W even for kSrcUpCat and kEpiPool; xs, xo, cu multiples of 8.
+constexpr int kSrcPlain = 0, kSrcUpCat = 1;
+constexpr int kEpiPlain = 0, kEpiPool = 1, kEpiKpn = 2;
+struct ConvArgs {
+  const uint16_t* x;
+  int xs, xo;
+  const uint16_t* xl;
+  int cu;
+  int H, W, cin;
+  const uint16_t* w;
+  const float* bias;
+  int cout, cout_pad;
+  uint16_t* y16;
+  float* y32;
+  int ys, yo, relu, accum;
+  uint16_t* pool16;
+  int ps;
+  const float* kin;
+  int inputs;
+  float* pred;
+};
+hipError_t conv(const ConvArgs& a, int src, int epi, hipStream_t s);
 // dw [9][cout][cin] += sum over pixels of g[p][co] * x[p + tap][ci]; part is
 // scratch of part_cap floats (at least 9 * cout * cin). Strides and xo are
 // multiples of 4 and the rows hold cin (cout) rounded up to 4 channels.

@@ -1,18 +1,24 @@
 // Kernels of the learned denoiser (a 3-level U-Net of 3x3 convolutions): BF16
 // activations and weights, FP32 accumulation on the matrix cores
-// (v_mfma_f32_32x32x8bf16_1k), FP32 gradients and Adam state. No reference
+// (v_mfma_f32_32x32x16_bf16 for the convolutions, v_mfma_f32_32x32x8bf16_1k
+// for the weight gradients), FP32 gradients and Adam state. No reference
 // counterpart (the reference renders plain accumulation only).
 //
 // Tensors are NHWC with a channel stride (so that a convolution can read or
 // write a slice of a concatenated tensor). A 3x3 convolution is an implicit
 // GEMM: M = pixels, N = output channels, K = 9 taps x input channels. One
-// wave computes a 32-pixel x 32-channel tile; the MFMA lane layout is
+// wave computes 32-pixel x 32-channel tiles; the MFMA lane layouts are
 //   A (32 x 8):  lane l holds A[l % 32][(l / 32) * 4 + j], j = 0..3
 //   B (8 x 32):  lane l holds B[(l / 32) * 4 + j][l % 32]
 //   D (32 x 32): lane l, register v holds D[(v / 4) * 8 + (l / 32) * 4 + v % 4][l % 32]
-// (checked on the device with known matrices).
+// (checked on the device with known matrices), and for the 16-deep K step
+// lane l holds K entries (l / 32) * 8 + j, j = 0..7, of its A row and B
+// column (the sum over K does not depend on which entries a lane holds, as
+// long as A and B agree), with the same D layout.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <atomic>
 
 #include "denoise_internal.hpp"
 
@@ -36,91 +42,421 @@ __device__ __forceinline__ s4 load4(const uint16_t* p, bool ok) {
   return s4{short(v.x & 0xffffu), short(v.x >> 16), short(v.y & 0xffffu), short(v.y >> 16)};
 }
 
-// y = conv3x3(x) (+ bias, ReLU): x [H*W][xs] (channels xo .. xo+cin), weights
-// w [9][cout_pad][cin] (tap-major, K contiguous), output channel n of pixel p
-// to y16[p * ys + yo + n] (bf16) and/or y32[p * ys + yo + n] (fp32, added to
-// the existing value when accum). cin is a multiple of 8, cout_pad of 32;
-// channels n >= cout are not written.
-// A workgroup (4 waves) computes R image rows x TP consecutive pixels x TN
-// output channels, TP x TN = 64 x 64 (NWN = 2 waves along the channels) or
-// 128 x 32 (NWN = 1). Per chunk of 32 input channels it stages the input's
-// R + 2 halo rows (TP + 2 pixels) and the 9 taps' weights of its TN channels
-// in LDS once; each wave then runs, per tap and K step, one weight fragment
-// against its R rows (R accumulators of 32 x 32). LDS rows are padded to 36
-// bf16 (72 bytes): 8-byte reads stay aligned and neighbouring lanes fall in
-// different banks.
-constexpr int kChunk = 32;        // input channels per staged chunk
-constexpr int kLdsRow = kChunk + 4;
-constexpr int kConvRows = 2;      // image rows per workgroup
+// Kernel-predicting output (Bako et al. 2017): per pixel, the softmax of
+// the kTaps logits (fp32 [n][kLogitStride]) weighs the input colour of the
+// kKernel x kKernel neighbourhood (coordinates clamped to the image).
+__device__ __forceinline__ int kpn_neighbour(int py, int px, int j, int H, int W) {
+  const int sy = min(max(py + j / kKernel - kKernelRadius, 0), H - 1);
+  const int sx = min(max(px + j % kKernel - kKernelRadius, 0), W - 1);
+  return sy * W + sx;
+}
+// softmax weights of one pixel's logits z into w (unnormalised) and their sum
+__device__ __forceinline__ float kpn_weights_z(const float* z, float* w) {
+  float m = z[0];
+#pragma unroll
+  for (int j = 1; j < kTaps; ++j) m = fmaxf(m, z[j]);
+  float sum = 0.f;
+#pragma unroll
+  for (int j = 0; j < kTaps; ++j) {
+    w[j] = __expf(z[j] - m);
+    sum += w[j];
+  }
+  return sum;
+}
+__device__ __forceinline__ float kpn_weights(const float* logits, int p, float* w) {
+  return kpn_weights_z(logits + size_t(p) * kLogitStride, w);
+}
+// out[0..2] = sum_j softmax(z)_j * rgb(neighbour j) for pixel (py, px)
+__device__ __forceinline__ void kpn_filter(const float* __restrict__ in, int inputs, const float* z,
+                                           int py, int px, int H, int W, float* out) {
+  float w[kTaps];
+  const float inv = 1.f / kpn_weights_z(z, w);
+  float r = 0.f, g = 0.f, b = 0.f;
+#pragma unroll
+  for (int j = 0; j < kTaps; ++j) {
+    const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * inputs;
+    r += w[j] * c[0];
+    g += w[j] * c[1];
+    b += w[j] * c[2];
+  }
+  out[0] = r * inv;
+  out[1] = g * inv;
+  out[2] = b * inv;
+}
 
-template <int NWN, int R>
-__global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t* __restrict__ x, int H, int W,
-                                                      int xs, int xo, int cin,
-                                                      const uint16_t* __restrict__ w,
-                                                      const float* __restrict__ bias, int cout,
-                                                      int cout_pad, uint16_t* y16, float* y32,
-                                                      int ys, int yo, int relu, int accum) {
-  constexpr int TN = 32 * NWN, TP = 32 * (4 / NWN), XP = TP + 2;
-  __shared__ __align__(16) uint16_t sA[(R + 2) * XP * kLdsRow];
-  __shared__ __align__(16) uint16_t sB[9 * TN * kLdsRow];
+// y = conv3x3(x) (+ bias, ReLU) on v_mfma_f32_32x32x16_bf16: weights w
+// [9][cout_pad][cin] (tap-major, K contiguous); output channel n of pixel p
+// to y16[p * ys + yo + n] (bf16) and/or y32[p * ys + yo + n] (fp32, added to
+// the existing value when accum); channels n >= cout are not written.
+//
+// A workgroup (4 waves, WX along x by WR along rows) computes R = 2 WR image
+// rows x TP = 32 WX pixels x TN = 32 NT output channels; each wave 2 rows x
+// 32 pixels x TN channels (2 NT accumulators of 32 x 32). Per chunk of 32
+// input channels the workgroup stages the R + 2 halo rows (TP + 2 pixels) and
+// the 9 taps' weights of its TN channels in LDS, one 16-byte granule (8
+// channels) per load; the next chunk's granules are loaded into registers
+// while the current one is multiplied (one barrier pair per chunk). LDS rows
+// hold 32 channels padded to 40 bf16 (80 bytes): 16-byte reads stay aligned
+// and any 8 consecutive rows cover the 32 banks once. With 16-channel K
+// steps, lane l supplies channels (l / 32) * 8 .. + 7 of its pixel (A) and of
+// its output channel's weights (B); D is the 32 x 32 layout above.
+//
+// Input sources: kSrcPlain reads x (stride xs, offset xo). kSrcUpCat reads
+// channel c < cu from xl, the tensor at half resolution (stride cu), at the
+// pixel (y / 2, x / 2) (nearest upsampling), and channel c >= cu from x at
+// c - cu: the U-Net's upsample-and-concatenate without materialising it.
+// Epilogues: kEpiPlain stores as above. kEpiPool also writes the 2 x 2 mean of
+// the stored bf16 output to pool16 (stride ps), as pool_kernel computes it.
+// kEpiKpn stages the fp32 logits of the tile in LDS and applies the
+// kernel-predicting filter (as kpn_apply_kernel) to write pred directly.
+constexpr int kChunk = 32;   // input channels per staged chunk
+constexpr int kPitch = 40;   // LDS row (bf16): 32 channels + 8 padding
+constexpr int kZPitch = kLogitStride + 1;  // LDS row (fp32) of one pixel's logits
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf8 lds_frag(const uint16_t* p) {
+  return __builtin_bit_cast(bf8, *reinterpret_cast<const uint4*>(p));
+}
+
+template <int WX, int WR, int NT>
+struct ConvShape {
+  static constexpr int RW = 2, R = RW * WR, TP = 32 * WX, XP = TP + 2, TN = 32 * NT;
+  static constexpr int kARows = (R + 2) * XP;             // staged pixels
+  static constexpr int kAGr = kARows * 4, kBGr = 9 * TN * 4;  // 16-byte granules
+  static constexpr int kAPer = (kAGr + 255) / 256, kBPer = (kBGr + 255) / 256;
+  static constexpr int kLds = (kARows + 9 * TN) * kPitch;  // bf16
+};
+
+// kpn_filter with the colours read from a staged tile: rgb points at the
+// pixel's top-left neighbour in a [rows][pitch][3] block of clamped input
+// colours (the same values kpn_neighbour would fetch), same arithmetic
+__device__ __forceinline__ void kpn_filter_tile(const float* z, const float* rgb, int pitch,
+                                                float* out) {
+  float w[kTaps];
+  const float inv = 1.f / kpn_weights_z(z, w);
+  float r = 0.f, g = 0.f, b = 0.f;
+#pragma unroll
+  for (int j = 0; j < kTaps; ++j) {
+    const float* c = rgb + ((j / kKernel) * pitch + j % kKernel) * 3;
+    r += w[j] * c[0];
+    g += w[j] * c[1];
+    b += w[j] * c[2];
+  }
+  out[0] = r * inv;
+  out[1] = g * inv;
+  out[2] = b * inv;
+}
+
+// The MFMAs of one staged chunk for waves of one 32-channel tile: 9 taps x
+// STEPS K16 steps, fully unrolled, with the next step's LDS fragments read
+// before the current step's MFMAs are issued (two fragment sets in
+// registers). Waves of two or three tiles have no registers to spare for
+// the second set and run the tap loop in the kernel.
+template <int WX, int WR, int NT, int STEPS>
+__device__ __forceinline__ void mma_chunk(const uint16_t* sA, const uint16_t* sB, int wr, int wx,
+                                          int r32, int kh, f16v (&acc)[2][NT]) {
+  using S = ConvShape<WX, WR, NT>;
+  constexpr int RW = S::RW, XP = S::XP, TN = S::TN, NS = 9 * STEPS;
+  const uint16_t* ab = &sA[((wr * RW) * XP + wx * 32 + r32) * kPitch + kh];
+  const uint16_t* bb = &sB[r32 * kPitch + kh];
+  bf8 fa[2][RW], fb[2][NT];
+  auto load = [&](int i, int buf) {
+    const int tap = i / STEPS, st = i % STEPS, dyi = tap / 3, dx = tap % 3;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) fb[buf][j] = lds_frag(bb + (tap * TN + j * 32) * kPitch + 16 * st);
+#pragma unroll
+    for (int r = 0; r < RW; ++r) fa[buf][r] = lds_frag(ab + ((r + dyi) * XP + dx) * kPitch + 16 * st);
+  };
+  load(0, 0);
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    if (i + 1 < NS) load(i + 1, (i + 1) & 1);
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+      acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[i & 1][0], fa[i & 1][r], acc[r][0], 0, 0, 0);
+    // keep the schedule to one step ahead (hoisting every read would need
+    // all fragments in registers at once)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// bias of channels n .. n + 3 (0 past cout or without bias); n % 4 == 0
+__device__ __forceinline__ float4 bias4(const float* bias, int n, int cout) {
+  if (!bias || n >= cout) return make_float4(0.f, 0.f, 0.f, 0.f);
+  if (n + 3 < cout) return *reinterpret_cast<const float4*>(bias + n);
+  return make_float4(bias[n], n + 1 < cout ? bias[n + 1] : 0.f, n + 2 < cout ? bias[n + 2] : 0.f, 0.f);
+}
+
+template <int WX, int WR, int NT, int SRC, int EPI>
+__global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
+  using S = ConvShape<WX, WR, NT>;
+  constexpr int RW = S::RW, R = S::R, TP = S::TP, XP = S::XP, TN = S::TN;
+  static_assert(EPI != kEpiKpn || (R * TP * kZPitch + (R + 2 * kKernelRadius) *
+                                                      (TP + 2 * kKernelRadius) * 3) * 4 <= S::kLds * 2,
+                "filter tile");
+  __shared__ __align__(16) uint16_t lds[S::kLds];
+  uint16_t* sA = lds;
+  uint16_t* sB = lds + S::kARows * kPitch;
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
-  const int tiles_x = (W + TP - 1) / TP;
-  const int y0 = (blockIdx.x / tiles_x) * R, x0 = (blockIdx.x % tiles_x) * TP;
+  const int tiles_x = (a.W + TP - 1) / TP, ntiles = tiles_x * ((a.H + R - 1) / R);
   const int n0 = blockIdx.y * TN;
-  const int wp = (wv % (4 / NWN)) * 32, wn = (wv / (4 / NWN)) * 32;  // wave's pixel / channel offset
-  const int r32 = l & 31, kq = (l >> 5) * 4;
-  f16v acc[R];
+  const int wx = wv % WX, wr = wv / WX;
+  const int r32 = l & 31, kh = (l >> 5) * 8, kq = (l >> 5) * 4;
+  const uint4 zero = {0u, 0u, 0u, 0u};
+  // Tiles of this workgroup: with a grid that is a multiple of the 8 XCDs
+  // (workgroup i runs on XCD i % 8), XCD x takes the x-th eighth of the
+  // tiles (row bands that share halo rows stay in one L2) and its workgroups
+  // stride through it; otherwise a plain grid stride.
+  int first, stride, last;
+  if (gridDim.x % 8 == 0) {
+    const int per = int(gridDim.x) / 8, span = (ntiles + 7) / 8, xcd = int(blockIdx.x) % 8;
+    first = xcd * span + int(blockIdx.x) / 8;
+    stride = per;
+    last = min(ntiles, (xcd + 1) * span);
+  } else {
+    first = blockIdx.x;
+    stride = gridDim.x;
+    last = ntiles;
+  }
+  if (first >= last) return;
+  // one chunk of input channels: the weights stay in LDS for every tile
+  // (not with the filter epilogue, whose logits overwrite them)
+  const bool b_once = EPI != kEpiKpn && a.cin <= kChunk;
+
+  uint4 pa[S::kAPer], pb[S::kBPer];
+  auto fetch = [&](int tile, int kc, bool with_b) {
+    const int y0 = (tile / tiles_x) * R, x0 = (tile % tiles_x) * TP;
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = f16v{};
-  for (int kc = 0; kc < cin; kc += kChunk) {
-    const int ch = min(kChunk, cin - kc);  // multiple of 8
-    const int ng = ch / 4;                 // 8-byte groups per pixel / weight row
-    for (int i = t; i < (R + 2) * XP * ng; i += 256) {  // input halo rows y0-1 .. y0+R
-      const int g = i % ng, q = i / ng, row = q / XP, px = q - row * XP;
-      const int sy = y0 + row - 1, sx = x0 + px - 1;
-      const bool in = sy >= 0 && sy < H && sx >= 0 && sx < W;
-      const s4 v = load4(x + (size_t(in ? sy * W + sx : 0) * xs + xo + kc + 4 * g), in);
-      *reinterpret_cast<s4*>(&sA[(row * XP + px) * kLdsRow + 4 * g]) = v;
+    for (int i = 0; i < S::kAPer; ++i) {
+      const int f = t + 256 * i;
+      pa[i] = zero;
+      if (f < S::kAGr) {
+        const int g = f & 3, pp = f >> 2, row = pp / XP, px = pp - row * XP;
+        const int sy = y0 + row - 1, sx = x0 + px - 1, c = kc + 8 * g;
+        if (sy >= 0 && sy < a.H && sx >= 0 && sx < a.W && c < a.cin) {
+          const uint16_t* p;
+          if (SRC == kSrcUpCat && c < a.cu)
+            p = a.xl + (size_t(sy >> 1) * (a.W >> 1) + (sx >> 1)) * a.cu + c;
+          else
+            p = a.x + (size_t(sy) * a.W + sx) * a.xs + a.xo + (SRC == kSrcUpCat ? c - a.cu : c);
+          pa[i] = *reinterpret_cast<const uint4*>(p);
+        }
+      }
     }
-    for (int i = t; i < 9 * TN * ng; i += 256) {  // weights: 9 taps x TN channels
-      const int g = i % ng, q = i / ng, tap = q / TN, n = q - tap * TN;
-      const s4 v = load4(w + (size_t(tap) * cout_pad + n0 + n) * cin + kc + 4 * g, true);
-      *reinterpret_cast<s4*>(&sB[(tap * TN + n) * kLdsRow + 4 * g]) = v;
+    if (!with_b) return;
+#pragma unroll
+    for (int i = 0; i < S::kBPer; ++i) {
+      const int f = t + 256 * i;
+      pb[i] = zero;
+      if (f < S::kBGr) {
+        const int g = f & 3, q = f >> 2, tap = q / TN, n = q - tap * TN, c = kc + 8 * g;
+        if (c < a.cin)
+          pb[i] = *reinterpret_cast<const uint4*>(a.w + (size_t(tap) * a.cout_pad + n0 + n) * a.cin + c);
+      }
     }
+  };
+  auto store = [&](bool with_b) {
+#pragma unroll
+    for (int i = 0; i < S::kAPer; ++i) {
+      const int f = t + 256 * i;
+      if (f < S::kAGr) *reinterpret_cast<uint4*>(&sA[(f >> 2) * kPitch + 8 * (f & 3)]) = pa[i];
+    }
+    if (!with_b) return;
+#pragma unroll
+    for (int i = 0; i < S::kBPer; ++i) {
+      const int f = t + 256 * i;
+      if (f < S::kBGr) *reinterpret_cast<uint4*>(&sB[(f >> 2) * kPitch + 8 * (f & 3)]) = pb[i];
+    }
+  };
+
+  f16v acc[RW][NT];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[r][j] = f16v{};
+  // stages (tile, chunk) in order; the next stage's granules are loaded while
+  // the current one is multiplied and while the finished tile is stored
+  int tile = first, kc = 0;
+  bool first_stage = true;
+  fetch(tile, 0, true);
+  while (true) {
+    store(first_stage || !b_once);
+    first_stage = false;
     __syncthreads();
+    int ntile = tile, nkc = kc + kChunk;
+    if (nkc >= a.cin) {
+      nkc = 0;
+      ntile += stride;
+    }
+    const bool more = ntile < last;
+    if (more) fetch(ntile, nkc, !b_once);
+    // K16 steps of the chunk (granules past cin are zero): 9 taps x 1 or 2
+    if constexpr (NT == 1) {
+      if (a.cin - kc > 16)
+        mma_chunk<WX, WR, NT, 2>(sA, sB, wr, wx, r32, kh, acc);
+      else
+        mma_chunk<WX, WR, NT, 1>(sA, sB, wr, wx, r32, kh, acc);
+    } else {
+      const int steps = a.cin - kc > 16 ? 2 : 1;
 #pragma unroll 1
-    for (int tap = 0; tap < 9; ++tap) {
-      const int dyi = tap / 3, dx = tap % 3;
-      const uint16_t* ar = &sA[(dyi * XP + wp + r32 + dx) * kLdsRow + kq];
-      const uint16_t* br = &sB[(tap * TN + wn + r32) * kLdsRow + kq];
-      for (int k = 0; k < ch; k += 8) {
-        const s4 bv = *reinterpret_cast<const s4*>(br + k);
+      for (int tap = 0; tap < 9; ++tap) {
+        const int dyi = tap / 3, dx = tap - dyi * 3;
+        const uint16_t* ar = &sA[((wr * RW + dyi) * XP + wx * 32 + r32 + dx) * kPitch + kh];
+        const uint16_t* br = &sB[(tap * TN + r32) * kPitch + kh];
+        for (int st = 0; st < steps; ++st) {
+          bf8 b[NT];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const s4 av = *reinterpret_cast<const s4*>(ar + r * XP * kLdsRow + k);
-          acc[r] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc[r], 0, 0, 0);
+          for (int j = 0; j < NT; ++j) b[j] = lds_frag(br + j * 32 * kPitch + 16 * st);
+#pragma unroll
+          for (int r = 0; r < RW; ++r) {
+            const bf8 av = lds_frag(ar + r * XP * kPitch + 16 * st);
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+              acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], av, acc[r][j], 0, 0, 0);
+          }
         }
       }
     }
     __syncthreads();
-  }
-  const int n = n0 + wn + r32;
-  if (n >= cout) return;
-  const float bn = bias ? bias[n] : 0.f;
+    if (nkc == 0) {  // the tile is complete
+      const int y0 = (tile / tiles_x) * R, x0 = (tile % tiles_x) * TP;
+      // acc[r][j]: D = weights x pixels, so lane l holds pixel r32 of row
+      // y0 + wr * RW + r and, in register v, output channel
+      // j * 32 + (v / 4) * 8 + kq + v % 4: four consecutive channels per
+      // register group
+      const int px = x0 + wx * 32 + r32;
+      if constexpr (EPI == kEpiKpn) {
+        // logits (fp32, kLogitStride per pixel; n0 == 0, TN == kLogitStride)
+        // of the tile into LDS, and after them the input colours of the tile
+        // with a kKernelRadius halo (coordinates clamped to the image); then
+        // one thread per pixel applies the filter
+        float* sz = reinterpret_cast<float*>(lds);
+        float* srgb = sz + R * TP * kZPitch;
+        constexpr int kRgbCols = TP + 2 * kKernelRadius, kRgbPx = (R + 2 * kKernelRadius) * kRgbCols;
+        constexpr int kRgbPer = (kRgbPx + 255) / 256;
+        float cr[kRgbPer][3];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int y = y0 + r;
-    if (y >= H) break;
+        for (int i = 0; i < kRgbPer; ++i) {
+          const int q = t + 256 * i;
+          if (q < kRgbPx) {
+            const int iy = q / kRgbCols, ix = q - iy * kRgbCols;
+            const int sy = min(max(y0 - kKernelRadius + iy, 0), a.H - 1);
+            const int sx = min(max(x0 - kKernelRadius + ix, 0), a.W - 1);
+            const float* c = a.kin + (size_t(sy) * a.W + sx) * a.inputs;
+            cr[i][0] = c[0];
+            cr[i][1] = c[1];
+            cr[i][2] = c[2];
+          }
+        }
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int xx = x0 + wp + (v >> 2) * 8 + kq + (v & 3);
-      if (xx >= W) continue;
-      float o32 = acc[r][v] + bn;
-      if (relu) o32 = fmaxf(o32, 0.f);
-      const size_t o = (size_t(y) * W + xx) * ys + yo + n;
-      if (y16) y16[o] = f2bf(o32);
-      if (y32) y32[o] = accum ? y32[o] + o32 : o32;
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+          for (int gq = 0; gq < 4; ++gq) {
+            const int n = j * 32 + gq * 8 + kq;
+            const float4 b4 = bias4(a.bias, n, a.cout);
+            const float bn[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+              const int pl = (wr * RW + r) * TP + wx * 32 + r32;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) sz[pl * kZPitch + n + e] = acc[r][j][gq * 4 + e] + bn[e];
+            }
+          }
+#pragma unroll
+        for (int i = 0; i < kRgbPer; ++i) {
+          const int q = t + 256 * i;
+          if (q < kRgbPx) {
+            srgb[q * 3 + 0] = cr[i][0];
+            srgb[q * 3 + 1] = cr[i][1];
+            srgb[q * 3 + 2] = cr[i][2];
+          }
+        }
+        __syncthreads();
+        for (int pl = t; pl < R * TP; pl += 256) {
+          const int ly = pl / TP, lx = pl % TP, py = y0 + ly, qx = x0 + lx;
+          if (py >= a.H || qx >= a.W) continue;
+          kpn_filter_tile(sz + pl * kZPitch, srgb + (ly * kRgbCols + lx) * 3, kRgbCols,
+                          a.pred + (size_t(py) * a.W + qx) * 3);
+        }
+        __syncthreads();  // before the next stage overwrites the logits
+      } else {
+        const int y = y0 + wr * RW;  // even: R and RW are
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+          for (int gq = 0; gq < 4; ++gq) {
+            const int n = n0 + j * 32 + gq * 8 + kq;  // channels n .. n + 3
+            const float4 b4 = bias4(a.bias, n, a.cout);
+            const float bn[4] = {b4.x, b4.y, b4.z, b4.w};
+            uint32_t o16[RW][2];  // bf16 pairs (n, n + 1), (n + 2, n + 3)
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+              float o32[4];
+              uint16_t h[4];
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                o32[e] = acc[r][j][gq * 4 + e] + bn[e];
+                if (a.relu) o32[e] = fmaxf(o32[e], 0.f);
+                h[e] = f2bf(o32[e]);
+              }
+              o16[r][0] = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
+              o16[r][1] = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+              if (y + r >= a.H || px >= a.W || n >= a.cout) continue;
+              const size_t o = (size_t(y + r) * a.W + px) * a.ys + a.yo + n;
+              if (n + 3 < a.cout) {  // 8-byte bf16 / 16-byte fp32 stores
+                if (a.y16) *reinterpret_cast<uint2*>(a.y16 + o) = make_uint2(o16[r][0], o16[r][1]);
+                if (a.y32) {
+                  float4* q = reinterpret_cast<float4*>(a.y32 + o);
+                  float4 v4 = make_float4(o32[0], o32[1], o32[2], o32[3]);
+                  if (a.accum) {
+                    const float4 old = *q;
+                    v4 = make_float4(old.x + v4.x, old.y + v4.y, old.z + v4.z, old.w + v4.w);
+                  }
+                  *q = v4;
+                }
+              } else {
+                for (int e = 0; n + e < a.cout; ++e) {
+                  if (a.y16) a.y16[o + e] = h[e];
+                  if (a.y32) a.y32[o + e] = a.accum ? a.y32[o + e] + o32[e] : o32[e];
+                }
+              }
+            }
+            if constexpr (EPI == kEpiPool) {
+              // pixel px + 1 is lane l + 1's; even lanes (px even) write the
+              // mean of rows y, y + 1 and pixels px, px + 1
+              uint32_t nb16[RW][2];
+#pragma unroll
+              for (int r = 0; r < RW; ++r)
+#pragma unroll
+                for (int d = 0; d < 2; ++d) nb16[r][d] = uint32_t(__shfl_xor(int(o16[r][d]), 1, 64));
+              if ((r32 & 1) == 0 && y < a.H && px < a.W && n < a.cout) {
+                uint16_t m[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                  const int d = e >> 1, sh = (e & 1) * 16;
+                  const float s = bf2f(uint16_t(o16[0][d] >> sh)) + bf2f(uint16_t(nb16[0][d] >> sh)) +
+                                  bf2f(uint16_t(o16[1][d] >> sh)) + bf2f(uint16_t(nb16[1][d] >> sh));
+                  m[e] = f2bf(0.25f * s);
+                }
+                const size_t o = (size_t(y >> 1) * (a.W >> 1) + (px >> 1)) * a.ps + n;
+                if (n + 3 < a.cout)
+                  *reinterpret_cast<uint2*>(a.pool16 + o) =
+                      make_uint2(uint32_t(m[0]) | (uint32_t(m[1]) << 16),
+                                 uint32_t(m[2]) | (uint32_t(m[3]) << 16));
+                else
+                  for (int e = 0; n + e < a.cout; ++e) a.pool16[o + e] = m[e];
+              }
+            }
+          }
+      }
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[r][j] = f16v{};
     }
+    if (!more) break;
+    tile = ntile;
+    kc = nkc;
   }
 }
 
@@ -312,32 +648,20 @@ __global__ void add_slice_kernel(const float* gy, int ys, int yo, int n, int C, 
 // network input: in (fp32 [n][inputs]) -> bf16 [n][in_pad] (channels from
 // inputs up zero)
 __global__ void pack_input_kernel(const float* in, int n, int inputs, int in_pad, uint16_t* x) {
+  // one thread per 8-channel granule (16-byte store); in_pad % 8 == 0
+  const int groups = in_pad / 8;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * in_pad) return;
-  const int p = i / in_pad, c = i - p * in_pad;
-  x[i] = c < inputs ? f2bf(in[size_t(p) * inputs + c]) : uint16_t(0);
-}
-// Kernel-predicting output (Bako et al. 2017): per pixel, the softmax of
-// the kTaps logits (fp32 [n][kLogitStride]) weighs the input colour of the
-// kKernel x kKernel neighbourhood (coordinates clamped to the image).
-__device__ __forceinline__ int kpn_neighbour(int py, int px, int j, int H, int W) {
-  const int sy = min(max(py + j / kKernel - kKernelRadius, 0), H - 1);
-  const int sx = min(max(px + j % kKernel - kKernelRadius, 0), W - 1);
-  return sy * W + sx;
-}
-// softmax weights of pixel p into w (unnormalised) and their sum
-__device__ __forceinline__ float kpn_weights(const float* logits, int p, float* w) {
-  const float* z = logits + size_t(p) * kLogitStride;
-  float m = z[0];
+  if (i >= n * groups) return;
+  const int p = i / groups, c0 = (i - p * groups) * 8;
+  const float* src = in + size_t(p) * inputs;
+  uint32_t w[4];
 #pragma unroll
-  for (int j = 1; j < kTaps; ++j) m = fmaxf(m, z[j]);
-  float sum = 0.f;
-#pragma unroll
-  for (int j = 0; j < kTaps; ++j) {
-    w[j] = __expf(z[j] - m);
-    sum += w[j];
+  for (int e = 0; e < 4; ++e) {
+    const int c = c0 + 2 * e;
+    const uint32_t lo = c < inputs ? f2bf(src[c]) : 0u, hi = c + 1 < inputs ? f2bf(src[c + 1]) : 0u;
+    w[e] = lo | (hi << 16);
   }
-  return sum;
+  *reinterpret_cast<uint4*>(x + size_t(p) * in_pad + c0) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 // pred [n][3] = sum_j softmax(z)_j * rgb(neighbour j)
 __global__ void kpn_apply_kernel(const float* __restrict__ in, int inputs,
@@ -346,19 +670,7 @@ __global__ void kpn_apply_kernel(const float* __restrict__ in, int inputs,
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= H * W) return;
   const int py = p / W, px = p - py * W;
-  float w[kTaps];
-  const float inv = 1.f / kpn_weights(logits, p, w);
-  float r = 0.f, g = 0.f, b = 0.f;
-#pragma unroll
-  for (int j = 0; j < kTaps; ++j) {
-    const float* c = in + size_t(kpn_neighbour(py, px, j, H, W)) * inputs;
-    r += w[j] * c[0];
-    g += w[j] * c[1];
-    b += w[j] * c[2];
-  }
-  pred[size_t(p) * 3 + 0] = r * inv;
-  pred[size_t(p) * 3 + 1] = g * inv;
-  pred[size_t(p) * 3 + 2] = b * inv;
+  kpn_filter(in, inputs, logits + size_t(p) * kLogitStride, py, px, H, W, pred + size_t(p) * 3);
 }
 // Squared error of pred against target: err += sum (pred - target)^2, and
 // the gradient of inv_count * that sum with respect to the logits,
@@ -427,25 +739,79 @@ __global__ void weight_layouts_kernel(const float* w, int cin, int cout, int cin
 
 inline unsigned nb(size_t n, unsigned t = 256) { return unsigned((n + t - 1) / t); }
 
+// compute units of the current device (cached per device)
+int cu_count() {
+  static std::atomic<int> counts[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int n = counts[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    counts[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
 }  // namespace
+
+hipError_t conv(const ConvArgs& a, int src, int epi, hipStream_t s) {
+  // 16-byte granules of input and weights
+  if (a.cin % 8 || a.cout_pad % 32 || a.xs % 8 || a.xo % 8 || a.ys % 4 || a.yo % 4)
+    return hipErrorInvalidValue;
+  if (src == kSrcPlain && a.xo + a.cin > a.xs) return hipErrorInvalidValue;
+  if (src == kSrcUpCat && (a.cu % 8 || a.cu >= a.cin || a.xo + a.cin - a.cu > a.xs || a.H % 2 ||
+                           a.W % 2 || !a.xl))
+    return hipErrorInvalidValue;
+  if (epi == kEpiPool && (a.H % 2 || a.W % 2 || !a.pool16 || !a.y16)) return hipErrorInvalidValue;
+  if (epi == kEpiKpn && (a.cout_pad != kLogitStride || !a.kin || !a.pred)) return hipErrorInvalidValue;
+  // workgroup shape by output channels: 64 per wave (2 x 2 waves of 32
+  // pixels x 2 rows), 96 per wave (same, for the 96-channel level), else 32
+  // per wave (4 waves along x)
+  const int shape = a.cout_pad % 64 == 0 ? 2 : a.cout_pad % 96 == 0 ? 3 : 1;
+  const int TP = shape == 1 ? 128 : 64, R = shape == 1 ? 2 : 4, TN = 32 * shape;
+  if (epi == kEpiKpn && shape != 2) return hipErrorInvalidValue;
+  // a resident grid (two workgroups per CU, one for the 96-channel shape)
+  // whose workgroups loop over the tiles
+  const int tiles = ((a.W + TP - 1) / TP) * ((a.H + R - 1) / R), ny = a.cout_pad / TN;
+  const int resident = std::max(1, cu_count() * (shape == 3 ? 1 : 2) / ny);
+  const dim3 grid(unsigned(std::min(tiles, resident)), unsigned(ny));
+  const int key = shape * 100 + src * 10 + epi;
+  switch (key) {
+    case 100: hipLaunchKernelGGL((conv_kernel<4, 1, 1, kSrcPlain, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 101: hipLaunchKernelGGL((conv_kernel<4, 1, 1, kSrcPlain, kEpiPool>), grid, dim3(256), 0, s, a); break;
+    case 110: hipLaunchKernelGGL((conv_kernel<4, 1, 1, kSrcUpCat, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 200: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcPlain, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 201: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcPlain, kEpiPool>), grid, dim3(256), 0, s, a); break;
+    case 202: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcPlain, kEpiKpn>), grid, dim3(256), 0, s, a); break;
+    case 210: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcUpCat, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 300: hipLaunchKernelGGL((conv_kernel<2, 2, 3, kSrcPlain, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, const uint16_t* w,
                    const float* bias, int cout, int cout_pad, uint16_t* y16, float* y32, int ys,
                    int yo, bool relu, bool accum, hipStream_t s) {
-  // 8-byte input and weight groups
-  if (cin % 8 || cout_pad % 32 || xs % 4 || xo % 4 || xo + cin > xs) return hipErrorInvalidValue;
-  constexpr int R = kConvRows;
-  const unsigned rows = unsigned((H + R - 1) / R);
-  if (cout_pad % 64 == 0) {
-    const dim3 grid(unsigned((W + 63) / 64) * rows, unsigned(cout_pad / 64));
-    hipLaunchKernelGGL((conv3x3_kernel<2, R>), grid, dim3(256), 0, s, x, H, W, xs, xo, cin, w,
-                       bias, cout, cout_pad, y16, y32, ys, yo, relu ? 1 : 0, accum ? 1 : 0);
-  } else {
-    const dim3 grid(unsigned((W + 127) / 128) * rows, unsigned(cout_pad / 32));
-    hipLaunchKernelGGL((conv3x3_kernel<1, R>), grid, dim3(256), 0, s, x, H, W, xs, xo, cin, w,
-                       bias, cout, cout_pad, y16, y32, ys, yo, relu ? 1 : 0, accum ? 1 : 0);
-  }
-  return hipGetLastError();
+  ConvArgs a{};
+  a.x = x;
+  a.xs = xs;
+  a.xo = xo;
+  a.H = H;
+  a.W = W;
+  a.cin = cin;
+  a.w = w;
+  a.bias = bias;
+  a.cout = cout;
+  a.cout_pad = cout_pad;
+  a.y16 = y16;
+  a.y32 = y32;
+  a.ys = ys;
+  a.yo = yo;
+  a.relu = relu ? 1 : 0;
+  a.accum = accum ? 1 : 0;
+  return conv(a, kSrcPlain, kEpiPlain, s);
 }
 
 hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
@@ -519,7 +885,8 @@ hipError_t add_slice(const float* gy, int ys, int yo, int n, int C, float* gx, h
   return hipGetLastError();
 }
 hipError_t pack_input(const float* in, int n, int inputs, int in_pad, uint16_t* x, hipStream_t s) {
-  hipLaunchKernelGGL(pack_input_kernel, dim3(nb(size_t(n) * in_pad)), dim3(256), 0, s, in, n,
+  if (in_pad % 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pack_input_kernel, dim3(nb(size_t(n) * (in_pad / 8))), dim3(256), 0, s, in, n,
                      inputs, in_pad, x);
   return hipGetLastError();
 }

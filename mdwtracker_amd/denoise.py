@@ -95,6 +95,11 @@ class Denoiser:
         self._chk(lib.mdwt_dn_infer(self._h, _f(x), _f(out), C.byref(ms)))
         return out, ms.value
 
+    def set_fused(self, on: bool = True) -> None:
+        """Inference as fused kernels (default) or as the training forward's
+        separate kernels; the outputs are identical."""
+        self._chk(lib.mdwt_dn_set_fused(self._h, 1 if on else 0))
+
     def gradient(self, x: np.ndarray, target: np.ndarray):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(target, np.float32)

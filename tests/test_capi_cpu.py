@@ -100,6 +100,7 @@ def test_denoiser_and_aov_entry_points_validate_without_gpu(capi):
     assert lib.mdwt_dn_get_params(None, fp) == 1
     assert lib.mdwt_dn_set_params(None, fp) == 1
     assert lib.mdwt_dn_infer(None, fp, fp, None) == 1
+    assert lib.mdwt_dn_set_fused(None, 1) == 1
     assert lib.mdwt_dn_gradient(None, fp, fp, fp, None) == 1
     assert lib.mdwt_dn_train_step(None, fp, fp, None, 1, 1e-3, None) == 1
     assert lib.mdwt_dn_last_error(None) == b""

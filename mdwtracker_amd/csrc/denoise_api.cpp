@@ -217,9 +217,10 @@ int fwd_fused(mdwt_denoiser* d, int i, mdwt::dn::ConvArgs a, int src, int epi) {
 
 // Inference forward: the same values as forward() (every fused step
 // computes what the kernel it replaces would have stored), without
-// materialising the pooled, upsampled and concatenated tensors or the
-// logits: c1 and c3 pool in their epilogue, c6 and c8 read their two
-// inputs directly, c10 applies the filter in its epilogue.
+// materialising the packed input, the pooled, upsampled and concatenated
+// tensors or the logits: c0 converts the fp32 input while staging it, c1
+// and c3 pool in their epilogue, c6 and c8 read their two inputs directly,
+// c10 applies the filter in its epilogue.
 int forward_fused(mdwt_denoiser* d) {
   using namespace mdwt::dn;
   const int H = d->H, W = d->W, n = H * W;
@@ -235,9 +236,11 @@ int forward_fused(mdwt_denoiser* d) {
     a.ys = ys;
     return a;
   };
-  DNCHK(d, pack_input(d->in32, n, d->inputs, d->in_pad, d->x0, d->stream));
-  if ((rc = fwd_fused(d, 0, args(d->x0, d->in_pad, H, W, d->a0, 32), kSrcPlain, kEpiPlain))) return rc;
-  ConvArgs a = args(d->a0, 32, H, W, d->a1, 32);
+  ConvArgs a = args(nullptr, 0, H, W, d->a0, 32);  // c0 reads the fp32 input
+  a.kin = d->in32;
+  a.inputs = d->inputs;
+  if ((rc = fwd_fused(d, 0, a, kSrcF32, kEpiPlain))) return rc;
+  a = args(d->a0, 32, H, W, d->a1, 32);
   a.pool16 = d->p1;
   a.ps = 32;
   if ((rc = fwd_fused(d, 1, a, kSrcPlain, kEpiPool))) return rc;

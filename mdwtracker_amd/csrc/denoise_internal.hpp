@@ -32,12 +32,14 @@ hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, con
 // The same convolution with a choice of input source and epilogue:
 // kSrcPlain reads x; kSrcUpCat reads channel c < cu from xl (H/2 x W/2,
 // stride cu) at (y / 2, x / 2) and channel c >= cu from x at c - cu (the
-// U-Net's nearest upsample and concatenation). kEpiPlain stores y16 / y32;
+// U-Net's nearest upsample and concatenation); kSrcF32 reads the fp32
+// network input kin (inputs values per pixel, zero up to cin) as
+// pack_input would convert it. kEpiPlain stores y16 / y32;
 // kEpiPool also stores the 2 x 2 mean of y16 into pool16 (stride ps), as
 // pool() would; kEpiKpn (cout_pad == kLogitStride) applies the
 // kernel-predicting filter to kin and writes pred instead of the logits.
 // H and W even for kSrcUpCat and kEpiPool; xs, xo, cu multiples of 8.
-constexpr int kSrcPlain = 0, kSrcUpCat = 1;
+constexpr int kSrcPlain = 0, kSrcUpCat = 1, kSrcF32 = 2;
 constexpr int kEpiPlain = 0, kEpiPool = 1, kEpiKpn = 2;
 struct ConvArgs {
   const uint16_t* x;

@@ -30,10 +30,13 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (finite inputs)
-  const uint32_t u = __float_as_uint(f);
-  return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32: two per instruction)
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+  const bf2v v = {__bf16(lo), __bf16(hi)};
+  return __builtin_bit_cast(uint32_t, v);
 }
+__device__ __forceinline__ uint16_t f2bf(float f) { return uint16_t(f2bf2(f, 0.f)); }
 
 // 4 consecutive bf16 (8 bytes, 8-byte aligned) or zeros
 __device__ __forceinline__ s4 load4(const uint16_t* p, bool ok) {
@@ -224,21 +227,37 @@ __global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
   const bool b_once = EPI != kEpiKpn && a.cin <= kChunk;
 
   uint4 pa[S::kAPer], pb[S::kBPer];
+  // kSrcF32: the fp32 network input, 8 values per granule, converted to
+  // bf16 when stored to LDS (as pack_input_kernel converts them)
+  constexpr int kF32 = SRC == kSrcF32 ? S::kAPer : 1;
+  float pf[kF32][8];
   auto fetch = [&](int tile, int kc, bool with_b) {
     const int y0 = (tile / tiles_x) * R, x0 = (tile % tiles_x) * TP;
 #pragma unroll
     for (int i = 0; i < S::kAPer; ++i) {
       const int f = t + 256 * i;
       pa[i] = zero;
+      if constexpr (SRC == kSrcF32)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pf[i][e] = 0.f;
       if (f < S::kAGr) {
         const int g = f & 3, pp = f >> 2, row = pp / XP, px = pp - row * XP;
         const int sy = y0 + row - 1, sx = x0 + px - 1, c = kc + 8 * g;
         if (sy >= 0 && sy < a.H && sx >= 0 && sx < a.W && c < a.cin) {
+          if constexpr (SRC == kSrcF32) {
+            const float* q = a.kin + (uint32_t(sy) * uint32_t(a.W) + uint32_t(sx)) * uint32_t(a.inputs);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              if (c + e < a.inputs) pf[i][e] = q[c + e];
+            continue;
+          }
           const uint16_t* p;
           if (SRC == kSrcUpCat && c < a.cu)
-            p = a.xl + (size_t(sy >> 1) * (a.W >> 1) + (sx >> 1)) * a.cu + c;
+            p = a.xl + (uint32_t(sy >> 1) * uint32_t(a.W >> 1) + uint32_t(sx >> 1)) * uint32_t(a.cu) +
+                uint32_t(c);
           else
-            p = a.x + (size_t(sy) * a.W + sx) * a.xs + a.xo + (SRC == kSrcUpCat ? c - a.cu : c);
+            p = a.x + (uint32_t(sy) * uint32_t(a.W) + uint32_t(sx)) * uint32_t(a.xs) +
+                uint32_t(a.xo + (SRC == kSrcUpCat ? c - a.cu : c));
           pa[i] = *reinterpret_cast<const uint4*>(p);
         }
       }
@@ -251,7 +270,8 @@ __global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
       if (f < S::kBGr) {
         const int g = f & 3, q = f >> 2, tap = q / TN, n = q - tap * TN, c = kc + 8 * g;
         if (c < a.cin)
-          pb[i] = *reinterpret_cast<const uint4*>(a.w + (size_t(tap) * a.cout_pad + n0 + n) * a.cin + c);
+          pb[i] = *reinterpret_cast<const uint4*>(
+              a.w + (uint32_t(tap) * uint32_t(a.cout_pad) + uint32_t(n0 + n)) * uint32_t(a.cin) + uint32_t(c));
       }
     }
   };
@@ -259,6 +279,13 @@ __global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
 #pragma unroll
     for (int i = 0; i < S::kAPer; ++i) {
       const int f = t + 256 * i;
+      if constexpr (SRC == kSrcF32) {
+        uint32_t w[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          w[e] = f2bf2(pf[i][2 * e], pf[i][2 * e + 1]);
+        pa[i] = make_uint4(w[0], w[1], w[2], w[3]);
+      }
       if (f < S::kAGr) *reinterpret_cast<uint4*>(&sA[(f >> 2) * kPitch + 8 * (f & 3)]) = pa[i];
     }
     if (!with_b) return;
@@ -392,17 +419,18 @@ __global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
 #pragma unroll
             for (int r = 0; r < RW; ++r) {
               float o32[4];
-              uint16_t h[4];
 #pragma unroll
               for (int e = 0; e < 4; ++e) {
                 o32[e] = acc[r][j][gq * 4 + e] + bn[e];
                 if (a.relu) o32[e] = fmaxf(o32[e], 0.f);
-                h[e] = f2bf(o32[e]);
               }
-              o16[r][0] = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-              o16[r][1] = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+              o16[r][0] = f2bf2(o32[0], o32[1]);
+              o16[r][1] = f2bf2(o32[2], o32[3]);
               if (y + r >= a.H || px >= a.W || n >= a.cout) continue;
-              const size_t o = (size_t(y + r) * a.W + px) * a.ys + a.yo + n;
+              // element offsets stay below 2^31 (at most 2^24 pixels x 96
+              // channels at full resolution, 160 at half)
+              const uint32_t o = (uint32_t(y + r) * uint32_t(a.W) + uint32_t(px)) * uint32_t(a.ys) +
+                                 uint32_t(a.yo + n);
               if (n + 3 < a.cout) {  // 8-byte bf16 / 16-byte fp32 stores
                 if (a.y16) *reinterpret_cast<uint2*>(a.y16 + o) = make_uint2(o16[r][0], o16[r][1]);
                 if (a.y32) {
@@ -416,7 +444,7 @@ __global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
                 }
               } else {
                 for (int e = 0; n + e < a.cout; ++e) {
-                  if (a.y16) a.y16[o + e] = h[e];
+                  if (a.y16) a.y16[o + e] = uint16_t(o16[r][e >> 1] >> ((e & 1) * 16));
                   if (a.y32) a.y32[o + e] = a.accum ? a.y32[o + e] + o32[e] : o32[e];
                 }
               }
@@ -428,23 +456,25 @@ __global__ __launch_bounds__(256, 2) void conv_kernel(ConvArgs a) {
 #pragma unroll
               for (int r = 0; r < RW; ++r)
 #pragma unroll
-                for (int d = 0; d < 2; ++d) nb16[r][d] = uint32_t(__shfl_xor(int(o16[r][d]), 1, 64));
+                for (int d = 0; d < 2; ++d)  // DPP quad_perm [1, 0, 3, 2]: lane l ^ 1
+                  nb16[r][d] = uint32_t(__builtin_amdgcn_mov_dpp(int(o16[r][d]), 0xB1, 0xF, 0xF, false));
               if ((r32 & 1) == 0 && y < a.H && px < a.W && n < a.cout) {
-                uint16_t m[4];
+                float mf[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                   const int d = e >> 1, sh = (e & 1) * 16;
                   const float s = bf2f(uint16_t(o16[0][d] >> sh)) + bf2f(uint16_t(nb16[0][d] >> sh)) +
                                   bf2f(uint16_t(o16[1][d] >> sh)) + bf2f(uint16_t(nb16[1][d] >> sh));
-                  m[e] = f2bf(0.25f * s);
+                  mf[e] = 0.25f * s;
                 }
-                const size_t o = (size_t(y >> 1) * (a.W >> 1) + (px >> 1)) * a.ps + n;
+                const uint32_t m01 = f2bf2(mf[0], mf[1]), m23 = f2bf2(mf[2], mf[3]);
+                const uint32_t o = (uint32_t(y >> 1) * uint32_t(a.W >> 1) + uint32_t(px >> 1)) *
+                                       uint32_t(a.ps) + uint32_t(n);
                 if (n + 3 < a.cout)
-                  *reinterpret_cast<uint2*>(a.pool16 + o) =
-                      make_uint2(uint32_t(m[0]) | (uint32_t(m[1]) << 16),
-                                 uint32_t(m[2]) | (uint32_t(m[3]) << 16));
+                  *reinterpret_cast<uint2*>(a.pool16 + o) = make_uint2(m01, m23);
                 else
-                  for (int e = 0; n + e < a.cout; ++e) a.pool16[o + e] = m[e];
+                  for (int e = 0; n + e < a.cout; ++e)
+                    a.pool16[o + e] = uint16_t((e < 2 ? m01 : m23) >> ((e & 1) * 16));
               }
             }
           }
@@ -763,13 +793,13 @@ hipError_t conv(const ConvArgs& a, int src, int epi, hipStream_t s) {
   if (src == kSrcUpCat && (a.cu % 8 || a.cu >= a.cin || a.xo + a.cin - a.cu > a.xs || a.H % 2 ||
                            a.W % 2 || !a.xl))
     return hipErrorInvalidValue;
+  if (src == kSrcF32 && (!a.kin || a.inputs > a.cin)) return hipErrorInvalidValue;
   if (epi == kEpiPool && (a.H % 2 || a.W % 2 || !a.pool16 || !a.y16)) return hipErrorInvalidValue;
   if (epi == kEpiKpn && (a.cout_pad != kLogitStride || !a.kin || !a.pred)) return hipErrorInvalidValue;
-  // workgroup shape by output channels: 64 per wave (2 x 2 waves of 32
-  // pixels x 2 rows), 96 per wave (same, for the 96-channel level), else 32
-  // per wave (4 waves along x)
+  // workgroup shape: 2 x 2 waves of 32 pixels x 2 rows (64 pixels x 4 rows),
+  // each with 64 output channels, 96 (the 96-channel level) or 32
   const int shape = a.cout_pad % 64 == 0 ? 2 : a.cout_pad % 96 == 0 ? 3 : 1;
-  const int TP = shape == 1 ? 128 : 64, R = shape == 1 ? 2 : 4, TN = 32 * shape;
+  const int TP = 64, R = 4, TN = 32 * shape;
   if (epi == kEpiKpn && shape != 2) return hipErrorInvalidValue;
   // a resident grid (two workgroups per CU, one for the 96-channel shape)
   // whose workgroups loop over the tiles
@@ -778,9 +808,10 @@ hipError_t conv(const ConvArgs& a, int src, int epi, hipStream_t s) {
   const dim3 grid(unsigned(std::min(tiles, resident)), unsigned(ny));
   const int key = shape * 100 + src * 10 + epi;
   switch (key) {
-    case 100: hipLaunchKernelGGL((conv_kernel<4, 1, 1, kSrcPlain, kEpiPlain>), grid, dim3(256), 0, s, a); break;
-    case 101: hipLaunchKernelGGL((conv_kernel<4, 1, 1, kSrcPlain, kEpiPool>), grid, dim3(256), 0, s, a); break;
-    case 110: hipLaunchKernelGGL((conv_kernel<4, 1, 1, kSrcUpCat, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 100: hipLaunchKernelGGL((conv_kernel<2, 2, 1, kSrcPlain, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 101: hipLaunchKernelGGL((conv_kernel<2, 2, 1, kSrcPlain, kEpiPool>), grid, dim3(256), 0, s, a); break;
+    case 110: hipLaunchKernelGGL((conv_kernel<2, 2, 1, kSrcUpCat, kEpiPlain>), grid, dim3(256), 0, s, a); break;
+    case 120: hipLaunchKernelGGL((conv_kernel<2, 2, 1, kSrcF32, kEpiPlain>), grid, dim3(256), 0, s, a); break;
     case 200: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcPlain, kEpiPlain>), grid, dim3(256), 0, s, a); break;
     case 201: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcPlain, kEpiPool>), grid, dim3(256), 0, s, a); break;
     case 202: hipLaunchKernelGGL((conv_kernel<2, 2, 2, kSrcPlain, kEpiKpn>), grid, dim3(256), 0, s, a); break;

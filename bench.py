@@ -180,6 +180,13 @@ def denoised_time_to_rmse(r, scene, ttr: dict, ref, ref_var, ms_per_frame: float
     out = {"frame_ms_with_aov": round(ms_aov, 4), "frame_ms_with_moments": round(ms_m2, 4),
            "aov_frames": n_aov,
            "network_ms": round(ms_net, 3),
+           # algorithmic FLOPs of the U-Net on the workload image
+           "network_gflop": round(denoise.network_gflop(scene.width, scene.height, scene.channels), 2),
+           "network_tflops": round(denoise.network_gflop(scene.width, scene.height, scene.channels)
+                                   / ms_net, 1),
+           "network_frac_of_bf16_peak": round(denoise.network_gflop(scene.width, scene.height,
+                                                                    scene.channels)
+                                              / ms_net / denoise.BF16_PEAK_TFLOPS, 4),
            "train": {"pairs": n_pairs, "size": args.dn_size, "steps": args.dn_steps,
                      "scenes": args.dn_scenes, "views": args.dn_views, "levels": list(denoise.TRAIN_LEVELS),
                      "final_loss": float(np.mean(losses[-20:])),

@@ -344,6 +344,29 @@ __global__ __launch_bounds__(256) void majorant_kernel(MajorantParams P) {
   }
 }
 
+// Brick copy of the volume (the samplers' layout, VolumeParams): output
+// voxel o of a channel is voxel (o & 3, (o >> 2) & 3, o >> 4) of brick
+// o >> 6 (bricks x fastest); voxels past the volume's edge are zero (the
+// samplers clamp their indices, so they are never read). One thread per
+// output voxel, grid (bricks * 64 / 256, channels).
+template <typename T>
+__global__ __launch_bounds__(256) void brick_kernel(const uint8_t* __restrict__ vol, uint64_t chan_stride,
+                                                    int nx, int ny, int nz, int nbx, int nby,
+                                                    uint64_t per, uint8_t* __restrict__ bricks,
+                                                    uint64_t brick_stride) {
+  const uint64_t o = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+  if (o >= per) return;
+  const uint32_t b = uint32_t(o >> 6), w = uint32_t(o & 63u);
+  const uint32_t bx = b % uint32_t(nbx), r = b / uint32_t(nbx);
+  const uint32_t by = r % uint32_t(nby), bz = r / uint32_t(nby);
+  const uint32_t x = bx * 4u + (w & 3u), y = by * 4u + ((w >> 2) & 3u), z = bz * 4u + (w >> 4);
+  const T* src = reinterpret_cast<const T*>(vol + size_t(blockIdx.y) * chan_stride);
+  T* dst = reinterpret_cast<T*>(bricks + size_t(blockIdx.y) * brick_stride);
+  dst[o] = (x < uint32_t(nx) && y < uint32_t(ny) && z < uint32_t(nz))
+               ? src[(size_t(z) * uint32_t(ny) + y) * uint32_t(nx) + x]
+               : T(0);
+}
+
 template <int DT>
 void launch_pencils(const RangeParams& p, hipStream_t s) {
   const dim3 grid(unsigned(p.mc[1]), unsigned(p.mc[2]), unsigned(p.channels));
@@ -375,6 +398,26 @@ hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s) {
     case MDWT_DT_U16: hipLaunchKernelGGL(range_kernel<MDWT_DT_U16>, grid, dim3(256), 0, s, p); break;
     case MDWT_DT_F16: hipLaunchKernelGGL(range_kernel<MDWT_DT_F16>, grid, dim3(256), 0, s, p); break;
     default: hipLaunchKernelGGL(range_kernel<MDWT_DT_F32>, grid, dim3(256), 0, s, p); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks(const VolumeParams& v, int dtype, hipStream_t s) {
+  const uint64_t per = uint64_t(v.nbx) * uint64_t(v.nby) * uint64_t((v.nz + 3) / 4) * 64u;
+  const dim3 grid(unsigned((per + 255) / 256), unsigned(v.channels));
+  switch (dtype_bytes(dtype)) {
+    case 1:
+      hipLaunchKernelGGL(brick_kernel<uint8_t>, grid, dim3(256), 0, s, v.vol, v.chan_stride, v.nx, v.ny,
+                         v.nz, v.nbx, v.nby, per, const_cast<uint8_t*>(v.bricks), v.brick_stride);
+      break;
+    case 2:
+      hipLaunchKernelGGL(brick_kernel<uint16_t>, grid, dim3(256), 0, s, v.vol, v.chan_stride, v.nx, v.ny,
+                         v.nz, v.nbx, v.nby, per, const_cast<uint8_t*>(v.bricks), v.brick_stride);
+      break;
+    default:
+      hipLaunchKernelGGL(brick_kernel<uint32_t>, grid, dim3(256), 0, s, v.vol, v.chan_stride, v.nx, v.ny,
+                         v.nz, v.nbx, v.nby, per, const_cast<uint8_t*>(v.bricks), v.brick_stride);
+      break;
   }
   return hipGetLastError();
 }

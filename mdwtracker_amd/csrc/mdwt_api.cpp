@@ -109,6 +109,8 @@ struct mdwt_renderer {
   // device memory
   uint8_t* d_vol = nullptr;
   uint64_t chan_stride = 0;
+  uint8_t* d_bricks = nullptr;  // brick copy of d_vol, read by the samplers
+  uint64_t brick_stride = 0;
   float* d_ranges = nullptr;
   float* d_maj = nullptr;
   Float4* d_tf = nullptr;
@@ -511,6 +513,10 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
   mdwt::VolumeParams v{};
   v.vol = h->d_vol;
   v.chan_stride = h->chan_stride;
+  v.bricks = h->d_bricks;
+  v.brick_stride = h->brick_stride;
+  v.nbx = (h->dims[0] + 3) / 4;
+  v.nby = (h->dims[1] + 3) / 4;
   v.nx = h->dims[0];
   v.ny = h->dims[1];
   v.nz = h->dims[2];
@@ -522,6 +528,22 @@ mdwt::VolumeParams volume_params(const mdwt_renderer* h) {
   v.bhi[1] = h->remap.y;
   v.bhi[2] = h->remap.z;
   return v;
+}
+
+// Allocate and fill the brick copy of the volume.
+int make_bricks(mdwt_renderer* h) {
+  if (h->d_bricks) return MDWT_OK;
+  const size_t per = size_t((h->dims[0] + 3) / 4) * size_t((h->dims[1] + 3) / 4) *
+                     size_t((h->dims[2] + 3) / 4) * 64u * bytes_per_voxel(h->dtype);
+  h->brick_stride = (per + 255) & ~size_t(255);
+  if (hipMalloc(&h->d_bricks, h->brick_stride * h->C) != hipSuccess) {
+    h->d_bricks = nullptr;
+    return fail(h, MDWT_ERR_OOM, "device allocation of the volume bricks failed");
+  }
+  mdwt::VolumeParams v = volume_params(h);
+  v.bricks = h->d_bricks;
+  HIPCHK(h, mdwt::launch_bricks(v, h->dtype, h->stream));
+  return MDWT_OK;
 }
 
 // The wave-coherent schedule covers -m 0 / -m 2 on at most four channels;
@@ -666,7 +688,7 @@ int mdwt_destroy(mdwt_renderer* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_frame(h);
     free_batch(h);
-    void* ptrs[] = {h->d_vol, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win};
+    void* ptrs[] = {h->d_vol, h->d_bricks, h->d_ranges, h->d_maj, h->d_tf, h->d_counters, h->d_win};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -799,6 +821,7 @@ int mdwt_init(mdwt_renderer* h, int mode, const int mc[3], int auto_camera) {
                              hipMemcpyHostToDevice, h->stream));
   int rc0 = build_range_grid(h, wx, wy, wz);
   if (rc0 != MDWT_OK) return rc0;
+  if ((rc0 = make_bricks(h)) != MDWT_OK) return rc0;
   HIPCHK(h, hipMalloc(&h->d_counters, 4 * sizeof(unsigned long long)));
   HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device));
   // this device's copy of the allocation-free read's staging array

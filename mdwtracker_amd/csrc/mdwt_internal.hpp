@@ -18,6 +18,8 @@ size_t wave_lds_bytes(const RenderParams& p);
 
 hipError_t launch_ranges(const RangeParams& p, int dtype, hipStream_t s);
 hipError_t launch_majorants(const MajorantParams& p, hipStream_t s);
+// v.bricks (brick_stride bytes per channel) = v.vol in 4 x 4 x 4 bricks
+hipError_t launch_bricks(const VolumeParams& v, int dtype, hipStream_t s);
 hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s);
 // batched views, launch form 2: the persistent frame kernel over p.fr.items
 // (grid workgroups, workgroup g takes items g, g + grid, ...) at the ids in

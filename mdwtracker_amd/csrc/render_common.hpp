@@ -30,8 +30,9 @@ struct HitRec {
 
 template <int DT>
 struct Ctx {
-  const uint8_t* vol;
-  uint64_t chan_stride;
+  const uint8_t* bricks;
+  uint64_t brick_stride;
+  uint32_t brow, bslab;  // voxels per row / slab of bricks: 64 nbx, 64 nbx nby
   int nx, ny, nz;
   float scale, offset;
   const float4* tf;    // LDS
@@ -43,8 +44,10 @@ struct Ctx {
 
   __device__ __forceinline__ void init(const RenderParams& P, const float4* tf_lds,
                                        const ChanTf* ch_lds) {
-    vol = P.vol.vol;
-    chan_stride = P.vol.chan_stride;
+    bricks = P.vol.bricks;
+    brick_stride = P.vol.brick_stride;
+    brow = 64u * uint32_t(P.vol.nbx);
+    bslab = brow * uint32_t(P.vol.nby);
     nx = P.vol.nx;
     ny = P.vol.ny;
     nz = P.vol.nz;
@@ -65,7 +68,6 @@ struct Ctx {
   __device__ __forceinline__ float sample(V3 p, int c) const {
     if (p.x < 0.0f || p.x > 1.0f || p.y < 0.0f || p.y > 1.0f || p.z < 0.0f || p.z > 1.0f)
       return qnan();
-    const uint8_t* base = vol + size_t(c) * chan_stride;
     const float x = p.x * float(nx) - 0.5f;
     const float y = p.y * float(ny) - 0.5f;
     const float z = p.z * float(nz) - 0.5f;
@@ -74,13 +76,18 @@ struct Ctx {
     const uint32_t x0 = uint32_t(clamp_index(flx, nx)), x1 = uint32_t(clamp_index_plus1(flx, nx));
     const uint32_t y0 = uint32_t(clamp_index(fly, ny)), y1 = uint32_t(clamp_index_plus1(fly, ny));
     const uint32_t z0 = uint32_t(clamp_index(flz, nz)), z1 = uint32_t(clamp_index_plus1(flz, nz));
-    const uint32_t sx = uint32_t(nx), sxy = uint32_t(nx) * uint32_t(ny);
-    const uint32_t r00 = y0 * sx + z0 * sxy, r10 = y1 * sx + z0 * sxy;
-    const uint32_t r01 = y0 * sx + z1 * sxy, r11 = y1 * sx + z1 * sxy;
-    const float v000 = fetch_raw<DT>(base, r00 + x0), v100 = fetch_raw<DT>(base, r00 + x1);
-    const float v010 = fetch_raw<DT>(base, r10 + x0), v110 = fetch_raw<DT>(base, r10 + x1);
-    const float v001 = fetch_raw<DT>(base, r01 + x0), v101 = fetch_raw<DT>(base, r01 + x1);
-    const float v011 = fetch_raw<DT>(base, r11 + x0), v111 = fetch_raw<DT>(base, r11 + x1);
+    // the voxels are read from the 4 x 4 x 4 brick copy: index = X(x) +
+    // Y(y) + Z(z) (a trilinear footprint touches about 1.8 cache lines
+    // instead of 4 in the planar layout; 3.5% faster at -m 2 on 4 x 256^3 u8)
+    const uint8_t* base = bricks + size_t(c) * brick_stride;
+    const uint32_t ax0 = ((x0 >> 2) << 6) | (x0 & 3u), ax1 = ((x1 >> 2) << 6) | (x1 & 3u);
+    const uint32_t ay0 = (y0 >> 2) * brow + ((y0 & 3u) << 2), ay1 = (y1 >> 2) * brow + ((y1 & 3u) << 2);
+    const uint32_t az0 = (z0 >> 2) * bslab + ((z0 & 3u) << 4), az1 = (z1 >> 2) * bslab + ((z1 & 3u) << 4);
+    const uint32_t r00 = ay0 + az0, r10 = ay1 + az0, r01 = ay0 + az1, r11 = ay1 + az1;
+    const float v000 = fetch_raw<DT>(base, r00 + ax0), v100 = fetch_raw<DT>(base, r00 + ax1);
+    const float v010 = fetch_raw<DT>(base, r10 + ax0), v110 = fetch_raw<DT>(base, r10 + ax1);
+    const float v001 = fetch_raw<DT>(base, r01 + ax0), v101 = fetch_raw<DT>(base, r01 + ax1);
+    const float v011 = fetch_raw<DT>(base, r11 + ax0), v111 = fetch_raw<DT>(base, r11 + ax1);
     const float c00 = v000 + fx * (v100 - v000);
     const float c10 = v010 + fx * (v110 - v010);
     const float c01 = v001 + fx * (v101 - v001);
@@ -205,7 +212,10 @@ __device__ __forceinline__ void flush_counts(const RenderParams& P, const Counts
 // different XCDs: every XCD gets an even share of the heavy image regions
 // (giving each XCD a contiguous run instead left half of them idle for half
 // the frame: the volume's footprint is not spread evenly along the curve),
-// and each XCD still works along the curve, every 8th block.
+// and each XCD still works along the curve, every 8th block. Dealing runs of
+// 2, 4, 8 or 16 consecutive blocks to each XCD in turn (more L2 sharing
+// between neighbouring blocks) measured 0.5-2% slower on 4 x 256^3 u8 at
+// 1024^2.
 __device__ __forceinline__ uint32_t block_slot(uint32_t b, uint32_t /*n*/) { return b; }
 
 // running mean of the colour (deviceCode.cu:463-469) and, when m2 is set, of

@@ -35,6 +35,24 @@ def input_width(channels: int) -> int:
 
 
 INPUTS = input_width(4)  # the 4-channel workloads
+
+# (input channels, output channels, level) of the eleven 3x3 convolutions
+# (denoise_api.cpp kLayers; None = the network input; level l runs at
+# 1 / 4^l of the pixels); the last one writes the kTaps = 49 filter logits
+NETWORK = [(None, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2), (96, 96, 2),
+           (160, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0), (32, 49, 0)]
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def network_gflop(width: int, height: int, channels: int = 4) -> float:
+    """Algorithmic GFLOP of one inference: 2 x 9 x cin x cout per output
+    pixel of each convolution, real (unpadded) channel counts."""
+    n = width * height
+    total = 0
+    for cin, cout, level in NETWORK:
+        cin = input_width(channels) if cin is None else cin
+        total += 2 * 9 * cin * cout * (n >> (2 * level))
+    return total * 1e-9
 DEPTH_SCALE = 0.25   # first-hit distances of the orbit cameras are ~1..4 world units
 STDERR_SCALE = 4.0   # standard error of the pixel mean, ~0.002..0.5
 FRAMES_SCALE = 1.0 / 16.0  # log2 of the frame count, 0..16
@@ -338,7 +356,11 @@ def _main() -> int:
     big = Denoiser(args.infer, args.infer, seed=1)
     x = rng.uniform(0, 1, (args.infer, args.infer, INPUTS)).astype(np.float32)
     ms = [big.infer(x)[1] for _ in range(args.infer_runs)]
-    print(f"inference {args.infer}^2: {float(np.median(ms)):.3f} ms (median of {len(ms)})", flush=True)
+    med = float(np.median(ms))
+    gf = network_gflop(args.infer, args.infer)
+    print(f"inference {args.infer}^2: {med:.3f} ms (median of {len(ms)}), {gf:.1f} GFLOP, "
+          f"{gf / med:.1f} TFLOP/s = {100 * gf / med / BF16_PEAK_TFLOPS:.1f}% of the BF16 peak",
+          flush=True)
     big.close()
     return 0
 

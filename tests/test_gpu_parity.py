@@ -127,6 +127,38 @@ def test_ranges_u8_row_widths(capi, po, shape, mc):
     g.close()
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float16, np.float32])
+def test_bricked_sampling_bit_exact(capi, po, dtype):
+    """The samplers read the volume's 4 x 4 x 4 brick copy: on a volume whose
+    sides are no multiple of the brick side (partial bricks on every face),
+    every voxel type and mode renders the oracle's image bit for bit."""
+    rng = np.random.default_rng(23)
+    shape = (13, 18, 27)  # (z, y, x)
+    if dtype == np.uint8:
+        vols = [scenes.quantize_u8(scenes.gaussian_blobs(shape, 3, rng)) for _ in range(2)]
+    elif dtype == np.uint16:
+        vols = [rng.integers(0, 65535, shape, dtype=np.uint16) for _ in range(2)]
+    else:
+        vols = [(rng.standard_normal(shape) * 3).astype(dtype) for _ in range(2)]
+    tfs = [scenes.piecewise_tf(6, rng, c / 2.0, n_texels=64) for c in range(2)]
+    for mode in (0, 2, 3):
+        print(f"[bricks] dtype={np.dtype(dtype).name} mode={mode}", flush=True)
+        g, o = capi.MdwtRenderer(0), po.OracleRenderer()
+        for r in (g, o):
+            for v in vols:
+                r.push_channel(v)
+            r.init(mode, (5, 4, 3), True)
+            for c in range(2):
+                r.set_tf(c, tfs[c], 3.0, (0.0, 1.0))
+            r.set_frame(80, 56, 1, True, (0.1, 0.2, 0.3))
+            r.set_camera([1.8, -1.1, 2.6], [0.5, 0.5, 0.5], [0, 0, 1], 42.0)
+        g.render(3)
+        o.render(3, threads=THREADS)
+        np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
+        g.close()
+        o.close()
+
+
 def _ragged_scene(n_ch, seed):
     """n_ch u8 blob channels on a 23 x 17 x 30 (x, y, z) grid: no side a power
     of two, macrocells (5, 4, 7) that do not divide it."""

@@ -714,6 +714,11 @@ int mdwt_push_channel(mdwt_renderer* h, const void* voxels, int dtype, int nx, i
   const uint64_t nvox = uint64_t(nx) * uint64_t(ny) * uint64_t(nz);
   if (nvox > 0xFFFFFFFFull / 2)
     return fail(h, MDWT_ERR_LIMIT, "a channel must hold fewer than 2^31 voxels");
+  // the samplers' brick copy: 32-bit voxel indices, 24-bit products by the
+  // bricks per slice
+  const uint64_t bricks_xy = uint64_t((nx + 3) / 4) * uint64_t((ny + 3) / 4);
+  if (bricks_xy >= (uint64_t(1) << 24) || bricks_xy * uint64_t((nz + 3) / 4) * 64u > 0xFFFFFFFFull)
+    return fail(h, MDWT_ERR_LIMIT, "volume too large for the brick layout (at most 2^24 bricks per slice)");
   if (h->C > 0 && (nx != h->dims[0] || ny != h->dims[1] || nz != h->dims[2]))
     return fail(h, MDWT_ERR_ARG, "all channels must share dims");
   if (h->C > 0 && dtype != h->dtype) return fail(h, MDWT_ERR_ARG, "all channels must share dtype");

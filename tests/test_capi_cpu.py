@@ -152,6 +152,10 @@ def test_frame_limit_moments_and_stream_entry_points_without_gpu(capi):
     r.set_moments(True)
     r.set_moments(False)
     assert lib.mdwt_set_moments(None, 1) == 1
+    # 16384 x 16384 voxels per slice: 2^24 bricks of 4 x 4 x 4 per slice,
+    # beyond the samplers' 24-bit brick products (refused before any read)
+    one = np.zeros(1, np.uint8)
+    assert lib.mdwt_push_channel(r._h, one.ctypes.data_as(C.c_void_p), 0, 16384, 16384, 1) == 4
     g = C.c_float()
     assert lib.mdwt_stream_read(None, 1 << 20, 1, C.byref(g)) == 1
     assert lib.mdwt_stream_read(r._h, 8, 1, C.byref(g)) == 1   # fewer than 16 bytes

@@ -81,6 +81,7 @@ def read_valu(workload: str, mode: int, kernel_ms: float):
     return {"achieved": round(achieved / 1e9, 1), "peak": round(peak / 1e9, 1),
             "unit": "G wave64 VALU instructions/s", "frac": round(achieved / peak, 4),
             "instructions_per_launch": d["valu_instructions_per_launch"],
+            "lane_utilisation": d.get("lane_utilisation"),
             "source": str(p.relative_to(ROOT))}
 
 

@@ -12,15 +12,20 @@
 //               all rays are packed into one list (K grows as rays finish, so
 //               a round always holds about kRecords records);
 //   2. enter:   the wave takes the list 64 records at a time: one lane per
-//               record loads the cell's majorants, draws the first free-flight
-//               step of every channel from one Philox block and decides
-//               "leaves the cell" / "reaches the end of the root box" /
-//               "needs a sample";
-//   3. sample:  the (record, channel) steps that need a sample are pushed on
-//               a per-wave LDS stack (ballot + mbcnt) and popped 64 at a time:
-//               threshold, trilinear sample, transfer function, and on a null
-//               collision the next step, pushed again while it stays inside;
-//   4. resolve: each ray reads the outcomes of its records in cell order with
+//               record loads the cell's majorants and draws the first
+//               free-flight distance of every channel from one Philox block;
+//               a conservative bound settles the (record, channel) pairs
+//               whose first step certainly leaves the cell, the others are
+//               pushed on the first-step stack (ballot + mbcnt);
+//   3. first:   full batches of 64 popped from that stack compute the exact
+//               first step and, when it lands inside the cell, its
+//               null-collision test; a rejected step's successor goes on the
+//               sample stack;
+//   4. sample:  full batches of 64 from the sample stack: threshold, trilinear
+//               sample, transfer function and, on a null collision, the next
+//               step, pushed again while it stays inside; the round's last,
+//               partial batch runs each lane to its item's end;
+//   5. resolve: each ray reads the outcomes of its records in cell order with
 //               the reference's rules and either ends (hit, or the root box's
 //               end) or continues in the next round.
 //

@@ -146,6 +146,13 @@ struct mdwt_renderer {
   uint32_t* d_items = nullptr;
   unsigned* d_work = nullptr;            // [1] frame id, [2] accumulation id of launch form 2
   unsigned long long* d_item_ticks = nullptr;
+  // launch form 2's item list: the items in the order the persistent grid
+  // takes them (workgroup g takes positions g, g + G, ...), re-dealt by the
+  // measured item costs after each call; pitems_measured when d_item_ticks
+  // holds the last form-2 frame's stamps of this list
+  std::vector<uint32_t> pitems;
+  uint32_t* d_pitems = nullptr;
+  bool pitems_measured = false;
   uint32_t batch_frame = 0, batch_accum = 0;
   int batch_grid = 0, batch_grid_key = -1;  // resident grid of launch form 2
   hipGraph_t graph = nullptr;               // launch form 2: one captured frame
@@ -1434,11 +1441,14 @@ void free_batch(mdwt_renderer* h) {
     if (v.m2) (void)hipFree(v.m2);
   }
   h->views.clear();
-  void* ptrs[] = {h->d_views, h->d_items, h->d_work, h->d_item_ticks};
+  void* ptrs[] = {h->d_views, h->d_items, h->d_pitems, h->d_work, h->d_item_ticks};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   h->d_views = nullptr;
   h->d_items = nullptr;
+  h->d_pitems = nullptr;
+  h->pitems.clear();
+  h->pitems_measured = false;
   h->d_work = nullptr;
   h->d_item_ticks = nullptr;
   h->items.clear();
@@ -1497,13 +1507,19 @@ int build_items(mdwt_renderer* h) {
       items.push_back((uint32_t(v) << 20) | ((o >> 16) << 10) | (o & 0xffffu));
   }
   if (h->d_items) (void)hipFree(h->d_items);
+  if (h->d_pitems) (void)hipFree(h->d_pitems);
   if (h->d_item_ticks) (void)hipFree(h->d_item_ticks);
   h->d_items = nullptr;
+  h->d_pitems = nullptr;
   h->d_item_ticks = nullptr;
   HIPCHK(h, hipMalloc(&h->d_items, items.size() * sizeof(uint32_t)));
+  HIPCHK(h, hipMalloc(&h->d_pitems, items.size() * sizeof(uint32_t)));
   HIPCHK(h, hipMalloc(&h->d_item_ticks, 2 * items.size() * sizeof(unsigned long long)));
   HIPCHK(h, hipMemcpy(h->d_items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_pitems, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(h->d_item_ticks, 0, 2 * items.size() * sizeof(unsigned long long)));
+  h->pitems = items;
+  h->pitems_measured = false;
   h->items = std::move(items);
   return build_view_orders(h);
 }
@@ -1611,10 +1627,10 @@ mdwt::RenderParams views_params(const mdwt_renderer* h) {
 }
 
 // Per-view time of the last one-launch frame from its workgroups' wall-clock
-// stamps (100 MHz): from the first workgroup's start to the view's last
-// workgroup's end.
-int views_times_from_ticks(mdwt_renderer* h) {
-  const size_t n = h->items.size();
+// stamps (100 MHz; stamp i belongs to list[i]): from the first workgroup's
+// start to the view's last workgroup's end.
+int views_times_from_ticks(mdwt_renderer* h, const std::vector<uint32_t>& list) {
+  const size_t n = list.size();
   std::vector<unsigned long long> t(2 * n);
   HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
                       hipMemcpyDeviceToHost));
@@ -1622,7 +1638,7 @@ int views_times_from_ticks(mdwt_renderer* h) {
   for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[2 * i]);
   std::vector<unsigned long long> end(h->views.size(), 0ull);
   for (size_t i = 0; i < n; ++i) {
-    const size_t v = h->items[i] >> 20;
+    const size_t v = list[i] >> 20;
     end[v] = std::max(end[v], t[2 * i + 1]);
   }
   h->view_us.assign(h->views.size(), 0.f);
@@ -1647,7 +1663,7 @@ int batch_render_one_launch(mdwt_renderer* h, int n_frames, float* ms_out) {
   float ms = 0.f;
   HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   if (n_frames > 0) {
-    const int rc = views_times_from_ticks(h);
+    const int rc = views_times_from_ticks(h, h->items);
     if (rc != MDWT_OK) return rc;
   }
   h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
@@ -1662,9 +1678,40 @@ mdwt::RenderParams persistent_params(const mdwt_renderer* h) {
   mdwt::RenderParams p = views_params(h);
   p.fr.frame_id = 0;  // read from work[1] / work[2] instead
   p.fr.accum_id = 0;
+  p.fr.items = h->d_pitems;
   p.fr.work = h->d_work;
   p.fr.item_ticks = h->d_item_ticks;
   return p;
+}
+
+// Re-deal form 2's items by the costs the last form-2 frame measured (its
+// per-item wall-clock stamps): items by decreasing cost, dealt to the G
+// resident workgroups in alternating directions per round (g, then G - 1 - g),
+// so that each workgroup's sum of costs is about the same (longest
+// processing time first, with a static assignment). The list changes in
+// place: the captured graph reads it at each replay.
+int deal_by_cost(mdwt_renderer* h, int G) {
+  const size_t n = h->pitems.size();
+  if (n == 0 || G <= 0) return MDWT_OK;
+  std::vector<unsigned long long> t(2 * n);
+  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+  std::vector<size_t> pos(n);
+  for (size_t i = 0; i < n; ++i) pos[i] = i;
+  auto cost = [&](size_t i) { return t[2 * i + 1] >= t[2 * i] ? t[2 * i + 1] - t[2 * i] : 0ull; };
+  std::stable_sort(pos.begin(), pos.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+  std::vector<uint32_t> dealt(n);
+  const size_t g_all = size_t(G);
+  for (size_t idx = 0; idx < n; ++idx) {
+    const size_t r = idx / g_all, g = idx % g_all;
+    const size_t cnt = std::min(g_all, n - r * g_all);
+    const size_t src = r * g_all + ((r & 1) ? cnt - 1 - g : g);
+    dealt[idx] = h->pitems[pos[src]];
+  }
+  h->pitems = std::move(dealt);
+  HIPCHK(h, hipMemcpyAsync(h->d_pitems, h->pitems.data(), n * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, h->stream));
+  return MDWT_OK;
 }
 
 // resident workgroups of the persistent launch (cached per kernel choice),
@@ -1714,6 +1761,7 @@ int batch_render_persistent(mdwt_renderer* h, int n_frames, float* ms_out) {
     h->graph_grid = grid;
     h->graph_valid = true;
   }
+  if (h->pitems_measured && (rc = deal_by_cost(h, grid)) != MDWT_OK) return rc;
   const unsigned ids[3] = {0u, h->batch_frame, h->batch_accum};
   HIPCHK(h, hipMemcpyAsync(h->d_work, ids, sizeof(ids), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
@@ -1725,8 +1773,9 @@ int batch_render_persistent(mdwt_renderer* h, int n_frames, float* ms_out) {
   h->batch_frame += uint32_t(n_frames);
   h->batch_accum = h->accumulate ? h->batch_accum + uint32_t(n_frames) : 0u;
   if (n_frames > 0) {
-    rc = views_times_from_ticks(h);
+    rc = views_times_from_ticks(h, h->pitems);
     if (rc != MDWT_OK) return rc;
+    h->pitems_measured = true;
   }
   h->last_kernel_ms = n_frames > 0 ? ms / float(n_frames) : 0.f;
   if (ms_out) *ms_out = ms;

@@ -516,39 +516,62 @@ __global__ __launch_bounds__(64) void wgrad_band_kernel(const uint16_t* __restri
   const int y0 = band * rows, y1 = min(H, y0 + rows);
   const int r32 = l & 31, kq = (l >> 5) * 4;
   f16v acc[3] = {f16v{}, f16v{}, f16v{}};
-  for (int y = y0; y < y1; ++y) {
-    const int sy = y + dyi - 1;
-    for (int x0 = 0; x0 < W; x0 += kSeg) {
-      for (int i = l; i < kSeg * 8; i += 64) {  // gradient: 32 px x 8 groups of 4 channels
-        const int px = i >> 3, grp = i & 7, c = co0 + grp * 4, xx = x0 + px;
-        const s4 v = load4(g + (size_t(y) * W + xx) * gs + c, xx < W && c < cout);
+  // stages: (row y, 32-pixel segment) in row order; the next stage's
+  // gradient (32 px x 8 groups of 4 channels) and input row y + dy - 1
+  // (34 px x 8 groups) are loaded into registers while this one is multiplied
+  constexpr int kGPer = kSeg * 8 / 64, kXPer = ((kSeg + 2) * 8 + 63) / 64;
+  const int segs = (W + kSeg - 1) / kSeg;
+  const int nst = (y1 - y0) * segs;
+  s4 pg[kGPer], pxv[kXPer];
+  auto fetch = [&](int st) {
+    const int y = y0 + st / segs, x0 = (st % segs) * kSeg, sy = y + dyi - 1;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          gT[(grp * 4 + j) * kRow + px] = c + j < cout ? uint16_t(v[j]) : uint16_t(0);
-      }
-      for (int i = l; i < (kSeg + 2) * 8; i += 64) {  // input row sy: 34 px x 8 groups
-        const int grp = i & 7, px = i >> 3;
-        const int sx = x0 + px - 1, c = ci0 + grp * 4;
-        const bool in = sy >= 0 && sy < H && sx >= 0 && sx < W && c < cin;
-        const s4 v = load4(x + (size_t(in ? sy * W + sx : 0) * xs + xo + (in ? c : 0)), in);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          xT[(grp * 4 + j) * kRow + px] = c + j < cin ? uint16_t(v[j]) : uint16_t(0);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < kSeg / 8; ++ks) {
-        const int kb = ks * 8 + kq;  // this lane's 4 pixels of the K step
-        const s4 a = *reinterpret_cast<const s4*>(&gT[r32 * kRow + kb]);
-#pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-          const uint16_t* xr = &xT[r32 * kRow + kb + dx];
-          const s4 b = s4{short(xr[0]), short(xr[1]), short(xr[2]), short(xr[3])};
-          acc[dx] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[dx], 0, 0, 0);
-        }
-      }
-      __syncthreads();
+    for (int i = 0; i < kGPer; ++i) {
+      const int idx = l + 64 * i, px = idx >> 3, grp = idx & 7, c = co0 + grp * 4, xx = x0 + px;
+      pg[i] = load4(g + (size_t(y) * W + xx) * gs + c, xx < W && c < cout);
     }
+#pragma unroll
+    for (int i = 0; i < kXPer; ++i) {
+      const int idx = l + 64 * i, grp = idx & 7, px = idx >> 3;
+      const int sx = x0 + px - 1, c = ci0 + grp * 4;
+      const bool in = idx < (kSeg + 2) * 8 && sy >= 0 && sy < H && sx >= 0 && sx < W && c < cin;
+      pxv[i] = load4(x + (size_t(in ? sy * W + sx : 0) * xs + xo + (in ? c : 0)), in);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < kGPer; ++i) {
+      const int idx = l + 64 * i, px = idx >> 3, grp = idx & 7, c = co0 + grp * 4;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        gT[(grp * 4 + j) * kRow + px] = c + j < cout ? uint16_t(pg[i][j]) : uint16_t(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kXPer; ++i) {
+      const int idx = l + 64 * i, grp = idx & 7, px = idx >> 3, c = ci0 + grp * 4;
+      if (idx >= (kSeg + 2) * 8) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        xT[(grp * 4 + j) * kRow + px] = c + j < cin ? uint16_t(pxv[i][j]) : uint16_t(0);
+    }
+  };
+  if (nst > 0) fetch(0);
+  for (int st = 0; st < nst; ++st) {
+    store();
+    __syncthreads();
+    if (st + 1 < nst) fetch(st + 1);
+#pragma unroll
+    for (int ks = 0; ks < kSeg / 8; ++ks) {
+      const int kb = ks * 8 + kq;  // this lane's 4 pixels of the K step
+      const s4 a = *reinterpret_cast<const s4*>(&gT[r32 * kRow + kb]);
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const uint16_t* xr = &xT[r32 * kRow + kb + dx];
+        const s4 b = s4{short(xr[0]), short(xr[1]), short(xr[2]), short(xr[3])};
+        acc[dx] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[dx], 0, 0, 0);
+      }
+    }
+    __syncthreads();
   }
   const int ci = ci0 + r32;
   if (ci >= cin) return;
@@ -845,6 +868,10 @@ hipError_t conv3x3(const uint16_t* x, int H, int W, int xs, int xo, int cin, con
   return conv(a, kSrcPlain, kEpiPlain, s);
 }
 
+// waves of one weight-gradient launch (8192 measured no faster: the band
+// reduction grows with them)
+constexpr int kWgradWaves = 2048;
+
 hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, int H, int W,
                  int cin, int cout, float* dw, float* part, size_t part_cap, hipStream_t s) {
   // 8-byte channel groups: strides and offsets in whole groups, and the
@@ -854,8 +881,8 @@ hipError_t wgrad(const uint16_t* g, int gs, const uint16_t* x, int xs, int xo, i
   const int tiles = ((cin + 31) / 32) * ((cout + 31) / 32);
   const size_t per_band = size_t(9) * cout * cin;
   if (per_band > part_cap) return hipErrorInvalidValue;
-  // about 2048 waves (3 tap rows per band), within the partial-sum buffer
-  int bands = std::max(1, std::min(H, std::min((683 + tiles - 1) / tiles, int(part_cap / per_band))));
+  // about kWgradWaves waves (3 tap rows per band), within the partial-sum buffer
+  int bands = std::max(1, std::min(H, std::min((kWgradWaves / 3 + tiles - 1) / tiles, int(part_cap / per_band))));
   const int rows = (H + bands - 1) / bands;
   bands = (H + rows - 1) / rows;
   const dim3 grid(unsigned((cin + 31) / 32), unsigned((cout + 31) / 32), unsigned(bands * 3));

@@ -270,11 +270,12 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
                   int n_threads, const int* tiles, int n_tiles, int frames, float* table,
                   int best[3]);
 /* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
- * loops, a workgroup per 16x16 block (default); 1 = flat step loop in
- * persistent waves that refill lanes from 8 interleaved pixel queues; 2 = flat
- * step loop, a workgroup per 16x16 block. Images do not depend on it. Modes
- * other than 0, 2 and 3 always run schedule 0. No reference counterpart (OptiX schedules the
- * reference's launch). */
+ * loops; 1 = wave-coherent tracking (a wave traces 64 rays and compacts their
+ * free-flight steps and null-collision tests into full 64-lane batches), the
+ * default. Schedule 1 covers -m 0 and -m 2 on at most four channels without
+ * lighting; everything else runs schedule 0. Images do not depend on it.
+ * Returns MDWT_ERR_ARG for other values. No reference counterpart (OptiX
+ * schedules the reference's launch). */
 int mdwt_set_schedule(mdwt_renderer* h, int schedule);
 /* Device copy of `bytes` bytes, `iters` times (one read and one write per
  * byte); GB/s of (read + write) bytes. No reference counterpart. */

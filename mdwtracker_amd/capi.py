@@ -340,9 +340,9 @@ class MdwtRenderer:
         return a
 
     def set_schedule(self, schedule: int) -> None:
-        """0: one thread per pixel, nested loops (default); 1: flat step loop in
-        persistent waves with lane refill; 2: flat step loop, a workgroup per
-        16x16 block; 3: wave-coherent tracking (-m 0 / -m 2 only)."""
+        """0: one thread per pixel, nested loops; 1: wave-coherent tracking
+        (the default; -m 0 / -m 2 on at most four channels without lighting,
+        everything else runs schedule 0). Images do not depend on it."""
         self._chk(lib.mdwt_set_schedule(self._h, schedule))
 
     # -- frames -------------------------------------------------------------

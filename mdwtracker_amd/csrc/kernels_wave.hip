@@ -56,21 +56,23 @@
 namespace mdwt {
 namespace {
 
-constexpr int kRecords = 160;  // records per wave and round
-constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most
+constexpr int kRecords = 316;  // records per wave and round
+constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most (-m 0: 16 x 4 box bits)
 constexpr int kStack = 128;    // each stack: < 64 left + one channel's 64 pushes
 // Safety factor of the certain-exit test (see the enter phase): far above the
 // few fp32 roundings (relative 2^-21) that it has to absorb.
 constexpr float kExitSafety = 1.0001f;
-
-enum : uint32_t { kLeave = 0u, kBoxEnd = 1u, kHit = 2u, kPending = 3u };
+constexpr uint32_t kVisitBits = 26;  // visits of one ray below 2^26 (checked on the host)
+constexpr uint32_t kVisitMask = (1u << kVisitBits) - 1u;
+constexpr uint64_t kNoEvent = ~0ull;
 
 // A record: the macrocell of one (ray, visit), its entry and exit t (grid
-// units) and the visit number, read and written as one 16-byte word.
+// units), the visit number and the ray's lane (vl = visit | lane << 26), read
+// and written as one 16-byte word.
 struct Record {
   uint32_t cid;
   float t0, t1;
-  uint32_t visit;
+  uint32_t vl;
 };
 // A ray of the wave (one per lane): world origin and direction, grid <-> world
 // t scales, end of the root box (world t), 1 / (dt * worldToGridT), pixel and
@@ -81,28 +83,26 @@ struct WaveRay {
   float tend, idw;
   uint32_t pixel, mask;
 };
-// A stack entry: slot | channel << 8 | ray lane << 10 | pair << 16, a t or a
-// uniform, and the channel's majorant in the record's cell.
+// A stack entry: record slot | channel << 9 | pair << 11, a t or a uniform,
+// and the channel's majorant in the record's cell.
 struct Item {
   uint32_t id;
   float x, m;
 };
 
-// Per-wave LDS. Record slot = k * n_live + rank for the k-th cell of the
-// rank-th live ray, so the lanes of a walk or a resolve step touch
-// consecutive records.
+// Per-wave LDS. The records of a round are dense and k-major: the k-th cells
+// of all rays that walk that far are consecutive, so the lanes of a walk step
+// or an enter batch touch consecutive records. A ray's first event of the
+// round (the smallest (visit, t, channel rule) key among its hits, and for
+// -m 2 the last channel's end of the root box) is an LDS atomic minimum, so
+// only events write: leaving a cell writes nothing.
 struct WaveLds {
   Record rec[kRecords];
-  uint16_t order[kRecords];  // the round's slots, ascending: slot | ray lane << 8
-  // first event of each (record, channel): 2-bit code per channel (bits 2c,
-  // 2c + 1; pending entries are settled with LDS atomic AND, so lanes that
-  // settle different channels of one record never overwrite each other) and
-  // the event's t (grid units)
-  uint32_t code[kRecords];
-  float4 tev[kRecords];
   Item sample[kStack];  // steps that need a null-collision test (x = t)
   Item first[kStack];   // first steps to compute exactly (x = uniform)
   WaveRay ray[64];
+  unsigned long long ev[64];   // per ray: first event key of the round
+  unsigned long long box[64];  // -m 0, per ray: end-of-box bit 4 (visit mod 16) + channel
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
@@ -115,9 +115,33 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
-// settles channel c of a record from pending to code cd
-__device__ __forceinline__ void settle(uint32_t* word, int c, uint32_t cd) {
-  atomicAnd(word, ~((kPending ^ cd) << (2 * c)));
+// fp32 -> uint32 with the same order (negative values below positive ones)
+__device__ __forceinline__ uint32_t order_bits(float t) {
+  const uint32_t b = __float_as_uint(t);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float from_order_bits(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+// Event key: visit, then a flag (0: hit, 1: the last channel's end of the
+// root box, -m 2), then t, then 2 channel bits: 3 - channel for -m 0 (ties to
+// the higher channel), the channel for -m 2 (ties to the lower one). Within a
+// visit any hit ends -m 2's ray before the end of the box does.
+__device__ __forceinline__ uint64_t event_key(uint32_t visit, uint32_t flag, float t, uint32_t ch) {
+  return (uint64_t(visit) << 35) | (uint64_t(flag) << 34) | (uint64_t(order_bits(t)) << 2) | ch;
+}
+template <int MODE>
+__device__ __forceinline__ void on_hit(WaveLds& L, const Record& R, int c, float t) {
+  const uint32_t ch = MODE == 2 ? uint32_t(c) : uint32_t(3 - c);
+  atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 0u, t, ch));
+}
+template <int MODE>
+__device__ __forceinline__ void on_box_end(WaveLds& L, const Record& R, int c, int C, float t) {
+  if (MODE == 2) {
+    if (c == C - 1) atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
+  } else {
+    atomicOr(&L.box[R.vl >> kVisitBits], 1ull << (((R.vl & 15u) << 2) | uint32_t(c)));
+  }
 }
 __device__ __forceinline__ float comp(const float4& v, int c) {
   return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
@@ -153,18 +177,17 @@ __device__ __forceinline__ float alpha_lookup(const float* s_a, const ChanTf& t,
 // The null-collision test of a step that landed at t inside its cell (before
 // the root box's end), then the next free-flight step; pair j of the
 // (visit, channel) stream. Returns true when the step needs another test (tn
-// inside the cell and before the box's end); otherwise the (record, channel)
-// is settled here.
-template <int DT>
+// inside the cell and before the box's end); otherwise its event, if any, is
+// recorded here.
+template <int MODE, int DT>
 __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                          const float* s_a, V3 w2u, uint32_t sample,
-                                          uint32_t slot, int c, const Record& R,
-                                          const WaveRay& Y, uint32_t j, float t, float m,
-                                          float& tn) {
+                                          const float* s_a, V3 w2u, uint32_t sample, int c,
+                                          const Record& R, const WaveRay& Y, uint32_t j, float t,
+                                          float m, float& tn) {
   const float4 ow = Y.org_w2g, dg = Y.dir_g2w;
   uint32_t w[4];
-  philox(P.fr.frame_id, sample, 1u + ((R.visit << 4) | uint32_t(c)), (j >> 1) << 1, P.fr.seed,
-         Y.pixel, w);
+  philox(P.fr.frame_id, sample, 1u + (((R.vl & kVisitMask) << 4) | uint32_t(c)), (j >> 1) << 1,
+         P.fr.seed, Y.pixel, w);
   const float thr = to_unit((j & 1u) ? w[2] : w[0]) * m;
   const float un = to_unit((j & 1u) ? w[3] : w[1]);
   const float tWorld = t * dg.w;
@@ -176,18 +199,13 @@ __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& 
     hit = a > 0.0f && thr < a;
   }
   if (hit) {
-    reinterpret_cast<float*>(&L.tev[slot])[c] = t;
-    settle(&L.code[slot], c, kHit);
+    on_hit<MODE>(L, R, c, t);
     return false;
   }
   tn = t - (plogf_poly(1.0f - un) / m) * X.dt * ow.w;
-  if (tn >= R.t1) {
-    settle(&L.code[slot], c, kLeave);
-    return false;
-  }
+  if (tn >= R.t1) return false;  // leaves the cell
   if (tn * dg.w >= Y.tend) {
-    reinterpret_cast<float*>(&L.tev[slot])[c] = tn;
-    settle(&L.code[slot], c, kBoxEnd);
+    on_box_end<MODE>(L, R, c, X.C, tn);
     return false;
   }
   return true;
@@ -197,7 +215,7 @@ __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& 
 // passes push the next step back; the last pass of a round (DRAIN, fewer
 // than 64 items) keeps every lane on its item until it is settled, so the
 // rare long runs of null collisions cost loop iterations instead of passes.
-template <int DT, bool DRAIN>
+template <int MODE, int DT, bool DRAIN>
 __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                             const float* s_a, uint32_t& n_items, V3 w2u,
                                             uint32_t sample) {
@@ -210,14 +228,13 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   wave_sync();
   bool push = false;
   float tn = 0.f;
-  uint32_t j = it.id >> 16;
+  uint32_t j = it.id >> 11;
   if (act) {
-    const uint32_t slot = it.id & 0xffu;
-    const int c = int((it.id >> 8) & 3u);
-    const Record R = L.rec[slot];
-    const WaveRay Y = L.ray[(it.id >> 10) & 63u];
+    const Record R = L.rec[it.id & 0x1ffu];
+    const int c = int((it.id >> 9) & 3u);
+    const WaveRay Y = L.ray[R.vl >> kVisitBits];
     float t = it.x;
-    while (null_test<DT>(P, X, L, s_a, w2u, sample, slot, c, R, Y, j, t, it.m, tn)) {
+    while (null_test<MODE, DT>(P, X, L, s_a, w2u, sample, c, R, Y, j, t, it.m, tn)) {
       ++j;
       if (!DRAIN) {
         push = true;
@@ -234,7 +251,7 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = (it.id & 0xffffu) | (j << 16);
+    o.id = (it.id & 0x7ffu) | (j << 11);
     o.x = tn;
     o.m = it.m;
     L.sample[base + lane_rank(pm)] = o;
@@ -248,7 +265,7 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
 // and, when it lands inside the cell before the box's end (nearly always for
 // these items), its null-collision test; a rejected step's successor goes on
 // the sample stack.
-template <int DT>
+template <int MODE, int DT>
 __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                            const float* s_a, uint32_t& n_first,
                                            uint32_t& n_items, V3 w2u, uint32_t sample) {
@@ -262,25 +279,22 @@ __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>&
   bool push = false;
   float tn = 0.f;
   if (act) {
-    const uint32_t slot = it.id & 0xffu;
-    const int c = int((it.id >> 8) & 3u);
-    const Record R = L.rec[slot];
-    const WaveRay Y = L.ray[(it.id >> 10) & 63u];
+    const Record R = L.rec[it.id & 0x1ffu];
+    const int c = int((it.id >> 9) & 3u);
+    const WaveRay Y = L.ray[R.vl >> kVisitBits];
     const float t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
-    if (t >= R.t1) {
-      settle(&L.code[slot], c, kLeave);
-    } else if (t * Y.dir_g2w.w >= Y.tend) {
-      reinterpret_cast<float*>(&L.tev[slot])[c] = t;
-      settle(&L.code[slot], c, kBoxEnd);
-    } else {
-      push = null_test<DT>(P, X, L, s_a, w2u, sample, slot, c, R, Y, 0u, t, it.m, tn);
+    if (t < R.t1) {  // otherwise it leaves the cell
+      if (t * Y.dir_g2w.w >= Y.tend)
+        on_box_end<MODE>(L, R, c, X.C, t);
+      else
+        push = null_test<MODE, DT>(P, X, L, s_a, w2u, sample, c, R, Y, 0u, t, it.m, tn);
     }
   }
   wave_sync();
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = (it.id & 0xffffu) | (1u << 16);  // pair 1
+    o.id = (it.id & 0x7ffu) | (1u << 11);  // pair 1
     o.x = tn;
     o.m = it.m;
     L.sample[n_items + lane_rank(pm)] = o;
@@ -382,37 +396,34 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
       const uint32_t n_live = uint32_t(__popcll(live));
       int K = int(uint32_t(kRecords) / n_live);
       K = K > kMaxAhead ? kMaxAhead : K;
-      const uint32_t rank = lane_rank(live);
-      // ---- 1. walk: up to K cells per live ray
-      uint32_t cnt = 0u;
+      const uint32_t visit0 = visit;
       if (alive) {
         L.ray[lane].mask = mask;
-        if (!exhausted) {
-          for (int k = 0; k < K; ++k) {
-            float t0, t1;
-            d.range(t0, t1);
-            Record R;
-            R.cid = uint32_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my);
-            R.t0 = t0;
-            R.t1 = t1;
-            R.visit = visit;
-            L.rec[uint32_t(k) * n_live + rank] = R;
-            ++visit;
-            ++cnt;
-            if (!d.advance()) {
-              exhausted = true;
-              break;
-            }
-          }
-        }
+        L.ev[lane] = kNoEvent;
+        L.box[lane] = 0ull;
       }
-      // the round's slots, ascending (k-major), each with its ray's lane
+      // ---- 1. walk: up to K cells per live ray, records dense in k-major order
+      bool walking = alive && !exhausted;
       uint32_t total = 0u;
       for (int k = 0; k < K; ++k) {
-        const uint64_t m = __ballot(uint32_t(k) < cnt);
-        if (uint32_t(k) < cnt)
-          L.order[total + lane_rank(m)] = uint16_t((uint32_t(k) * n_live + rank) | (uint32_t(lane) << 8));
-        total += uint32_t(__popcll(m));
+        const uint64_t wm = __ballot(walking);
+        if (wm == 0ull) break;
+        if (walking) {
+          float t0, t1;
+          d.range(t0, t1);
+          Record R;
+          R.cid = uint32_t(d.cx + d.cy * X.mx + d.cz * X.mx * X.my);
+          R.t0 = t0;
+          R.t1 = t1;
+          R.vl = visit | (uint32_t(lane) << kVisitBits);
+          L.rec[total + lane_rank(wm)] = R;
+          ++visit;
+          if (!d.advance()) {
+            exhausted = true;
+            walking = false;
+          }
+        }
+        total += uint32_t(__popcll(wm));
       }
       wave_sync();
       // ---- 2. enter: first step of every channel of every record. The
@@ -429,21 +440,17 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
       for (uint32_t b = 0; b < total; b += 64u) {
         const uint32_t i = b + uint32_t(lane);
         const bool act = i < total;
-        uint32_t head = 0u;
         float4 uq = make_float4(0.f, 0.f, 0.f, 0.f), m4 = uq;
         uint32_t need = 0u;
         if (act) {
-          const uint32_t o = L.order[i];
-          const uint32_t sl = o & 0xffu, rl = o >> 8;
-          head = sl | (rl << 10);
-          const Record R = L.rec[sl];
-          const WaveRay& Y = L.ray[rl];
+          const Record R = L.rec[i];
+          const WaveRay& Y = L.ray[R.vl >> kVisitBits];
           const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
           m4 = cell_majorants(X, R.cid);
           uint32_t w[4];
-          philox(P.fr.frame_id, uint32_t(s), 1u + ((R.visit << 4) | 12u), 0u, P.fr.seed, Y.pixel, w);
+          philox(P.fr.frame_id, uint32_t(s), 1u + (((R.vl & kVisitMask) << 4) | 12u), 0u, P.fr.seed,
+                 Y.pixel, w);
           const uint32_t msk = Y.mask;
-          uint32_t codes = 0u;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const float m = comp(m4, c);
@@ -452,20 +459,17 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
               if (!(u >= q * m)) {
                 set_comp(uq, c, u);
                 need |= 1u << c;
-                codes |= kPending << (2 * c);
               }
             }
           }
-          L.code[sl] = codes;
         }
-        wave_sync();
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const bool nc = (need >> c) & 1u;
           const uint64_t pm = __ballot(nc);
           if (nc) {
             Item e;
-            e.id = head | (uint32_t(c) << 8);
+            e.id = i | (uint32_t(c) << 9);
             e.x = comp(uq, c);
             e.m = comp(m4, c);
             L.first[n_first + lane_rank(pm)] = e;
@@ -474,67 +478,65 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           wave_sync();
           // full batches as soon as there are any
           while (n_first >= 64u) {
-            first_pass<DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
+            first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
             // ---- 3. sample
             while (n_items >= 64u)
-              sample_pass<DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+              sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
           }
         }
       }
       if (n_first > 0u) {  // fewer than 64 left
-        first_pass<DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
-        while (n_items >= 64u) sample_pass<DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+        first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
+        while (n_items >= 64u)
+          sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
       }
-      if (n_items > 0u) sample_pass<DT, true>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+      if (n_items > 0u) sample_pass<MODE, DT, true>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
       wave_sync();
-      // ---- 4. resolve: the ray's records in cell order
+      // ---- 4. resolve: the ray's first event of the round
       if (alive) {
+        const uint64_t e = L.ev[lane];
         bool done = false;
-        for (uint32_t k = 0; k < cnt && !done; ++k) {
-          const uint32_t sl = k * n_live + rank;
-          const uint32_t codes = L.code[sl];
-          if (codes == 0u) continue;  // every channel left the cell
-          const float4 te = L.tev[sl];
-          if (MODE == 2) {
-            float minT = L.rec[sl].t1;
-            int hc = -1;
-            bool term = false;
-            for (int c = 0; c < C; ++c) {
-              const uint32_t cd = (codes >> (2 * c)) & 3u;
-              const float tc = comp(te, c);
-              if (cd == kHit && tc < minT) {
-                hc = c;
-                minT = tc;
-              } else if (cd == kBoxEnd && tc < minT && c == C - 1) {
-                term = true;
-              }
-            }
-            if (hc >= 0) {
+        if (MODE == 2) {
+          // a hit in the first visit with an event (smallest t, ties to the
+          // lower channel), or else the last channel's end of the box there
+          if (e != kNoEvent) {
+            done = true;
+            if (!((e >> 34) & 1ull)) {
               hit = true;
-              hit_c = hc;
-              hit_t = minT;
+              hit_c = int(e & 3ull);
+              hit_t = from_order_bits(uint32_t(e >> 2));
             }
-            done = hc >= 0 || term;
+          }
+        } else {
+          // channel c leaves the ray at its first end-of-box visit; the ray
+          // ends at its first hit (smallest t, ties to the higher channel)
+          // unless every channel it still tracks left before that visit
+          const uint32_t rot = (visit0 & 15u) << 2;
+          const uint64_t bx0 = L.box[lane];
+          const uint64_t bx = rot ? ((bx0 >> rot) | (bx0 << (64u - rot))) : bx0;  // bit 4k + c
+          const uint32_t k_hit = e != kNoEvent ? uint32_t(e >> 35) - visit0 : 0xffffffffu;
+          uint32_t k_all = 0u, left = 0u;
+          bool all = true;
+          for (int c = 0; c < C; ++c) {
+            if (!((mask >> c) & 1u)) continue;
+            const uint64_t cb = bx & (0x1111111111111111ull << c);
+            if (cb == 0ull) {
+              all = false;
+            } else {
+              const uint32_t kc = uint32_t(__builtin_ctzll(cb)) >> 2;
+              k_all = kc > k_all ? kc : k_all;
+              left |= 1u << c;
+            }
+          }
+          if (e != kNoEvent && (!all || k_hit <= k_all)) {
+            done = true;
+            hit = true;
+            hit_c = 3 - int(e & 3ull);
+            hit_t = from_order_bits(uint32_t(e >> 2));
+          } else if (all) {
+            done = true;
           } else {
-            float best = 1e20f;
-            int hc = -1;
-            for (int c = 0; c < C; ++c) {
-              if (!((mask >> c) & 1u)) continue;
-              const uint32_t cd = (codes >> (2 * c)) & 3u;
-              const float tc = comp(te, c);
-              if (cd == kHit && tc <= best) {
-                hc = c;
-                best = tc;
-              } else if (cd == kBoxEnd) {
-                mask &= ~(1u << c);
-              }
-            }
-            if (hc >= 0) {
-              hit = true;
-              hit_c = hc;
-              hit_t = best;
-            }
-            done = hc >= 0 || mask == 0u;
+            mask &= ~left;
           }
         }
         if (done || exhausted) alive = false;
@@ -583,7 +585,10 @@ RenderFn wave_fn(int dtype) {
 }  // namespace
 
 bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit) {
-  return (mode == 0 || mode == 2) && !count && !lit && p.vol.channels <= 4;
+  // a ray visits at most mx + my + mz cells: visit numbers fit the records' 26 bits
+  const uint64_t max_visits = uint64_t(p.mc[0]) + uint64_t(p.mc[1]) + uint64_t(p.mc[2]);
+  return (mode == 0 || mode == 2) && !count && !lit && p.vol.channels <= 4 &&
+         max_visits < (uint64_t(1) << kVisitBits);
 }
 
 RenderFn select_wave_kernel(int mode, int dtype) {

@@ -304,6 +304,53 @@ __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>&
   n_items += uint32_t(__popcll(pm));
 }
 
+// The round's last pass: the fewer than 64 items left on the first-step stack
+// and, in the remaining lanes, as many items of the sample stack as fit. Every
+// lane stays on its item until it is settled, so a run of null collisions
+// costs loop iterations instead of passes.
+template <int MODE, int DT>
+__device__ __forceinline__ void tail_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
+                                          const float* s_a, uint32_t& n_first,
+                                          uint32_t& n_items, V3 w2u, uint32_t sample) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nf = n_first;  // < 64
+  const uint32_t ns = n_items < 64u - nf ? n_items : 64u - nf;
+  const uint32_t sbase = n_items - ns;
+  const bool is_first = lane < nf;
+  const bool act = lane < nf + ns;
+  Item it = {0u, 0.f, 0.f};
+  if (is_first)
+    it = L.first[lane];
+  else if (act)
+    it = L.sample[sbase + (lane - nf)];
+  wave_sync();
+  if (act) {
+    const Record R = L.rec[it.id & 0x1ffu];
+    const int c = int((it.id >> 9) & 3u);
+    const WaveRay Y = L.ray[R.vl >> kVisitBits];
+    float t = it.x, tn = 0.f;
+    uint32_t j = it.id >> 11;
+    bool go = true;
+    if (is_first) {
+      t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
+      go = false;
+      if (t < R.t1) {  // otherwise it leaves the cell
+        if (t * Y.dir_g2w.w >= Y.tend)
+          on_box_end<MODE>(L, R, c, X.C, t);
+        else
+          go = true;
+      }
+    }
+    while (go && null_test<MODE, DT>(P, X, L, s_a, w2u, sample, c, R, Y, j, t, it.m, tn)) {
+      ++j;
+      t = tn;
+    }
+  }
+  wave_sync();
+  n_first = 0u;
+  n_items = sbase;
+}
+
 // Transfer-function opacities and per-channel parameters -> LDS (the caller
 // synchronises the workgroup afterwards). Colours stay in HBM: only a ray's
 // hit reads them.
@@ -475,21 +522,22 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
             L.first[n_first + lane_rank(pm)] = e;
           }
           n_first += uint32_t(__popcll(pm));
-          wave_sync();
           // full batches as soon as there are any
-          while (n_first >= 64u) {
-            first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
-            // ---- 3. sample
-            while (n_items >= 64u)
-              sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+          if (n_first >= 64u) {
+            wave_sync();
+            do {
+              first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
+              // ---- 3. sample
+              while (n_items >= 64u)
+                sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+            } while (n_first >= 64u);
           }
         }
       }
-      if (n_first > 0u) {  // fewer than 64 left
-        first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
-        while (n_items >= 64u)
-          sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
-      }
+      wave_sync();
+      // fewer than 64 first steps and 64 tests left: one pass for both, and
+      // a draining pass for the sample items that did not fit
+      if (n_first + n_items > 0u) tail_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
       if (n_items > 0u) sample_pass<MODE, DT, true>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
       wave_sync();
       // ---- 4. resolve: the ray's first event of the round

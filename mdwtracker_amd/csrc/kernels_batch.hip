@@ -10,6 +10,9 @@
 // grid): a static split with no device work counter, no atomics and no
 // barrier inside the item loop, so every workgroup's loop ends after a fixed
 // number of items.
+// Under the wave-coherent schedule, -m 0 / -m 2 views on at most four
+// channels without lighting run kernels_wave.hip's render_views_wave and
+// render_views_wave_persistent instead, with the same items and ids.
 // No reference counterpart: the reference renders one camera per owlLaunch2D
 // (renderer.cpp:949).
 #include "mdwt_internal.hpp"
@@ -236,11 +239,17 @@ namespace {
 bool wide_views(const RenderParams& p, int dtype) {
   return p.vol.channels <= 4 && (dtype == MDWT_DT_U8 || dtype == MDWT_DT_U16);
 }
+// batched views on the wave-coherent kernels (schedule 1 where it applies)
+bool wave_views(const RenderParams& p, int mode, bool count) {
+  return p.fr.schedule == 1 && wave_schedule_applies(p, mode, count, is_lit(p));
+}
 }  // namespace
 
 hipError_t views_persistent_blocks_per_cu(const RenderParams& p, int mode, int dtype, int* out) {
-  const size_t lds = render_lds_bytes(p);
-  const RenderFn f = select_persistent(mode, dtype, is_lit(p), wide_views(p, dtype));
+  const bool wave = wave_views(p, mode, false);
+  const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
+  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 2)
+                          : select_persistent(mode, dtype, is_lit(p), wide_views(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, f, p.fr.bw * p.fr.bh, lds);
@@ -249,8 +258,10 @@ hipError_t views_persistent_blocks_per_cu(const RenderParams& p, int mode, int d
 hipError_t launch_views_persistent(const RenderParams& p, int mode, int dtype, int grid,
                                    hipStream_t s) {
   if (p.fr.n_items <= 0 || grid <= 0 || !p.fr.work) return hipErrorInvalidValue;
-  const size_t lds = render_lds_bytes(p);
-  const RenderFn f = select_persistent(mode, dtype, is_lit(p), wide_views(p, dtype));
+  const bool wave = wave_views(p, mode, false);
+  const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
+  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 2)
+                          : select_persistent(mode, dtype, is_lit(p), wide_views(p, dtype));
   hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(f, dim3(unsigned(grid)), dim3(unsigned(p.fr.bw * p.fr.bh)), lds, s, p);
@@ -261,8 +272,10 @@ hipError_t launch_views_persistent(const RenderParams& p, int mode, int dtype, i
 
 hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, hipStream_t s) {
   if (p.fr.n_items <= 0) return hipSuccess;
-  const size_t lds = render_lds_bytes(p);
-  const RenderFn f = select_views(mode, dtype, count, is_lit(p), wide_views(p, dtype));
+  const bool wave = wave_views(p, mode, count);
+  const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
+  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 1)
+                          : select_views(mode, dtype, count, is_lit(p), wide_views(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(f, dim3(unsigned(p.fr.n_items)), dim3(unsigned(p.fr.bw * p.fr.bh)), lds, s, p);

@@ -89,6 +89,10 @@ struct Item {
   uint32_t id;
   float x, m;
 };
+// The random streams of one sample of a view: frame id, sample index, seed.
+struct Draw {
+  uint32_t frame, sample, seed;
+};
 
 // Per-wave LDS. The records of a round are dense and k-major: the k-th cells
 // of all rays that walk that far are consecutive, so the lanes of a walk step
@@ -181,13 +185,13 @@ __device__ __forceinline__ float alpha_lookup(const float* s_a, const ChanTf& t,
 // recorded here.
 template <int MODE, int DT>
 __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                          const float* s_a, V3 w2u, uint32_t sample, int c,
+                                          const float* s_a, V3 w2u, const Draw& D, int c,
                                           const Record& R, const WaveRay& Y, uint32_t j, float t,
                                           float m, float& tn) {
   const float4 ow = Y.org_w2g, dg = Y.dir_g2w;
   uint32_t w[4];
-  philox(P.fr.frame_id, sample, 1u + (((R.vl & kVisitMask) << 4) | uint32_t(c)), (j >> 1) << 1,
-         P.fr.seed, Y.pixel, w);
+  philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | uint32_t(c)), (j >> 1) << 1, D.seed,
+         Y.pixel, w);
   const float thr = to_unit((j & 1u) ? w[2] : w[0]) * m;
   const float un = to_unit((j & 1u) ? w[3] : w[1]);
   const float tWorld = t * dg.w;
@@ -218,7 +222,7 @@ __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& 
 template <int MODE, int DT, bool DRAIN>
 __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                             const float* s_a, uint32_t& n_items, V3 w2u,
-                                            uint32_t sample) {
+                                            const Draw& D) {
   const int lane = threadIdx.x & 63;
   const uint32_t n = n_items < 64u ? n_items : 64u;
   const uint32_t base = n_items - n;
@@ -234,7 +238,7 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
     const int c = int((it.id >> 9) & 3u);
     const WaveRay Y = L.ray[R.vl >> kVisitBits];
     float t = it.x;
-    while (null_test<MODE, DT>(P, X, L, s_a, w2u, sample, c, R, Y, j, t, it.m, tn)) {
+    while (null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, j, t, it.m, tn)) {
       ++j;
       if (!DRAIN) {
         push = true;
@@ -268,7 +272,7 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
 template <int MODE, int DT>
 __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                            const float* s_a, uint32_t& n_first,
-                                           uint32_t& n_items, V3 w2u, uint32_t sample) {
+                                           uint32_t& n_items, V3 w2u, const Draw& D) {
   const int lane = threadIdx.x & 63;
   const uint32_t n = n_first < 64u ? n_first : 64u;
   const uint32_t base = n_first - n;
@@ -287,7 +291,7 @@ __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>&
       if (t * Y.dir_g2w.w >= Y.tend)
         on_box_end<MODE>(L, R, c, X.C, t);
       else
-        push = null_test<MODE, DT>(P, X, L, s_a, w2u, sample, c, R, Y, 0u, t, it.m, tn);
+        push = null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, 0u, t, it.m, tn);
     }
   }
   wave_sync();
@@ -311,7 +315,7 @@ __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>&
 template <int MODE, int DT>
 __device__ __forceinline__ void tail_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                           const float* s_a, uint32_t& n_first,
-                                          uint32_t& n_items, V3 w2u, uint32_t sample) {
+                                          uint32_t& n_items, V3 w2u, const Draw& D) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nf = n_first;  // < 64
   const uint32_t ns = n_items < 64u - nf ? n_items : 64u - nf;
@@ -341,7 +345,7 @@ __device__ __forceinline__ void tail_pass(const RenderParams& P, const Ctx<DT>& 
           go = true;
       }
     }
-    while (go && null_test<MODE, DT>(P, X, L, s_a, w2u, sample, c, R, Y, j, t, it.m, tn)) {
+    while (go && null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, j, t, it.m, tn)) {
       ++j;
       t = tn;
     }
@@ -374,46 +378,36 @@ __device__ __forceinline__ void load_alpha_lds(const RenderParams& P, float* s_a
   }
 }
 
+// The 8x8 pixel quad of this wave in pixel block (bx, by) of view V (a
+// workgroup's bw x bh block has one quad per wave).
 template <int MODE, int DT>
-__device__ __forceinline__ void render_wave_body(const RenderParams& P) {
-  extern __shared__ float s_alpha[];
-  __shared__ ChanTf s_ch[kMaxChannels];
-  __shared__ WaveLds s_wave[kThreads / 64];
-  const uint64_t t_begin = P.fr.wave_cost ? wall_clock64() : 0ull;
-  load_alpha_lds(P, s_alpha, s_ch);
-  __syncthreads();  // the only workgroup barrier: the waves run independently from here
-
-  Ctx<DT> X;
-  // colours of a hit are looked up in the HBM copy of the tables
-  X.init(P, reinterpret_cast<const float4*>(P.tf.tab), s_ch);
+__device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
+                                           const float* s_alpha, const ViewCtx& V, int bx, int by) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  WaveLds& L = s_wave[wave];
   const int C = X.C;
-
-  const uint32_t slot_b = block_slot(blockIdx.x, uint32_t(P.fr.n_blocks));
-  const uint32_t bxy = P.fr.block_order[slot_b];
   const int qs = P.fr.bw == 16 ? 1 : 0;
-  const int px = int(bxy & 0xffffu) * P.fr.bw + (wave & qs) * 8 + (lane & 7);
-  const int py = int(bxy >> 16) * P.fr.bh + (wave >> qs) * 8 + (lane >> 3);
-  const bool valid = px < P.fr.width && py < P.fr.height;
-  const uint32_t pixel = valid ? uint32_t(px) + uint32_t(P.fr.width) * uint32_t(py) : 0u;
+  const int px = bx * P.fr.bw + (wave & qs) * 8 + (lane & 7);
+  const int py = by * P.fr.bh + (wave >> qs) * 8 + (lane >> 3);
+  const bool valid = px < V.W && py < V.H;
+  const uint32_t pixel = valid ? uint32_t(px) + uint32_t(V.W) * uint32_t(py) : 0u;
   const V3 blo = load3(P.vol.blo), bhi = load3(P.vol.bhi);
   const V3 ext = bhi - blo;
   const V3 w2u = v3(1.f / ext.x, 1.f / ext.y, 1.f / ext.z);
   const uint32_t all_ch = (1u << C) - 1u;
 
   float fr = 0.f, fg = 0.f, fb = 0.f, fa = 1.f;
-  for (int s = 0; s < P.fr.spp; ++s) {
+  for (int s = 0; s < V.spp; ++s) {
+    const Draw D = {V.frame_id, uint32_t(s), V.seed};
     // ---- ray set-up (mainRG, deviceCode.cu:293-321; the program's grid ray)
     Dda d;
     bool alive = false;
     float idw = 0.f;
     {
       Rng rng;
-      rng.init(P.fr.seed, pixel, P.fr.frame_id, uint32_t(s), 0u);
+      rng.init(V.seed, pixel, V.frame_id, uint32_t(s), 0u);
       const float jit = rng.jitter();
       V3 org, dir;
-      camera_ray(P.cam, px, py, P.fr.width, P.fr.height, jit, org, dir);
+      camera_ray(*V.cam, px, py, V.W, V.H, jit, org, dir);
       float c0, c1;
       WaveRay Y;
       Y.org_w2g = make_float4(org.x, org.y, org.z, 0.f);
@@ -495,8 +489,7 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
           m4 = cell_majorants(X, R.cid);
           uint32_t w[4];
-          philox(P.fr.frame_id, uint32_t(s), 1u + (((R.vl & kVisitMask) << 4) | 12u), 0u, P.fr.seed,
-                 Y.pixel, w);
+          philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | 12u), 0u, D.seed, Y.pixel, w);
           const uint32_t msk = Y.mask;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
@@ -526,10 +519,10 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
           if (n_first >= 64u) {
             wave_sync();
             do {
-              first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
+              first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, D);
               // ---- 3. sample
               while (n_items >= 64u)
-                sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+                sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, D);
             } while (n_first >= 64u);
           }
         }
@@ -537,8 +530,8 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
       wave_sync();
       // fewer than 64 first steps and 64 tests left: one pass for both, and
       // a draining pass for the sample items that did not fit
-      if (n_first + n_items > 0u) tail_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, uint32_t(s));
-      if (n_items > 0u) sample_pass<MODE, DT, true>(P, X, L, s_alpha, n_items, w2u, uint32_t(s));
+      if (n_first + n_items > 0u) tail_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, D);
+      if (n_items > 0u) sample_pass<MODE, DT, true>(P, X, L, s_alpha, n_items, w2u, D);
       wave_sync();
       // ---- 4. resolve: the ray's first event of the round
       if (alive) {
@@ -610,23 +603,119 @@ __device__ __forceinline__ void render_wave_body(const RenderParams& P) {
     composite_sample(P, hit, h.r, h.g, h.b, h.a, 1.0f, 0.0f, 1.0f, h.channel, fr, fg, fb, fa);
     wave_sync();
   }
-  if (valid) finish_pixel(P, P.fr.accum, P.fr.m2, P.fr.accum_id, P.fr.spp, pixel, fr, fg, fb, fa);
-  if (P.fr.wave_cost && lane == 0)
-    P.fr.wave_cost[(blockDim.x >> 6) * slot_b + uint32_t(wave)] = uint32_t(wall_clock64() - t_begin);
+  if (valid) finish_pixel(P, V.accum, V.m2, V.accum_id, V.spp, pixel, fr, fg, fb, fa);
 }
 
+// Per-workgroup set-up: the opacity table and channel parameters in LDS (the
+// only workgroup barrier: the waves run independently from here) and the
+// per-wave LDS. The per-wave LDS is static, sized for a 256-thread block (with
+// a dynamic array behind the opacity table the frame ran 15% slower); smaller
+// blocks leave part of it unused.
+#define MDWT_WAVE_PROLOGUE                                                        \
+  extern __shared__ float s_alpha[];                                              \
+  __shared__ ChanTf s_ch[kMaxChannels];                                           \
+  __shared__ WaveLds s_wave[kThreads / 64];                                       \
+  load_alpha_lds(P, s_alpha, s_ch);                                               \
+  __syncthreads();                                                                \
+  Ctx<DT> X;                                                                      \
+  /* colours of a hit are looked up in the HBM copy of the tables */            \
+  X.init(P, reinterpret_cast<const float4*>(P.tf.tab), s_ch);                     \
+  WaveLds& L = s_wave[threadIdx.x >> 6];
+
+// One view: workgroup i renders the pixel block at position block_slot(i) of
+// the frame's block order.
 template <int MODE, int DT>
 __global__ __launch_bounds__(256) void render_wave(RenderParams P) {
-  render_wave_body<MODE, DT>(P);
+  const uint64_t t_begin = P.fr.wave_cost ? wall_clock64() : 0ull;
+  MDWT_WAVE_PROLOGUE
+  const uint32_t slot_b = block_slot(blockIdx.x, uint32_t(P.fr.n_blocks));
+  const uint32_t bxy = P.fr.block_order[slot_b];
+  ViewCtx V;
+  V.cam = &P.cam;
+  V.W = P.fr.width;
+  V.H = P.fr.height;
+  V.spp = P.fr.spp;
+  V.seed = P.fr.seed;
+  V.frame_id = P.fr.frame_id;
+  V.accum_id = P.fr.accum_id;
+  V.accum = P.fr.accum;
+  V.m2 = P.fr.m2;
+  wave_block<MODE, DT>(P, X, L, s_alpha, V, int(bxy & 0xffffu), int(bxy >> 16));
+  if (P.fr.wave_cost && (threadIdx.x & 63) == 0)
+    P.fr.wave_cost[(blockDim.x >> 6) * slot_b + (threadIdx.x >> 6)] =
+        uint32_t(wall_clock64() - t_begin);
 }
 
+__device__ __forceinline__ ViewCtx batch_view(const RenderParams& P, uint32_t code,
+                                              uint32_t frame_id, uint32_t accum_id) {
+  const ViewParams* vp = P.fr.views + (code >> 20);
+  ViewCtx V;
+  V.cam = &vp->cam;
+  V.W = vp->width;
+  V.H = vp->height;
+  V.spp = vp->spp;
+  V.seed = vp->seed;
+  V.frame_id = frame_id;
+  V.accum_id = accum_id;
+  V.accum = vp->accum;
+  V.m2 = vp->m2;
+  return V;
+}
+
+// Batched views, launch form 1: workgroup i renders item i (view << 20 |
+// by << 10 | bx) at the launch's frame and accumulation ids; with item_ticks
+// set it records its start and end wall clock.
+template <int MODE, int DT>
+__global__ __launch_bounds__(256) void render_views_wave(RenderParams P) {
+  const uint64_t t_begin = P.fr.item_ticks ? wall_clock64() : 0ull;
+  MDWT_WAVE_PROLOGUE
+  const uint32_t code = P.fr.items[blockIdx.x];
+  const ViewCtx V = batch_view(P, code, P.fr.frame_id, P.fr.accum_id);
+  wave_block<MODE, DT>(P, X, L, s_alpha, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu));
+  if (P.fr.item_ticks) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      P.fr.item_ticks[2 * blockIdx.x] = t_begin;
+      P.fr.item_ticks[2 * blockIdx.x + 1] = wall_clock64();
+    }
+  }
+}
+
+// Batched views, launch form 2 (persistent): workgroup g renders items g,
+// g + G, ... at the frame and accumulation ids in work[1] / work[2]; each wave
+// moves to its next item as soon as its quad is done. Wave 0 records each
+// item's start and end wall clock.
+template <int MODE, int DT>
+__global__ __launch_bounds__(256) void render_views_wave_persistent(RenderParams P) {
+  MDWT_WAVE_PROLOGUE
+  const uint32_t fid = P.fr.work[1], aid = P.fr.work[2];
+  const uint32_t n = uint32_t(P.fr.n_items);
+  for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
+    const uint64_t t_item = P.fr.item_ticks ? wall_clock64() : 0ull;
+    const uint32_t code = P.fr.items[it];
+    const ViewCtx V = batch_view(P, code, fid, aid);
+    wave_block<MODE, DT>(P, X, L, s_alpha, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu));
+    if (P.fr.item_ticks && threadIdx.x == 0) {
+      P.fr.item_ticks[2 * it] = t_item;
+      P.fr.item_ticks[2 * it + 1] = wall_clock64();
+    }
+  }
+}
+#undef MDWT_WAVE_PROLOGUE
+
+// form: 0 one view, 1 batched views (one launch), 2 batched views (persistent)
+template <int MODE, int DT>
+RenderFn wave_fn_form(int form) {
+  return form == 0 ? render_wave<MODE, DT>
+                   : (form == 1 ? render_views_wave<MODE, DT> : render_views_wave_persistent<MODE, DT>);
+}
 template <int MODE>
-RenderFn wave_fn(int dtype) {
+RenderFn wave_fn(int dtype, int form) {
   switch (dtype) {
-    case MDWT_DT_U8: return render_wave<MODE, MDWT_DT_U8>;
-    case MDWT_DT_U16: return render_wave<MODE, MDWT_DT_U16>;
-    case MDWT_DT_F16: return render_wave<MODE, MDWT_DT_F16>;
-    default: return render_wave<MODE, MDWT_DT_F32>;
+    case MDWT_DT_U8: return wave_fn_form<MODE, MDWT_DT_U8>(form);
+    case MDWT_DT_U16: return wave_fn_form<MODE, MDWT_DT_U16>(form);
+    case MDWT_DT_F16: return wave_fn_form<MODE, MDWT_DT_F16>(form);
+    default: return wave_fn_form<MODE, MDWT_DT_F32>(form);
   }
 }
 
@@ -639,11 +728,12 @@ bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit
          max_visits < (uint64_t(1) << kVisitBits);
 }
 
-RenderFn select_wave_kernel(int mode, int dtype) {
-  return mode == 0 ? wave_fn<0>(dtype) : wave_fn<2>(dtype);
+RenderFn select_wave_kernel(int mode, int dtype, int form) {
+  return mode == 0 ? wave_fn<0>(dtype, form) : wave_fn<2>(dtype, form);
 }
 
 size_t wave_lds_bytes(const RenderParams& p) {
+  static_assert(sizeof(WaveLds) % 16 == 0, "the opacity table follows the per-wave LDS");
   return size_t(p.vol.channels) * size_t(p.tf.stride) * sizeof(float);
 }
 

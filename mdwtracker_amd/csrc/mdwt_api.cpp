@@ -1539,7 +1539,7 @@ mdwt::RenderParams view_params(const mdwt_renderer* h, size_t i) {
   p.fr.m2 = v.m2;
   p.fr.block_order = h->view_order[i];
   p.fr.n_blocks = h->view_blocks[i];
-  p.fr.schedule = 0;
+  p.fr.schedule = eff_schedule(h);
   p.fr.wave_cost = nullptr;
   p.fr.aov = nullptr;  // batched views have no auxiliary buffers
   return p;
@@ -1613,7 +1613,7 @@ int batch_render_views(mdwt_renderer* h, int n_frames, float* ms_out) {
 // the batch's frame and accumulation ids.
 mdwt::RenderParams views_params(const mdwt_renderer* h) {
   mdwt::RenderParams p = render_params(h);
-  p.fr.schedule = 0;
+  p.fr.schedule = eff_schedule(h);
   p.fr.wave_cost = nullptr;
   p.fr.views = h->d_views;
   p.fr.items = h->d_items;

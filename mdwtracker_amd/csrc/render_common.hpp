@@ -282,6 +282,16 @@ __device__ __forceinline__ void composite_sample(const RenderParams& P, bool hit
 }
 
 // Divides the pixel sums (alpha started at 1) by spp and accumulates.
+// What one pixel block needs from its view: camera, size, samples, seed,
+// frame / accumulation ids and accumulators.
+struct ViewCtx {
+  const CameraParams* cam;
+  int W, H, spp;
+  uint32_t seed, frame_id, accum_id;
+  void* accum;
+  void* m2;
+};
+
 __device__ __forceinline__ void finish_pixel(const RenderParams& P, void* accum, void* m2,
                                              uint32_t accum_id, int spp_i, uint32_t pixel,
                                              float fr, float fg, float fb, float fa) {

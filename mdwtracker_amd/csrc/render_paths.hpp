@@ -582,16 +582,6 @@ __device__ void shade_gradient(const RenderParams& P, const Ctx<DT>& X, V3 org, 
   }
 }
 
-// What one pixel block needs from its view: camera, size, samples, seed,
-// frame / accumulation ids and accumulators.
-struct ViewCtx {
-  const CameraParams* cam;
-  int W, H, spp;
-  uint32_t seed, frame_id, accum_id;
-  void* accum;
-  void* m2;
-};
-
 // Denoiser inputs of one sample folded into the pixel's running means:
 // first-hit distance (0 on a miss), transmittance (1 on a miss, 0 on a hit)
 // and a one-hot of the hit channel, aov_groups(C) float4 per pixel.

@@ -10,8 +10,8 @@
 // grid): a static split with no device work counter, no atomics and no
 // barrier inside the item loop, so every workgroup's loop ends after a fixed
 // number of items.
-// Under the wave-coherent schedule, -m 0 / -m 2 views on at most four
-// channels without lighting run kernels_wave.hip's render_views_wave and
+// Under the wave-coherent schedule, -m 2 views and -m 0 views on at most four
+// channels, without lighting, run kernels_wave.hip's render_views_wave and
 // render_views_wave_persistent instead, with the same items and ids.
 // No reference counterpart: the reference renders one camera per owlLaunch2D
 // (renderer.cpp:949).
@@ -248,7 +248,7 @@ bool wave_views(const RenderParams& p, int mode, bool count) {
 hipError_t views_persistent_blocks_per_cu(const RenderParams& p, int mode, int dtype, int* out) {
   const bool wave = wave_views(p, mode, false);
   const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
-  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 2)
+  const RenderFn f = wave ? select_wave_kernel(p, mode, dtype, 2)
                           : select_persistent(mode, dtype, is_lit(p), wide_views(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
@@ -260,7 +260,7 @@ hipError_t launch_views_persistent(const RenderParams& p, int mode, int dtype, i
   if (p.fr.n_items <= 0 || grid <= 0 || !p.fr.work) return hipErrorInvalidValue;
   const bool wave = wave_views(p, mode, false);
   const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
-  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 2)
+  const RenderFn f = wave ? select_wave_kernel(p, mode, dtype, 2)
                           : select_persistent(mode, dtype, is_lit(p), wide_views(p, dtype));
   hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;
@@ -274,7 +274,7 @@ hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, 
   if (p.fr.n_items <= 0) return hipSuccess;
   const bool wave = wave_views(p, mode, count);
   const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
-  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 1)
+  const RenderFn f = wave ? select_wave_kernel(p, mode, dtype, 1)
                           : select_views(mode, dtype, count, is_lit(p), wide_views(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;

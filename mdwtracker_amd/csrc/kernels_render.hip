@@ -288,7 +288,7 @@ hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count,
   const bool lit = is_lit(p);
   const bool wave = p.fr.schedule == 1 && wave_schedule_applies(p, mode, count, lit);
   const size_t lds = wave ? wave_lds_bytes(p) : render_lds_bytes(p);
-  const RenderFn f = wave ? select_wave_kernel(mode, dtype, 0)
+  const RenderFn f = wave ? select_wave_kernel(p, mode, dtype, 0)
                           : select_render(mode, dtype, count, lit, six_waves(p, dtype));
   const hipError_t e = prepare(f, lds);
   if (e != hipSuccess) return e;

@@ -1,5 +1,5 @@
-// Wave-coherent Woodcock tracking for -m 0 (N-DDA) and -m 2 (syncDDA) on at
-// most four channels (frame-kernel schedule 1).
+// Wave-coherent Woodcock tracking for -m 2 (syncDDA, up to twelve channels)
+// and -m 0 (N-DDA, at most four channels) (frame-kernel schedule 1).
 //
 // The one-thread-per-pixel kernel nests cell -> channel -> free-flight step
 // loops per lane, so a wave64 runs every loop level as long as its slowest
@@ -65,6 +65,10 @@ constexpr float kExitSafety = 1.0001f;
 constexpr uint32_t kVisitBits = 26;  // visits of one ray below 2^26 (checked on the host)
 constexpr uint32_t kVisitMask = (1u << kVisitBits) - 1u;
 constexpr uint64_t kNoEvent = ~0ull;
+// stack entry id: record slot (9 bits) | channel << 9 (4 bits) | pair << 13
+constexpr uint32_t kSlotMask = 0x1ffu;
+constexpr uint32_t kChanShift = 9;
+constexpr uint32_t kPairShift = 13;
 
 // A record: the macrocell of one (ray, visit), its entry and exit t (grid
 // units), the visit number and the ray's lane (vl = visit | lane << 26), read
@@ -83,7 +87,7 @@ struct WaveRay {
   float tend, idw;
   uint32_t pixel, mask;
 };
-// A stack entry: record slot | channel << 9 | pair << 11, a t or a uniform,
+// A stack entry: record slot | channel << 9 | pair << 13, a t or a uniform,
 // and the channel's majorant in the record's cell.
 struct Item {
   uint32_t id;
@@ -128,15 +132,15 @@ __device__ __forceinline__ float from_order_bits(uint32_t o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 // Event key: visit, then a flag (0: hit, 1: the last channel's end of the
-// root box, -m 2), then t, then 2 channel bits: 3 - channel for -m 0 (ties to
+// root box, -m 2), then t, then 4 channel bits: 15 - channel for -m 0 (ties to
 // the higher channel), the channel for -m 2 (ties to the lower one). Within a
 // visit any hit ends -m 2's ray before the end of the box does.
 __device__ __forceinline__ uint64_t event_key(uint32_t visit, uint32_t flag, float t, uint32_t ch) {
-  return (uint64_t(visit) << 35) | (uint64_t(flag) << 34) | (uint64_t(order_bits(t)) << 2) | ch;
+  return (uint64_t(visit) << 37) | (uint64_t(flag) << 36) | (uint64_t(order_bits(t)) << 4) | ch;
 }
 template <int MODE>
 __device__ __forceinline__ void on_hit(WaveLds& L, const Record& R, int c, float t) {
-  const uint32_t ch = MODE == 2 ? uint32_t(c) : uint32_t(3 - c);
+  const uint32_t ch = MODE == 2 ? uint32_t(c) : uint32_t(15 - c);
   atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 0u, t, ch));
 }
 template <int MODE>
@@ -157,11 +161,14 @@ __device__ __forceinline__ void set_comp(float4& v, int c, float x) {
   else v.w = x;
 }
 
+// The majorants of channels 4g .. 4g + 3 in cell cid (0 past the last channel).
 template <int DT>
-__device__ __forceinline__ float4 cell_majorants(const Ctx<DT>& X, uint32_t cid) {
+__device__ __forceinline__ float4 cell_majorants(const Ctx<DT>& X, uint32_t cid, int g) {
   if (X.C == 4) return *reinterpret_cast<const float4*>(X.maj + size_t(cid) * 4);
-  const float* p = X.maj + size_t(cid) * X.C;
-  return make_float4(p[0], X.C > 1 ? p[1] : 0.f, X.C > 2 ? p[2] : 0.f, 0.f);
+  const int c0 = 4 * g;
+  const float* p = X.maj + size_t(cid) * X.C + c0;
+  return make_float4(p[0], X.C > c0 + 1 ? p[1] : 0.f, X.C > c0 + 2 ? p[2] : 0.f,
+                     X.C > c0 + 3 ? p[3] : 0.f);
 }
 
 // transferFunction's opacity (deviceCode.cu:27-42) from the LDS table of
@@ -232,10 +239,10 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   wave_sync();
   bool push = false;
   float tn = 0.f;
-  uint32_t j = it.id >> 11;
+  uint32_t j = it.id >> kPairShift;
   if (act) {
-    const Record R = L.rec[it.id & 0x1ffu];
-    const int c = int((it.id >> 9) & 3u);
+    const Record R = L.rec[it.id & kSlotMask];
+    const int c = int((it.id >> kChanShift) & 15u);
     const WaveRay Y = L.ray[R.vl >> kVisitBits];
     float t = it.x;
     while (null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, j, t, it.m, tn)) {
@@ -255,7 +262,7 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = (it.id & 0x7ffu) | (j << 11);
+    o.id = (it.id & ((1u << kPairShift) - 1u)) | (j << kPairShift);
     o.x = tn;
     o.m = it.m;
     L.sample[base + lane_rank(pm)] = o;
@@ -283,8 +290,8 @@ __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>&
   bool push = false;
   float tn = 0.f;
   if (act) {
-    const Record R = L.rec[it.id & 0x1ffu];
-    const int c = int((it.id >> 9) & 3u);
+    const Record R = L.rec[it.id & kSlotMask];
+    const int c = int((it.id >> kChanShift) & 15u);
     const WaveRay Y = L.ray[R.vl >> kVisitBits];
     const float t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
     if (t < R.t1) {  // otherwise it leaves the cell
@@ -298,7 +305,7 @@ __device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>&
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = (it.id & 0x7ffu) | (1u << 11);  // pair 1
+    o.id = (it.id & ((1u << kPairShift) - 1u)) | (1u << kPairShift);  // pair 1
     o.x = tn;
     o.m = it.m;
     L.sample[n_items + lane_rank(pm)] = o;
@@ -329,11 +336,11 @@ __device__ __forceinline__ void tail_pass(const RenderParams& P, const Ctx<DT>& 
     it = L.sample[sbase + (lane - nf)];
   wave_sync();
   if (act) {
-    const Record R = L.rec[it.id & 0x1ffu];
-    const int c = int((it.id >> 9) & 3u);
+    const Record R = L.rec[it.id & kSlotMask];
+    const int c = int((it.id >> kChanShift) & 15u);
     const WaveRay Y = L.ray[R.vl >> kVisitBits];
     float t = it.x, tn = 0.f;
-    uint32_t j = it.id >> 11;
+    uint32_t j = it.id >> kPairShift;
     bool go = true;
     if (is_first) {
       t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
@@ -379,8 +386,10 @@ __device__ __forceinline__ void load_alpha_lds(const RenderParams& P, float* s_a
 }
 
 // The 8x8 pixel quad of this wave in pixel block (bx, by) of view V (a
-// workgroup's bw x bh block has one quad per wave).
-template <int MODE, int DT>
+// workgroup's bw x bh block has one quad per wave). WIDE: more than four
+// channels (-m 2), drawn in groups of four; a separate instantiation, since
+// the group loop's code in the four-channel kernel cost it 3%.
+template <int MODE, int DT, bool WIDE>
 __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
                                            const float* s_alpha, const ViewCtx& V, int bx, int by) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -478,53 +487,63 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       // pass it with a few instructions; the rest are computed exactly, in
       // full batches, from the first-step stack.
       uint32_t n_items = 0u, n_first = 0u;
+      const int groups = WIDE ? (C + 3) >> 2 : 1;  // channels in groups of 4: one Philox block each
       for (uint32_t b = 0; b < total; b += 64u) {
         const uint32_t i = b + uint32_t(lane);
         const bool act = i < total;
-        float4 uq = make_float4(0.f, 0.f, 0.f, 0.f), m4 = uq;
-        uint32_t need = 0u;
-        if (act) {
-          const Record R = L.rec[i];
-          const WaveRay& Y = L.ray[R.vl >> kVisitBits];
-          const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
-          m4 = cell_majorants(X, R.cid);
-          uint32_t w[4];
-          philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | 12u), 0u, D.seed, Y.pixel, w);
-          const uint32_t msk = Y.mask;
+        // the first steps of channels 4g .. 4g + 3 of record i
+        auto enter_group = [&](const int g) {
+          float4 uq = make_float4(0.f, 0.f, 0.f, 0.f), m4 = uq;
+          uint32_t need = 0u;
+          if (act) {
+            const Record R = L.rec[i];
+            const WaveRay& Y = L.ray[R.vl >> kVisitBits];
+            const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
+            m4 = cell_majorants(X, R.cid, g);
+            uint32_t w[4];
+            philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | (12u + uint32_t(g))), 0u,
+                   D.seed, Y.pixel, w);
+            const uint32_t msk = Y.mask >> (4 * g);
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const float m = comp(m4, c);
-            if (c < C && m != 0.0f && ((msk >> c) & 1u)) {
-              const float u = to_unit(w[c]);
-              if (!(u >= q * m)) {
-                set_comp(uq, c, u);
-                need |= 1u << c;
+            for (int k = 0; k < 4; ++k) {
+              const float m = comp(m4, k);
+              if (4 * g + k < C && m != 0.0f && ((msk >> k) & 1u)) {
+                const float u = to_unit(w[k]);
+                if (!(u >= q * m)) {
+                  set_comp(uq, k, u);
+                  need |= 1u << k;
+                }
               }
             }
           }
-        }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const bool nc = (need >> c) & 1u;
-          const uint64_t pm = __ballot(nc);
-          if (nc) {
-            Item e;
-            e.id = i | (uint32_t(c) << 9);
-            e.x = comp(uq, c);
-            e.m = comp(m4, c);
-            L.first[n_first + lane_rank(pm)] = e;
+          for (int k = 0; k < 4; ++k) {
+            const bool nc = (need >> k) & 1u;
+            const uint64_t pm = __ballot(nc);
+            if (nc) {
+              Item e;
+              e.id = i | (uint32_t(4 * g + k) << kChanShift);
+              e.x = comp(uq, k);
+              e.m = comp(m4, k);
+              L.first[n_first + lane_rank(pm)] = e;
+            }
+            n_first += uint32_t(__popcll(pm));
+            // full batches as soon as there are any
+            if (n_first >= 64u) {
+              wave_sync();
+              do {
+                first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, D);
+                // ---- 3. sample
+                while (n_items >= 64u)
+                  sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, D);
+              } while (n_first >= 64u);
+            }
           }
-          n_first += uint32_t(__popcll(pm));
-          // full batches as soon as there are any
-          if (n_first >= 64u) {
-            wave_sync();
-            do {
-              first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, D);
-              // ---- 3. sample
-              while (n_items >= 64u)
-                sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, D);
-            } while (n_first >= 64u);
-          }
+        };
+        if (WIDE) {
+          for (int g = 0; g < groups; ++g) enter_group(g);
+        } else {
+          enter_group(0);
         }
       }
       wave_sync();
@@ -542,10 +561,10 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
           // lower channel), or else the last channel's end of the box there
           if (e != kNoEvent) {
             done = true;
-            if (!((e >> 34) & 1ull)) {
+            if (!((e >> 36) & 1ull)) {
               hit = true;
-              hit_c = int(e & 3ull);
-              hit_t = from_order_bits(uint32_t(e >> 2));
+              hit_c = int(e & 15ull);
+              hit_t = from_order_bits(uint32_t(e >> 4));
             }
           }
         } else {
@@ -555,7 +574,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
           const uint32_t rot = (visit0 & 15u) << 2;
           const uint64_t bx0 = L.box[lane];
           const uint64_t bx = rot ? ((bx0 >> rot) | (bx0 << (64u - rot))) : bx0;  // bit 4k + c
-          const uint32_t k_hit = e != kNoEvent ? uint32_t(e >> 35) - visit0 : 0xffffffffu;
+          const uint32_t k_hit = e != kNoEvent ? uint32_t(e >> 37) - visit0 : 0xffffffffu;
           uint32_t k_all = 0u, left = 0u;
           bool all = true;
           for (int c = 0; c < C; ++c) {
@@ -572,8 +591,8 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
           if (e != kNoEvent && (!all || k_hit <= k_all)) {
             done = true;
             hit = true;
-            hit_c = 3 - int(e & 3ull);
-            hit_t = from_order_bits(uint32_t(e >> 2));
+            hit_c = 15 - int(e & 15ull);
+            hit_t = from_order_bits(uint32_t(e >> 4));
           } else if (all) {
             done = true;
           } else {
@@ -624,7 +643,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
 
 // One view: workgroup i renders the pixel block at position block_slot(i) of
 // the frame's block order.
-template <int MODE, int DT>
+template <int MODE, int DT, bool WIDE>
 __global__ __launch_bounds__(256) void render_wave(RenderParams P) {
   const uint64_t t_begin = P.fr.wave_cost ? wall_clock64() : 0ull;
   MDWT_WAVE_PROLOGUE
@@ -640,7 +659,7 @@ __global__ __launch_bounds__(256) void render_wave(RenderParams P) {
   V.accum_id = P.fr.accum_id;
   V.accum = P.fr.accum;
   V.m2 = P.fr.m2;
-  wave_block<MODE, DT>(P, X, L, s_alpha, V, int(bxy & 0xffffu), int(bxy >> 16));
+  wave_block<MODE, DT, WIDE>(P, X, L, s_alpha, V, int(bxy & 0xffffu), int(bxy >> 16));
   if (P.fr.wave_cost && (threadIdx.x & 63) == 0)
     P.fr.wave_cost[(blockDim.x >> 6) * slot_b + (threadIdx.x >> 6)] =
         uint32_t(wall_clock64() - t_begin);
@@ -665,13 +684,13 @@ __device__ __forceinline__ ViewCtx batch_view(const RenderParams& P, uint32_t co
 // Batched views, launch form 1: workgroup i renders item i (view << 20 |
 // by << 10 | bx) at the launch's frame and accumulation ids; with item_ticks
 // set it records its start and end wall clock.
-template <int MODE, int DT>
+template <int MODE, int DT, bool WIDE>
 __global__ __launch_bounds__(256) void render_views_wave(RenderParams P) {
   const uint64_t t_begin = P.fr.item_ticks ? wall_clock64() : 0ull;
   MDWT_WAVE_PROLOGUE
   const uint32_t code = P.fr.items[blockIdx.x];
   const ViewCtx V = batch_view(P, code, P.fr.frame_id, P.fr.accum_id);
-  wave_block<MODE, DT>(P, X, L, s_alpha, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu));
+  wave_block<MODE, DT, WIDE>(P, X, L, s_alpha, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu));
   if (P.fr.item_ticks) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -685,7 +704,7 @@ __global__ __launch_bounds__(256) void render_views_wave(RenderParams P) {
 // g + G, ... at the frame and accumulation ids in work[1] / work[2]; each wave
 // moves to its next item as soon as its quad is done. Wave 0 records each
 // item's start and end wall clock.
-template <int MODE, int DT>
+template <int MODE, int DT, bool WIDE>
 __global__ __launch_bounds__(256) void render_views_wave_persistent(RenderParams P) {
   MDWT_WAVE_PROLOGUE
   const uint32_t fid = P.fr.work[1], aid = P.fr.work[2];
@@ -694,7 +713,7 @@ __global__ __launch_bounds__(256) void render_views_wave_persistent(RenderParams
     const uint64_t t_item = P.fr.item_ticks ? wall_clock64() : 0ull;
     const uint32_t code = P.fr.items[it];
     const ViewCtx V = batch_view(P, code, fid, aid);
-    wave_block<MODE, DT>(P, X, L, s_alpha, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu));
+    wave_block<MODE, DT, WIDE>(P, X, L, s_alpha, V, int(code & 0x3ffu), int((code >> 10) & 0x3ffu));
     if (P.fr.item_ticks && threadIdx.x == 0) {
       P.fr.item_ticks[2 * it] = t_item;
       P.fr.item_ticks[2 * it + 1] = wall_clock64();
@@ -704,18 +723,19 @@ __global__ __launch_bounds__(256) void render_views_wave_persistent(RenderParams
 #undef MDWT_WAVE_PROLOGUE
 
 // form: 0 one view, 1 batched views (one launch), 2 batched views (persistent)
-template <int MODE, int DT>
+template <int MODE, int DT, bool WIDE>
 RenderFn wave_fn_form(int form) {
-  return form == 0 ? render_wave<MODE, DT>
-                   : (form == 1 ? render_views_wave<MODE, DT> : render_views_wave_persistent<MODE, DT>);
+  return form == 0 ? render_wave<MODE, DT, WIDE>
+                   : (form == 1 ? render_views_wave<MODE, DT, WIDE>
+                                : render_views_wave_persistent<MODE, DT, WIDE>);
 }
-template <int MODE>
+template <int MODE, bool WIDE>
 RenderFn wave_fn(int dtype, int form) {
   switch (dtype) {
-    case MDWT_DT_U8: return wave_fn_form<MODE, MDWT_DT_U8>(form);
-    case MDWT_DT_U16: return wave_fn_form<MODE, MDWT_DT_U16>(form);
-    case MDWT_DT_F16: return wave_fn_form<MODE, MDWT_DT_F16>(form);
-    default: return wave_fn_form<MODE, MDWT_DT_F32>(form);
+    case MDWT_DT_U8: return wave_fn_form<MODE, MDWT_DT_U8, WIDE>(form);
+    case MDWT_DT_U16: return wave_fn_form<MODE, MDWT_DT_U16, WIDE>(form);
+    case MDWT_DT_F16: return wave_fn_form<MODE, MDWT_DT_F16, WIDE>(form);
+    default: return wave_fn_form<MODE, MDWT_DT_F32, WIDE>(form);
   }
 }
 
@@ -724,12 +744,15 @@ RenderFn wave_fn(int dtype, int form) {
 bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit) {
   // a ray visits at most mx + my + mz cells: visit numbers fit the records' 26 bits
   const uint64_t max_visits = uint64_t(p.mc[0]) + uint64_t(p.mc[1]) + uint64_t(p.mc[2]);
-  return (mode == 0 || mode == 2) && !count && !lit && p.vol.channels <= 4 &&
+  // -m 0's end-of-box word holds 16 visits x 4 channels
+  const int max_channels = mode == 0 ? 4 : kMaxChannels;
+  return (mode == 0 || mode == 2) && !count && !lit && p.vol.channels <= max_channels &&
          max_visits < (uint64_t(1) << kVisitBits);
 }
 
-RenderFn select_wave_kernel(int mode, int dtype, int form) {
-  return mode == 0 ? wave_fn<0>(dtype, form) : wave_fn<2>(dtype, form);
+RenderFn select_wave_kernel(const RenderParams& p, int mode, int dtype, int form) {
+  if (mode == 0) return wave_fn<0, false>(dtype, form);
+  return p.vol.channels > 4 ? wave_fn<2, true>(dtype, form) : wave_fn<2, false>(dtype, form);
 }
 
 size_t wave_lds_bytes(const RenderParams& p) {

@@ -58,8 +58,8 @@ inline mdwt::ShadeParams default_shade() {
   return s;
 }
 
-// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 0 /
-// -m 2 on at most four channels (kernels_wave.hip), the thread-per-pixel
+// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 2 and
+// for -m 0 on at most four channels (kernels_wave.hip), the thread-per-pixel
 // kernel everywhere else.
 constexpr int kDefaultSchedule = 1;
 
@@ -553,12 +553,13 @@ int make_bricks(mdwt_renderer* h) {
   return MDWT_OK;
 }
 
-// The wave-coherent schedule covers -m 0 / -m 2 on at most four channels;
-// every other mode, shadow rays, gradient shading and auxiliary outputs run
-// the thread-per-pixel kernel (the launcher checks the same conditions).
+// The wave-coherent schedule covers -m 0 on at most four channels and -m 2 on
+// any number; every other mode, shadow rays, gradient shading and auxiliary
+// outputs run the thread-per-pixel kernel (the launcher checks the same
+// conditions).
 int eff_schedule(const mdwt_renderer* h) {
   if (h->sh.shadows || h->sh.gradient || h->aov_on) return 0;
-  return ((h->mode == 0 || h->mode == 2) && h->C <= 4) ? h->schedule : 0;
+  return ((h->mode == 0 && h->C <= 4) || h->mode == 2) ? h->schedule : 0;
 }
 
 mdwt::RenderParams render_params(const mdwt_renderer* h) {

@@ -261,3 +261,30 @@ def test_persistent_graph_recaptures_on_changes(capi, scene):
     for (a1, b1), (a2, b2) in [(out[0], out[1])]:
         for x, y in zip(a1 + b1, a2 + b2):
             np.testing.assert_array_equal(_bits(x), _bits(y))
+
+
+@pytest.mark.parametrize("launch", [0, 1, 2])
+def test_ten_channel_batch_equals_single_view_renders(capi, launch):
+    """-m 2 on ten f32 channels runs the wave-coherent kernel's twelve-channel
+    instantiation in every launch form (one Philox block per four channels per
+    cell record); each batched view equals its single-view render bit for
+    bit, and the single-view ten-channel render is pinned against the oracle
+    in test_gpu_configs.py."""
+    sc = scenes.make_scene("large_10ch_f32", seed=1234, scale=8)
+    views = _views(4, seed=11)
+    print(f"[batch 10ch] launch={launch} views={[(v['width'], v['height'], v['spp']) for v in views]}",
+          flush=True)
+    g = capi.MdwtRenderer(0)
+    g.set_moments(True)
+    scenes.load_scene(g, sc, 2)
+    g.batch_set_launch(launch)
+    g.batch_set_views(views)
+    g.batch_render(2)
+    g.batch_render(1)
+    for i, v in enumerate(views):
+        (a_s, m_s), _ = _single(capi, sc, 2, v, 3)
+        a_b, m_b = g.batch_read_view(i, with_m2=True)
+        np.testing.assert_array_equal(_bits(a_b), _bits(a_s))
+        np.testing.assert_array_equal(_bits(m_b), _bits(m_s))
+        assert a_s[..., :3].max() > 0.0  # the views see the volume
+    g.close()

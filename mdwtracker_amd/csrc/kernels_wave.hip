@@ -140,12 +140,14 @@ __device__ __forceinline__ uint64_t event_key(uint32_t visit, uint32_t flag, flo
 }
 template <int MODE>
 __device__ __forceinline__ void on_hit(WaveLds& L, const Record& R, int c, float t) {
-  const uint32_t ch = MODE == 2 ? uint32_t(c) : uint32_t(15 - c);
+  const uint32_t ch = MODE == 0 ? uint32_t(15 - c) : uint32_t(c);
   atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 0u, t, ch));
 }
 template <int MODE>
 __device__ __forceinline__ void on_box_end(WaveLds& L, const Record& R, int c, int C, float t) {
-  if (MODE == 2) {
+  if (MODE == 3) {
+    atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
+  } else if (MODE == 2) {
     if (c == C - 1) atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
   } else {
     atomicOr(&L.box[R.vl >> kVisitBits], 1ull << (((R.vl & 15u) << 2) | uint32_t(c)));
@@ -203,15 +205,31 @@ __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& 
   const float un = to_unit((j & 1u) ? w[3] : w[1]);
   const float tWorld = t * dg.w;
   const V3 xT = (v3(ow.x, ow.y, ow.z) + tWorld * v3(dg.x, dg.y, dg.z)) * w2u;
-  const float value = X.sample(xT, c);
-  bool hit = false;
-  if (!(value != value)) {
-    const float a = alpha_lookup(s_a, X.chtf[c], value);
-    hit = a > 0.0f && thr < a;
-  }
-  if (hit) {
-    on_hit<MODE>(L, R, c, t);
-    return false;
+  if (MODE == 3) {
+    // cummilativeDTCH (deviceCode.cu:585-609): the channels in order take
+    // their opacity off the threshold until one exceeds what is left
+    float th = thr;
+    for (int k = 0; k < X.C; ++k) {
+      const float value = X.sample(xT, k);
+      if (value != value) continue;
+      const float a = alpha_lookup(s_a, X.chtf[k], value);
+      if (a > 0.0f && th < a) {
+        on_hit<MODE>(L, R, k, t);
+        return false;
+      }
+      th -= a;
+    }
+  } else {
+    const float value = X.sample(xT, c);
+    bool hit = false;
+    if (!(value != value)) {
+      const float a = alpha_lookup(s_a, X.chtf[c], value);
+      hit = a > 0.0f && thr < a;
+    }
+    if (hit) {
+      on_hit<MODE>(L, R, c, t);
+      return false;
+    }
   }
   tn = t - (plogf_poly(1.0f - un) / m) * X.dt * ow.w;
   if (tn >= R.t1) return false;  // leaves the cell
@@ -499,25 +517,42 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
             const Record R = L.rec[i];
             const WaveRay& Y = L.ray[R.vl >> kVisitBits];
             const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
-            m4 = cell_majorants(X, R.cid, g);
-            uint32_t w[4];
-            philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | (12u + uint32_t(g))), 0u,
-                   D.seed, Y.pixel, w);
-            const uint32_t msk = Y.mask >> (4 * g);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              const float m = comp(m4, k);
-              if (4 * g + k < C && m != 0.0f && ((msk >> k) & 1u)) {
-                const float u = to_unit(w[k]);
+            if (MODE == 3) {
+              // one summed majorant per cell, tracked at channel 0's positions
+              const float m = X.maj[R.cid];
+              if (m != 0.0f) {
+                uint32_t w[4];
+                philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | 12u), 0u, D.seed,
+                       Y.pixel, w);
+                const float u = to_unit(w[0]);
+                m4.x = m;
                 if (!(u >= q * m)) {
-                  set_comp(uq, k, u);
-                  need |= 1u << k;
+                  uq.x = u;
+                  need = 1u;
+                }
+              }
+            } else {
+              m4 = cell_majorants(X, R.cid, g);
+              uint32_t w[4];
+              philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | (12u + uint32_t(g))),
+                     0u, D.seed, Y.pixel, w);
+              const uint32_t msk = Y.mask >> (4 * g);
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                const float m = comp(m4, k);
+                if (4 * g + k < C && m != 0.0f && ((msk >> k) & 1u)) {
+                  const float u = to_unit(w[k]);
+                  if (!(u >= q * m)) {
+                    set_comp(uq, k, u);
+                    need |= 1u << k;
+                  }
                 }
               }
             }
           }
+          constexpr int kPush = MODE == 3 ? 1 : 4;  // tracked channels per record
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
+          for (int k = 0; k < kPush; ++k) {
             const bool nc = (need >> k) & 1u;
             const uint64_t pm = __ballot(nc);
             if (nc) {
@@ -556,9 +591,10 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       if (alive) {
         const uint64_t e = L.ev[lane];
         bool done = false;
-        if (MODE == 2) {
+        if (MODE == 2 || MODE == 3) {
           // a hit in the first visit with an event (smallest t, ties to the
-          // lower channel), or else the last channel's end of the box there
+          // lower channel; -m 3 has at most one per visit), or else the end
+          // of the box there (-m 2: the last channel's)
           if (e != kNoEvent) {
             done = true;
             if (!((e >> 36) & 1ull)) {
@@ -746,12 +782,13 @@ bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit
   const uint64_t max_visits = uint64_t(p.mc[0]) + uint64_t(p.mc[1]) + uint64_t(p.mc[2]);
   // -m 0's end-of-box word holds 16 visits x 4 channels
   const int max_channels = mode == 0 ? 4 : kMaxChannels;
-  return (mode == 0 || mode == 2) && !count && !lit && p.vol.channels <= max_channels &&
-         max_visits < (uint64_t(1) << kVisitBits);
+  return (mode == 0 || mode == 2 || mode == 3) && !count && !lit &&
+         p.vol.channels <= max_channels && max_visits < (uint64_t(1) << kVisitBits);
 }
 
 RenderFn select_wave_kernel(const RenderParams& p, int mode, int dtype, int form) {
   if (mode == 0) return wave_fn<0, false>(dtype, form);
+  if (mode == 3) return wave_fn<3, false>(dtype, form);
   return p.vol.channels > 4 ? wave_fn<2, true>(dtype, form) : wave_fn<2, false>(dtype, form);
 }
 

@@ -559,7 +559,7 @@ int make_bricks(mdwt_renderer* h) {
 // conditions).
 int eff_schedule(const mdwt_renderer* h) {
   if (h->sh.shadows || h->sh.gradient || h->aov_on) return 0;
-  return ((h->mode == 0 && h->C <= 4) || h->mode == 2) ? h->schedule : 0;
+  return ((h->mode == 0 && h->C <= 4) || h->mode == 2 || h->mode == 3) ? h->schedule : 0;
 }
 
 mdwt::RenderParams render_params(const mdwt_renderer* h) {

@@ -175,6 +175,7 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
   if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
   const float step = X.dt;
   float majorant = X.maj[d.cx + d.cy * X.mx + d.cz * X.mx * X.my];
+  uint32_t visit = 0u;  // the (visit, channel 0) positions of the stream
   while (true) {
     cnt.visits++;
     float t0, t1;
@@ -184,8 +185,12 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
     const float mnext = more ? X.maj[dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my] : 0.f;
     if (majorant != 0.00f) {
       float t = t0;
+      uint32_t cw[4];
+      rng.cell_block(visit, 0u, cw);
+      rng.start_pairs(visit, 0u);
+      float u = to_unit(cw[0]);
       while (true) {
-        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - u) / majorant) * step * g.worldToGridT;
         if (t >= t1) break;
         const float tWorld = t * g.gridToWorldT;
         const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -193,7 +198,8 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
           cnt.rejections++;
           return;
         }
-        float thr = rng.thr() * majorant;
+        float thr = rng.pair_thr() * majorant;
+        u = rng.pair_dist();
         for (int c = 0; c < X.C; ++c) {
           const float value = X.sample(xT, c);
           cnt.samples++;
@@ -215,6 +221,7 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
     if (!more) return;
     d = dn;
     majorant = mnext;
+    ++visit;
   }
 }
 

@@ -91,7 +91,8 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 //
 // -m 0 and -m 2 track each (macrocell visit v, channel c) from the cell's
 // entry (v counts the cells of the ray's DDA walk, the same cells for every
-// channel pass of -m 0):
+// channel pass of -m 0); -m 3 tracks its one summed majorant per visit v with
+// the positions of channel 0:
 //   * the first free-flight distance of every channel of group g = c / 4 at
 //     visit v is word c mod 4 of the cell block (frame, sample,
 //     1 + 16 v + 12 + g, stream): one block serves up to four channels, and
@@ -100,7 +101,7 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 //     its null-collision threshold and the next step's distance: words
 //     2 (j mod 2) and 2 (j mod 2) + 1 of block (frame, sample, 1 + 16 v + c,
 //     stream | 2 floor(j / 2)).
-// The other modes count free-flight steps from the ray's start: step j owns
+// The other modes (1, 4, 5) count free-flight steps from the ray's start: step j owns
 // words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold, read only if
 // the step lands inside the cell) of block (frame, sample, 1, stream | 2
 // floor(j / 2)).
@@ -919,18 +920,26 @@ void trace_sync(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& r
 }
 
 // deviceCode.cu:513-617 cummilativeDTCH (-m 3, cumulative majorant)
+// The draws per visit: the first distance, then per step that lands inside
+// the cell its threshold and the next distance (first_dist / pair_thr /
+// pair_dist, channel 0's positions of the visit); in the faithful path the
+// same calls take the running stream in the reference's order (distance,
+// threshold, distance, ...).
 void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng,
                       Hit& prd) {
   const int C = int(o->ch.size());
   const float unit = o->dt;
+  uint32_t visit = 0;  // macrocells entered so far: the stream's block offset
   auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
     prd.visits++;
     const int cid = cell_id(o, cellIdx);
+    const uint32_t vis = visit++;
     const float majorant = o->maj[size_t(cid)];
     if (majorant == 0.00f) return true;
     float t = t0;
+    float u = rng.first_dist(vis, 0u);
     while (true) {
-      t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (flight_log(o, 1.0f - u) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -938,7 +947,8 @@ void trace_cumulative(const oracle_renderer* o, const GridRay& g, float prd_t1, 
         prd.rejections++;
         return false;
       }
-      float nullColThreshold = rng.thr() * majorant;
+      float nullColThreshold = rng.pair_thr() * majorant;
+      u = rng.pair_dist();
       for (int c = 0; c < C; ++c) {
         const float value = sample_volume(o, xTexture, c);
         prd.samples++;

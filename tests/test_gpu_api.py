@@ -93,19 +93,25 @@ def test_homogeneous_medium_full_size(capi, mode):
     a = g.read_accum()
     sigma = float(np.float32(alpha) ** np.float32(3))
     mu = sigma / g.get_dt()
-    # chord of the jitter-free pixel centre rays (the pixel footprint is tiny
-    # against the chord's variation at 1024^2)
+    # hit probability averaged over the renderer's diagonal jitter (camera ray
+    # at (px + jit) / W, (py + jit) / H, jit uniform in [0, 1); midpoint rule):
+    # near the silhouette the chord changes within a pixel
     org, llc, hor, ver = g.get_camera().astype(np.float64)
     ys, xs = np.mgrid[0:H, 0:W]
-    p = llc + ((xs + 0.5) / W)[..., None] * hor + ((ys + 0.5) / H)[..., None] * ver
-    d = p - org
-    d /= np.linalg.norm(d, axis=-1, keepdims=True)
-    with np.errstate(divide="ignore"):
-        ta, tb = (0.0 - org) / d, (1.0 - org) / d
-    tn = np.max(np.minimum(ta, tb), axis=-1)
-    tf = np.min(np.maximum(ta, tb), axis=-1)
-    L = np.where(tf > tn, tf - tn, 0.0)
-    prob = 1.0 - np.exp(-mu * L)
+    prob = np.zeros((H, W))
+    n_jit = 16
+    for k in range(n_jit):
+        jit = (k + 0.5) / n_jit
+        p = llc + ((xs + jit) / W)[..., None] * hor + ((ys + jit) / H)[..., None] * ver
+        d = p - org
+        d /= np.linalg.norm(d, axis=-1, keepdims=True)
+        with np.errstate(divide="ignore"):
+            ta, tb = (0.0 - org) / d, (1.0 - org) / d
+        tn = np.max(np.minimum(ta, tb), axis=-1)
+        tf = np.min(np.maximum(ta, tb), axis=-1)
+        L = np.where(tf > tn, tf - tn, 0.0)
+        prob += 1.0 - np.exp(-mu * L)
+    prob /= n_jit
     inside = (prob > 0.05) & (prob < 0.95)
     mean = a[..., 0] / 1.1
     z = (mean[inside].sum() - prob[inside].sum()) / np.sqrt(

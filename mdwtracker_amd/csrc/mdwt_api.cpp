@@ -58,9 +58,9 @@ inline mdwt::ShadeParams default_shade() {
   return s;
 }
 
-// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 2 and
-// for -m 0 on at most four channels (kernels_wave.hip), the thread-per-pixel
-// kernel everywhere else.
+// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 2,
+// -m 3 and for -m 0 on at most four channels (kernels_wave.hip), the
+// thread-per-pixel kernel everywhere else.
 constexpr int kDefaultSchedule = 1;
 
 struct mdwt_renderer {
@@ -553,10 +553,10 @@ int make_bricks(mdwt_renderer* h) {
   return MDWT_OK;
 }
 
-// The wave-coherent schedule covers -m 0 on at most four channels and -m 2 on
-// any number; every other mode, shadow rays, gradient shading and auxiliary
-// outputs run the thread-per-pixel kernel (the launcher checks the same
-// conditions).
+// The wave-coherent schedule covers -m 0 on at most four channels and -m 2 and
+// -m 3 on any number; every other mode, shadow rays, gradient shading and
+// auxiliary outputs run the thread-per-pixel kernel (the launcher checks the
+// same conditions).
 int eff_schedule(const mdwt_renderer* h) {
   if (h->sh.shadows || h->sh.gradient || h->aov_on) return 0;
   return ((h->mode == 0 && h->C <= 4) || h->mode == 2 || h->mode == 3) ? h->schedule : 0;

@@ -554,11 +554,13 @@ int make_bricks(mdwt_renderer* h) {
 }
 
 // The wave-coherent schedule covers -m 0 on at most four channels and -m 2 and
-// -m 3 on any number; every other mode, shadow rays, gradient shading and
-// auxiliary outputs run the thread-per-pixel kernel (the launcher checks the
-// same conditions).
+// -m 3 on any number; every other mode, shadow rays, gradient shading and the
+// frames that write auxiliary outputs (the first aov_frames of an
+// accumulation) run the thread-per-pixel kernel (the launcher checks the same
+// conditions). Both give the same image bits, so an accumulation may switch.
 int eff_schedule(const mdwt_renderer* h) {
-  if (h->sh.shadows || h->sh.gradient || h->aov_on) return 0;
+  if (h->sh.shadows || h->sh.gradient) return 0;
+  if (h->aov_on && (h->aov_frames == 0 || h->accum_id < uint32_t(h->aov_frames))) return 0;
   return ((h->mode == 0 && h->C <= 4) || h->mode == 2 || h->mode == 3) ? h->schedule : 0;
 }
 

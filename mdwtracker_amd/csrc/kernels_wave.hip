@@ -139,14 +139,26 @@ __device__ __forceinline__ float from_order_bits(uint32_t o) {
 __device__ __forceinline__ uint64_t event_key(uint32_t visit, uint32_t flag, float t, uint32_t ch) {
   return (uint64_t(visit) << 37) | (uint64_t(flag) << 36) | (uint64_t(order_bits(t)) << 4) | ch;
 }
+// -m 1: the channels' clocks fire in time order, the lower channel first on a
+// tie, and any channel's clock past the box's end ends the ray: key visit,
+// then t, then the channel, then the flag.
+__device__ __forceinline__ uint64_t clock_key(uint32_t visit, uint32_t flag, float t, uint32_t ch) {
+  return (uint64_t(visit) << 37) | (uint64_t(order_bits(t)) << 5) | (uint64_t(ch) << 1) | flag;
+}
 template <int MODE>
 __device__ __forceinline__ void on_hit(WaveLds& L, const Record& R, int c, float t) {
+  if (MODE == 1) {
+    atomicMin(&L.ev[R.vl >> kVisitBits], clock_key(R.vl & kVisitMask, 0u, t, uint32_t(c)));
+    return;
+  }
   const uint32_t ch = MODE == 0 ? uint32_t(15 - c) : uint32_t(c);
   atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 0u, t, ch));
 }
 template <int MODE>
 __device__ __forceinline__ void on_box_end(WaveLds& L, const Record& R, int c, int C, float t) {
-  if (MODE == 3) {
+  if (MODE == 1) {
+    atomicMin(&L.ev[R.vl >> kVisitBits], clock_key(R.vl & kVisitMask, 1u, t, uint32_t(c)));
+  } else if (MODE == 3) {
     atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
   } else if (MODE == 2) {
     if (c == C - 1) atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
@@ -592,7 +604,18 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       if (alive) {
         const uint64_t e = L.ev[lane];
         bool done = false;
-        if (MODE == 2 || MODE == 3) {
+        if (MODE == 1) {
+          // the earliest event of any channel's clock: a hit, or its clock
+          // past the box's end
+          if (e != kNoEvent) {
+            done = true;
+            if (!(e & 1ull)) {
+              hit = true;
+              hit_c = int((e >> 1) & 15ull);
+              hit_t = from_order_bits(uint32_t(e >> 5));
+            }
+          }
+        } else if (MODE == 2 || MODE == 3) {
           // a hit in the first visit with an event (smallest t, ties to the
           // lower channel; -m 3 has at most one per visit), or else the end
           // of the box there (-m 2: the last channel's)
@@ -783,13 +806,15 @@ bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit
   const uint64_t max_visits = uint64_t(p.mc[0]) + uint64_t(p.mc[1]) + uint64_t(p.mc[2]);
   // -m 0's end-of-box word holds 16 visits x 4 channels
   const int max_channels = mode == 0 ? 4 : kMaxChannels;
-  return (mode == 0 || mode == 2 || mode == 3) && !count && !lit &&
+  return mode >= 0 && mode <= 3 && !count && !lit &&
          p.vol.channels <= max_channels && max_visits < (uint64_t(1) << kVisitBits);
 }
 
 RenderFn select_wave_kernel(const RenderParams& p, int mode, int dtype, int form) {
   if (mode == 0) return wave_fn<0, false>(dtype, form);
   if (mode == 3) return wave_fn<3, false>(dtype, form);
+  if (mode == 1)
+    return p.vol.channels > 4 ? wave_fn<1, true>(dtype, form) : wave_fn<1, false>(dtype, form);
   return p.vol.channels > 4 ? wave_fn<2, true>(dtype, form) : wave_fn<2, false>(dtype, form);
 }
 

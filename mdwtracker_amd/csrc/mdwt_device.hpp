@@ -67,9 +67,10 @@ __device__ __forceinline__ float to_unit(uint32_t w) { return float(w >> 8) * 0x
 
 // Word layout of a (pixel, frame, sample, stream) stream: the jitter is word 0
 // of block (frame, sample, 0, stream).
-// -m 0 / -m 2 track every (macrocell visit v, channel c) from the cell's
-// entry (v counts the cells of the ray's DDA walk); -m 3 tracks its summed
-// majorant per visit at the positions of channel 0:
+// -m 0 / -m 1 / -m 2 track every (macrocell visit v, channel c) from the
+// cell's entry (v counts the cells of the ray's DDA walk; -m 1 runs the
+// channels' clocks on these positions); -m 3 tracks its summed majorant per
+// visit at the positions of channel 0:
 //   * the first free-flight distance of channel c is word c mod 4 of the cell
 //     block (frame, sample, 1 + 16 v + 12 + c / 4, stream): one Philox block
 //     gives the first step of up to four channels, and most (v, c) steps
@@ -79,7 +80,7 @@ __device__ __forceinline__ float to_unit(uint32_t w) { return float(w >> 8) * 0x
 //     2 (j mod 2) + 1 of block (frame, sample, 1 + 16 v + c, stream |
 //     2 floor(j / 2)).
 // Every (v, c) step is thus a function of (v, c, j) alone, whichever lane
-// or loop computes it. The other modes (1, 4, 5) count steps from the ray's start:
+// or loop computes it. The other modes (4, 5) count steps from the ray's start:
 // step j owns words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold)
 // of block (frame, sample, 1, stream | 2 floor(j / 2)).
 struct Rng {
@@ -146,6 +147,14 @@ struct Rng {
   __device__ __forceinline__ float pair_dist() {
     const uint32_t p = k++;
     return to_unit((p & 1u) ? b3 : b1);
+  }
+  // -m 1: pair j of the (visit v, channel c) stream, without pair state
+  __device__ __forceinline__ void vc_pair(uint32_t v, uint32_t c, uint32_t j, float& th,
+                                          float& dist) const {
+    uint32_t o[4];
+    philox(frame, sample, 1u + ((v << 4) | c), stream | ((j >> 1) << 1), k0, k1, o);
+    th = to_unit((j & 1u) ? o[2] : o[0]);
+    dist = to_unit((j & 1u) ? o[3] : o[1]);
   }
 };
 

@@ -227,8 +227,10 @@ __device__ void trace_cumulative(const Ctx<DT>& X, const GridRay& g, float prd_t
 
 // -m 1, multiMajDTCH (deviceCode.cu:619-745): per cell one exponential clock
 // per channel, the earliest fires; only the fired channel's clock advances.
-// Clocks and majorants live in registers: the loops run over kMaxChannels
-// with constant indices and stop at X.C, so no private-memory arrays appear.
+// Channel c's clock starts at the first distance of (visit, c) and its j-th
+// event takes pair j of the (visit, c) stream. Clocks, majorants and pair
+// counts live in registers: the loops run over kMaxChannels with constant
+// indices and stop at X.C, so no private-memory arrays appear.
 template <int DT>
 __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rng& rng, HitRec& h,
                             Counts& cnt) {
@@ -236,6 +238,8 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
   if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
   const float step = X.dt;
   float maj[kMaxChannels], ts[kMaxChannels];
+  uint32_t pj[kMaxChannels];
+  uint32_t visit = 0u;
   while (true) {
     cnt.visits++;
     float t0, t1;
@@ -251,6 +255,7 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
         maj[i] = i < 4 && i < X.C ? pick(m4, i) : 0.0f;
         if (i < X.C) sum += maj[i];
         ts[i] = t0;
+        pj[i] = 0u;
       }
     } else {
 #pragma unroll
@@ -262,14 +267,18 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
           maj[i] = 0.0f;
         }
         ts[i] = t0;
+        pj[i] = 0u;
       }
     }
     if (!(sum <= 0.0000001f)) {
+      uint32_t cw[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
       for (int i = 0; i < kMaxChannels; ++i) {
         if (i < X.C) {
+          if ((i & 3) == 0) rng.cell_block(visit, uint32_t(i >> 2), cw);
           if (maj[i] > 0.0f)
-            ts[i] = ts[i] - (plogf_poly(1.0f - rng.step()) / maj[i]) * step * g.worldToGridT;
+            ts[i] = ts[i] - (plogf_poly(1.0f - to_unit(cw[i & 3])) / maj[i]) * step *
+                                g.worldToGridT;
           else
             ts[i] = 1e20f;
         }
@@ -277,13 +286,14 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
       while (true) {
         float minT = ts[0], mj = maj[0];
         int sel = 0;
-        const float rand = rng.step();
+        uint32_t j = pj[0];
 #pragma unroll
         for (int i = 1; i < kMaxChannels; ++i) {
           if (i < X.C && ts[i] < minT && maj[i] > 0.f) {
             sel = i;
             minT = ts[i];
             mj = maj[i];
+            j = pj[i];
           }
         }
         if (minT >= t1) break;
@@ -293,7 +303,9 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
           cnt.rejections++;
           return;
         }
-        const float thr = rng.thr() * mj;
+        float th, rand;
+        rng.vc_pair(visit, uint32_t(sel), j, th, rand);
+        const float thr = th * mj;
         const float value = X.sample(xT, sel);
         cnt.samples++;
         if (!(value != value)) {
@@ -310,11 +322,16 @@ __device__ void trace_multi(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
         cnt.rejections++;
         const float nt = minT - (plogf_poly(1.0f - rand) / mj) * step * g.worldToGridT;
 #pragma unroll
-        for (int i = 0; i < kMaxChannels; ++i)
-          if (i == sel) ts[i] = nt;
+        for (int i = 0; i < kMaxChannels; ++i) {
+          if (i == sel) {
+            ts[i] = nt;
+            pj[i] = j + 1u;
+          }
+        }
       }
     }
     if (!d.advance()) return;
+    ++visit;
   }
 }
 

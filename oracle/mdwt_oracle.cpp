@@ -101,7 +101,10 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 //     its null-collision threshold and the next step's distance: words
 //     2 (j mod 2) and 2 (j mod 2) + 1 of block (frame, sample, 1 + 16 v + c,
 //     stream | 2 floor(j / 2)).
-// The other modes (1, 4, 5) count free-flight steps from the ray's start: step j owns
+// -m 1 runs one exponential clock per channel in a cell on the (v, c)
+// positions of -m 2: the clock of channel c starts at its first distance and
+// its j-th event uses pair j of the (v, c) stream.
+// The other modes (4, 5) count free-flight steps from the ray's start: step j owns
 // words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold, read only if
 // the step lands inside the cell) of block (frame, sample, 1, stream | 2
 // floor(j / 2)).
@@ -178,6 +181,16 @@ struct Rng {
   float thr() {
     if (seq) return next();
     return unit(block(1u + base, stream | ((cur >> 1) << 1))[2 * (cur & 1u) + 1]);
+  }
+  // -m 1: the first distance of channel c at visit v (no pair state) and
+  // pair j of the (v, c) stream (its threshold and the next distance)
+  float cell_word(uint32_t v, uint32_t c) {
+    return unit(block(1u + ((v << 4) | (12u + (c >> 2))), stream)[c & 3u]);
+  }
+  void vc_pair(uint32_t v, uint32_t c, uint32_t j, float& th, float& dist) {
+    const uint32_t* b = block(1u + ((v << 4) | c), stream | ((j >> 1) << 1));
+    th = unit(b[2 * (j & 1u)]);
+    dist = unit(b[2 * (j & 1u) + 1]);
   }
 };
 
@@ -824,26 +837,35 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
   const float unit = o->dt;
   float majorants[kMaxChannels];
   float ts[kMaxChannels];
+  uint32_t pairs[kMaxChannels];  // events of each channel's clock so far in the cell
+  uint32_t visit = 0;            // macrocells entered so far: the streams' block offset
   auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
     prd.visits++;
     const int cid = cell_id(o, cellIdx);
+    const uint32_t vis = visit++;
     float majorantSum = 0.0f;
     for (int i = 0; i < C; ++i) {
       majorants[i] = o->maj[size_t(cid) * C + i];
       majorantSum += majorants[i];
       ts[i] = t0;
+      pairs[i] = 0;
     }
     if (majorantSum <= 0.0000001f) return true;
+    // the reference draws one running stream in this order: a distance per
+    // channel, then per event a distance (before the selection) and, for an
+    // event inside the cell, its threshold
     for (int i = 0; i < C; ++i) {
-      if (majorants[i] > 0.0f)
-        ts[i] = ts[i] - (flight_log(o, 1.0f - rng.step()) / majorants[i]) * unit * g.worldToGridT;
-      else
+      if (majorants[i] > 0.0f) {
+        const float u = rng.seq ? rng.step() : rng.cell_word(vis, uint32_t(i));
+        ts[i] = ts[i] - (flight_log(o, 1.0f - u) / majorants[i]) * unit * g.worldToGridT;
+      } else {
         ts[i] = 1e20f;
+      }
     }
     while (true) {
       float minT = ts[0];
       int sel = 0;
-      const float rand = rng.step();
+      const float rand_seq = rng.seq ? rng.step() : 0.0f;
       for (int i = 1; i < C; ++i) {
         if (ts[i] < minT && majorants[i] > 0.f) {
           sel = i;
@@ -857,7 +879,14 @@ void trace_multi(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
         prd.rejections++;
         return false;
       }
-      const float nullColThreshold = rng.thr() * majorants[sel];
+      float th, rand;
+      if (rng.seq) {
+        th = rng.thr();
+        rand = rand_seq;
+      } else {
+        rng.vc_pair(vis, uint32_t(sel), pairs[sel]++, th, rand);
+      }
+      const float nullColThreshold = th * majorants[sel];
       const float value = sample_volume(o, xTexture, sel);
       prd.samples++;
       if (!std::isnan(value)) {

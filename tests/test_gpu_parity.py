@@ -353,8 +353,8 @@ def test_tile_order_and_spp_and_no_accumulation(capi, po):
 
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_schedules_agree_bit_exact(capi, mode):
-    """The wave-coherent schedule (-m 0 / -m 2 / -m 3; -m 1 falls back to the
-    thread-per-pixel kernel) and the thread-per-pixel schedule give
+    """The wave-coherent schedule (the default for -m 0 to -m 3) and the
+    thread-per-pixel schedule give
     identical images, second moments and counters, with two samples per
     pixel on a frame whose sides are not multiples of the 16-pixel block."""
     sc = _scene("sync_4ch256", 2)

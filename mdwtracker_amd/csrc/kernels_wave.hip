@@ -10,25 +10,25 @@
 //
 //   1. walk:    each live ray walks its DDA up to K cells ahead and writes one
 //               record (cell, entry/exit t, visit) per cell; the records of
-//               all rays are packed into one list (K grows as rays finish, so
-//               a round always holds about kRecords records);
+//               all rays are packed into one dense, k-major list (K grows as
+//               rays finish, so a round holds up to kRecords records);
 //   2. enter:   the wave takes the list 64 records at a time: one lane per
 //               record loads the cell's majorants and draws the first
 //               free-flight distance of every channel from one Philox block;
 //               a conservative bound settles the (record, channel) pairs
 //               whose first step certainly leaves the cell, the others are
-//               pushed on the first-step stack (ballot + mbcnt);
-//   3. first:   full batches of 64 popped from that stack compute the exact
-//               first step and, when it lands inside the cell, its
-//               null-collision test; a rejected step's successor goes on the
-//               sample stack;
-//   4. sample:  full batches of 64 from the sample stack: threshold, trilinear
-//               sample, transfer function and, on a null collision, the next
-//               step, pushed again while it stays inside; the round's last,
-//               partial batch runs each lane to its item's end;
-//   5. resolve: each ray reads the outcomes of its records in cell order with
-//               the reference's rules and either ends (hit, or the root box's
-//               end) or continues in the next round.
+//               pushed on the item stack (ballot + mbcnt);
+//   3. passes:  full batches of 64 items popped from the stack: a first step
+//               is computed exactly and, when it lands inside the cell, gets
+//               its null-collision test (threshold, trilinear sample,
+//               transfer function); a rejected step's successor is pushed
+//               back as a test item;
+//   4. tail:    the round's last, partial batch runs each lane to its item's
+//               end;
+//   5. resolve: only events write: a hit, or the end of the root box, is an
+//               LDS atomic minimum on the ray's 64-bit event key, so each ray
+//               reads its first event of the round and either ends or
+//               continues in the next round.
 //
 // Exactness. With the per-(visit, channel) random streams of mdwt_device.hpp,
 // the steps of every (cell, channel) of a ray are a function of (visit,
@@ -57,9 +57,9 @@
 namespace mdwt {
 namespace {
 
-constexpr int kRecords = 316;  // records per wave and round
+constexpr int kRecords = 412;  // records per wave and round
 constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most (-m 0: 16 x 4 box bits)
-constexpr int kStack = 128;    // each stack: < 64 left + one channel's 64 pushes
+constexpr int kStack = 128;    // the item stack: < 64 left + one channel's 64 pushes
 // Safety factor of the certain-exit test (see the enter phase): far above the
 // few fp32 roundings (relative 2^-21) that it has to absorb.
 constexpr float kExitSafety = 1.0001f;
@@ -69,7 +69,8 @@ constexpr uint64_t kNoEvent = ~0ull;
 // stack entry id: record slot (9 bits) | channel << 9 (4 bits) | pair << 13
 constexpr uint32_t kSlotMask = 0x1ffu;
 constexpr uint32_t kChanShift = 9;
-constexpr uint32_t kPairShift = 13;
+constexpr uint32_t kFirstBit = 1u << 13;  // a first step (x = uniform), not a test (x = t)
+constexpr uint32_t kPairShift = 14;
 
 // A record: the macrocell of one (ray, visit), its entry and exit t (grid
 // units), the visit number and the ray's lane (vl = visit | lane << 26), read
@@ -107,8 +108,7 @@ struct Draw {
 // only events write: leaving a cell writes nothing.
 struct WaveLds {
   Record rec[kRecords];
-  Item sample[kStack];  // steps that need a null-collision test (x = t)
-  Item first[kStack];   // first steps to compute exactly (x = uniform)
+  Item stk[kStack];  // first steps (x = uniform) and null-collision tests (x = t)
   WaveRay ray[64];
   unsigned long long ev[64];   // per ray: first event key of the round
   unsigned long long box[64];  // -m 0, per ray: end-of-box bit 4 (visit mod 16) + channel
@@ -253,20 +253,22 @@ __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& 
   return true;
 }
 
-// Pops up to 64 items of the sample stack (n_items) and tests each. Full
-// passes push the next step back; the last pass of a round (DRAIN, fewer
-// than 64 items) keeps every lane on its item until it is settled, so the
-// rare long runs of null collisions cost loop iterations instead of passes.
+// Pops up to 64 items of the stack (n_items): a first step is computed
+// exactly, as the sequential loop does, and when it lands inside the cell
+// before the box's end gets its null-collision test; a test item is tested at
+// its t. Full passes push a rejected step's successor back; the round's last
+// pass (DRAIN, fewer than 64 items) keeps every lane on its item until it is
+// settled.
 template <int MODE, int DT, bool DRAIN>
-__device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                            const float* s_a, uint32_t& n_items, V3 w2u,
-                                            const Draw& D) {
+__device__ __forceinline__ void item_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
+                                          const float* s_a, uint32_t& n_items, V3 w2u,
+                                          const Draw& D) {
   const int lane = threadIdx.x & 63;
   const uint32_t n = n_items < 64u ? n_items : 64u;
   const uint32_t base = n_items - n;
   const bool act = uint32_t(lane) < n;
   Item it = {0u, 0.f, 0.f};
-  if (act) it = L.sample[base + lane];
+  if (act) it = L.stk[base + lane];
   wave_sync();
   bool push = false;
   float tn = 0.f;
@@ -276,7 +278,18 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
     const int c = int((it.id >> kChanShift) & 15u);
     const WaveRay Y = L.ray[R.vl >> kVisitBits];
     float t = it.x;
-    while (null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, j, t, it.m, tn)) {
+    bool go = true;
+    if (it.id & kFirstBit) {
+      t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
+      go = false;
+      if (t < R.t1) {  // otherwise it leaves the cell
+        if (t * Y.dir_g2w.w >= Y.tend)
+          on_box_end<MODE>(L, R, c, X.C, t);
+        else
+          go = true;
+      }
+    }
+    while (go && null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, j, t, it.m, tn)) {
       ++j;
       if (!DRAIN) {
         push = true;
@@ -293,104 +306,13 @@ __device__ __forceinline__ void sample_pass(const RenderParams& P, const Ctx<DT>
   const uint64_t pm = __ballot(push);
   if (push) {
     Item o;
-    o.id = (it.id & ((1u << kPairShift) - 1u)) | (j << kPairShift);
+    o.id = (it.id & (kFirstBit - 1u)) | (j << kPairShift);
     o.x = tn;
     o.m = it.m;
-    L.sample[base + lane_rank(pm)] = o;
+    L.stk[base + lane_rank(pm)] = o;
   }
   wave_sync();
   n_items = base + uint32_t(__popcll(pm));
-}
-
-// Pops up to 64 items of the first-step stack (n_first): the first free-flight
-// step of (record, channel) computed exactly, as the sequential loop does,
-// and, when it lands inside the cell before the box's end (nearly always for
-// these items), its null-collision test; a rejected step's successor goes on
-// the sample stack.
-template <int MODE, int DT>
-__device__ __forceinline__ void first_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                           const float* s_a, uint32_t& n_first,
-                                           uint32_t& n_items, V3 w2u, const Draw& D) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t n = n_first < 64u ? n_first : 64u;
-  const uint32_t base = n_first - n;
-  const bool act = uint32_t(lane) < n;
-  Item it = {0u, 0.f, 0.f};
-  if (act) it = L.first[base + lane];
-  wave_sync();
-  bool push = false;
-  float tn = 0.f;
-  if (act) {
-    const Record R = L.rec[it.id & kSlotMask];
-    const int c = int((it.id >> kChanShift) & 15u);
-    const WaveRay Y = L.ray[R.vl >> kVisitBits];
-    const float t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
-    if (t < R.t1) {  // otherwise it leaves the cell
-      if (t * Y.dir_g2w.w >= Y.tend)
-        on_box_end<MODE>(L, R, c, X.C, t);
-      else
-        push = null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, 0u, t, it.m, tn);
-    }
-  }
-  wave_sync();
-  const uint64_t pm = __ballot(push);
-  if (push) {
-    Item o;
-    o.id = (it.id & ((1u << kPairShift) - 1u)) | (1u << kPairShift);  // pair 1
-    o.x = tn;
-    o.m = it.m;
-    L.sample[n_items + lane_rank(pm)] = o;
-  }
-  wave_sync();
-  n_first = base;
-  n_items += uint32_t(__popcll(pm));
-}
-
-// The round's last pass: the fewer than 64 items left on the first-step stack
-// and, in the remaining lanes, as many items of the sample stack as fit. Every
-// lane stays on its item until it is settled, so a run of null collisions
-// costs loop iterations instead of passes.
-template <int MODE, int DT>
-__device__ __forceinline__ void tail_pass(const RenderParams& P, const Ctx<DT>& X, WaveLds& L,
-                                          const float* s_a, uint32_t& n_first,
-                                          uint32_t& n_items, V3 w2u, const Draw& D) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nf = n_first;  // < 64
-  const uint32_t ns = n_items < 64u - nf ? n_items : 64u - nf;
-  const uint32_t sbase = n_items - ns;
-  const bool is_first = lane < nf;
-  const bool act = lane < nf + ns;
-  Item it = {0u, 0.f, 0.f};
-  if (is_first)
-    it = L.first[lane];
-  else if (act)
-    it = L.sample[sbase + (lane - nf)];
-  wave_sync();
-  if (act) {
-    const Record R = L.rec[it.id & kSlotMask];
-    const int c = int((it.id >> kChanShift) & 15u);
-    const WaveRay Y = L.ray[R.vl >> kVisitBits];
-    float t = it.x, tn = 0.f;
-    uint32_t j = it.id >> kPairShift;
-    bool go = true;
-    if (is_first) {
-      t = R.t0 - (plogf_poly(1.0f - it.x) / it.m) * X.dt * Y.org_w2g.w;
-      go = false;
-      if (t < R.t1) {  // otherwise it leaves the cell
-        if (t * Y.dir_g2w.w >= Y.tend)
-          on_box_end<MODE>(L, R, c, X.C, t);
-        else
-          go = true;
-      }
-    }
-    while (go && null_test<MODE, DT>(P, X, L, s_a, w2u, D, c, R, Y, j, t, it.m, tn)) {
-      ++j;
-      t = tn;
-    }
-  }
-  wave_sync();
-  n_first = 0u;
-  n_items = sbase;
 }
 
 // Transfer-function opacities and per-channel parameters -> LDS (the caller
@@ -517,7 +439,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       // which the test below over-estimates by the safety factor. Most steps
       // pass it with a few instructions; the rest are computed exactly, in
       // full batches, from the first-step stack.
-      uint32_t n_items = 0u, n_first = 0u;
+      uint32_t n_items = 0u;
       const int groups = WIDE ? (C + 3) >> 2 : 1;  // channels in groups of 4: one Philox block each
       for (uint32_t b = 0; b < total; b += 64u) {
         const uint32_t i = b + uint32_t(lane);
@@ -570,21 +492,18 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
             const uint64_t pm = __ballot(nc);
             if (nc) {
               Item e;
-              e.id = i | (uint32_t(4 * g + k) << kChanShift);
+              e.id = i | (uint32_t(4 * g + k) << kChanShift) | kFirstBit;
               e.x = comp(uq, k);
               e.m = comp(m4, k);
-              L.first[n_first + lane_rank(pm)] = e;
+              L.stk[n_items + lane_rank(pm)] = e;
             }
-            n_first += uint32_t(__popcll(pm));
+            n_items += uint32_t(__popcll(pm));
             // full batches as soon as there are any
-            if (n_first >= 64u) {
+            if (n_items >= 64u) {
               wave_sync();
               do {
-                first_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, D);
-                // ---- 3. sample
-                while (n_items >= 64u)
-                  sample_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, D);
-              } while (n_first >= 64u);
+                item_pass<MODE, DT, false>(P, X, L, s_alpha, n_items, w2u, D);
+              } while (n_items >= 64u);
             }
           }
         };
@@ -595,10 +514,8 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
         }
       }
       wave_sync();
-      // fewer than 64 first steps and 64 tests left: one pass for both, and
-      // a draining pass for the sample items that did not fit
-      if (n_first + n_items > 0u) tail_pass<MODE, DT>(P, X, L, s_alpha, n_first, n_items, w2u, D);
-      if (n_items > 0u) sample_pass<MODE, DT, true>(P, X, L, s_alpha, n_items, w2u, D);
+      // fewer than 64 items left: one looping pass
+      if (n_items > 0u) item_pass<MODE, DT, true>(P, X, L, s_alpha, n_items, w2u, D);
       wave_sync();
       // ---- 4. resolve: the ray's first event of the round
       if (alive) {

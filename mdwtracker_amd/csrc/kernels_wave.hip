@@ -52,7 +52,7 @@
 // counters come from the counting pass of that kernel (steps tracked past a
 // ray's end here are speculation, not the reference's work).
 #include "mdwt_internal.hpp"
-#include "render_common.hpp"
+#include "render_paths.hpp"  // blend_channels (-m 4 / 5)
 
 namespace mdwt {
 namespace {
@@ -158,7 +158,7 @@ template <int MODE>
 __device__ __forceinline__ void on_box_end(WaveLds& L, const Record& R, int c, int C, float t) {
   if (MODE == 1) {
     atomicMin(&L.ev[R.vl >> kVisitBits], clock_key(R.vl & kVisitMask, 1u, t, uint32_t(c)));
-  } else if (MODE == 3) {
+  } else if (MODE >= 3) {
     atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
   } else if (MODE == 2) {
     if (c == C - 1) atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
@@ -218,7 +218,14 @@ __device__ __forceinline__ bool null_test(const RenderParams& P, const Ctx<DT>& 
   const float un = to_unit((j & 1u) ? w[3] : w[1]);
   const float tWorld = t * dg.w;
   const V3 xT = (v3(ow.x, ow.y, ow.z) + tWorld * v3(dg.x, dg.y, dg.z)) * w2u;
-  if (MODE == 3) {
+  if (MODE == 4 || MODE == 5) {
+    // blendDTCH (deviceCode.cu:972-1065): one blended sample of all channels
+    const float4 sb = blend_channels<MODE, DT>(X, xT);
+    if (sb.w > 0.0f && thr < sb.w) {
+      on_hit<MODE>(L, R, 0, t);
+      return false;
+    }
+  } else if (MODE == 3) {
     // cummilativeDTCH (deviceCode.cu:585-609): the channels in order take
     // their opacity off the threshold until one exceeds what is left
     float th = thr;
@@ -452,7 +459,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
             const Record R = L.rec[i];
             const WaveRay& Y = L.ray[R.vl >> kVisitBits];
             const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
-            if (MODE == 3) {
+            if (MODE >= 3) {
               // one summed majorant per cell, tracked at channel 0's positions
               const float m = X.maj[R.cid];
               if (m != 0.0f) {
@@ -485,7 +492,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
               }
             }
           }
-          constexpr int kPush = MODE == 3 ? 1 : 4;  // tracked channels per record
+          constexpr int kPush = MODE >= 3 ? 1 : 4;  // tracked channels per record
 #pragma unroll
           for (int k = 0; k < kPush; ++k) {
             const bool nc = (need >> k) & 1u;
@@ -532,7 +539,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
               hit_t = from_order_bits(uint32_t(e >> 5));
             }
           }
-        } else if (MODE == 2 || MODE == 3) {
+        } else if (MODE >= 2) {
           // a hit in the first visit with an event (smallest t, ties to the
           // lower channel; -m 3 has at most one per visit), or else the end
           // of the box there (-m 2: the last channel's)
@@ -591,9 +598,17 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       const float tWorld = hit_t * Y.dir_g2w.w;
       const V3 xT = (v3(Y.org_w2g.x, Y.org_w2g.y, Y.org_w2g.z) +
                      tWorld * v3(Y.dir_g2w.x, Y.dir_g2w.y, Y.dir_g2w.z)) * w2u;
-      float r2;
-      (void)X.tf_alpha(X.sample(xT, hit_c), hit_c, r2);
-      X.tf_rgb(r2, hit_c, h);
+      if (MODE == 4 || MODE == 5) {
+        // the blended sample's colour; the hit keeps channel 0
+        const float4 sb = blend_channels<MODE, DT>(X, xT);
+        h.r = sb.x;
+        h.g = sb.y;
+        h.b = sb.z;
+      } else {
+        float r2;
+        (void)X.tf_alpha(X.sample(xT, hit_c), hit_c, r2);
+        X.tf_rgb(r2, hit_c, h);
+      }
       h.tHit = tWorld;
     }
     composite_sample(P, hit, h.r, h.g, h.b, h.a, 1.0f, 0.0f, 1.0f, h.channel, fr, fg, fb, fa);
@@ -723,13 +738,15 @@ bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit
   const uint64_t max_visits = uint64_t(p.mc[0]) + uint64_t(p.mc[1]) + uint64_t(p.mc[2]);
   // -m 0's end-of-box word holds 16 visits x 4 channels
   const int max_channels = mode == 0 ? 4 : kMaxChannels;
-  return mode >= 0 && mode <= 3 && !count && !lit &&
+  return mode >= 0 && mode <= 5 && !count && !lit &&
          p.vol.channels <= max_channels && max_visits < (uint64_t(1) << kVisitBits);
 }
 
 RenderFn select_wave_kernel(const RenderParams& p, int mode, int dtype, int form) {
   if (mode == 0) return wave_fn<0, false>(dtype, form);
   if (mode == 3) return wave_fn<3, false>(dtype, form);
+  if (mode == 4) return wave_fn<4, false>(dtype, form);
+  if (mode == 5) return wave_fn<5, false>(dtype, form);
   if (mode == 1)
     return p.vol.channels > 4 ? wave_fn<1, true>(dtype, form) : wave_fn<1, false>(dtype, form);
   return p.vol.channels > 4 ? wave_fn<2, true>(dtype, form) : wave_fn<2, false>(dtype, form);

@@ -400,6 +400,7 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
   if (!d.begin(g.org, g.dir, 1e20f, X.mx, X.my, X.mz)) return;
   const float step = X.dt;
   float majorant = X.maj[d.cx + d.cy * X.mx + d.cz * X.mx * X.my];
+  uint32_t visit = 0u;  // the (visit, channel 0) positions of the stream, as -m 3
   while (true) {
     cnt.visits++;
     float t0, t1;
@@ -409,8 +410,12 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
     const float mnext = more ? X.maj[dn.cx + dn.cy * X.mx + dn.cz * X.mx * X.my] : 0.f;
     if (majorant != 0.00f) {
       float t = t0;
+      uint32_t cw[4];
+      rng.cell_block(visit, 0u, cw);
+      rng.start_pairs(visit, 0u);
+      float u = to_unit(cw[0]);
       while (true) {
-        t = t - (plogf_poly(1.0f - rng.step()) / majorant) * step * g.worldToGridT;
+        t = t - (plogf_poly(1.0f - u) / majorant) * step * g.worldToGridT;
         if (t >= t1) break;
         const float tWorld = t * g.gridToWorldT;
         const V3 xT = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -420,7 +425,9 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
         }
         const float4 s = blend_channels<MODE, DT>(X, xT);
         cnt.samples++;
-        if (s.w > 0.0f && rng.thr() * majorant < s.w) {
+        const float th = rng.pair_thr();
+        u = rng.pair_dist();
+        if (s.w > 0.0f && th * majorant < s.w) {
           h.missed = false;
           h.tHit = tWorld;
           h.channel = 0;
@@ -435,6 +442,7 @@ __device__ void trace_blend(const Ctx<DT>& X, const GridRay& g, float prd_t1, Rn
     if (!more) return;
     d = dn;
     majorant = mnext;
+    ++visit;
   }
 }
 

@@ -91,8 +91,8 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 //
 // -m 0 and -m 2 track each (macrocell visit v, channel c) from the cell's
 // entry (v counts the cells of the ray's DDA walk, the same cells for every
-// channel pass of -m 0); -m 3 tracks its one summed majorant per visit v with
-// the positions of channel 0:
+// channel pass of -m 0); -m 3 / 4 / 5 track their one summed majorant per
+// visit v with the positions of channel 0:
 //   * the first free-flight distance of every channel of group g = c / 4 at
 //     visit v is word c mod 4 of the cell block (frame, sample,
 //     1 + 16 v + 12 + g, stream): one block serves up to four channels, and
@@ -103,11 +103,12 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 //     stream | 2 floor(j / 2)).
 // -m 1 runs one exponential clock per channel in a cell on the (v, c)
 // positions of -m 2: the clock of channel c starts at its first distance and
-// its j-th event uses pair j of the (v, c) stream.
-// The other modes (4, 5) count free-flight steps from the ray's start: step j owns
-// words 2 (j mod 2) (distance) and 2 (j mod 2) + 1 (threshold, read only if
-// the step lands inside the cell) of block (frame, sample, 1, stream | 2
-// floor(j / 2)).
+// its j-th event uses pair j of the (v, c) stream. -m 4 / 5 track their summed
+// majorant per visit like -m 3 (channel 0's positions).
+// step() / thr() count free-flight steps from the ray's start (step j: words
+// 2 (j mod 2) and 2 (j mod 2) + 1 of block (frame, sample, 1, stream | 2
+// floor(j / 2))); no mode uses that layout now, and in the faithful path they
+// take the running stream.
 // The reference draws the same quantities from one sequential LCG stream
 // (deviceCode.cu:296-298, 563, 583, 797, 830, 913, 931); fixed positions let
 // the GPU compute one Philox block for all first steps of a cell, and draw
@@ -1039,13 +1040,19 @@ vec4f blend_channels(const oracle_renderer* o, vec3f p) {
 // payload's channel at 0 for these modes.
 void trace_blend(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& rng, Hit& prd) {
   const float unit = o->dt;
+  uint32_t visit = 0;  // macrocells entered so far: the stream's block offset
   auto lambda = [&](vec3i cellIdx, float t0, float t1) -> bool {
     prd.visits++;
+    const uint32_t vis = visit++;
     const float majorant = o->maj[size_t(cell_id(o, cellIdx))];
     if (majorant == 0.00f) return true;
     float t = t0;
+    // channel 0's positions of the visit, as -m 3; the faithful path draws
+    // the threshold only for a positive blended opacity (the reference's
+    // short-circuit), then the next distance
+    float u = rng.first_dist(vis, 0u);
     while (true) {
-      t = t - (flight_log(o, 1.0f - rng.step()) / majorant) * unit * g.worldToGridT;
+      t = t - (flight_log(o, 1.0f - u) / majorant) * unit * g.worldToGridT;
       if (t >= t1) break;
       const float tWorld = t * g.gridToWorldT;
       const vec3f xTexture = (g.worldOrg + tWorld * g.worldDir) * g.worldToUnit;
@@ -1055,10 +1062,11 @@ void trace_blend(const oracle_renderer* o, const GridRay& g, float prd_t1, Rng& 
       }
       const vec4f s = blend_channels(o, xTexture);
       prd.samples++;
-      if (s.w > 0.0f && rng.thr() * majorant < s.w) {
+      if (s.w > 0.0f && rng.pair_thr() * majorant < s.w) {
         record_hit(prd, tWorld, s, 0);
         return false;
       }
+      u = rng.pair_dist();
       prd.rejections++;
     }
     return true;

@@ -60,7 +60,7 @@ def _single(capi, scene, mode, v, frames):
 
 
 @pytest.mark.parametrize("launch", [0, 1, 2])
-@pytest.mark.parametrize("mode", [2, 0, 1, 3, 7])
+@pytest.mark.parametrize("mode", [2, 0, 1, 3, 4, 7])
 def test_batched_views_equal_single_view_renders(capi, scene, mode, launch):
     views = _views(6, seed=mode)
     print(f"[batch] launch={launch} mode={mode} "

@@ -351,9 +351,9 @@ def test_tile_order_and_spp_and_no_accumulation(capi, po):
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
 def test_schedules_agree_bit_exact(capi, mode):
-    """The wave-coherent schedule (the default for -m 0 to -m 3) and the
+    """The wave-coherent schedule (the default for -m 0 to -m 5) and the
     thread-per-pixel schedule give
     identical images, second moments and counters, with two samples per
     pixel on a frame whose sides are not multiples of the 16-pixel block."""

@@ -58,7 +58,7 @@ namespace mdwt {
 namespace {
 
 constexpr int kRecords = 412;  // records per wave and round
-constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most (-m 0: 16 x 4 box bits)
+constexpr int kMaxAhead = 16;  // cells a ray walks ahead per round, at most (-m 0: 64 / C above 4 channels)
 constexpr int kStack = 128;    // the item stack: < 64 left + one channel's 64 pushes
 // Safety factor of the certain-exit test (see the enter phase): far above the
 // few fp32 roundings (relative 2^-21) that it has to absorb.
@@ -163,7 +163,12 @@ __device__ __forceinline__ void on_box_end(WaveLds& L, const Record& R, int c, i
   } else if (MODE == 2) {
     if (c == C - 1) atomicMin(&L.ev[R.vl >> kVisitBits], event_key(R.vl & kVisitMask, 1u, t, 0u));
   } else {
-    atomicOr(&L.box[R.vl >> kVisitBits], 1ull << (((R.vl & 15u) << 2) | uint32_t(c)));
+    // bit (visit mod 16) * 4 + c; above four channels, (visit mod kv) * C + c
+    // with kv = 64 / C visits in the word
+    const uint32_t v = R.vl & kVisitMask;
+    const uint32_t bit = C <= 4 ? (((v & 15u) << 2) | uint32_t(c))
+                                : (v % uint32_t(64 / C)) * uint32_t(C) + uint32_t(c);
+    atomicOr(&L.box[R.vl >> kVisitBits], 1ull << bit);
   }
 }
 __device__ __forceinline__ float comp(const float4& v, int c) {
@@ -406,6 +411,7 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       const uint32_t n_live = uint32_t(__popcll(live));
       int K = int(uint32_t(kRecords) / n_live);
       K = K > kMaxAhead ? kMaxAhead : K;
+      if (MODE == 0 && WIDE && K > 64 / C) K = 64 / C;  // the end-of-box word's visits
       const uint32_t visit0 = visit;
       if (alive) {
         L.ray[lane].mask = mask;
@@ -555,22 +561,39 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
           // channel c leaves the ray at its first end-of-box visit; the ray
           // ends at its first hit (smallest t, ties to the higher channel)
           // unless every channel it still tracks left before that visit
-          const uint32_t rot = (visit0 & 15u) << 2;
           const uint64_t bx0 = L.box[lane];
-          const uint64_t bx = rot ? ((bx0 >> rot) | (bx0 << (64u - rot))) : bx0;  // bit 4k + c
           const uint32_t k_hit = e != kNoEvent ? uint32_t(e >> 37) - visit0 : 0xffffffffu;
           uint32_t k_all = 0u, left = 0u;
           bool all = true;
-          for (int c = 0; c < C; ++c) {
-            if (!((mask >> c) & 1u)) continue;
-            const uint64_t cb = bx & (0x1111111111111111ull << c);
-            if (cb == 0ull) {
-              all = false;
-            } else {
-              const uint32_t kc = uint32_t(__builtin_ctzll(cb)) >> 2;
-              k_all = kc > k_all ? kc : k_all;
-              left |= 1u << c;
+          if (!WIDE) {
+            const uint32_t rot = (visit0 & 15u) << 2;
+            const uint64_t bx = rot ? ((bx0 >> rot) | (bx0 << (64u - rot))) : bx0;  // bit 4k + c
+            for (int c = 0; c < C; ++c) {
+              if (!((mask >> c) & 1u)) continue;
+              const uint64_t cb = bx & (0x1111111111111111ull << c);
+              if (cb == 0ull) {
+                all = false;
+              } else {
+                const uint32_t kc = uint32_t(__builtin_ctzll(cb)) >> 2;
+                k_all = kc > k_all ? kc : k_all;
+                left |= 1u << c;
+              }
             }
+          } else {
+            // a ring of kv = 64 / C visits of C bits; the round's k-th visit
+            // sits at slot (visit0 + k) mod kv
+            const uint32_t kv = uint32_t(64 / C), cmask = (1u << C) - 1u;
+            const uint32_t s0 = visit0 % kv;
+            for (uint32_t k = 0; k < uint32_t(K); ++k) {
+              const uint32_t slot = (s0 + k) % kv;
+              const uint32_t chunk = uint32_t(bx0 >> (slot * uint32_t(C))) & cmask & mask;
+              const uint32_t fresh = chunk & ~left;
+              if (fresh) {
+                left |= fresh;
+                k_all = k;
+              }
+            }
+            all = left == mask;
           }
           if (e != kNoEvent && (!all || k_hit <= k_all)) {
             done = true;
@@ -736,14 +759,13 @@ RenderFn wave_fn(int dtype, int form) {
 bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit) {
   // a ray visits at most mx + my + mz cells: visit numbers fit the records' 26 bits
   const uint64_t max_visits = uint64_t(p.mc[0]) + uint64_t(p.mc[1]) + uint64_t(p.mc[2]);
-  // -m 0's end-of-box word holds 16 visits x 4 channels
-  const int max_channels = mode == 0 ? 4 : kMaxChannels;
-  return mode >= 0 && mode <= 5 && !count && !lit &&
-         p.vol.channels <= max_channels && max_visits < (uint64_t(1) << kVisitBits);
+  return mode >= 0 && mode <= 5 && !count && !lit && p.vol.channels <= kMaxChannels &&
+         max_visits < (uint64_t(1) << kVisitBits);
 }
 
 RenderFn select_wave_kernel(const RenderParams& p, int mode, int dtype, int form) {
-  if (mode == 0) return wave_fn<0, false>(dtype, form);
+  if (mode == 0)
+    return p.vol.channels > 4 ? wave_fn<0, true>(dtype, form) : wave_fn<0, false>(dtype, form);
   if (mode == 3) return wave_fn<3, false>(dtype, form);
   if (mode == 4) return wave_fn<4, false>(dtype, form);
   if (mode == 5) return wave_fn<5, false>(dtype, form);

@@ -561,7 +561,7 @@ int make_bricks(mdwt_renderer* h) {
 int eff_schedule(const mdwt_renderer* h) {
   if (h->sh.shadows || h->sh.gradient) return 0;
   if (h->aov_on && (h->aov_frames == 0 || h->accum_id < uint32_t(h->aov_frames))) return 0;
-  return ((h->mode == 0 && h->C <= 4) || (h->mode >= 1 && h->mode <= 5)) ? h->schedule : 0;
+  return (h->mode >= 0 && h->mode <= 5) ? h->schedule : 0;
 }
 
 mdwt::RenderParams render_params(const mdwt_renderer* h) {

@@ -50,7 +50,7 @@ def large8():
     return scenes.make_scene("large_10ch_f32", seed=1234, scale=8)
 
 
-@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_ten_channel_f32_plain_kernels_bit_exact(capi, po, large8, mode):
     sc = large8
     print(f"[10ch f32] mode={mode} dims={sc.volumes[0].shape} mc={sc.mc} image={sc.width}", flush=True)
@@ -101,7 +101,7 @@ def _hit_probability(cam, W, H, M, n=16):
     return ph / n
 
 
-@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_ten_channel_full_size_homogeneous(capi, mode):
     """Ten constant f32 channels of 512x512x128 at 2048^2: per pixel the hit
     probability is 1 - exp(-M L) with M the summed extinction, and a hit is

@@ -440,7 +440,7 @@ def main() -> int:
                 "measured_copy_gbps": round(copy_bw, 1),
                 "frac_of_measured": round(achieved / bw, 5) if bw > 0 else None,
                 "kernel": ((f"render_wave<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")
-                           if args.schedule == 1 and (1 <= args.mode <= 5 or (args.mode == 0 and scene.channels <= 4)) else
+                           if args.schedule == 1 and args.mode <= 5 else
                            f"render_kernel<{args.mode},{VOXEL_NAMES[str(scene.volumes[0].dtype)]}>")}
     if traffic_src:
         roofline["traffic_source"] = traffic_src

@@ -272,9 +272,9 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
 /* Frame-kernel schedule: 0 = one thread per pixel, nested cell/channel/step
  * loops; 1 = wave-coherent tracking (a wave traces 64 rays and compacts their
  * free-flight steps and null-collision tests into full 64-lane batches), the
- * default. Schedule 1 covers -m 1 to -m 5, and -m 0 on at most four
- * channels, without lighting; the ray marchers (-m 6-9), lighting and the
- * counting pass run schedule 0. Images do not depend on it.
+ * default. Schedule 1 covers the tracking modes -m 0 to -m 5 without
+ * lighting; the ray marchers (-m 6-9), lighting and the counting pass run
+ * schedule 0. Images do not depend on it.
  * Returns MDWT_ERR_ARG for other values. No reference counterpart (OptiX
  * schedules the reference's launch). */
 int mdwt_set_schedule(mdwt_renderer* h, int schedule);

@@ -341,9 +341,8 @@ class MdwtRenderer:
 
     def set_schedule(self, schedule: int) -> None:
         """0: one thread per pixel, nested loops; 1: wave-coherent tracking
-        (the default; -m 1 to -m 5, and -m 0 on at most four channels,
-        without lighting; the ray marchers and lighting run schedule 0).
-        Images do not depend on it."""
+        (the default; -m 0 to -m 5 without lighting; the ray marchers and
+        lighting run schedule 0). Images do not depend on it."""
         self._chk(lib.mdwt_set_schedule(self._h, schedule))
 
     # -- frames -------------------------------------------------------------

@@ -10,8 +10,8 @@
 // grid): a static split with no device work counter, no atomics and no
 // barrier inside the item loop, so every workgroup's loop ends after a fixed
 // number of items.
-// Under the wave-coherent schedule, -m 1 to -m 5 views and -m 0 views on at
-// most four channels, without lighting, run kernels_wave.hip's
+// Under the wave-coherent schedule, -m 0 to -m 5 views without lighting run
+// kernels_wave.hip's
 // render_views_wave and render_views_wave_persistent instead, with the same
 // items and ids.
 // No reference counterpart: the reference renders one camera per owlLaunch2D

@@ -1,6 +1,6 @@
-// Wave-coherent Woodcock tracking for -m 2 (syncDDA) and -m 1 (one clock per
-// channel), up to twelve channels, -m 3 (cumulative majorant), -m 4 / 5 (blend
-// tracking) and -m 0 (N-DDA, at most four channels) (frame-kernel schedule 1).
+// Wave-coherent Woodcock tracking for the tracking modes, up to twelve
+// channels: -m 0 (N-DDA), -m 1 (one clock per channel), -m 2 (syncDDA), -m 3
+// (cumulative majorant) and -m 4 / 5 (blend tracking) (frame-kernel schedule 1).
 //
 // The one-thread-per-pixel kernel nests cell -> channel -> free-flight step
 // loops per lane, so a wave64 runs every loop level as long as its slowest

@@ -58,9 +58,8 @@ inline mdwt::ShadeParams default_shade() {
   return s;
 }
 
-// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 1 to
-// -m 5 and for -m 0 on at most four channels (kernels_wave.hip), the
-// thread-per-pixel kernel everywhere else.
+// Frame-kernel schedule of a new handle: the wave-coherent kernel for -m 0 to
+// -m 5 (kernels_wave.hip), the thread-per-pixel kernel everywhere else.
 constexpr int kDefaultSchedule = 1;
 
 struct mdwt_renderer {
@@ -553,8 +552,8 @@ int make_bricks(mdwt_renderer* h) {
   return MDWT_OK;
 }
 
-// The wave-coherent schedule covers -m 0 on at most four channels and -m 1
-// to -m 5 on any number; the ray marchers, shadow rays, gradient shading and the
+// The wave-coherent schedule covers the tracking modes -m 0 to -m 5; the ray
+// marchers, shadow rays, gradient shading and the
 // frames that write auxiliary outputs (the first aov_frames of an
 // accumulation) run the thread-per-pixel kernel (the launcher checks the same
 // conditions). Both give the same image bits, so an accumulation may switch.

@@ -46,7 +46,14 @@
 //     same cells with the same visit numbers; a channel's pass ends at its
 //     first event (a box-end event removes the channel from later cells), and
 //     a hit is kept when t <= the nearest hit so far, so the ray ends in the
-//     first cell with a hit, the smallest t, ties to the higher channel.
+//     first cell with a hit, the smallest t, ties to the higher channel;
+//   * multiMajDTCH (-m 1, deviceCode.cu:619-745): the channels' clocks fire
+//     in time order, the lower channel first on a tie, so the ray ends at the
+//     earliest hit or clock past the box's end of any channel;
+//   * cummilativeDTCH / blendDTCH (-m 3 / 4 / 5, deviceCode.cu:513-617,
+//     972-1065): one summed majorant per cell, tracked as a single channel
+//     whose null-collision test samples every channel; the first hit or box
+//     end ends the ray.
 // The hit's colour is recomputed from the same sample expression. Images are
 // bit-identical to the one-thread-per-pixel kernel and the oracle; work
 // counters come from the counting pass of that kernel (steps tracked past a
@@ -66,7 +73,8 @@ constexpr float kExitSafety = 1.0001f;
 constexpr uint32_t kVisitBits = 26;  // visits of one ray below 2^26 (checked on the host)
 constexpr uint32_t kVisitMask = (1u << kVisitBits) - 1u;
 constexpr uint64_t kNoEvent = ~0ull;
-// stack entry id: record slot (9 bits) | channel << 9 (4 bits) | pair << 13
+// stack entry id: record slot (9 bits) | channel << 9 (4 bits) | first-step
+// flag << 13 | pair << 14
 constexpr uint32_t kSlotMask = 0x1ffu;
 constexpr uint32_t kChanShift = 9;
 constexpr uint32_t kFirstBit = 1u << 13;  // a first step (x = uniform), not a test (x = t)
@@ -89,8 +97,8 @@ struct WaveRay {
   float tend, idw;
   uint32_t pixel, mask;
 };
-// A stack entry: record slot | channel << 9 | pair << 13, a t or a uniform,
-// and the channel's majorant in the record's cell.
+// A stack entry: the id above, a t (a test) or a uniform (a first step), and
+// the channel's majorant in the record's cell.
 struct Item {
   uint32_t id;
   float x, m;
@@ -103,15 +111,15 @@ struct Draw {
 // Per-wave LDS. The records of a round are dense and k-major: the k-th cells
 // of all rays that walk that far are consecutive, so the lanes of a walk step
 // or an enter batch touch consecutive records. A ray's first event of the
-// round (the smallest (visit, t, channel rule) key among its hits, and for
-// -m 2 the last channel's end of the root box) is an LDS atomic minimum, so
-// only events write: leaving a cell writes nothing.
+// round (the smallest key among its hits and the box ends that end it, see
+// event_key / clock_key) is an LDS atomic minimum, and -m 0's box ends set
+// bits of a word, so only events write: leaving a cell writes nothing.
 struct WaveLds {
   Record rec[kRecords];
   Item stk[kStack];  // first steps (x = uniform) and null-collision tests (x = t)
   WaveRay ray[64];
   unsigned long long ev[64];   // per ray: first event key of the round
-  unsigned long long box[64];  // -m 0, per ray: end-of-box bit 4 (visit mod 16) + channel
+  unsigned long long box[64];  // -m 0, per ray: end-of-box bits (see on_box_end)
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
@@ -132,10 +140,11 @@ __device__ __forceinline__ uint32_t order_bits(float t) {
 __device__ __forceinline__ float from_order_bits(uint32_t o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
-// Event key: visit, then a flag (0: hit, 1: the last channel's end of the
-// root box, -m 2), then t, then 4 channel bits: 15 - channel for -m 0 (ties to
-// the higher channel), the channel for -m 2 (ties to the lower one). Within a
-// visit any hit ends -m 2's ray before the end of the box does.
+// Event key (-m 0 / 2 / 3 / 4 / 5): visit, then a flag (0: hit, 1: the end of
+// the root box: -m 2's last channel, -m 3-5's summed majorant), then t, then
+// 4 channel bits: 15 - channel for -m 0 (ties to the higher channel), the
+// channel otherwise (ties to the lower one). Within a visit any hit ends
+// -m 2's ray before the end of the box does.
 __device__ __forceinline__ uint64_t event_key(uint32_t visit, uint32_t flag, float t, uint32_t ch) {
   return (uint64_t(visit) << 37) | (uint64_t(flag) << 36) | (uint64_t(order_bits(t)) << 4) | ch;
 }

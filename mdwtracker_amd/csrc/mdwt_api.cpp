@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_deal.hpp"
 #include "mdwt.h"
 #include "mdwt_internal.hpp"
 #include "mdwt_types.hpp"
@@ -147,10 +148,12 @@ struct mdwt_renderer {
   unsigned long long* d_item_ticks = nullptr;
   // launch form 2's item list: the items in the order the persistent grid
   // takes them (workgroup g takes positions g, g + G, ...), re-dealt by the
-  // measured item costs after each call; pitems_measured when d_item_ticks
-  // holds the last form-2 frame's stamps of this list
+  // measured item costs after each call; pitems_measured when pitem_ticks
+  // holds the last form-2 frame's stamps of this list (a host copy, so a
+  // form-1 call in between does not change them)
   std::vector<uint32_t> pitems;
   uint32_t* d_pitems = nullptr;
+  std::vector<unsigned long long> pitem_ticks;
   bool pitems_measured = false;
   uint32_t batch_frame = 0, batch_accum = 0;
   int batch_grid = 0, batch_grid_key = -1;  // resident grid of launch form 2
@@ -1450,6 +1453,7 @@ void free_batch(mdwt_renderer* h) {
   h->d_items = nullptr;
   h->d_pitems = nullptr;
   h->pitems.clear();
+  h->pitem_ticks.clear();
   h->pitems_measured = false;
   h->d_work = nullptr;
   h->d_item_ticks = nullptr;
@@ -1521,6 +1525,7 @@ int build_items(mdwt_renderer* h) {
   HIPCHK(h, hipMemcpy(h->d_pitems, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(h->d_item_ticks, 0, 2 * items.size() * sizeof(unsigned long long)));
   h->pitems = items;
+  h->pitem_ticks.clear();
   h->pitems_measured = false;
   h->items = std::move(items);
   return build_view_orders(h);
@@ -1630,8 +1635,10 @@ mdwt::RenderParams views_params(const mdwt_renderer* h) {
 
 // Per-view time of the last one-launch frame from its workgroups' wall-clock
 // stamps (100 MHz; stamp i belongs to list[i]): from the first workgroup's
-// start to the view's last workgroup's end.
-int views_times_from_ticks(mdwt_renderer* h, const std::vector<uint32_t>& list) {
+// start to the view's last workgroup's end. The stamps read stay in *keep
+// when given.
+int views_times_from_ticks(mdwt_renderer* h, const std::vector<uint32_t>& list,
+                           std::vector<unsigned long long>* keep = nullptr) {
   const size_t n = list.size();
   std::vector<unsigned long long> t(2 * n);
   HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
@@ -1646,6 +1653,7 @@ int views_times_from_ticks(mdwt_renderer* h, const std::vector<uint32_t>& list) 
   h->view_us.assign(h->views.size(), 0.f);
   for (size_t v = 0; v < end.size(); ++v)
     h->view_us[v] = end[v] >= t0 ? float(double(end[v] - t0) / 100.0) : 0.f;
+  if (keep) keep->swap(t);
   return MDWT_OK;
 }
 
@@ -1687,29 +1695,15 @@ mdwt::RenderParams persistent_params(const mdwt_renderer* h) {
 }
 
 // Re-deal form 2's items by the costs the last form-2 frame measured (its
-// per-item wall-clock stamps): items by decreasing cost, dealt to the G
-// resident workgroups in alternating directions per round (g, then G - 1 - g),
-// so that each workgroup's sum of costs is about the same (longest
-// processing time first, with a static assignment). The list changes in
-// place: the captured graph reads it at each replay.
+// per-item wall-clock stamps, kept on the host): longest processing time
+// first over the G resident workgroups (mdwt::deal_sources). The list
+// changes in place: the captured graph reads it at each replay.
 int deal_by_cost(mdwt_renderer* h, int G) {
   const size_t n = h->pitems.size();
-  if (n == 0 || G <= 0) return MDWT_OK;
-  std::vector<unsigned long long> t(2 * n);
-  HIPCHK(h, hipMemcpy(t.data(), h->d_item_ticks, t.size() * sizeof(unsigned long long),
-                      hipMemcpyDeviceToHost));
-  std::vector<size_t> pos(n);
-  for (size_t i = 0; i < n; ++i) pos[i] = i;
-  auto cost = [&](size_t i) { return t[2 * i + 1] >= t[2 * i] ? t[2 * i + 1] - t[2 * i] : 0ull; };
-  std::stable_sort(pos.begin(), pos.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+  if (n == 0 || G <= 0 || h->pitem_ticks.size() != 2 * n) return MDWT_OK;
+  const std::vector<uint32_t> src = mdwt::deal_sources(h->pitem_ticks.data(), n, size_t(G));
   std::vector<uint32_t> dealt(n);
-  const size_t g_all = size_t(G);
-  for (size_t idx = 0; idx < n; ++idx) {
-    const size_t r = idx / g_all, g = idx % g_all;
-    const size_t cnt = std::min(g_all, n - r * g_all);
-    const size_t src = r * g_all + ((r & 1) ? cnt - 1 - g : g);
-    dealt[idx] = h->pitems[pos[src]];
-  }
+  for (size_t idx = 0; idx < n; ++idx) dealt[idx] = h->pitems[src[idx]];
   h->pitems = std::move(dealt);
   HIPCHK(h, hipMemcpyAsync(h->d_pitems, h->pitems.data(), n * sizeof(uint32_t),
                            hipMemcpyHostToDevice, h->stream));
@@ -1775,7 +1769,7 @@ int batch_render_persistent(mdwt_renderer* h, int n_frames, float* ms_out) {
   h->batch_frame += uint32_t(n_frames);
   h->batch_accum = h->accumulate ? h->batch_accum + uint32_t(n_frames) : 0u;
   if (n_frames > 0) {
-    rc = views_times_from_ticks(h, h->pitems);
+    rc = views_times_from_ticks(h, h->pitems, &h->pitem_ticks);
     if (rc != MDWT_OK) return rc;
     h->pitems_measured = true;
   }

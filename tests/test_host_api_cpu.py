@@ -1,6 +1,7 @@
 """Host-side API checks that need no GPU: the RAW loader's file-name parsing and
 value conversions (rawFile.cpp:31-128), the C++ header compiling on its own,
-and the offline driver's argument handling (viewer.cpp:70-161, 204-224)."""
+the offline driver's argument handling (viewer.cpp:70-161, 204-224), and the
+host-side deal of the persistent batched launch's work items."""
 import subprocess
 from pathlib import Path
 
@@ -35,6 +36,64 @@ def test_cpp_header_compiles_standalone(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
                         f"-I{ROOT / 'include'}", str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+DEAL_CHECK = r"""
+#include <algorithm>
+#include <cstdio>
+#include <random>
+#include "batch_deal.hpp"
+int main() {
+  std::mt19937_64 g(7);
+  for (int trial = 0; trial < 200; ++trial) {
+    const size_t n = trial < 10 ? size_t(trial) : 1 + g() % 3000;
+    const size_t G = 1 + g() % 300;
+    std::vector<unsigned long long> t(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+      const unsigned long long a = g() % 1000000;
+      const int kind = trial % 4;
+      const unsigned long long c = kind == 0 ? g() % 16 : kind == 1 ? g() % 300000
+                                 : kind == 2 ? g() : (g() % 3 == 0 ? 0 : g() % 5000);
+      t[2 * i] = a;
+      t[2 * i + 1] = a + c;
+      if (kind == 3 && g() % 7 == 0) std::swap(t[2 * i], t[2 * i + 1]);
+    }
+    std::vector<size_t> pos(n);
+    for (size_t i = 0; i < n; ++i) pos[i] = i;
+    std::stable_sort(pos.begin(), pos.end(), [&](size_t x, size_t y) {
+      return mdwt::item_cost(t.data(), x) > mdwt::item_cost(t.data(), y);
+    });
+    const std::vector<uint32_t> order = mdwt::order_by_decreasing_cost(t.data(), n);
+    const std::vector<uint32_t> src = mdwt::deal_sources(t.data(), n, G);
+    std::vector<int> seen(n, 0);
+    for (size_t idx = 0; idx < n; ++idx) {
+      if (order[idx] != pos[idx]) { printf("order differs: trial %d at %zu\n", trial, idx); return 1; }
+      const size_t r = idx / G, w = idx % G, cnt = std::min(G, n - r * G);
+      if (src[idx] != pos[r * G + ((r & 1) ? cnt - 1 - w : w)]) { printf("deal differs\n"); return 1; }
+      ++seen[src[idx]];
+    }
+    for (size_t i = 0; i < n; ++i)
+      if (seen[i] != 1) { printf("not a permutation\n"); return 1; }
+  }
+  printf("ok\n");
+  return 0;
+}
+"""
+
+
+def test_batch_deal_orders_like_a_stable_sort(tmp_path):
+    """Launch form 2's re-deal: the radix order by decreasing cost equals a
+    stable comparison sort's (ties, zero and out-of-order stamps, 64-bit
+    costs), and the dealt list is a permutation in alternating rounds of G."""
+    src = tmp_path / "deal.cpp"
+    src.write_text(DEAL_CHECK)
+    exe = tmp_path / "deal"
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Werror",
+                        f"-I{ROOT / 'mdwtracker_amd' / 'csrc'}", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
 
 
 def test_offline_cli_argument_errors():

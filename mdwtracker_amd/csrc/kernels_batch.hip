@@ -11,9 +11,8 @@
 // barrier inside the item loop, so every workgroup's loop ends after a fixed
 // number of items.
 // Under the wave-coherent schedule, -m 0 to -m 5 views without lighting run
-// kernels_wave.hip's
-// render_views_wave and render_views_wave_persistent instead, with the same
-// items and ids.
+// kernels_wave.hip's render_views_wave and render_views_wave_persistent
+// instead, with the same items and ids.
 // No reference counterpart: the reference renders one camera per owlLaunch2D
 // (renderer.cpp:949).
 #include "mdwt_internal.hpp"

@@ -9,9 +9,9 @@ namespace mdwt {
 
 // host stub of a frame kernel
 using RenderFn = void (*)(RenderParams);
-// wave-coherent frame kernels (kernels_wave.hip, schedule 1): -m 0 on at most
-// four channels, -m 2 on any number, no lighting, not the counting pass; form 0
-// one view, 1 batched views in one launch, 2 batched views in a persistent launch
+// wave-coherent frame kernels (kernels_wave.hip, schedule 1): -m 0 to -m 5 on
+// at most twelve channels, no lighting, not the counting pass; form 0 one view,
+// 1 batched views in one launch, 2 batched views in a persistent launch
 bool wave_schedule_applies(const RenderParams& p, int mode, bool count, bool lit);
 RenderFn select_wave_kernel(const RenderParams& p, int mode, int dtype, int form);
 // dynamic LDS of a wave-coherent launch (transfer-function opacities)

@@ -81,15 +81,15 @@ struct FrameParams {
   unsigned long long* counters;  // samples, visits, rejections, rays
   const uint32_t* block_order;   // (bx | by << 16) per 16x16 pixel block, Hilbert order
   int n_blocks;
-  int schedule;  // 0: one thread per pixel, 1: wave-coherent (-m 0 / -m 2, <= 4 channels)
+  int schedule;  // 0: one thread per pixel, 1: wave-coherent (-m 0 to -m 5, no lighting)
   // cost-sorted order: wall-clock ticks (100 MHz) of each wave, indexed by
   // waves_per_block * (position in block_order) + wave; nullptr = not recorded
   uint32_t* wave_cost;
   int bw, bh;  // pixel block of a workgroup: 8x8, 16x8 or 16x16 (bw*bh/64 waves of 8x8 pixels)
-  // batched views (views == nullptr: the single view described above). The
-  // workgroups of one launch pull items (view << 20 | by << 10 | bx) from
-  // work[0]; work[1] / work[2] hold the frame and accumulation ids, advanced
-  // on the device so that a captured graph can be replayed frame after frame.
+  // batched views (views == nullptr: the single view described above). An
+  // item is one pixel block of one view (view << 20 | by << 10 | bx); launch
+  // form 2 reads the frame and accumulation ids from work[1] / work[2], which
+  // the device advances so that a captured graph replays frame after frame.
   const ViewParams* views;
   const uint32_t* items;
   int n_items;

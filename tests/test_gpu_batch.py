@@ -138,24 +138,28 @@ def test_block_shapes_and_tiles_do_not_change_images(capi, scene, threads, tile)
 
 @pytest.mark.parametrize("threads", [64, 128])
 def test_one_launch_batch_block_shapes(capi, scene, threads):
-    """The one-launch batch with 8x8 and 16x8 blocks (items rebuilt for the
-    block shape) equals the default 16x16 per-view launches."""
+    """The one-launch and persistent batches with 8x8 and 16x8 blocks (items
+    rebuilt for the block shape; the persistent grid's items re-dealt by cost
+    after each call) equal the default 16x16 per-view launches."""
     views = _views(5, seed=21)
-    print(f"[batch] one launch, {threads} threads", flush=True)
+    print(f"[batch] one launch and persistent, {threads} threads", flush=True)
     out = []
-    for launch, th in ((0, 256), (1, threads)):
+    for launch, th in ((0, 256), (1, threads), (2, threads)):
         g = capi.MdwtRenderer(0)
         g.set_block(th)
         scenes.load_scene(g, scene, 2)
         g.batch_set_launch(launch)
         g.batch_set_views(views)
-        g.batch_render(2)
+        g.batch_render(1)
+        g.batch_render(1)
+        g.batch_render(1)
         out.append([g.batch_read_view(i) for i in range(len(views))])
         t = g.batch_view_times()
         assert t.shape == (len(views),) and np.all(t > 0)
         g.close()
-    for a, b in zip(*out):
+    for a, b, c in zip(*out):
         np.testing.assert_array_equal(_bits(a), _bits(b))
+        np.testing.assert_array_equal(_bits(a), _bits(c))
 
 
 def test_set_macrocells_matches_fresh_init(capi, scene):

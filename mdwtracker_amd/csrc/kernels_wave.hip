@@ -463,20 +463,37 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
       // full batches, from the first-step stack.
       uint32_t n_items = 0u;
       const int groups = WIDE ? (C + 3) >> 2 : 1;  // channels in groups of 4: one Philox block each
+      // -m 3 / 4 / 5 (one summed majorant per cell): each batch's record and
+      // majorant are read one batch ahead, so their latency overlaps the batch
+      // before (records do not change during the round). 3 % faster there; the
+      // same for the per-channel majorants of -m 0 / 1 / 2 measured 1 % slower.
+      constexpr bool kAhead = !WIDE && MODE >= 3;
+      Record r_nx = {0u, 0.f, 0.f, 0u};
+      float m_nx = 0.f;
+      auto fetch_next = [&](const uint32_t i) {
+        if (kAhead && i < total) {
+          r_nx = L.rec[i];
+          m_nx = X.maj[r_nx.cid];
+        }
+      };
+      fetch_next(uint32_t(lane));
       for (uint32_t b = 0; b < total; b += 64u) {
         const uint32_t i = b + uint32_t(lane);
         const bool act = i < total;
+        const Record r_cur = r_nx;
+        const float m_cur = m_nx;
+        fetch_next(i + 64u);
         // the first steps of channels 4g .. 4g + 3 of record i
         auto enter_group = [&](const int g) {
           float4 uq = make_float4(0.f, 0.f, 0.f, 0.f), m4 = uq;
           uint32_t need = 0u;
           if (act) {
-            const Record R = L.rec[i];
+            const Record R = kAhead ? r_cur : L.rec[i];
             const WaveRay& Y = L.ray[R.vl >> kVisitBits];
             const float q = ((R.t1 - R.t0) + R.t1 * 0x1p-23f) * Y.idw * kExitSafety;
             if (MODE >= 3) {
               // one summed majorant per cell, tracked at channel 0's positions
-              const float m = X.maj[R.cid];
+              const float m = kAhead ? m_cur : X.maj[R.cid];
               if (m != 0.0f) {
                 uint32_t w[4];
                 philox(D.frame, D.sample, 1u + (((R.vl & kVisitMask) << 4) | 12u), 0u, D.seed,

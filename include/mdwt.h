@@ -278,6 +278,13 @@ int mdwt_autotune(mdwt_renderer* h, const int* mc_sizes, int n_mc, const int* th
  * Returns MDWT_ERR_ARG for other values. No reference counterpart (OptiX
  * schedules the reference's launch). */
 int mdwt_set_schedule(mdwt_renderer* h, int schedule);
+/* Pixel jitter: 0 (the default) jitters x and y by one uniform, as the
+ * reference's raygen does (deviceCode.cu:297-298: one random() added to both
+ * coordinates); 1 draws y's jitter from the next word of the same Philox
+ * block, a box filter over the pixel. The estimator then converges to the
+ * pixel-area average instead of the reference's diagonal average. No
+ * reference counterpart (an option beside the reference's behaviour). */
+int mdwt_set_jitter_2d(mdwt_renderer* h, int on);
 /* Device copy of `bytes` bytes, `iters` times (one read and one write per
  * byte); GB/s of (read + write) bytes. No reference counterpart. */
 int mdwt_stream_copy(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps);

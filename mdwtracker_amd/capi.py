@@ -67,6 +67,7 @@ SIGNATURES = {
     "mdwt_last_kernel_ms": [_h, _fp],
     "mdwt_set_tile_order": [_h, C.c_int, C.c_int],
     "mdwt_set_schedule": [_h, C.c_int],
+    "mdwt_set_jitter_2d": [_h, C.c_int],
     "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_stream_read": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_set_moments": [_h, C.c_int],
@@ -344,6 +345,11 @@ class MdwtRenderer:
         (the default; -m 0 to -m 5 without lighting; the ray marchers and
         lighting run schedule 0). Images do not depend on it."""
         self._chk(lib.mdwt_set_schedule(self._h, schedule))
+
+    def set_jitter_2d(self, on: bool = True) -> None:
+        """y jitter from its own random word (a box filter over the pixel)
+        instead of the reference's one scalar for both axes (the default)."""
+        self._chk(lib.mdwt_set_jitter_2d(self._h, int(bool(on))))
 
     # -- frames -------------------------------------------------------------
     def render(self, n_frames: int = 1) -> float:

@@ -87,15 +87,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void r
 
 // ---------------------------------------------------------- test hooks -----
 __global__ void gen_rays_kernel(CameraParams cam, int W, int H, uint32_t seed, uint32_t frame,
-                                uint32_t sample, float* org, float* dir) {
+                                uint32_t sample, int jitter2d, float* org, float* dir) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W * H) return;
   const int px = i % W, py = i / W;
   Rng rng;
   rng.init(seed, uint32_t(i), frame, sample, 0u);
-  const float jit = rng.jitter();
+  float jx, jy;
+  rng.jitter2(jitter2d != 0, jx, jy);
   V3 o, d;
-  camera_ray(cam, px, py, W, H, jit, o, d);
+  camera_ray(cam, px, py, W, H, jx, jy, o, d);
   org[3 * i] = o.x;
   org[3 * i + 1] = o.y;
   org[3 * i + 2] = o.z;
@@ -298,10 +299,10 @@ hipError_t launch_render(const RenderParams& p, int mode, int dtype, bool count,
 }
 
 hipError_t launch_gen_rays(const CameraParams& cam, int W, int H, uint32_t seed, uint32_t frame,
-                           uint32_t sample, float* org, float* dir, hipStream_t s) {
+                           uint32_t sample, int jitter2d, float* org, float* dir, hipStream_t s) {
   const int n = W * H;
   hipLaunchKernelGGL(gen_rays_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, cam, W, H,
-                     seed, frame, sample, org, dir);
+                     seed, frame, sample, jitter2d, org, dir);
   return hipGetLastError();
 }
 

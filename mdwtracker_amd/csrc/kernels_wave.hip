@@ -388,9 +388,10 @@ __device__ __forceinline__ void wave_block(const RenderParams& P, const Ctx<DT>&
     {
       Rng rng;
       rng.init(V.seed, pixel, V.frame_id, uint32_t(s), 0u);
-      const float jit = rng.jitter();
+      float jx, jy;
+      rng.jitter2(P.fr.jitter2d != 0, jx, jy);
       V3 org, dir;
-      camera_ray(*V.cam, px, py, V.W, V.H, jit, org, dir);
+      camera_ray(*V.cam, px, py, V.W, V.H, jx, jy, org, dir);
       float c0, c1;
       WaveRay Y;
       Y.org_w2g = make_float4(org.x, org.y, org.z, 0.f);

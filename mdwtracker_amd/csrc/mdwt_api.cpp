@@ -133,6 +133,7 @@ struct mdwt_renderer {
 
   // frame-kernel schedule (0: thread per pixel, 1: wave-coherent task queues)
   int schedule = kDefaultSchedule;
+  bool jitter2d = false;  // y jitter from its own word (mdwt_set_jitter_2d)
   int cu_count = 0;
 
   // workgroup pixel block (64: 8x8, 128: 16x8, 256: 16x16 threads/pixels) and
@@ -584,6 +585,7 @@ mdwt::RenderParams render_params(const mdwt_renderer* h) {
   p.fr.light_intensity = h->light;
   p.fr.ambient = h->ambient;
   p.fr.seed = h->seed;
+  p.fr.jitter2d = h->jitter2d ? 1 : 0;
   p.fr.frame_id = h->frame_id;
   p.fr.accum_id = h->accum_id;
   p.fr.accum = h->d_accum;
@@ -1245,7 +1247,8 @@ int mdwt_gen_rays(mdwt_renderer* h, uint32_t frame, uint32_t sample, float* org,
   hipError_t e = hipMalloc(&d_org, n * 12);
   if (e == hipSuccess) e = hipMalloc(&d_dir, n * 12);
   if (e == hipSuccess)
-    e = mdwt::launch_gen_rays(h->cam, h->W, h->H, h->seed, frame, sample, d_org, d_dir, h->stream);
+    e = mdwt::launch_gen_rays(h->cam, h->W, h->H, h->seed, frame, sample, h->jitter2d ? 1 : 0,
+                              d_org, d_dir, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess) e = hipMemcpy(org, d_org, n * 12, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(dir, d_dir, n * 12, hipMemcpyDeviceToHost);
@@ -1318,6 +1321,12 @@ int mdwt_set_schedule(mdwt_renderer* h, int schedule) {
   if (schedule < 0 || schedule > 1)
     return fail(h, MDWT_ERR_ARG, "schedule must be 0 (thread per pixel) or 1 (wave-coherent)");
   h->schedule = schedule;
+  return MDWT_OK;
+}
+
+int mdwt_set_jitter_2d(mdwt_renderer* h, int on) {
+  if (!h) return MDWT_ERR_ARG;
+  h->jitter2d = on != 0;
   return MDWT_OK;
 }
 

@@ -66,7 +66,8 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
 __device__ __forceinline__ float to_unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }
 
 // Word layout of a (pixel, frame, sample, stream) stream: the jitter is word 0
-// of block (frame, sample, 0, stream).
+// of block (frame, sample, 0, stream), and with two-axis jitter the y jitter
+// is word 1 of it.
 // -m 0 / -m 1 / -m 2 track every (macrocell visit v, channel c) from the
 // cell's entry (v counts the cells of the ray's DDA walk; -m 1 runs the
 // channels' clocks on these positions); -m 3 tracks its summed majorant per
@@ -102,10 +103,13 @@ struct Rng {
     cur = 0;
     base = 0;
   }
-  __device__ __forceinline__ float jitter() const {
+  // the pixel jitter of x and y: word 0 of the jitter block for both axes
+  // (deviceCode.cu:298), or words 0 and 1 with two_d
+  __device__ __forceinline__ void jitter2(bool two_d, float& jx, float& jy) const {
     uint32_t o[4];
     philox(frame, sample, 0u, stream, k0, k1, o);
-    return to_unit(o[0]);
+    jx = to_unit(o[0]);
+    jy = two_d ? to_unit(o[1]) : jx;
   }
   __device__ __forceinline__ float step() {
     cur = k;
@@ -419,9 +423,9 @@ __device__ __forceinline__ bool clip_root(V3 org, V3 dir, V3 lb, V3 rt, float& t
 
 // Camera ray (deviceCode.cu:293-300 jitter, generateRay 61-76).
 __device__ __forceinline__ void camera_ray(const CameraParams& cam, int px, int py, int W, int H,
-                                           float jit, V3& org, V3& dir) {
-  const float su = (float(px) + jit) / float(W);
-  const float sv = (float(py) + jit) / float(H);
+                                           float jx, float jy, V3& org, V3& dir) {
+  const float su = (float(px) + jx) / float(W);
+  const float sv = (float(py) + jy) / float(H);
   org = load3(cam.org);
   V3 d = load3(cam.llc) + su * load3(cam.hor) + sv * load3(cam.ver);
   if (fabsf(d.x) < 1e-5f) d.x = 1e-5f;

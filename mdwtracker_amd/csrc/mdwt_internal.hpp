@@ -36,7 +36,7 @@ hipError_t launch_views(const RenderParams& p, int mode, int dtype, bool count, 
 // dynamic LDS of a frame launch (transfer-function texels)
 size_t render_lds_bytes(const RenderParams& p);
 hipError_t launch_gen_rays(const CameraParams& cam, int W, int H, uint32_t seed, uint32_t frame,
-                           uint32_t sample, float* org, float* dir, hipStream_t s);
+                           uint32_t sample, int jitter2d, float* org, float* dir, hipStream_t s);
 hipError_t launch_clip(const VolumeParams& vol, const float* org, const float* dir, int n,
                        float* t01, int* hit, hipStream_t s);
 hipError_t launch_trace_cells(const VolumeParams& vol, const int mc[3], const float* org,

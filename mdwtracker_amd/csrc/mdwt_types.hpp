@@ -76,6 +76,9 @@ struct FrameParams {
   float light_intensity, ambient;
   ShadeParams sh;
   uint32_t seed, frame_id, accum_id;
+  // 1: the y jitter is word 1 of the jitter block; 0: both axes take word 0,
+  // the reference's one scalar (deviceCode.cu:298)
+  int jitter2d;
   void* accum;  // float4 per pixel
   void* m2;     // float4 per pixel
   unsigned long long* counters;  // samples, visits, rejections, rays

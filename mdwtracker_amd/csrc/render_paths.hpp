@@ -655,9 +655,10 @@ __device__ __forceinline__ void render_block(const RenderParams& P, const Ctx<DT
   for (int s = 0; s < V.spp; ++s) {
     Rng rng;
     rng.init(V.seed, pixel, V.frame_id, uint32_t(s), 0u);
-    const float jit = rng.jitter();
+    float jx, jy;
+    rng.jitter2(P.fr.jitter2d != 0, jx, jy);
     V3 org, dir;
-    camera_ray(*V.cam, px, py, V.W, V.H, jit, org, dir);
+    camera_ray(*V.cam, px, py, V.W, V.H, jx, jy, org, dir);
     HitRec h;
     h.missed = true;
     h.a = 1.f;

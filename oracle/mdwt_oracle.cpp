@@ -87,7 +87,8 @@ inline void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 
 // One stream of uniforms in [0,1) per (pixel, frame, sample, stream): Philox
 // blocks under key (seed, pixel), words converted as (w >> 8) * 2^-24. Fixed
-// word layout: the pixel jitter is word 0 of block (frame, sample, 0, stream).
+// word layout: the pixel jitter is word 0 of block (frame, sample, 0, stream)
+// (and with two-axis jitter, oracle_set_jitter_2d, the y jitter word 1).
 //
 // -m 0 and -m 2 track each (macrocell visit v, channel c) from the cell's
 // entry (v counts the cells of the ray's DDA walk, the same cells for every
@@ -143,7 +144,13 @@ struct Rng {
     return buf;
   }
   static float unit(uint32_t w) { return float(w >> 8) * 0x1p-24f; }  // exact
-  float jitter() { return unit(block(0u, stream)[0]); }
+  // deviceCode.cu:297-298 adds one uniform to both pixel coordinates; with
+  // two_d the y jitter is the block's next word
+  void jitter2(bool two_d, float& jx, float& jy) {
+    const uint32_t* w = block(0u, stream);
+    jx = unit(w[0]);
+    jy = two_d ? unit(w[1]) : jx;
+  }
   void seek(uint32_t b) {
     if (seq) return;
     base = b;
@@ -398,6 +405,7 @@ struct oracle_renderer {
   // reference-faithful tracking (oracle_set_faithful): libm log, the two
   // divisions of the TF remap, one sequential RNG stream per ray
   bool faithful = false;
+  bool jitter2d = false;  // oracle_set_jitter_2d
 };
 
 namespace {
@@ -1273,9 +1281,10 @@ void render_pixel(oracle_renderer* o, int px, int py, uint64_t cnt[4]) {
   for (int sampleID = 0; sampleID < o->spp; ++sampleID) {
     Rng rng(o->seed, pixel, o->frame_id, uint32_t(sampleID), 0u);
     rng.seq = o->faithful;
-    const float jit = rng.jitter();
-    const float su = (float(px) + jit) / float(o->W);
-    const float sv = (float(py) + jit) / float(o->H);
+    float jx, jy;
+    rng.jitter2(o->jitter2d, jx, jy);
+    const float su = (float(px) + jx) / float(o->W);
+    const float sv = (float(py) + jy) / float(o->H);
     vec3f org, dir;
     generate_ray(o, su, sv, org, dir);
     vec4f color = {0.f, 0.f, 0.f, 0.f};
@@ -1540,6 +1549,11 @@ int oracle_set_faithful(oracle_renderer* o, int on) {
   return 0;
 }
 
+int oracle_set_jitter_2d(oracle_renderer* o, int on) {
+  o->jitter2d = on != 0;
+  return 0;
+}
+
 int oracle_set_threads(oracle_renderer* o, int n) {
   o->build_threads = n < 1 ? 1 : n;
   return 0;
@@ -1699,9 +1713,10 @@ int oracle_gen_rays(oracle_renderer* o, uint32_t frame, uint32_t sample, float* 
     for (int px = 0; px < o->W; ++px) {
       const size_t i = size_t(px) + size_t(o->W) * py;
       Rng rng(o->seed, uint32_t(i), frame, sample, 0u);
-      const float jit = rng.jitter();
-      const float su = (float(px) + jit) / float(o->W);
-      const float sv = (float(py) + jit) / float(o->H);
+      float jx, jy;
+      rng.jitter2(o->jitter2d, jx, jy);
+      const float su = (float(px) + jx) / float(o->W);
+      const float sv = (float(py) + jy) / float(o->H);
       vec3f ro, rd;
       generate_ray(o, su, sv, ro, rd);
       org[3 * i] = ro.x; org[3 * i + 1] = ro.y; org[3 * i + 2] = ro.z;

@@ -63,6 +63,10 @@ int oracle_set_material(oracle_renderer* o, int ch, float diffuse, float specula
  * checks the default path against the reference's semantics (the default
  * path's images are what the GPU must match bit for bit). */
 int oracle_set_faithful(oracle_renderer* o, int on);
+/* Pixel jitter of y from word 1 of the jitter block instead of the one
+ * scalar both axes take by default (deviceCode.cu:297-298); mirrors
+ * mdwt_set_jitter_2d. */
+int oracle_set_jitter_2d(oracle_renderer* o, int on);
 /* OpenMP threads for the range-grid build (result does not depend on it) */
 int oracle_set_threads(oracle_renderer* o, int n);
 /* Render n_frames frames; rows [y0, y1) only (y1 <= 0 means all rows).

@@ -41,6 +41,7 @@ _SIG = {
     "oracle_set_material": ([_h, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _fp], C.c_int),
     "oracle_set_threads": ([_h, C.c_int], C.c_int),
     "oracle_set_faithful": ([_h, C.c_int], C.c_int),
+    "oracle_set_jitter_2d": ([_h, C.c_int], C.c_int),
     "oracle_render": ([_h, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "oracle_reset_accumulation": ([_h], C.c_int),
     "oracle_read_accum": ([_h, _fp, _fp], C.c_int),
@@ -189,6 +190,10 @@ class OracleRenderer:
         """Reference-faithful tracking: libm log, the TF remap's divisions,
         one sequential RNG stream per ray (see oracle_set_faithful)."""
         self._chk(lib.oracle_set_faithful(self._h, int(bool(on))))
+
+    def set_jitter_2d(self, on=True):
+        """y jitter from word 1 of the jitter block (see oracle_set_jitter_2d)."""
+        self._chk(lib.oracle_set_jitter_2d(self._h, int(bool(on))))
 
     def set_lighting(self, light_intensity, ambient):
         self._chk(lib.oracle_set_lighting(self._h, light_intensity, ambient))

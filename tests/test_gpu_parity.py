@@ -258,6 +258,34 @@ def test_camera_rays_and_clip_bit_exact(capi, po):
         assert 0 < gh.sum() < gh.size
 
 
+@pytest.mark.parametrize("mode", [2, 0, 3])
+def test_two_axis_jitter_bit_exact(capi, po, mode):
+    """mdwt_set_jitter_2d: camera rays and accumulated images equal the
+    oracle's with the same option, and differ from the one-scalar default."""
+    sc = _scene("ndda_cpu_2ch64")
+    print(f"[jitter2d] ndda_cpu_2ch64 mode={mode}", flush=True)
+    g1 = capi.MdwtRenderer(0)
+    scenes.load_scene(g1, sc, mode)
+    g1.render(4)
+    a_one = g1.read_accum()
+    g1.close()
+    g, o = _pair(capi, po, sc, mode)
+    g.set_jitter_2d(True)
+    o.set_jitter_2d(True)
+    go, gd = g.gen_rays(5, 2)
+    oo, od = o.gen_rays(5, 2)
+    np.testing.assert_array_equal(_bits(go), _bits(oo))
+    np.testing.assert_array_equal(_bits(gd), _bits(od))
+    g.render(4)
+    o.render(4, threads=THREADS)
+    a_g, m_g = g.read_accum(with_m2=True)
+    a_o, m_o = o.read_accum(with_m2=True)
+    np.testing.assert_array_equal(_bits(a_g), _bits(a_o))
+    np.testing.assert_array_equal(_bits(m_g), _bits(m_o))
+    assert not np.array_equal(_bits(a_g), _bits(a_one))
+    g.close()
+
+
 def test_dda_cells_bit_exact_64k_rays(capi, po):
     sc = _scene("sync_4ch256", 4)  # 64^3 volume, mc 4^3 -> use a finer grid below
     rng = np.random.default_rng(23)

@@ -170,3 +170,28 @@ def test_camera_rays_follow_philox_jitter():
         np.testing.assert_allclose(d[i], e, atol=2e-6)
         np.testing.assert_allclose(org[i], cam[0], atol=0)
     assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+
+
+def test_camera_rays_two_axis_jitter():
+    """With two-axis jitter, x keeps word 0 of the jitter block and y takes
+    word 1; switching it off again restores the one-scalar rays."""
+    r = _grid_renderer((2, 2, 2))
+    r.set_frame(16, 8)
+    r.set_seed(99)
+    org1, d1 = r.gen_rays(frame=3, sample=1)
+    r.set_jitter_2d(True)
+    org, d = r.gen_rays(frame=3, sample=1)
+    cam = r.get_camera().astype(np.float64)
+    for i in [0, 5, 17, 64, 127]:
+        px, py = i % 16, i // 16
+        w = po.philox4x32([3, 1, 0, 0], [99, i])
+        jx = (int(w[0]) >> 8) * 2.0 ** -24
+        jy = (int(w[1]) >> 8) * 2.0 ** -24
+        p = cam[1] + (px + jx) / 16 * cam[2] + (py + jy) / 8 * cam[3]
+        e = (p - cam[0]) / np.linalg.norm(p - cam[0])
+        np.testing.assert_allclose(d[i], e, atol=2e-6)
+        np.testing.assert_allclose(org[i], cam[0], atol=0)
+    assert (np.abs(d - d1).max(axis=1) > 0).mean() > 0.9
+    r.set_jitter_2d(False)
+    _, d0 = r.gen_rays(frame=3, sample=1)
+    np.testing.assert_array_equal(d0, d1)

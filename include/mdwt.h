@@ -285,6 +285,14 @@ int mdwt_set_schedule(mdwt_renderer* h, int schedule);
  * pixel-area average instead of the reference's diagonal average. No
  * reference counterpart (an option beside the reference's behaviour). */
 int mdwt_set_jitter_2d(mdwt_renderer* h, int on);
+/* Majorants: 0 (the default) as the reference computes them (renderer.cu:
+ * 159-203), which can fall below the opacities a TF lookup interpolates by up
+ * to half a texel of lerp; 1 the largest opacity of every texel a lookup of
+ * the cell's value range reads (the range widened by 2^-12 (1 + |r|), the
+ * maximum by 1 + 2^-20): an upper bound, so Woodcock tracking is unbiased.
+ * Rebuilds the majorant grid and restarts the accumulation when the setting
+ * changes. No reference counterpart (an option beside its behaviour). */
+int mdwt_set_conservative_majorants(mdwt_renderer* h, int on);
 /* Device copy of `bytes` bytes, `iters` times (one read and one write per
  * byte); GB/s of (read + write) bytes. No reference counterpart. */
 int mdwt_stream_copy(mdwt_renderer* h, uint64_t bytes, int iters, float* gbps);

@@ -68,6 +68,7 @@ SIGNATURES = {
     "mdwt_set_tile_order": [_h, C.c_int, C.c_int],
     "mdwt_set_schedule": [_h, C.c_int],
     "mdwt_set_jitter_2d": [_h, C.c_int],
+    "mdwt_set_conservative_majorants": [_h, C.c_int],
     "mdwt_stream_copy": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_stream_read": [_h, C.c_uint64, C.c_int, _fp],
     "mdwt_set_moments": [_h, C.c_int],
@@ -350,6 +351,12 @@ class MdwtRenderer:
         """y jitter from its own random word (a box filter over the pixel)
         instead of the reference's one scalar for both axes (the default)."""
         self._chk(lib.mdwt_set_jitter_2d(self._h, int(bool(on))))
+
+    def set_conservative_majorants(self, on: bool = True) -> None:
+        """Majorants that bound every opacity a TF lookup of the cell's value
+        range interpolates (the default keeps the reference's, which can be
+        half a texel low); rebuilds them and restarts the accumulation."""
+        self._chk(lib.mdwt_set_conservative_majorants(self._h, int(bool(on))))
 
     # -- frames -------------------------------------------------------------
     def render(self, n_frames: int = 1) -> float:

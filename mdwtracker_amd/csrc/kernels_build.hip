@@ -313,6 +313,27 @@ __device__ __forceinline__ float channel_majorant(const float4* tab, int n, floa
   return m;
 }
 
+// An option beside renderer.cu:159-203, whose majorant can fall below the
+// opacities a lookup interpolates by up to half a texel of lerp: the largest
+// opacity of every texel that a lookup of a value in [mn, mx] reads. The
+// lookup interval is widened by 2^-12 (1 + |r|) on each side, which covers the
+// render's remap by reciprocals against the divisions here, and the maximum
+// is scaled by 1 + 2^-20 for the lerp's rounding. A lookup at u reads
+// texels floor(u n - 0.5) and the next one, clamped to the table.
+__device__ __forceinline__ float channel_majorant_conservative(const float4* tab, int n, float os,
+                                                               float xlo, float xhi, float vlo,
+                                                               float vhi, float mn, float mx) {
+  const float r1 = ((mn - vlo) / (vhi - vlo) - xlo) / (xhi - xlo);
+  const float r2 = ((mx - vlo) / (vhi - vlo) - xlo) / (xhi - xlo);
+  const float a = fminf(r1, r2), b = fmaxf(r1, r2);
+  const float lo = a - 0x1p-12f * (1.0f + fabsf(a)), hi = b + 0x1p-12f * (1.0f + fabsf(b));
+  const int i0 = clamp_index(floorf(lo * float(n) - 0.5f), n);
+  const int i1 = clamp_index_plus1(floorf(hi * float(n) - 0.5f), n);
+  float m = 0.f;
+  for (int i = i0; i <= i1; ++i) m = fmaxf(m, tab[i].w);
+  return m * os * (1.0f + 0x1p-20f);
+}
+
 // renderer.cu:206-274: modes 0/2 keep one majorant per (cell, channel)
 // (renderer.cu:159-203), mode 3 their sum per cell (renderer.cu:111-157).
 __global__ __launch_bounds__(256) void majorant_kernel(MajorantParams P) {
@@ -326,8 +347,13 @@ __global__ __launch_bounds__(256) void majorant_kernel(MajorantParams P) {
     const float2 r = rg[i];
     float m = 0.f;
     if (!(r.y < r.x))
-      m = channel_majorant(tab + size_t(c) * P.tf.stride, P.tf.n[c], P.tf.opacity[c], P.tf.xf_lo[c],
-                           P.tf.xf_hi[c], P.tf.dom_lo[c], P.tf.dom_hi[c], r.x, r.y);
+      m = P.conservative
+              ? channel_majorant_conservative(tab + size_t(c) * P.tf.stride, P.tf.n[c],
+                                              P.tf.opacity[c], P.tf.xf_lo[c], P.tf.xf_hi[c],
+                                              P.tf.dom_lo[c], P.tf.dom_hi[c], r.x, r.y)
+              : channel_majorant(tab + size_t(c) * P.tf.stride, P.tf.n[c], P.tf.opacity[c],
+                                 P.tf.xf_lo[c], P.tf.xf_hi[c], P.tf.dom_lo[c], P.tf.dom_hi[c], r.x,
+                                 r.y);
     P.maj[i] = m;
   } else {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,9 +362,13 @@ __global__ __launch_bounds__(256) void majorant_kernel(MajorantParams P) {
     for (int c = 0; c < C; ++c) {
       const float2 r = rg[size_t(cell) * C + c];
       if (r.y < r.x) continue;
-      sum += channel_majorant(tab + size_t(c) * P.tf.stride, P.tf.n[c], P.tf.opacity[c],
-                              P.tf.xf_lo[c], P.tf.xf_hi[c], P.tf.dom_lo[c], P.tf.dom_hi[c], r.x,
-                              r.y);
+      sum += P.conservative
+                 ? channel_majorant_conservative(tab + size_t(c) * P.tf.stride, P.tf.n[c],
+                                                 P.tf.opacity[c], P.tf.xf_lo[c], P.tf.xf_hi[c],
+                                                 P.tf.dom_lo[c], P.tf.dom_hi[c], r.x, r.y)
+                 : channel_majorant(tab + size_t(c) * P.tf.stride, P.tf.n[c], P.tf.opacity[c],
+                                    P.tf.xf_lo[c], P.tf.xf_hi[c], P.tf.dom_lo[c], P.tf.dom_hi[c],
+                                    r.x, r.y);
     }
     P.maj[cell] = sum;
   }

@@ -134,6 +134,7 @@ struct mdwt_renderer {
   // frame-kernel schedule (0: thread per pixel, 1: wave-coherent task queues)
   int schedule = kDefaultSchedule;
   bool jitter2d = false;  // y jitter from its own word (mdwt_set_jitter_2d)
+  bool conservative_maj = false;  // mdwt_set_conservative_majorants
   int cu_count = 0;
 
   // workgroup pixel block (64: 8x8, 128: 16x8, 256: 16x16 threads/pixels) and
@@ -512,6 +513,7 @@ int rebuild_majorants(mdwt_renderer* h) {
   p.channels = h->C;
   p.n_cells = h->mc[0] * h->mc[1] * h->mc[2];
   p.mode = h->mode;
+  p.conservative = h->conservative_maj ? 1 : 0;
   p.maj = h->d_maj;
   HIPCHK(h, mdwt::launch_majorants(p, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1322,6 +1324,16 @@ int mdwt_set_schedule(mdwt_renderer* h, int schedule) {
     return fail(h, MDWT_ERR_ARG, "schedule must be 0 (thread per pixel) or 1 (wave-coherent)");
   h->schedule = schedule;
   return MDWT_OK;
+}
+
+int mdwt_set_conservative_majorants(mdwt_renderer* h, int on) {
+  if (!h) return MDWT_ERR_ARG;
+  if (h->conservative_maj == (on != 0)) return MDWT_OK;
+  h->conservative_maj = on != 0;
+  if (!h->inited) return MDWT_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  reset_acc(h);
+  return rebuild_majorants(h);
 }
 
 int mdwt_set_jitter_2d(mdwt_renderer* h, int on) {

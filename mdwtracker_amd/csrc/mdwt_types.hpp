@@ -133,6 +133,7 @@ struct MajorantParams {
   int channels;
   int n_cells;
   int mode;
+  int conservative;  // 1: channel_majorant_conservative (an option), 0: the reference's
   float* maj;
 };
 

@@ -406,6 +406,7 @@ struct oracle_renderer {
   // divisions of the TF remap, one sequential RNG stream per ray
   bool faithful = false;
   bool jitter2d = false;  // oracle_set_jitter_2d
+  bool conservative_maj = false;  // oracle_set_conservative_majorants
 };
 
 namespace {
@@ -591,6 +592,26 @@ float channel_majorant(const oracle_renderer* o, int c, float mn, float mx) {
   return m;
 }
 
+// The conservative option (oracle_set_conservative_majorants), no reference
+// counterpart: the largest opacity of every texel a lookup of a value in
+// [mn, mx] reads, with the lookup interval widened by 2^-12 (1 + |r|) on each
+// side and the maximum scaled by 1 + 2^-20. A lookup at u reads texels
+// floor(u n - 0.5) and the next one (tf_lookup_u).
+float channel_majorant_conservative(const oracle_renderer* o, int c, float mn, float mx) {
+  const TF& tf = o->tf[c];
+  const float vlo = o->dom_lo[c], vhi = o->dom_hi[c];
+  const int n = int(tf.texels.size());
+  const float r1 = ((mn - vlo) / (vhi - vlo) - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  const float r2 = ((mx - vlo) / (vhi - vlo) - tf.xf_lo) / (tf.xf_hi - tf.xf_lo);
+  const float a = fminf(r1, r2), b = fmaxf(r1, r2);
+  const float lo = a - 0x1p-12f * (1.0f + fabsf(a)), hi = b + 0x1p-12f * (1.0f + fabsf(b));
+  const int i0 = clamp_index(floorf(lo * float(n) - 0.5f), n);
+  const int i1 = clamp_index_plus1(floorf(hi * float(n) - 0.5f), n);
+  float m = 0.f;
+  for (int i = i0; i <= i1; ++i) m = fmaxf(m, tf.texels[i].w);
+  return m * tf.opacity_scale * (1.0f + 0x1p-20f);
+}
+
 // renderer.cu:206-274 RecalculateDensityRanges (grid part)
 void build_majorants(oracle_renderer* o) {
   const int C = int(o->ch.size());
@@ -603,7 +624,8 @@ void build_majorants(oracle_renderer* o) {
       const float mn = o->ranges[2 * (cell * C + c)];
       const float mx = o->ranges[2 * (cell * C + c) + 1];
       if (mx < mn) continue;  // empty box
-      const float m = channel_majorant(o, c, mn, mx);
+      const float m = o->conservative_maj ? channel_majorant_conservative(o, c, mn, mx)
+                                          : channel_majorant(o, c, mn, mx);
       if (per_channel)
         o->maj[cell * C + c] = m;
       else
@@ -1551,6 +1573,14 @@ int oracle_set_faithful(oracle_renderer* o, int on) {
 
 int oracle_set_jitter_2d(oracle_renderer* o, int on) {
   o->jitter2d = on != 0;
+  return 0;
+}
+
+int oracle_set_conservative_majorants(oracle_renderer* o, int on) {
+  if (o->conservative_maj == (on != 0)) return 0;
+  o->conservative_maj = on != 0;
+  o->accum_id = 0;
+  if (o->inited && tfs_ready(o)) build_majorants(o);
   return 0;
 }
 

@@ -67,6 +67,11 @@ int oracle_set_faithful(oracle_renderer* o, int on);
  * scalar both axes take by default (deviceCode.cu:297-298); mirrors
  * mdwt_set_jitter_2d. */
 int oracle_set_jitter_2d(oracle_renderer* o, int on);
+/* Majorants bounding every opacity a TF lookup of the cell's value range
+ * interpolates, instead of renderer.cu:159-203's (half a texel low at
+ * worst); mirrors mdwt_set_conservative_majorants. Rebuilds them when the
+ * setting changes and restarts the accumulation. */
+int oracle_set_conservative_majorants(oracle_renderer* o, int on);
 /* OpenMP threads for the range-grid build (result does not depend on it) */
 int oracle_set_threads(oracle_renderer* o, int n);
 /* Render n_frames frames; rows [y0, y1) only (y1 <= 0 means all rows).

@@ -42,6 +42,7 @@ _SIG = {
     "oracle_set_threads": ([_h, C.c_int], C.c_int),
     "oracle_set_faithful": ([_h, C.c_int], C.c_int),
     "oracle_set_jitter_2d": ([_h, C.c_int], C.c_int),
+    "oracle_set_conservative_majorants": ([_h, C.c_int], C.c_int),
     "oracle_render": ([_h, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "oracle_reset_accumulation": ([_h], C.c_int),
     "oracle_read_accum": ([_h, _fp, _fp], C.c_int),
@@ -194,6 +195,11 @@ class OracleRenderer:
     def set_jitter_2d(self, on=True):
         """y jitter from word 1 of the jitter block (see oracle_set_jitter_2d)."""
         self._chk(lib.oracle_set_jitter_2d(self._h, int(bool(on))))
+
+    def set_conservative_majorants(self, on=True):
+        """Majorants that bound every interpolated TF opacity of the cell's
+        value range (see oracle_set_conservative_majorants)."""
+        self._chk(lib.oracle_set_conservative_majorants(self._h, int(bool(on))))
 
     def set_lighting(self, light_intensity, ambient):
         self._chk(lib.oracle_set_lighting(self._h, light_intensity, ambient))

@@ -8,7 +8,8 @@ and majorant grid (renderer.cu:159-203, 111-157).
   that macrocell's range (conservative) and the range never exceeds the
   window widened by one more voxel (tight);
 * majorants: brute force over the texel interval, empty cells are 0,
-  mode 3 is the channel sum, opacity scale is linear."""
+  mode 3 is the channel sum, opacity scale is linear; the conservative
+  option bounds every lookup where the reference's falls half a texel low."""
 import numpy as np
 import pytest
 
@@ -112,6 +113,37 @@ def test_majorants_bound_the_texel_interval():
             for f in np.linspace(mn, mx, 9):
                 ta = r.tf(c, f)[3]
                 assert ta <= max(m, a[max(i0 - 1, 0)]) * (1 + 1e-6)
+
+
+def test_conservative_majorants_bound_every_lookup():
+    """oracle_set_conservative_majorants: on a ramp volume (narrow value range
+    per cell) with one opacity spike, the reference's majorants fall below
+    some lookups of a cell's values (the half texel its interval omits); the
+    conservative ones bound every lookup and are never below the reference's.
+    Rebuilt in place when the option changes, for modes 2 and 3."""
+    n, M = 64, 16
+    ramp = np.broadcast_to((np.arange(n, dtype=np.uint8) * 4)[None, None, :], (n, n, n)).copy()
+    tf = np.zeros((256, 4), np.float32)
+    tf[:, :3] = 1.0
+    tf[[23, 87, 150], 3] = 1.0  # just below the cells' range starts 24, 88, 152
+    for mode in (2, 3):
+        r = _build([ramp], (M, M, M), mode=mode, tfs=[tf], os_=1.0)
+        rg = r.read_ranges().reshape(-1, 2)
+        m_ref = r.read_majorants().copy()
+        r.set_conservative_majorants(True)
+        m_con = r.read_majorants()
+        assert np.all(m_con >= m_ref)
+        low = 0
+        for cell in range(M ** 3):
+            mn, mx = rg[cell]
+            if mx < mn:
+                continue
+            ta = np.array([r.tf(0, f)[3] for f in np.linspace(mn, mx, 33)])
+            assert np.all(ta <= m_con[cell]), (cell, ta.max(), m_con[cell])
+            low += int(np.any(ta > m_ref[cell] * (1 + 1e-6)))
+        assert low > 0  # the option changes something the reference gets wrong
+        r.set_conservative_majorants(False)
+        np.testing.assert_array_equal(r.read_majorants(), m_ref)
 
 
 def test_majorants_empty_cells_sum_and_scale():

@@ -290,6 +290,8 @@ int mdwt_set_jitter_2d(mdwt_renderer* h, int on);
  * to half a texel of lerp; 1 the largest opacity of every texel a lookup of
  * the cell's value range reads (the range widened by 2^-12 (1 + |r|), the
  * maximum by 1 + 2^-20): an upper bound, so Woodcock tracking is unbiased.
+ * Where the reference's interval runs a texel past the lookups it is also
+ * lower than the reference's majorant.
  * Rebuilds the majorant grid and restarts the accumulation when the setting
  * changes. No reference counterpart (an option beside its behaviour). */
 int mdwt_set_conservative_majorants(mdwt_renderer* h, int on);

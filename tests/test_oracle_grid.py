@@ -117,9 +117,10 @@ def test_majorants_bound_the_texel_interval():
 
 def test_conservative_majorants_bound_every_lookup():
     """oracle_set_conservative_majorants: on a ramp volume (narrow value range
-    per cell) with one opacity spike, the reference's majorants fall below
-    some lookups of a cell's values (the half texel its interval omits); the
-    conservative ones bound every lookup and are never below the reference's.
+    per cell) with opacity spikes, the reference's majorants fall below some
+    lookups of a cell's values (the half texel its interval omits); the
+    conservative ones bound every lookup. They can also be lower than the
+    reference's, whose interval may run a texel past what lookups read.
     Rebuilt in place when the option changes, for modes 2 and 3."""
     n, M = 64, 16
     ramp = np.broadcast_to((np.arange(n, dtype=np.uint8) * 4)[None, None, :], (n, n, n)).copy()
@@ -132,7 +133,6 @@ def test_conservative_majorants_bound_every_lookup():
         m_ref = r.read_majorants().copy()
         r.set_conservative_majorants(True)
         m_con = r.read_majorants()
-        assert np.all(m_con >= m_ref)
         low = 0
         for cell in range(M ** 3):
             mn, mx = rg[cell]

@@ -239,6 +239,38 @@ def test_autotune_single_view(capi, scene):
     g.close()
 
 
+@pytest.mark.parametrize("launch", [0, 1, 2])
+def test_batch_with_jitter_2d_and_conservative_majorants(capi, scene, launch):
+    """The two options (two-axis jitter, conservative majorants) reach every
+    launch form: each batched view equals a single-view render of the same
+    camera, size, spp and seed with the same options."""
+    views = _views(4, seed=41)
+    print(f"[batch] options launch={launch}", flush=True)
+
+    def setup(r):
+        r.set_jitter_2d(True)
+        r.set_conservative_majorants(True)
+
+    g = capi.MdwtRenderer(0)
+    scenes.load_scene(g, scene, 2)
+    setup(g)
+    g.batch_set_launch(launch)
+    g.batch_set_views(views)
+    g.batch_render(2)
+    g.batch_render(1)
+    for i, v in enumerate(views):
+        s = capi.MdwtRenderer(0)
+        scenes.load_scene(s, scene, 2)
+        setup(s)
+        s.set_frame(v["width"], v["height"], v["spp"], True, (0.0, 0.0, 0.0))
+        s.set_camera(v["from"], v["at"], v["up"], v["fovy"])
+        s.set_seed(v["seed"])
+        s.render(3)
+        np.testing.assert_array_equal(_bits(g.batch_read_view(i)), _bits(s.read_accum()))
+        s.close()
+    g.close()
+
+
 def test_persistent_graph_recaptures_on_changes(capi, scene):
     """Launch form 2 replays one captured frame; changing the frame's
     parameters (lighting, accumulation off, block shape) makes the next call

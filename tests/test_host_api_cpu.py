@@ -74,6 +74,16 @@ int main() {
     }
     for (size_t i = 0; i < n; ++i)
       if (seen[i] != 1) { printf("not a permutation\n"); return 1; }
+    // balance: any two workgroups' sums differ by at most the largest cost
+    std::vector<unsigned long long> sum(G, 0ull);
+    unsigned long long cmax = 0;
+    for (size_t idx = 0; idx < n; ++idx) {
+      const unsigned long long c = mdwt::item_cost(t.data(), src[idx]);
+      sum[idx % G] += c;
+      cmax = std::max(cmax, c);
+    }
+    const auto mm = std::minmax_element(sum.begin(), sum.end());
+    if (trial % 4 != 2 && *mm.second - *mm.first > cmax) { printf("unbalanced\n"); return 1; }
   }
   printf("ok\n");
   return 0;
@@ -84,7 +94,9 @@ int main() {
 def test_batch_deal_orders_like_a_stable_sort(tmp_path):
     """Launch form 2's re-deal: the radix order by decreasing cost equals a
     stable comparison sort's (ties, zero and out-of-order stamps, 64-bit
-    costs), and the dealt list is a permutation in alternating rounds of G."""
+    costs), the dealt list is a permutation in alternating rounds of G, and
+    any two workgroups' cost sums differ by at most the largest cost (except
+    for full 64-bit costs, whose sums overflow)."""
     src = tmp_path / "deal.cpp"
     src.write_text(DEAL_CHECK)
     exe = tmp_path / "deal"
